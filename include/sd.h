@@ -1,0 +1,210 @@
+/*
+ * sd.h -- C ABI of libsd_hip.so, the MI355X (gfx950) implementation of
+ * sparseDist's blocked exact-kNN and SNN path.
+ *
+ * The reference binds its native code through Rcpp-generated .Call routines
+ * (src/RcppExports.cpp:306-333).  Each entry point below names the reference
+ * interface it replaces.  Only plain C types cross this boundary: no R, Rcpp,
+ * Armadillo or torch types, no exceptions.  Every function returns an int
+ * status (SD_OK = 0); the message of the last failure on the calling thread is
+ * available from sd_last_error().
+ *
+ * Conventions shared by every entry point:
+ *   - Sparse input is CSC (dgCMatrix layout): colptr int64[ncol+1], rowind
+ *     int32[nnz] strictly ascending within a column, values double[nnz].
+ *     Explicit zeros are dropped at upload (the reference's dgCMatrix ->
+ *     arma::sp_mat conversion stores none), so Jaccard counts structural
+ *     non-zeros; "degenerate" (all-zero) columns are detected on the values
+ *     as R/sparseKNN.R:214-217 does.
+ *   - kNN results are FNN-style: idx int32 and dist double, nq x k
+ *     column-major (entry (q, t) at t*nq + q), 1-based indices, best first.
+ *     Empty slots hold R's NA_integer_ (INT32_MIN) in idx and R's NA_real_
+ *     (bit pattern 0x7FF00000000007A2) in dist, as src/topK.cpp:96-101, 140-145.
+ *   - Ties are broken by the lower column index across every tile, slab and
+ *     launch (src/topK.cpp:42-49).  Results never depend on block_size, tile
+ *     shape or batching.
+ *   - Output pointers are host memory unless a parameter says otherwise.
+ *   - One context may be used from one host thread at a time; sd_cancel() is
+ *     the exception and may be called from any thread.
+ */
+#ifndef SD_H
+#define SD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes --------------------------------------------------------- */
+enum {
+  SD_OK = 0,
+  SD_EINVAL = 1,       /* invalid argument (text as the reference's R/C++ errors) */
+  SD_ENONFINITE = 2,   /* "Cosine requires finite input (found NA, NaN or Inf)." */
+  SD_ENOMEM = 3,       /* device or host allocation failed */
+  SD_ENOCAND = 4,      /* "No candidate neighbours available." */
+  SD_ECAP = 5,         /* caller buffer too small */
+  SD_ECANCELLED = 6,   /* "Computation interrupted by the user." */
+  SD_EDEVICE = 7,      /* HIP runtime error */
+  SD_EUNSUPPORTED = 8, /* method or option not provided by this library */
+  SD_EBUDGET = 9       /* serving: request larger than the memory budget */
+};
+
+/* ---- methods (R/sparseKNN.R:156-160; "jaccard" is an alias of binary) ----- */
+enum {
+  SD_BINARY = 0,
+  SD_COSINE = 1,
+  SD_EUCLIDEAN = 2,
+  SD_MANHATTAN = 3,
+  SD_PEARSON = 4,
+  SD_COVARIANCE = 5
+};
+
+/* ---- slab arithmetic ------------------------------------------------------ */
+enum {
+  SD_PREC_AUTO = 0,  /* BF16 MFMA + FP64 re-rank where it applies, else exact */
+  SD_PREC_EXACT = 1, /* FP64 merge walk for every pair (the reference's order) */
+  SD_PREC_FP32 = 2,  /* plain FP32 dense slab + FP64 re-rank (baseline path) */
+  SD_PREC_BF16 = 3   /* BF16 MFMA, FP32 accumulate, FP64 re-rank + certificate */
+};
+
+typedef struct sd_context sd_context;
+typedef struct sd_matrix sd_matrix;
+typedef struct sd_graph sd_graph;
+
+const char* sd_last_error(void);
+
+/* Library version string (also records the build's offload target). */
+const char* sd_version(void);
+
+/* ---- context -------------------------------------------------------------- */
+/* Owns the HIP stream, scratch memory and statistics.  device = HIP ordinal. */
+int sd_context_create(int device, sd_context** out);
+int sd_context_destroy(sd_context* ctx);
+
+/* Request cancellation of the running call (reference Progress::check_abort /
+ * stop_if_aborted, src/sparseDist.cpp:177-179).  The running call returns
+ * SD_ECANCELLED and writes no partial output.  Callable from any thread. */
+int sd_cancel(sd_context* ctx);
+
+/* Kernel statistics since the last reset: total device time in ms of the
+ * fused slab kernel and its launch count, measured with HIP events on the
+ * context's stream; uncertified query columns sent to the exact fallback. */
+typedef struct {
+  double slab_ms;
+  int64_t slab_launches;
+  double rerank_ms;
+  int64_t rerank_launches;
+  double snn_ms;
+  int64_t snn_launches;
+  int64_t fallback_queries;
+  int64_t certified_queries;
+} sd_stats;
+int sd_context_stats(sd_context* ctx, sd_stats* out, int reset);
+/* Enable/disable event timing of kernels (off by default). */
+int sd_context_set_timing(sd_context* ctx, int on);
+/* The context's hipStream_t, for callers that order their own work. */
+void* sd_context_stream(sd_context* ctx);
+
+/* ---- matrices ------------------------------------------------------------- */
+/* Copy a CSC matrix to device memory.  Replaces the per-call dgCMatrix ->
+ * arma::sp_mat conversion at the reference's .Call boundary
+ * (src/RcppExports.cpp:124-137, 229-242; R/sparseDist.R:198-202): the matrix
+ * stays resident and its derived layouts (norms, densified tiles) are cached
+ * with it. */
+int sd_matrix_upload_csc(sd_context* ctx, int64_t nrow, int64_t ncol, const int64_t* colptr,
+                         const int32_t* rowind, const double* values, sd_matrix** out);
+int sd_matrix_free(sd_matrix* m);
+/* Drop cached derived layouts (the CSC itself stays resident). */
+int sd_matrix_drop_cache(sd_matrix* m);
+
+/* ---- kNN ------------------------------------------------------------------ */
+typedef struct {
+  int32_t k;               /* neighbours requested (>= 1) */
+  int32_t method;          /* SD_BINARY ... SD_COVARIANCE */
+  int32_t dist;            /* 1 = distance form, 0 = similarity form */
+  int32_t include_self;    /* self-search only (ignored when Y != NULL) */
+  int32_t drop_degenerate; /* mask all-zero columns on both sides */
+  int32_t precision;       /* SD_PREC_* */
+  int32_t margin;          /* extra candidates re-ranked in FP64 (0 = default) */
+  int32_t reserved;
+  int64_t block_size;      /* query columns per slab: a memory hint only */
+} sd_knn_params;
+
+/* Blocked exact kNN of the columns of X among the columns of Y (Y == NULL:
+ * self-search among X).  Replaces R/sparseKNN.R:110-313 after argument
+ * validation, i.e. the block loop over sparseDist() (fastJacc2
+ * src/sparseDist.cpp:559-617, fastCosine2 967-1000, fastEuclidean2 833-878,
+ * fastManhattan2 734-779, fastCorr2 403-441, fastCov2 481-510) and topKBlock
+ * (src/topK.cpp:67-149), fused so the slab never leaves the device.
+ * idx/dist have room for X.ncol * params.k entries; the result occupies the
+ * first X.ncol * (*k_eff) entries.  *warn = 1 when k was clamped to the
+ * number of available candidates (the caller raises the reference's
+ * "k (K) exceeds the A available neighbours; returning A." warning). */
+int sd_knn(sd_context* ctx, const sd_matrix* X, const sd_matrix* Y, const sd_knn_params* params,
+           int32_t* idx, double* dist, int32_t* k_eff, int32_t* warn);
+
+/* Both directions (R/sparseKNNCross.R:44-199): forward = kNN of X among Y,
+ * reverse = kNN of Y among X, each bit-identical to the corresponding sd_knn
+ * call (the reference's equivalence contract, R/sparseKNNCross.R:33-40).
+ * Replaces the slab + topKBlock + topKRowsAccum (src/topKRows.cpp:104-248)
+ * loop.  warn bit 0: forward k clamped; bit 1: reverse k clamped. */
+int sd_knn_cross(sd_context* ctx, const sd_matrix* X, const sd_matrix* Y,
+                 const sd_knn_params* params, int32_t* fwd_idx, double* fwd_dist, int32_t* k_fwd,
+                 int32_t* rev_idx, double* rev_dist, int32_t* k_rev, int32_t* warn);
+
+/* ---- one-to-one replacements of the reference's native helpers ------------ */
+/* Dense slab D (X.ncol x Y.ncol, column-major, host memory) in FP64 with the
+ * reference's operation order.  Replaces fastJacc2 / fastCosine2 /
+ * fastEuclidean2 / fastManhattan2 / fastCorr2 / fastCov2
+ * (src/RcppExports.cpp:80-93, 110-137, 199-242; R/RcppExports.R:24-66) with
+ * m = X and m2 = Y. */
+int sd_slab(sd_context* ctx, int32_t method, const sd_matrix* X, const sd_matrix* Y, int32_t dist,
+            double* D);
+
+/* Column-wise top-k of a dense host block D (nrow x ncol).  Replaces
+ * topKBlock (src/topK.cpp:67-149, src/RcppExports.cpp:273-286).  self_row:
+ * per column, the 1-based row to exclude or 0.  idx/val are ncol x k. */
+int sd_topk_block(sd_context* ctx, const double* D, int64_t nrow, int64_t ncol, int32_t k,
+                  int32_t decreasing, const int32_t* self_row, int32_t* idx, double* val);
+
+/* Fold a dense host block into a running per-row top-k held in host memory
+ * (best_idx/best_val, nrow x k, updated in place).  Replaces topKRowsAccum
+ * (src/topKRows.cpp:104-248, src/RcppExports.cpp:288-304). */
+int sd_topk_rows_accum(sd_context* ctx, const double* D, int64_t nrow, int64_t ncol, int32_t k,
+                       int32_t decreasing, int32_t* best_idx, double* best_val,
+                       int64_t col_offset, const int32_t* self_col);
+
+/* ---- SNN ------------------------------------------------------------------ */
+/* Jaccard weights over the kNN edge set: idx n x k (1-based, NA_integer_
+ * padding) -> w n x k (NA_real_ where undefined).  Replaces snnJaccard
+ * (src/snn.cpp:198-250, src/RcppExports.cpp:35-47). */
+int sd_snn_jaccard(sd_context* ctx, const int32_t* idx, int32_t n, int32_t k,
+                   int32_t include_self, double* w);
+
+/* The SNN graph as an n x n sparse matrix on the device.  edges: 0 = "knn"
+ * (weights over the kNN edge set, strict w > prune, duplicates collapsed,
+ * optional union symmetrisation; R/sparseSNN.R:221-250), 1 = "shared"
+ * (every pair sharing a neighbour, pruned in the kernel; replaces
+ * snnJaccardShared src/snn.cpp:299-447 and R/sparseSNN.R:206-218).
+ * idx_on_device: idx is a device pointer (e.g. an sd_knn result kept on the
+ * GPU) rather than host memory. */
+int sd_snn(sd_context* ctx, const int32_t* idx, int32_t n, int32_t k, int32_t include_self,
+           double prune, int32_t symmetrise, int32_t edges, int32_t idx_on_device,
+           sd_graph** out);
+/* Size of the graph: n and stored entries. */
+int sd_graph_info(const sd_graph* g, int64_t* n, int64_t* nnz);
+/* Copy the graph to host memory in dgCMatrix layout: colptr int64[n+1],
+ * rowind int32[nnz] (0-based, ascending per column), val double[nnz]. */
+int sd_graph_copy(const sd_graph* g, int64_t* colptr, int32_t* rowind, double* val);
+int sd_graph_free(sd_graph* g);
+
+/* Number of usable host cores minus one, at least 1 (R/sparseDist.R:221-224
+ * over detectCoresCpp, src/detectCores.cpp:29-34).  The GPU paths ignore
+ * ncores; it is kept for the R signatures. */
+int sd_detect_cores(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SD_H */

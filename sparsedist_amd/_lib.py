@@ -1,0 +1,98 @@
+"""ctypes binding of libsd_hip.so (the C ABI declared in include/sd.h).
+
+The library is built in-tree (``make -C sparsedist_amd``, or
+``__graft_entry__.build()``).  There is no fallback: if the shared library is
+missing or no GPU is visible, loading raises ``SparseDistError``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import threading
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsd_hip.so")
+
+SD_OK, SD_EINVAL, SD_ENONFINITE, SD_ENOMEM, SD_ENOCAND, SD_ECAP, SD_ECANCELLED, \
+    SD_EDEVICE, SD_EUNSUPPORTED, SD_EBUDGET = range(10)
+SD_BINARY, SD_COSINE, SD_EUCLIDEAN, SD_MANHATTAN, SD_PEARSON, SD_COVARIANCE = range(6)
+SD_PREC_AUTO, SD_PREC_EXACT, SD_PREC_FP32, SD_PREC_BF16 = range(4)
+
+# every symbol include/sd.h declares
+EXPORTS = (
+    "sd_last_error", "sd_version", "sd_context_create", "sd_context_destroy", "sd_cancel",
+    "sd_context_stats", "sd_context_set_timing", "sd_context_stream", "sd_matrix_upload_csc",
+    "sd_matrix_free", "sd_matrix_drop_cache", "sd_knn", "sd_knn_cross", "sd_slab",
+    "sd_topk_block", "sd_topk_rows_accum", "sd_snn_jaccard", "sd_snn", "sd_graph_info",
+    "sd_graph_copy", "sd_graph_free", "sd_detect_cores",
+)
+
+
+class SparseDistError(RuntimeError):
+    """An error reported by libsd_hip.so; ``code`` is the SD_* status."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(msg)
+        self.code = code
+
+
+class KnnParams(C.Structure):
+    _fields_ = [("k", C.c_int32), ("method", C.c_int32), ("dist", C.c_int32),
+                ("include_self", C.c_int32), ("drop_degenerate", C.c_int32),
+                ("precision", C.c_int32), ("margin", C.c_int32), ("reserved", C.c_int32),
+                ("block_size", C.c_int64)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("slab_ms", C.c_double), ("slab_launches", C.c_int64),
+                ("rerank_ms", C.c_double), ("rerank_launches", C.c_int64),
+                ("snn_ms", C.c_double), ("snn_launches", C.c_int64),
+                ("fallback_queries", C.c_int64), ("certified_queries", C.c_int64)]
+
+
+_lib = None
+_lock = threading.Lock()
+
+
+def load(path: str = LIB_PATH):
+    """Load the shared library once; raise if it is not built."""
+    global _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(path):
+            raise SparseDistError(SD_EDEVICE, f"{path} is not built; run make -C sparsedist_amd")
+        L = C.CDLL(path)
+        vp, i32, i64, dbl = C.c_void_p, C.c_int32, C.c_int64, C.c_double
+        L.sd_last_error.restype = C.c_char_p
+        L.sd_version.restype = C.c_char_p
+        L.sd_context_create.argtypes = [C.c_int, C.POINTER(vp)]
+        L.sd_context_destroy.argtypes = [vp]
+        L.sd_cancel.argtypes = [vp]
+        L.sd_context_stats.argtypes = [vp, C.POINTER(Stats), C.c_int]
+        L.sd_context_set_timing.argtypes = [vp, C.c_int]
+        L.sd_context_stream.argtypes = [vp]
+        L.sd_context_stream.restype = vp
+        L.sd_matrix_upload_csc.argtypes = [vp, i64, i64, vp, vp, vp, C.POINTER(vp)]
+        L.sd_matrix_free.argtypes = [vp]
+        L.sd_matrix_drop_cache.argtypes = [vp]
+        L.sd_knn.argtypes = [vp, vp, vp, C.POINTER(KnnParams), vp, vp, C.POINTER(i32),
+                             C.POINTER(i32)]
+        L.sd_knn_cross.argtypes = [vp, vp, vp, C.POINTER(KnnParams), vp, vp, C.POINTER(i32), vp,
+                                   vp, C.POINTER(i32), C.POINTER(i32)]
+        L.sd_slab.argtypes = [vp, i32, vp, vp, i32, vp]
+        L.sd_topk_block.argtypes = [vp, vp, i64, i64, i32, i32, vp, vp, vp]
+        L.sd_topk_rows_accum.argtypes = [vp, vp, i64, i64, i32, i32, vp, vp, i64, vp]
+        L.sd_snn_jaccard.argtypes = [vp, vp, i32, i32, i32, vp]
+        L.sd_snn.argtypes = [vp, vp, i32, i32, i32, dbl, i32, i32, i32, C.POINTER(vp)]
+        L.sd_graph_info.argtypes = [vp, C.POINTER(i64), C.POINTER(i64)]
+        L.sd_graph_copy.argtypes = [vp, vp, vp, vp]
+        L.sd_graph_free.argtypes = [vp]
+        L.sd_detect_cores.argtypes = []
+        _lib = L
+        return L
+
+
+def check(rc: int) -> None:
+    if rc != SD_OK:
+        raise SparseDistError(rc, load().sd_last_error().decode())

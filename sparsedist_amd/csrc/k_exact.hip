@@ -1,0 +1,179 @@
+// k_exact.hip -- FP64 slab with the reference's operation order, and the
+// exact column-wise top-k used by every path that materialises a slab
+// (the exact method path, the certificate fallback and the one-to-one
+// replacements of topKBlock / topKRowsAccum).
+#include "sd_internal.h"
+#include "pair_math.h"
+#include "topk_block.h"
+
+namespace sd {
+namespace {
+
+constexpr int kStageMax = 2048;  // query entries staged in LDS
+
+// grid: (ceil(nref / 256), nb); thread = reference column.
+__global__ __launch_bounds__(256) void k_exact_slab(ExactArgs A, const int32_t* __restrict__ qcols,
+                                                    double* __restrict__ D, int64_t ld,
+                                                    int mask) {
+  __shared__ int32_t s_r[kStageMax];
+  __shared__ double s_v[kStageMax];
+  const int64_t b = blockIdx.y;
+  const int64_t j = qcols ? qcols[b] : b;
+  const int64_t q0 = A.qry.p[j], qn = A.qry.p[j + 1] - q0;
+  const double* qvals = (A.method == SD_COSINE) ? A.qry_unit : A.qry.x;
+  const int32_t* qr = A.qry.i + q0;
+  const double* qv = qvals + q0;
+  if (qn <= kStageMax) {
+    for (int64_t t = threadIdx.x; t < qn; t += blockDim.x) {
+      s_r[t] = qr[t];
+      s_v[t] = qv[t];
+    }
+    __syncthreads();
+    qr = s_r;
+    qv = s_v;
+  }
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= A.ref.ncol) return;
+  double v;
+  if (mask && A.ref_deg && A.ref_deg[i]) {
+    v = __longlong_as_double((long long)kNaRealBits);
+  } else if (mask && A.self_mode && i == j) {
+    if (A.include_self) {
+      // exact self value restored (R/sparseKNN.R:270-279)
+      if (A.method == SD_BINARY || A.method == SD_COSINE || A.method == SD_PEARSON)
+        v = A.dist ? 0.0 : 1.0;
+      else if (A.method == SD_EUCLIDEAN || A.method == SD_MANHATTAN)
+        v = 0.0;
+      else
+        v = exact_pair(A, i, qr, qv, qn, j);
+    } else {
+      v = __longlong_as_double((long long)kNaRealBits);  // excluded like self_row
+    }
+  } else {
+    v = exact_pair(A, i, qr, qv, qn, j);
+  }
+  D[i + b * ld] = v;
+}
+
+// One workgroup per column of D.
+template <int CAP>
+__global__ __launch_bounds__(256) void k_topk_dense(const double* __restrict__ D, int64_t nrow,
+                                                    int64_t ld, int k, int decreasing,
+                                                    const int32_t* __restrict__ skip,
+                                                    int32_t* __restrict__ out_idx,
+                                                    double* __restrict__ out_val, int64_t out_ld,
+                                                    const int32_t* __restrict__ outcol) {
+  __shared__ BlockTopKStorage<CAP> st;
+  const int64_t b = blockIdx.x;
+  const double* col = D + b * ld;
+  const int64_t sk = skip ? (int64_t)skip[b] : -1;
+  BlockTopK<CAP> tk(st, k, decreasing);
+  tk.run(nrow, [&](int64_t r, double& v, int64_t& id) {
+    v = col[r];
+    id = r;
+    return r != sk;
+  });
+  const int64_t oc = outcol ? (int64_t)outcol[b] : b;
+  tk.write(out_idx + oc, out_val + oc, out_ld, 1);
+}
+
+// Running row-wise top-k: row r of D (strided by ld across columns) merged
+// with its seed state (nrow x k, column-major, 1-based global indices).
+template <int CAP>
+__global__ __launch_bounds__(256) void k_topk_rows(const double* __restrict__ D, int64_t nrow,
+                                                   int64_t ncol, int k, int decreasing,
+                                                   int32_t* __restrict__ best_idx,
+                                                   double* __restrict__ best_val,
+                                                   int64_t col_offset,
+                                                   const int32_t* __restrict__ self_col) {
+  __shared__ BlockTopKStorage<CAP> st;
+  const int64_t r = blockIdx.x;
+  BlockTopK<CAP> tk(st, k, decreasing);
+  // seed: slots fill left to right until the first NA (src/topKRows.cpp:187-199)
+  int nseed = 0;
+  while (nseed < k) {
+    const int32_t gi = best_idx[(int64_t)nseed * nrow + r];
+    if (gi == kNaInt || gi <= 0) break;
+    ++nseed;
+  }
+  const int64_t sc = self_col[r];
+  tk.run(nseed + ncol, [&](int64_t t, double& v, int64_t& id) {
+    if (t < nseed) {
+      v = best_val[t * nrow + r];
+      id = (int64_t)best_idx[t * nrow + r] - 1;
+      return true;
+    }
+    const int64_t b = t - nseed;
+    v = D[r + b * nrow];
+    id = col_offset + b;
+    return (id + 1) != sc;
+  });
+  tk.write(best_idx + r, best_val + r, nrow, 1);
+}
+
+}  // namespace
+
+int exact_slab(sd_context* ctx, const ExactArgs& a, const int32_t* d_qcols, int64_t nb, double* D,
+               int64_t ld, int mask) {
+  if (nb <= 0 || a.ref.ncol <= 0) return SD_OK;
+  const int64_t chunk = 65535;  // grid.y limit
+  for (int64_t b0 = 0; b0 < nb; b0 += chunk) {
+    const int64_t n = (nb - b0 < chunk) ? nb - b0 : chunk;
+    dim3 grid((unsigned)((a.ref.ncol + 255) / 256), (unsigned)n);
+    hipLaunchKernelGGL(k_exact_slab, grid, dim3(256), 0, ctx->stream, a,
+                       d_qcols ? d_qcols + b0 : nullptr, D + b0 * ld, ld, mask);
+    SD_HIP(hipGetLastError());
+  }
+  return SD_OK;
+}
+
+static int cap_for(int k) {
+  if (k + 256 <= 512) return 512;
+  if (k + 256 <= 1024) return 1024;
+  if (k + 256 <= 2048) return 2048;
+  return 0;
+}
+
+int topk_dense(sd_context* ctx, const double* D, int64_t nrow, int64_t ncol, int64_t ld, int k,
+               int decreasing, const int32_t* d_skip, int32_t* out_idx, double* out_val,
+               int64_t out_ld, const int32_t* d_outcol) {
+  if (ncol <= 0) return SD_OK;
+  const int cap = cap_for(k);
+  if (!cap) return set_error(SD_EUNSUPPORTED, "k = %d exceeds this library's limit of 1792", k);
+  for (int64_t c0 = 0; c0 < ncol; c0 += 1 << 30) {
+    const int64_t n = (ncol - c0 < (1 << 30)) ? ncol - c0 : (1 << 30);
+#define SD_LAUNCH_TOPK(C)                                                                        \
+  hipLaunchKernelGGL(k_topk_dense<C>, dim3((unsigned)n), dim3(256), 0, ctx->stream, D + c0 * ld, \
+                     nrow, ld, k, decreasing, d_skip ? d_skip + c0 : nullptr,                    \
+                     d_outcol ? out_idx : out_idx + c0, d_outcol ? out_val : out_val + c0, out_ld, \
+                     d_outcol ? d_outcol + c0 : nullptr)
+    switch (cap) {
+      case 512: SD_LAUNCH_TOPK(512); break;
+      case 1024: SD_LAUNCH_TOPK(1024); break;
+      default: SD_LAUNCH_TOPK(2048); break;
+    }
+#undef SD_LAUNCH_TOPK
+    SD_HIP(hipGetLastError());
+  }
+  return SD_OK;
+}
+
+int topk_rows(sd_context* ctx, const double* D, int64_t nrow, int64_t ncol, int k, int decreasing,
+              int32_t* best_idx, double* best_val, int64_t col_offset, const int32_t* self_col) {
+  if (nrow <= 0) return SD_OK;
+  const int cap = cap_for(k);
+  if (!cap) return set_error(SD_EUNSUPPORTED, "k = %d exceeds this library's limit of 1792", k);
+#define SD_LAUNCH_ROWS(C)                                                                   \
+  hipLaunchKernelGGL(k_topk_rows<C>, dim3((unsigned)nrow), dim3(256), 0, ctx->stream, D, nrow, \
+                     ncol, k, decreasing, best_idx, best_val, col_offset, self_col)
+  switch (cap) {
+    case 512: SD_LAUNCH_ROWS(512); break;
+    case 1024: SD_LAUNCH_ROWS(1024); break;
+    default: SD_LAUNCH_ROWS(2048); break;
+  }
+#undef SD_LAUNCH_ROWS
+  SD_HIP(hipGetLastError());
+  return SD_OK;
+}
+
+}  // namespace sd

@@ -1,0 +1,232 @@
+// k_prep.hip -- per-column statistics and densified layouts of a resident
+// CSC matrix.  Everything here runs once per matrix and is cached with it.
+#include <hip/hip_bf16.h>
+
+#include "sd_internal.h"
+
+namespace sd {
+namespace {
+
+// Cosine unit normalisation, one thread per column (reference UnitCsc /
+// unit_csc, src/sparseDist.cpp:235-284).  The scaled sum-of-squares
+// recurrence is order dependent in FP64, so it runs sequentially over the
+// column's stored values, exactly as the reference does; the division by
+// scale and then by sqrt(ssq) is elementwise and runs in a second kernel.
+__global__ void k_unit_scale(const int64_t* __restrict__ p, const double* __restrict__ x,
+                             int64_t ncol, double* __restrict__ scale_out,
+                             double* __restrict__ sq_out) {
+  const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (j >= ncol) return;
+  double scale = 0.0, ssq = 1.0;
+  for (int64_t e = p[j], ee = p[j + 1]; e < ee; ++e) {
+    const double v = x[e];
+    if (v == 0.0) continue;
+    const double a = fabs(v);
+    if (scale < a) {
+      const double r = scale / a;
+      ssq = 1.0 + ssq * r * r;
+      scale = a;
+    } else {
+      const double r = a / scale;
+      ssq += r * r;
+    }
+  }
+  scale_out[j] = scale;
+  sq_out[j] = sqrt(ssq);
+}
+
+// One wave per column: u = (v / scale) / sqrt(ssq), 0 for an all-zero column.
+__global__ void k_unit_vals(const int64_t* __restrict__ p, const double* __restrict__ x,
+                            int64_t ncol, const double* __restrict__ scale,
+                            const double* __restrict__ sq, double* __restrict__ u) {
+  const int lane = threadIdx.x & 63;
+  const int64_t j = blockIdx.x * (int64_t)(blockDim.x / 64) + (threadIdx.x >> 6);
+  if (j >= ncol) return;
+  const double s = scale[j], q = sq[j];
+  for (int64_t e = p[j] + lane, ee = p[j + 1]; e < ee; e += 64)
+    u[e] = (s == 0.0) ? 0.0 : (x[e] / s) / q;
+}
+
+// Column mean and centred sum of squares over all nrow entries, sequential
+// in stored order (this project's pearson / covariance definition; the
+// reference calls Armadillo's cor()/cov(), src/sparseDist.cpp:425, 503).
+__global__ void k_centre(const int64_t* __restrict__ p, const double* __restrict__ x, int64_t ncol,
+                         int64_t nrow, double* __restrict__ mean, double* __restrict__ css) {
+  const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (j >= ncol) return;
+  const double r = (double)nrow;
+  double s = 0.0;
+  for (int64_t e = p[j], ee = p[j + 1]; e < ee; ++e) s += x[e];
+  const double mu = s / r;
+  double q = 0.0;
+  for (int64_t e = p[j], ee = p[j + 1]; e < ee; ++e) {
+    const double d = x[e] - mu;
+    q += d * d;
+  }
+  q += (double)(nrow - (p[j + 1] - p[j])) * (mu * mu);
+  mean[j] = mu;
+  css[j] = q;
+}
+
+// Sum of squares per column (euclidean candidate filter only; not on the
+// exact path, so the order of summation is free).
+__global__ void k_sqnorm(const int64_t* __restrict__ p, const double* __restrict__ x, int64_t ncol,
+                         double* __restrict__ out) {
+  const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (j >= ncol) return;
+  double s = 0.0;
+  for (int64_t e = p[j], ee = p[j + 1]; e < ee; ++e) s += x[e] * x[e];
+  out[j] = s;
+}
+
+__device__ inline uint16_t f32_to_bf16_rne(float f) {
+  uint32_t b = __float_as_uint(f);
+  const uint32_t lsb = (b >> 16) & 1u;
+  b += 0x7FFFu + lsb;  // round to nearest even (inputs are finite)
+  return (uint16_t)(b >> 16);
+}
+
+// Densify column chunks into [ncol][kpad] column-contiguous storage.  One
+// workgroup per (column, K chunk): the chunk is zeroed in LDS, the column's
+// stored entries that fall into it are scattered, and the chunk is written
+// out with 16-byte stores.  ELEM bytes per element.
+template <int KIND, int ELEM>
+__global__ __launch_bounds__(256) void k_densify(const int64_t* __restrict__ p,
+                                                 const int32_t* __restrict__ ri,
+                                                 const double* __restrict__ vals, int64_t ncol,
+                                                 int64_t kpad, int64_t chunk,
+                                                 uint8_t* __restrict__ out) {
+  extern __shared__ __align__(16) uint8_t lds[];
+  const int64_t nchunks = (kpad + chunk - 1) / chunk;
+  const int64_t j = blockIdx.x / nchunks;
+  const int64_t c = blockIdx.x % nchunks;
+  if (j >= ncol) return;
+  const int64_t k0 = c * chunk;
+  const int64_t len = (kpad - k0 < chunk) ? (kpad - k0) : chunk;  // multiple of 64
+  const int64_t bytes = len * ELEM;
+  uint4* l4 = reinterpret_cast<uint4*>(lds);
+  for (int64_t t = threadIdx.x; t < bytes / 16; t += blockDim.x) l4[t] = make_uint4(0, 0, 0, 0);
+  __syncthreads();
+  for (int64_t e = p[j] + threadIdx.x, ee = p[j + 1]; e < ee; e += blockDim.x) {
+    const int64_t r = ri[e];
+    if (r < k0 || r >= k0 + len) continue;
+    const int64_t o = r - k0;
+    if (KIND == kDenseI8Pattern) {
+      lds[o] = 1;
+    } else if (KIND == kDenseBf16Unit || KIND == kDenseBf16Raw) {
+      reinterpret_cast<uint16_t*>(lds)[o] = f32_to_bf16_rne((float)vals[e]);
+    } else if (KIND == kDenseF32Pattern) {
+      reinterpret_cast<float*>(lds)[o] = 1.0f;
+    } else {
+      reinterpret_cast<float*>(lds)[o] = (float)vals[e];
+    }
+  }
+  __syncthreads();
+  uint4* dst = reinterpret_cast<uint4*>(out + (j * kpad + k0) * ELEM);
+  for (int64_t t = threadIdx.x; t < bytes / 16; t += blockDim.x) dst[t] = l4[t];
+}
+
+}  // namespace
+
+int prep_unit(sd_context* ctx, sd_matrix* m) {
+  if (m->unit_ready) return SD_OK;
+  SD_TRY(m->unit.alloc(sizeof(double) * (size_t)(m->nnz > 0 ? m->nnz : 1)));
+  SD_TRY(m->scale.alloc(sizeof(double) * (size_t)(m->ncol + 1)));
+  DevBuf sq;
+  SD_TRY(sq.alloc(sizeof(double) * (size_t)(m->ncol + 1)));
+  const int64_t nb = (m->ncol + 255) / 256;
+  hipLaunchKernelGGL(k_unit_scale, dim3((unsigned)nb), dim3(256), 0, ctx->stream, m->p.as<int64_t>(),
+                     m->x.as<double>(), m->ncol, m->scale.as<double>(), sq.as<double>());
+  const int64_t nw = (m->ncol + 3) / 4;
+  hipLaunchKernelGGL(k_unit_vals, dim3((unsigned)nw), dim3(256), 0, ctx->stream, m->p.as<int64_t>(),
+                     m->x.as<double>(), m->ncol, m->scale.as<double>(), sq.as<double>(),
+                     m->unit.as<double>());
+  SD_HIP(hipGetLastError());
+  SD_HIP(hipStreamSynchronize(ctx->stream));
+  m->unit_ready = true;
+  return SD_OK;
+}
+
+int prep_centre(sd_context* ctx, sd_matrix* m) {
+  if (m->centre_ready) return SD_OK;
+  SD_TRY(m->mean.alloc(sizeof(double) * (size_t)(m->ncol + 1)));
+  SD_TRY(m->css.alloc(sizeof(double) * (size_t)(m->ncol + 1)));
+  const int64_t nb = (m->ncol + 255) / 256;
+  hipLaunchKernelGGL(k_centre, dim3((unsigned)nb), dim3(256), 0, ctx->stream, m->p.as<int64_t>(),
+                     m->x.as<double>(), m->ncol, m->nrow, m->mean.as<double>(), m->css.as<double>());
+  SD_HIP(hipGetLastError());
+  SD_HIP(hipStreamSynchronize(ctx->stream));
+  m->centre_ready = true;
+  return SD_OK;
+}
+
+int prep_sq(sd_context* ctx, sd_matrix* m) {
+  if (m->sq_ready) return SD_OK;
+  SD_TRY(m->sqn.alloc(sizeof(double) * (size_t)(m->ncol + 1)));
+  const int64_t nb = (m->ncol + 255) / 256;
+  hipLaunchKernelGGL(k_sqnorm, dim3((unsigned)nb), dim3(256), 0, ctx->stream, m->p.as<int64_t>(),
+                     m->x.as<double>(), m->ncol, m->sqn.as<double>());
+  SD_HIP(hipGetLastError());
+  SD_HIP(hipStreamSynchronize(ctx->stream));
+  m->sq_ready = true;
+  return SD_OK;
+}
+
+int prep_dense(sd_context* ctx, sd_matrix* m, int kind, int64_t kpad) {
+  if (kind < 0 || kind >= kDenseKinds) return set_error(SD_EINVAL, "internal: dense kind %d", kind);
+  if (m->dense_kpad[kind] == kpad && m->dense[kind].ptr) return SD_OK;
+  const int elem = (kind == kDenseI8Pattern) ? 1 : (kind == kDenseBf16Unit || kind == kDenseBf16Raw) ? 2 : 4;
+  m->dense_kpad[kind] = 0;
+  SD_TRY(m->dense[kind].alloc((size_t)m->ncol * (size_t)kpad * (size_t)elem + 256));
+  uint8_t* dst = m->dense[kind].as<uint8_t>();
+  const double* vals = m->x.as<double>();
+  if (kind == kDenseBf16Unit || kind == kDenseF32Unit) {
+    SD_TRY(prep_unit(ctx, m));
+    vals = m->unit.as<double>();
+  }
+  const int64_t chunk_bytes = 64 * 1024;
+  const int64_t chunk = chunk_bytes / elem;  // multiple of 64 elements
+  const int64_t nchunks = (kpad + chunk - 1) / chunk;
+  const int64_t grid = m->ncol * nchunks;
+  const size_t lds = (size_t)((kpad < chunk ? kpad : chunk) * elem);
+  if (grid > 0) {
+    switch (kind) {
+      case kDenseI8Pattern:
+        hipLaunchKernelGGL((k_densify<kDenseI8Pattern, 1>), dim3((unsigned)grid), dim3(256), lds,
+                           ctx->stream, m->p.as<int64_t>(), m->i.as<int32_t>(), vals, m->ncol, kpad,
+                           chunk, dst);
+        break;
+      case kDenseBf16Unit:
+        hipLaunchKernelGGL((k_densify<kDenseBf16Unit, 2>), dim3((unsigned)grid), dim3(256), lds,
+                           ctx->stream, m->p.as<int64_t>(), m->i.as<int32_t>(), vals, m->ncol, kpad,
+                           chunk, dst);
+        break;
+      case kDenseBf16Raw:
+        hipLaunchKernelGGL((k_densify<kDenseBf16Raw, 2>), dim3((unsigned)grid), dim3(256), lds,
+                           ctx->stream, m->p.as<int64_t>(), m->i.as<int32_t>(), vals, m->ncol, kpad,
+                           chunk, dst);
+        break;
+      case kDenseF32Unit:
+        hipLaunchKernelGGL((k_densify<kDenseF32Unit, 4>), dim3((unsigned)grid), dim3(256), lds,
+                           ctx->stream, m->p.as<int64_t>(), m->i.as<int32_t>(), vals, m->ncol, kpad,
+                           chunk, dst);
+        break;
+      case kDenseF32Pattern:
+        hipLaunchKernelGGL((k_densify<kDenseF32Pattern, 4>), dim3((unsigned)grid), dim3(256), lds,
+                           ctx->stream, m->p.as<int64_t>(), m->i.as<int32_t>(), vals, m->ncol, kpad,
+                           chunk, dst);
+        break;
+      default:
+        hipLaunchKernelGGL((k_densify<kDenseF32Raw, 4>), dim3((unsigned)grid), dim3(256), lds,
+                           ctx->stream, m->p.as<int64_t>(), m->i.as<int32_t>(), vals, m->ncol, kpad,
+                           chunk, dst);
+        break;
+    }
+    SD_HIP(hipGetLastError());
+  }
+  SD_HIP(hipStreamSynchronize(ctx->stream));
+  m->dense_kpad[kind] = kpad;
+  return SD_OK;
+}
+
+}  // namespace sd

@@ -1,0 +1,596 @@
+// k_slab.hip -- the fused slab kernel: densified reference x query tiles on
+// MFMA with a per-query running top-L kept in registers, so the
+// n_ref x n_query slab (the reference's dense block, R/sparseKNN.R:256-258)
+// never exists in memory.  Plus the kernels that finish a search: merging
+// the per-split candidate lists, the FP64 re-rank with its certificate, and
+// writing FNN-layout results.
+//
+// Workgroup = WM x 4 waves.  Wave (wm, wn) computes 128 reference columns x
+// 64 query columns with 32x32 MFMA blocks (4 along refs, 2 along queries).
+// After each reference tile the two lanes holding the same query exchange
+// values (lane l owns query block 0, lane l+32 query block 1), so one lane
+// owns one query and keeps its L best candidates sorted in registers.
+#include <hip/hip_bf16.h>
+
+#include "pair_math.h"
+#include "sd_internal.h"
+#include "topk_block.h"
+
+namespace sd {
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v16i __attribute__((ext_vector_type(16)));
+typedef float v16f __attribute__((ext_vector_type(16)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+
+namespace {
+
+constexpr int kTileRefs = 128;  // reference columns per wave row
+constexpr int kRowBytes = 64;   // K bytes per row per chunk
+constexpr int kLdsStride = 80;  // padded LDS row stride (bytes)
+
+// ---- MFMA operand types ------------------------------------------------------
+struct OpI8 {  // 0/1 patterns, exact int32 counts
+  using Acc = v16i;
+  static constexpr int kElem = 1;
+  __device__ static void mma(Acc& c, const uint4 a[2], const uint4 b[2]) {
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      v4i av, bv;
+      __builtin_memcpy(&av, &a[s], 16);
+      __builtin_memcpy(&bv, &b[s], 16);
+      c = __builtin_amdgcn_mfma_i32_32x32x32_i8(av, bv, c, 0, 0, 0);
+    }
+  }
+};
+
+struct OpBf16 {  // BF16 values, FP32 accumulation
+  using Acc = v16f;
+  static constexpr int kElem = 2;
+  __device__ static void mma(Acc& c, const uint4 a[2], const uint4 b[2]) {
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      v8bf av, bv;
+      __builtin_memcpy(&av, &a[s], 16);
+      __builtin_memcpy(&bv, &b[s], 16);
+      c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, c, 0, 0, 0);
+    }
+  }
+};
+
+struct OpF32 {  // plain FP32 values and products
+  using Acc = v16f;
+  static constexpr int kElem = 4;
+  __device__ static void mma(Acc& c, const uint4 a[2], const uint4 b[2]) {
+    float av[8], bv[8];
+    __builtin_memcpy(av, a, 32);
+    __builtin_memcpy(bv, b, 32);
+#pragma unroll
+    for (int e = 0; e < 8; ++e) c = __builtin_amdgcn_mfma_f32_32x32x2f32(av[e], bv[e], c, 0, 0, 0);
+  }
+};
+
+// ---- candidate keys --------------------------------------------------------------
+// Jaccard: the exact rational sim = c / d (d > 0; two empty columns are 1/1,
+// src/sparseDist.cpp:551-556).  Distance and similarity forms rank by the
+// same order of sim (1 - sim is monotone, and distinct rationals with
+// denominators below 2^26 stay distinct and ordered in FP64), so ranking by
+// the rational is ranking by the reference's double.
+struct KeyJ {
+  uint32_t c, d;
+};
+__device__ inline bool better(const KeyJ& a, uint32_t ia, const KeyJ& b, uint32_t ib) {
+  const uint64_t l = (uint64_t)a.c * b.d, r = (uint64_t)b.c * a.d;
+  return l > r || (l == r && ia < ib);
+}
+__device__ inline KeyJ worst_key(KeyJ*) { return KeyJ{0u, 1u}; }
+__device__ inline double key_to_double(const KeyJ& k) { return (double)k.c / (double)k.d; }
+
+// Approximate similarity (larger is better).
+__device__ inline bool better(float a, uint32_t ia, float b, uint32_t ib) {
+  return a > b || (a == b && ia < ib);
+}
+__device__ inline float worst_key(float*) { return -INFINITY; }
+__device__ inline double key_to_double(float k) { return (double)k; }
+
+template <class K>
+struct Entry {
+  K key;
+  uint32_t idx;
+};
+
+}  // namespace
+
+struct FusedKernelArgs {
+  const uint8_t* ref_dense;
+  const uint8_t* qry_dense;
+  int64_t nref, nqry, kpad_bytes;
+  const int64_t* ref_p;
+  const int64_t* qry_p;
+  const float* ref_aux;
+  const float* qry_aux;
+  const uint8_t* ref_deg;
+  int self_mode, include_self, method;
+  int splits, qtiles, tiles_per_split;
+  int64_t ntiles;
+  double* cand_key;
+  int32_t* cand_idx;
+};
+
+namespace {
+
+// XCD-aware block mapping: blocks are dispatched round-robin over the 8
+// XCDs, so consecutive block ids sit on different L2s.  Give each XCD whole
+// groups of (query tile, every split) so that the concurrent workgroups of
+// one XCD share their query tiles (B) and reference tiles (A) in its L2.
+__device__ inline void map_block(int b, int qtiles, int splits, int& qt, int& sp) {
+  const int nblk = qtiles * splits;
+  int lin = b;
+  if ((nblk & 7) == 0) lin = (b & 7) * (nblk >> 3) + (b >> 3);  // contiguous range per XCD
+  qt = lin / splits;
+  sp = lin % splits;
+}
+
+template <class Op, class K, int L, int WM>
+__global__ __launch_bounds__(WM * 256, 1) void k_fused(FusedKernelArgs P) {
+  constexpr int WAVES = WM * 4;
+  constexpr int NT = WAVES * 64;
+  constexpr int TM = kTileRefs * WM;  // refs per workgroup tile
+  constexpr int TN = 256;             // queries per workgroup
+  // A/B staging (double-buffered) shares its LDS with the epilogue's lane
+  // exchange area (NT lanes x 65 words, padded against bank conflicts).
+  constexpr int AB_BYTES = 2 * TM * kLdsStride + 2 * TN * kLdsStride;
+  constexpr int XCH_BYTES = NT * 65 * 4;
+  constexpr int UNION_BYTES = AB_BYTES > XCH_BYTES ? AB_BYTES : XCH_BYTES;
+  __shared__ __align__(16) uint8_t lds_u[UNION_BYTES];
+  uint8_t (*lds_a)[TM * kLdsStride] = reinterpret_cast<uint8_t (*)[TM * kLdsStride]>(lds_u);
+  uint8_t (*lds_b)[TN * kLdsStride] =
+      reinterpret_cast<uint8_t (*)[TN * kLdsStride]>(lds_u + 2 * TM * kLdsStride);
+  uint32_t* xch = reinterpret_cast<uint32_t*>(lds_u);
+  __shared__ int32_t s_rnnz[TM];
+  __shared__ float s_raux[TM];
+  __shared__ uint8_t s_rdeg[TM];
+
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6, lane = tid & 63;
+  const int wm = wave / 4, wn = wave % 4;
+  const int h = lane >> 5, n = lane & 31;
+  int qt, sp;
+  map_block(blockIdx.x, P.qtiles, P.splits, qt, sp);
+  const int64_t q_base = (int64_t)qt * TN;
+  const int64_t my_q = q_base + wn * 64 + h * 32 + n;  // the query this lane owns
+  const bool q_ok = my_q < P.nqry;
+  uint32_t nj = 0;
+  float qaux = 0.f;
+  if (q_ok) {
+    nj = (uint32_t)(P.qry_p[my_q + 1] - P.qry_p[my_q]);
+    if (P.qry_aux) qaux = P.qry_aux[my_q];
+  }
+
+  Entry<K> list[L];
+#pragma unroll
+  for (int t = 0; t < L; ++t) {
+    list[t].key = worst_key((K*)nullptr);
+    list[t].idx = 0xFFFFFFFFu;
+  }
+
+  const int64_t t0 = (int64_t)sp * P.tiles_per_split;
+  const int64_t t1 = (t0 + P.tiles_per_split < P.ntiles) ? t0 + P.tiles_per_split : P.ntiles;
+  const int64_t nchunks = P.kpad_bytes / kRowBytes;
+
+  // global -> register staging: A has TM*4 16-byte segments, B has TN*4
+  constexpr int SEG_A = TM * 4 / NT;  // = 2
+  constexpr int SEG_B = TN * 4 / NT;  // = 4 / WM
+  uint4 ra[SEG_A], rb[SEG_B];
+
+  for (int64_t tile = t0; tile < t1; ++tile) {
+    const int64_t r_base = tile * TM;
+    for (int t = tid; t < TM; t += NT) {
+      const int64_t r = r_base + t;
+      if (r < P.nref) {
+        s_rnnz[t] = (int32_t)(P.ref_p[r + 1] - P.ref_p[r]);
+        s_raux[t] = P.ref_aux ? P.ref_aux[r] : 0.f;
+        s_rdeg[t] = P.ref_deg ? P.ref_deg[r] : 0;
+      } else {
+        s_rnnz[t] = 0;
+        s_raux[t] = 0.f;
+        s_rdeg[t] = 1;
+      }
+    }
+
+    typename Op::Acc acc[4][2];
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[b][j][r] = 0;
+
+    auto gload = [&](int64_t c) {
+#pragma unroll
+      for (int u = 0; u < SEG_A; ++u) {
+        const int s = tid + u * NT;
+        const int row = s >> 2, seg = s & 3;
+        const int64_t r = r_base + row;
+        if (r < P.nref)
+          ra[u] = *reinterpret_cast<const uint4*>(P.ref_dense + r * P.kpad_bytes + c * kRowBytes +
+                                                  seg * 16);
+        else
+          ra[u] = make_uint4(0, 0, 0, 0);
+      }
+#pragma unroll
+      for (int u = 0; u < SEG_B; ++u) {
+        const int s = tid + u * NT;
+        const int row = s >> 2, seg = s & 3;
+        const int64_t q = q_base + row;
+        if (q < P.nqry)
+          rb[u] = *reinterpret_cast<const uint4*>(P.qry_dense + q * P.kpad_bytes + c * kRowBytes +
+                                                  seg * 16);
+        else
+          rb[u] = make_uint4(0, 0, 0, 0);
+      }
+    };
+    auto lstore = [&](int buf) {
+#pragma unroll
+      for (int u = 0; u < SEG_A; ++u) {
+        const int s = tid + u * NT;
+        *reinterpret_cast<uint4*>(&lds_a[buf][(s >> 2) * kLdsStride + (s & 3) * 16]) = ra[u];
+      }
+#pragma unroll
+      for (int u = 0; u < SEG_B; ++u) {
+        const int s = tid + u * NT;
+        *reinterpret_cast<uint4*>(&lds_b[buf][(s >> 2) * kLdsStride + (s & 3) * 16]) = rb[u];
+      }
+    };
+
+    gload(0);
+    __syncthreads();  // previous tile's readers of lds_* and s_r* are done
+    lstore(0);
+    __syncthreads();
+    for (int64_t c = 0; c < nchunks; ++c) {
+      const int buf = (int)(c & 1);
+      if (c + 1 < nchunks) gload(c + 1);
+      uint4 fa[4][2], fb[2][2];
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        const uint8_t* p = &lds_a[buf][(wm * kTileRefs + b * 32 + n) * kLdsStride + h * 32];
+        fa[b][0] = *reinterpret_cast<const uint4*>(p);
+        fa[b][1] = *reinterpret_cast<const uint4*>(p + 16);
+      }
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const uint8_t* p = &lds_b[buf][(wn * 64 + j * 32 + n) * kLdsStride + h * 32];
+        fb[j][0] = *reinterpret_cast<const uint4*>(p);
+        fb[j][1] = *reinterpret_cast<const uint4*>(p + 16);
+      }
+#pragma unroll
+      for (int b = 0; b < 4; ++b)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) Op::mma(acc[b][j], fa[b], fb[j]);
+      if (c + 1 < nchunks) lstore(buf ^ 1);
+      __syncthreads();
+    }
+
+    // ---- epilogue: this lane owns query my_q and must see its 128 values.
+    // acc[b][j][r] belongs to ref 32b + 8(r/4) + 4h + (r%4), query block j.
+    // Phase 0 scans the lane's own block-h values, phase 1 the block-h
+    // values of lane l ^ 32 (the other four rows of every group of eight).
+    // Values pass through the (now idle) staging LDS so the scan is a plain
+    // loop; rows are padded to 65 words against bank conflicts.
+    const int rloc_base = wm * kTileRefs;
+#pragma unroll
+    for (int phase = 0; phase < 2; ++phase) {
+#pragma unroll
+      for (int b = 0; b < 4; ++b)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const auto v0 = acc[b][0][r], v1 = acc[b][1][r];
+          const auto x = (phase == 0) ? (h ? v1 : v0) : (h ? v0 : v1);
+          uint32_t bits;
+          if constexpr (sizeof(K) == 8)
+            bits = (uint32_t)x;  // exact integer count (int or float accumulator)
+          else
+            __builtin_memcpy(&bits, &x, 4);
+          xch[((phase == 0) ? tid : (tid ^ 32)) * 65 + b * 16 + r] = bits;
+        }
+      __syncthreads();
+      const int hh = (phase == 0) ? h : 1 - h;
+      if (q_ok) {
+        for (int v = 0; v < 64; ++v) {
+          const int b = v >> 4, rr = v & 15;
+          const int rloc = rloc_base + b * 32 + 8 * (rr >> 2) + 4 * hh + (rr & 3);
+          const int64_t rg = r_base + rloc;
+          if (s_rdeg[rloc] || (P.self_mode && rg == my_q && !P.include_self)) continue;
+          const uint32_t bits = xch[tid * 65 + v];
+          K key;
+          if constexpr (sizeof(K) == 8) {
+            const uint32_t c = bits;
+            const uint32_t d = (uint32_t)s_rnnz[rloc] + nj - c;
+            key = (d == 0) ? KeyJ{1u, 1u} : KeyJ{c, d};
+          } else {
+            float sv;
+            __builtin_memcpy(&sv, &bits, 4);
+            if (P.method == SD_EUCLIDEAN)
+              key = -(qaux + s_raux[rloc] - 2.f * sv);
+            else
+              key = sv;
+            if (P.self_mode && rg == my_q) key = 2.0f;  // restored self similarity
+          }
+          if (better(key, (uint32_t)rg, list[L - 1].key, list[L - 1].idx)) {
+            Entry<K> carry{key, (uint32_t)rg};
+#pragma unroll
+            for (int p = 0; p < L; ++p) {
+              if (better(carry.key, carry.idx, list[p].key, list[p].idx)) {
+                const Entry<K> tmp = list[p];
+                list[p] = carry;
+                carry = tmp;
+              }
+            }
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+
+  if (q_ok) {
+    const int64_t slot = ((int64_t)sp * WM + wm) * P.nqry + my_q;
+#pragma unroll
+    for (int t = 0; t < L; ++t) {
+      const bool valid = list[t].idx != 0xFFFFFFFFu;
+      P.cand_key[slot * L + t] = valid ? key_to_double(list[t].key) : 0.0;
+      P.cand_idx[slot * L + t] = valid ? (int32_t)list[t].idx : -1;
+    }
+  }
+}
+
+// ---- merge per-split lists: top-L by (key desc, idx asc) --------------------
+__global__ __launch_bounds__(256) void k_merge_lists(const double* __restrict__ key,
+                                                     const int32_t* __restrict__ idx, int nlists,
+                                                     int64_t nqry, int L,
+                                                     double* __restrict__ out_key,
+                                                     int32_t* __restrict__ out_idx) {
+  __shared__ BlockTopKStorage<512> st;
+  const int64_t q = blockIdx.x;
+  BlockTopK<512> tk(st, L, 1);
+  tk.run((int64_t)nlists * L, [&](int64_t t, double& v, int64_t& id) {
+    const int64_t s = t / L, e = t % L;
+    const int64_t at = (s * nqry + q) * L + e;
+    const int32_t i = idx[at];
+    v = key[at];
+    id = i;
+    return i >= 0;
+  });
+  __syncthreads();
+  const int cnt = st.cnt;
+  for (int t = threadIdx.x; t < L; t += blockDim.x) {
+    out_key[q * L + t] = (t < cnt) ? st.val[t] : 0.0;
+    out_idx[q * L + t] = (t < cnt) ? (int32_t)st.idx[t] : -1;
+  }
+}
+
+// ---- Jaccard results straight from exact lists ------------------------------------
+__global__ void k_finish_exact(const double* __restrict__ key, const int32_t* __restrict__ idx,
+                               int64_t nqry, int L, int k, int dist,
+                               const uint8_t* __restrict__ qdeg, int32_t* __restrict__ out_idx,
+                               double* __restrict__ out_val) {
+  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (e >= nqry * k) return;
+  const int64_t q = e % nqry, t = e / nqry;
+  const int32_t i = idx[q * L + t];
+  if (i < 0 || (qdeg && qdeg[q])) {
+    out_idx[e] = kNaInt;
+    out_val[e] = __longlong_as_double((long long)kNaRealBits);
+  } else {
+    const double sim = key[q * L + t];
+    out_idx[e] = i + 1;
+    out_val[e] = dist ? (1.0 - sim) : sim;
+  }
+}
+
+}  // namespace
+
+
+
+namespace {
+
+__device__ inline uint64_t wave_shfl_u64(uint64_t v, int src) {
+  const uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
+  return (uint64_t)__shfl((int)lo, src) | ((uint64_t)(uint32_t)__shfl((int)hi, src) << 32);
+}
+
+// One wave per query: exact FP64 values of the L candidates (the reference's
+// cosine arithmetic), exact order, certificate against the best value any
+// excluded column could have.
+__global__ __launch_bounds__(256) void k_rerank(RerankArgs R) {
+  const int lane = threadIdx.x & 63;
+  const int64_t j = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (j >= R.nqry) return;
+  const int L = R.L;
+  const int32_t ci = (lane < L) ? R.cand_idx[j * L + lane] : -1;
+  const bool deg_q = R.qdeg && R.qdeg[j];
+  const int64_t q0 = R.ex.qry.p[j], qn = R.ex.qry.p[j + 1] - q0;
+  double c = -2.0, v = 0.0;
+  const bool valid = ci >= 0 && !deg_q;
+  if (valid) {
+    if (R.ex.self_mode && R.ex.include_self && ci == j) {
+      c = 1.0;  // restored self value (R/sparseKNN.R:270-279)
+    } else {
+      const int64_t a0 = R.ex.ref.p[ci], na = R.ex.ref.p[ci + 1] - a0;
+      c = cos_value(R.ex.ref.i + a0, R.ex.ref_unit + a0, na, R.ex.ref_scale[ci],
+                    R.ex.qry.i + q0, R.ex.qry_unit + q0, qn, R.ex.qry_scale[j]);
+    }
+    v = R.ex.dist ? (1.0 - c) : c;
+    if (v == 0.0) v = 0.0;
+  }
+  // sort lanes by (final value in the method's direction, idx)
+  uint64_t key = valid ? order_key(v, R.decreasing) : ~0ULL;
+  uint32_t id = valid ? (uint32_t)ci : 0xFFFFFFFFu;
+  double cv = c, vv = v;
+  for (int size = 2; size <= 64; size <<= 1) {
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      const int partner = lane ^ stride;
+      const uint64_t pk = wave_shfl_u64(key, partner);
+      const uint32_t pi = (uint32_t)__shfl((int)id, partner);
+      const double pc = __shfl(cv, partner);
+      const double pv = __shfl(vv, partner);
+      const bool is_lo = (lane & stride) == 0;
+      const bool asc = (lane & size) == 0;
+      const bool mine_less = key < pk || (key == pk && id < pi);
+      const bool take_min = (is_lo == asc);
+      const bool keep_mine = take_min ? mine_less : !mine_less;
+      if (!keep_mine) {
+        key = pk;
+        id = pi;
+        cv = pc;
+        vv = pv;
+      }
+    }
+  }
+  // number of valid candidates and the list's worst approximate key
+  const unsigned long long vmask = __ballot(valid);
+  const int nvalid = __popcll(vmask);
+  bool certified = true;
+  if (nvalid >= L && R.k <= nvalid) {
+    const double sL = R.cand_key[j * L + (L - 1)];
+    const double ck = __shfl(cv, R.k - 1);
+    const double ub = R.in_round;
+    const double m = (double)qn;
+    const double A = (2.0 * ub + ub * ub) + (1.0 + ub) * (1.0 + ub) * m * R.acc_round;
+    double U;
+    if (R.nonneg)
+      U = (sL > 0.0) ? sL / (1.0 - A) : sL + A * 1e-300;
+    else
+      U = sL + A;
+    U += 1e-12;
+    certified = (A < 0.5) && (ck > U);
+  }
+  if (deg_q) certified = true;
+  if (!certified) {
+    if (lane == 0) {
+      const int at = atomicAdd(R.fail_count, 1);
+      R.fail_list[at] = (int32_t)j;
+    }
+    return;
+  }
+  if (lane < R.k) {
+    const int64_t e = (int64_t)lane * R.nqry + j;
+    if (lane < nvalid && !deg_q) {
+      R.out_idx[e] = (int32_t)id + 1;
+      R.out_val[e] = vv;
+    } else {
+      R.out_idx[e] = kNaInt;
+      R.out_val[e] = __longlong_as_double((long long)kNaRealBits);
+    }
+  }
+}
+
+}  // namespace
+
+// ----------------------------------------------------------------------------
+// host launchers
+// ----------------------------------------------------------------------------
+
+template <class Op, class K, int L, int WM>
+static int launch_fused_t(sd_context* ctx, FusedKernelArgs& P) {
+  const int nblk = P.qtiles * P.splits;
+  hipLaunchKernelGGL((k_fused<Op, K, L, WM>), dim3((unsigned)nblk), dim3(WM * 256), 0, ctx->stream,
+                     P);
+  SD_HIP(hipGetLastError());
+  return SD_OK;
+}
+
+template <class Op, class K, int WM>
+static int launch_fused_L(sd_context* ctx, FusedKernelArgs& P, int L) {
+  switch (L) {
+    case 8: return launch_fused_t<Op, K, 8, WM>(ctx, P);
+    case 16: return launch_fused_t<Op, K, 16, WM>(ctx, P);
+    case 24: return launch_fused_t<Op, K, 24, WM>(ctx, P);
+    case 32: return launch_fused_t<Op, K, 32, WM>(ctx, P);
+    case 40: return launch_fused_t<Op, K, 40, WM>(ctx, P);
+    case 48: return launch_fused_t<Op, K, 48, WM>(ctx, P);
+    case 64: return launch_fused_t<Op, K, 64, WM>(ctx, P);
+    default: return set_error(SD_EINVAL, "internal: unsupported list length %d", L);
+  }
+}
+
+int fused_list_len(int want) {
+  static const int sizes[] = {8, 16, 24, 32, 40, 48, 64};
+  for (int s : sizes)
+    if (want <= s) return s;
+  return 0;
+}
+
+int fused_slab_topl(sd_context* ctx, const FusedArgs& a, double* cand_key, int32_t* cand_idx) {
+  FusedKernelArgs P{};
+  P.ref_dense = static_cast<const uint8_t*>(a.ref_dense);
+  P.qry_dense = static_cast<const uint8_t*>(a.qry_dense);
+  P.nref = a.nref;
+  P.nqry = a.nqry;
+  const int elem = (a.method == SD_BINARY && a.prec != SD_PREC_FP32) ? 1
+                   : (a.prec == SD_PREC_FP32)                         ? 4
+                                                                      : 2;
+  P.kpad_bytes = a.kpad * elem;
+  P.ref_p = a.ref_p;
+  P.qry_p = a.qry_p;
+  P.ref_aux = a.ref_aux;
+  P.qry_aux = a.qry_aux;
+  P.ref_deg = a.ref_deg;
+  P.self_mode = a.self_mode;
+  P.include_self = a.include_self;
+  P.method = a.method;
+  constexpr int WM = 1;
+  const int64_t TM = (int64_t)kTileRefs * WM;
+  P.ntiles = (a.nref + TM - 1) / TM;
+  P.qtiles = (int)((a.nqry + 255) / 256);
+  int splits = a.splits < 1 ? 1 : a.splits;
+  if (splits > P.ntiles) splits = (int)P.ntiles;
+  P.tiles_per_split = (int)((P.ntiles + splits - 1) / splits);
+  P.splits = (int)((P.ntiles + P.tiles_per_split - 1) / P.tiles_per_split);
+  P.cand_key = cand_key;
+  P.cand_idx = cand_idx;
+  KernelTimer timer(ctx, &ctx->stats.slab_ms, &ctx->stats.slab_launches);
+  if (a.method == SD_BINARY) {
+    if (a.prec == SD_PREC_FP32) return launch_fused_L<OpF32, KeyJ, WM>(ctx, P, a.L);
+    return launch_fused_L<OpI8, KeyJ, WM>(ctx, P, a.L);
+  }
+  if (a.prec == SD_PREC_FP32) return launch_fused_L<OpF32, float, WM>(ctx, P, a.L);
+  return launch_fused_L<OpBf16, float, WM>(ctx, P, a.L);
+}
+
+int fused_effective_splits(const FusedArgs& a) {
+  const int64_t ntiles = (a.nref + kTileRefs - 1) / kTileRefs;
+  int splits = a.splits < 1 ? 1 : a.splits;
+  if (splits > ntiles) splits = (int)ntiles;
+  const int64_t tps = (ntiles + splits - 1) / splits;
+  return (int)((ntiles + tps - 1) / tps);
+}
+
+int merge_lists(sd_context* ctx, const double* key, const int32_t* idx, int nlists, int64_t nqry,
+                int L, double* out_key, int32_t* out_idx) {
+  if (nqry <= 0) return SD_OK;
+  hipLaunchKernelGGL(k_merge_lists, dim3((unsigned)nqry), dim3(256), 0, ctx->stream, key, idx,
+                     nlists, nqry, L, out_key, out_idx);
+  SD_HIP(hipGetLastError());
+  return SD_OK;
+}
+
+int finish_exact_lists(sd_context* ctx, const double* key, const int32_t* idx, int64_t nqry, int L,
+                       int k, int dist, const uint8_t* qdeg, int32_t* out_idx, double* out_val) {
+  const int64_t n = nqry * k;
+  if (n <= 0) return SD_OK;
+  hipLaunchKernelGGL(k_finish_exact, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream,
+                     key, idx, nqry, L, k, dist, qdeg, out_idx, out_val);
+  SD_HIP(hipGetLastError());
+  return SD_OK;
+}
+
+int rerank_cosine(sd_context* ctx, const RerankArgs& R) {
+  if (R.nqry <= 0) return SD_OK;
+  KernelTimer timer(ctx, &ctx->stats.rerank_ms, &ctx->stats.rerank_launches);
+  hipLaunchKernelGGL(k_rerank, dim3((unsigned)((R.nqry + 3) / 4)), dim3(256), 0, ctx->stream, R);
+  SD_HIP(hipGetLastError());
+  return SD_OK;
+}
+
+}  // namespace sd

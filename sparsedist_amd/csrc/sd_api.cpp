@@ -1,0 +1,764 @@
+// sd_api.cpp -- the C ABI of libsd_hip.so: contexts, resident matrices, and
+// the drivers that replace the reference's R block loops (R/sparseKNN.R,
+// R/sparseKNNCross.R, R/sparseSNN.R) with device pipelines.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <new>
+#include <thread>
+
+#include "sd_internal.h"
+
+namespace sd {
+
+static thread_local std::string g_err;
+
+int set_error(int code, const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+void clear_error() { g_err.clear(); }
+
+int DevBuf::alloc(size_t n) {
+  release();
+  if (n == 0) n = 16;
+  hipError_t e = hipMalloc(&ptr, n);
+  if (e != hipSuccess) {
+    ptr = nullptr;
+    (void)hipGetLastError();
+    return set_error(SD_ENOMEM, "device allocation of %zu bytes failed: %s", n,
+                     hipGetErrorString(e));
+  }
+  bytes = n;
+  return SD_OK;
+}
+
+int DevBuf::ensure(size_t n) {
+  if (ptr && bytes >= n) return SD_OK;
+  return alloc(n);
+}
+
+void DevBuf::release() {
+  if (ptr) (void)hipFree(ptr);
+  ptr = nullptr;
+  bytes = 0;
+}
+
+KernelTimer::KernelTimer(sd_context* c, double* a, int64_t* n)
+    : ctx(c), acc(a), count(n), on(c->timing) {
+  if (on) (void)hipEventRecord(ctx->ev_a, ctx->stream);
+}
+
+KernelTimer::~KernelTimer() {
+  if (!on) return;
+  (void)hipEventRecord(ctx->ev_b, ctx->stream);
+  if (hipEventSynchronize(ctx->ev_b) == hipSuccess) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, ctx->ev_a, ctx->ev_b) == hipSuccess) *acc += ms;
+    *count += 1;
+  }
+}
+
+}  // namespace sd
+
+using namespace sd;
+
+namespace {
+
+constexpr double kNaReal() {
+  return __builtin_bit_cast(double, kNaRealBits);
+}
+
+int check_cancel(sd_context* ctx) {
+  if (ctx->cancel.load(std::memory_order_relaxed))
+    return set_error(SD_ECANCELLED, "Computation interrupted by the user.");
+  return SD_OK;
+}
+
+// Ranking direction (R/sparseKNN.R:168-181).
+int method_direction(int method, int dist, int* dec) {
+  switch (method) {
+    case SD_BINARY:
+    case SD_COSINE:
+    case SD_PEARSON:
+      *dec = !dist;
+      return SD_OK;
+    case SD_COVARIANCE:
+      *dec = 1;
+      return SD_OK;
+    case SD_EUCLIDEAN:
+    case SD_MANHATTAN:
+      if (!dist)
+        return set_error(SD_EINVAL,
+                         "'dist = FALSE' is only meaningful for methods \"binary\", \"cosine\" and "
+                         "\"pearson\"; %s has no similarity form.",
+                         method == SD_EUCLIDEAN ? "euclidean" : "manhattan");
+      *dec = 0;
+      return SD_OK;
+    default:
+      return set_error(SD_EUNSUPPORTED, "method %d is not provided by this library", method);
+  }
+}
+
+int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
+
+struct DevOut {
+  DevBuf idx, val;
+};
+
+// Prepare per-method statistics of a matrix.
+int prep_method(sd_context* ctx, sd_matrix* m, int method) {
+  if (method == SD_COSINE) {
+    if (m->any_nonfinite)
+      return set_error(SD_ENONFINITE, "Cosine requires finite input (found NA, NaN or Inf).");
+    return prep_unit(ctx, m);
+  }
+  if (method == SD_PEARSON || method == SD_COVARIANCE) return prep_centre(ctx, m);
+  return SD_OK;
+}
+
+ExactArgs exact_args(const sd_matrix* ref, const sd_matrix* qry, int method, int dist) {
+  ExactArgs a;
+  a.method = method;
+  a.dist = dist;
+  a.ref = ref->view();
+  a.qry = qry->view();
+  a.ref_unit = ref->unit.as<double>();
+  a.ref_scale = ref->scale.as<double>();
+  a.qry_unit = qry->unit.as<double>();
+  a.qry_scale = qry->scale.as<double>();
+  a.ref_mean = ref->mean.as<double>();
+  a.ref_css = ref->css.as<double>();
+  a.qry_mean = qry->mean.as<double>();
+  a.qry_css = qry->css.as<double>();
+  return a;
+}
+
+// Exact FP64 slab + top-k for the query columns listed in d_qcols (device),
+// writing results into out (nq x k column-major) at those columns.  Memory
+// is bounded by processing the columns in slabs.
+int exact_knn_columns(sd_context* ctx, const ExactArgs& a, const int32_t* d_qcols, int64_t ncols,
+                      int64_t nq_total, int k, int dec, int64_t block_hint, int32_t* d_out_idx,
+                      double* d_out_val) {
+  const int64_t nref = a.ref.ncol;
+  const int64_t budget = (int64_t)4 << 30;  // bytes of slab per pass
+  int64_t nb = budget / (int64_t)std::max<int64_t>(1, nref * 8);
+  nb = std::max<int64_t>(1, std::min<int64_t>(nb, 65535));
+  if (block_hint > 0) nb = std::min<int64_t>(nb, std::max<int64_t>(block_hint, 1));
+  nb = std::min<int64_t>(nb, ncols);
+  DevBuf& D = ctx->scratch[0];
+  SD_TRY(D.ensure(sizeof(double) * (size_t)nref * (size_t)nb + 64));
+  for (int64_t c0 = 0; c0 < ncols; c0 += nb) {
+    SD_TRY(check_cancel(ctx));
+    const int64_t n = std::min(nb, ncols - c0);
+    SD_TRY(exact_slab(ctx, a, d_qcols + c0, n, D.as<double>(), nref, 1));
+    SD_TRY(topk_dense(ctx, D.as<double>(), nref, n, nref, k, dec, nullptr, d_out_idx, d_out_val,
+                      nq_total, d_qcols + c0));
+  }
+  SD_HIP(hipStreamSynchronize(ctx->stream));
+  return SD_OK;
+}
+
+struct KnnPlan {
+  int k = 0, dec = 0, warn = 0;
+  bool selfmode = false;
+};
+
+// Validation, degenerate masks and k clamp (R/sparseKNN.R:114-231).
+int plan_knn(const sd_matrix* X, const sd_matrix* Y, const sd_knn_params* p, KnnPlan* out,
+             int64_t* avail_out) {
+  if (p->k < 1)
+    return set_error(SD_EINVAL,
+                     "'k' must be a positive integer no larger than .Machine$integer.max.");
+  if (p->block_size < 0)
+    return set_error(SD_EINVAL,
+                     "'block_size' must be a positive integer no larger than .Machine$integer.max.");
+  SD_TRY(method_direction(p->method, p->dist, &out->dec));
+  out->selfmode = (Y == nullptr);
+  if (X->ncol < 1) return set_error(SD_EINVAL, "'X' must have at least one column.");
+  if (!out->selfmode && X->nrow != Y->nrow)
+    return set_error(SD_EINVAL, "'X' and 'Y' must have the same number of rows.");
+  const sd_matrix* ref = out->selfmode ? X : Y;
+  int64_t avail = 0;
+  for (int64_t j = 0; j < ref->ncol; ++j) avail += !(p->drop_degenerate && ref->hdeg[j]);
+  if (out->selfmode && !p->include_self) avail -= 1;
+  *avail_out = avail;
+  if (avail < 1) return set_error(SD_ENOCAND, "No candidate neighbours available.");
+  out->k = p->k;
+  if ((int64_t)out->k > avail) {
+    out->warn = 1;
+    out->k = (int)avail;
+  }
+  return SD_OK;
+}
+
+int run_knn(sd_context* ctx, sd_matrix* X, sd_matrix* Y, const sd_knn_params* p, const KnnPlan& plan,
+            int32_t* h_idx, double* h_val) {
+  sd_matrix* ref = plan.selfmode ? X : Y;
+  const int method = p->method;
+  const int k = plan.k;
+  const int64_t nq = X->ncol, nref = ref->ncol;
+  SD_TRY(prep_method(ctx, ref, method));
+  if (!plan.selfmode) SD_TRY(prep_method(ctx, X, method));
+  SD_TRY(check_cancel(ctx));
+
+  const uint8_t* ref_deg = p->drop_degenerate ? ref->deg.as<uint8_t>() : nullptr;
+  const uint8_t* qry_deg = p->drop_degenerate ? X->deg.as<uint8_t>() : nullptr;
+
+  DevBuf& out_idx = ctx->scratch[1];
+  DevBuf& out_val = ctx->scratch[2];
+  SD_TRY(out_idx.ensure(sizeof(int32_t) * (size_t)nq * (size_t)k + 64));
+  SD_TRY(out_val.ensure(sizeof(double) * (size_t)nq * (size_t)k + 64));
+
+  int prec = p->precision;
+  int margin = p->margin > 0 ? p->margin : 16;
+  const bool fused_ok_binary = (method == SD_BINARY) && prec != SD_PREC_EXACT &&
+                               fused_list_len(k) > 0 && ref->nrow < (1 << 25);
+  const bool fused_ok_cosine = (method == SD_COSINE) && prec != SD_PREC_EXACT &&
+                               fused_list_len(k + margin) > 0 && nref >= 1;
+  ExactArgs ea = exact_args(ref, X, method, p->dist);
+  ea.ref_deg = ref_deg;
+  ea.self_mode = plan.selfmode;
+  ea.include_self = p->include_self;
+
+  if (fused_ok_binary || fused_ok_cosine) {
+    const bool use_f32 = (prec == SD_PREC_FP32);
+    const int64_t kpad = round_up(std::max<int64_t>(ref->nrow, 1), 64);
+    int kind;
+    if (method == SD_BINARY)
+      kind = use_f32 ? kDenseF32Pattern : kDenseI8Pattern;
+    else
+      kind = use_f32 ? kDenseF32Unit : kDenseBf16Unit;
+    SD_TRY(prep_dense(ctx, ref, kind, kpad));
+    if (!plan.selfmode) SD_TRY(prep_dense(ctx, X, kind, kpad));
+    SD_TRY(check_cancel(ctx));
+
+    FusedArgs fa;
+    fa.method = method;
+    fa.prec = use_f32 ? SD_PREC_FP32 : SD_PREC_BF16;
+    fa.nref = nref;
+    fa.nqry = nq;
+    fa.kpad = kpad;
+    fa.ref_dense = ref->dense[kind].ptr;
+    fa.qry_dense = X->dense[kind].ptr;
+    fa.ref_p = ref->p.as<int64_t>();
+    fa.qry_p = X->p.as<int64_t>();
+    fa.ref_deg = ref_deg;
+    fa.self_mode = plan.selfmode;
+    fa.include_self = p->include_self;
+    fa.L = fused_list_len(method == SD_BINARY ? k : k + margin);
+    const int64_t qtiles = (nq + 255) / 256;
+    int64_t splits = (2 * (int64_t)ctx->cu_count + qtiles - 1) / qtiles;
+    splits = round_up(std::max<int64_t>(splits, 1), 8);
+    fa.splits = (int)std::min<int64_t>(splits, 4096);
+    const int nlists = fused_effective_splits(fa);
+    const int L = fa.L;
+    DevBuf& ck = ctx->scratch[3];
+    DevBuf& ci = ctx->scratch[4];
+    SD_TRY(ck.ensure(sizeof(double) * (size_t)nlists * (size_t)nq * (size_t)L + 64));
+    SD_TRY(ci.ensure(sizeof(int32_t) * (size_t)nlists * (size_t)nq * (size_t)L + 64));
+    SD_TRY(fused_slab_topl(ctx, fa, ck.as<double>(), ci.as<int32_t>()));
+    const double* lkey = ck.as<double>();
+    const int32_t* lidx = ci.as<int32_t>();
+    DevBuf& mk = ctx->scratch[5];
+    DevBuf& mi = ctx->scratch[6];
+    if (nlists > 1) {
+      SD_TRY(mk.ensure(sizeof(double) * (size_t)nq * (size_t)L + 64));
+      SD_TRY(mi.ensure(sizeof(int32_t) * (size_t)nq * (size_t)L + 64));
+      SD_TRY(merge_lists(ctx, lkey, lidx, nlists, nq, L, mk.as<double>(), mi.as<int32_t>()));
+      lkey = mk.as<double>();
+      lidx = mi.as<int32_t>();
+    }
+    if (method == SD_BINARY) {
+      SD_TRY(finish_exact_lists(ctx, lkey, lidx, nq, L, k, p->dist,
+                                p->drop_degenerate ? X->deg.as<uint8_t>() : nullptr,
+                                out_idx.as<int32_t>(), out_val.as<double>()));
+    } else {
+      DevBuf& fl = ctx->scratch[7];
+      SD_TRY(fl.ensure(sizeof(int32_t) * (size_t)(nq + 16)));
+      int32_t* fail_count = fl.as<int32_t>() + nq;
+      SD_HIP(hipMemsetAsync(fail_count, 0, sizeof(int32_t), ctx->stream));
+      RerankArgs R;
+      R.ex = ea;
+      R.cand_key = lkey;
+      R.cand_idx = lidx;
+      R.nqry = nq;
+      R.L = L;
+      R.k = k;
+      R.decreasing = plan.dec;
+      R.nonneg = !ref->any_negative && !X->any_negative;
+      R.in_round = use_f32 ? std::ldexp(1.0, -24) : std::ldexp(1.0, -8) + std::ldexp(1.0, -23);
+      R.acc_round = std::ldexp(1.0, -23);
+      R.qdeg = qry_deg;
+      R.out_idx = out_idx.as<int32_t>();
+      R.out_val = out_val.as<double>();
+      R.fail_list = fl.as<int32_t>();
+      R.fail_count = fail_count;
+      SD_TRY(rerank_cosine(ctx, R));
+      int32_t nfail = 0;
+      SD_HIP(hipMemcpyAsync(&nfail, fail_count, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+      SD_HIP(hipStreamSynchronize(ctx->stream));
+      ctx->stats.fallback_queries += nfail;
+      ctx->stats.certified_queries += nq - nfail;
+      if (nfail > 0)
+        SD_TRY(exact_knn_columns(ctx, ea, fl.as<int32_t>(), nfail, nq, k, plan.dec, 0,
+                                 out_idx.as<int32_t>(), out_val.as<double>()));
+    }
+  } else {
+    // exact FP64 path over every query column
+    DevBuf& qc = ctx->scratch[7];
+    SD_TRY(qc.ensure(sizeof(int32_t) * (size_t)(nq + 16)));
+    std::vector<int32_t> iota((size_t)nq);
+    for (int64_t j = 0; j < nq; ++j) iota[(size_t)j] = (int32_t)j;
+    SD_HIP(hipMemcpyAsync(qc.ptr, iota.data(), sizeof(int32_t) * (size_t)nq, hipMemcpyHostToDevice,
+                          ctx->stream));
+    SD_TRY(exact_knn_columns(ctx, ea, qc.as<int32_t>(), nq, nq, k, plan.dec, p->block_size,
+                             out_idx.as<int32_t>(), out_val.as<double>()));
+  }
+  SD_TRY(check_cancel(ctx));
+  SD_HIP(hipMemcpyAsync(h_idx, out_idx.ptr, sizeof(int32_t) * (size_t)nq * (size_t)k,
+                        hipMemcpyDeviceToHost, ctx->stream));
+  SD_HIP(hipMemcpyAsync(h_val, out_val.ptr, sizeof(double) * (size_t)nq * (size_t)k,
+                        hipMemcpyDeviceToHost, ctx->stream));
+  SD_HIP(hipStreamSynchronize(ctx->stream));
+  // a degenerate query gets no neighbours (R/sparseKNN.R:298-301)
+  if (p->drop_degenerate) {
+    const double na = kNaReal();
+    for (int64_t q = 0; q < nq; ++q)
+      if (X->hdeg[(size_t)q])
+        for (int t = 0; t < k; ++t) {
+          h_idx[(int64_t)t * nq + q] = kNaInt;
+          h_val[(int64_t)t * nq + q] = na;
+        }
+  }
+  return SD_OK;
+}
+
+template <class F>
+int guarded(F&& f) {
+  try {
+    clear_error();
+    return f();
+  } catch (const std::bad_alloc&) {
+    return set_error(SD_ENOMEM, "host allocation failed");
+  } catch (...) {
+    return set_error(SD_EDEVICE, "unexpected internal error");
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* sd_last_error(void) { return g_err.c_str(); }
+
+const char* sd_version(void) { return "sparsedist_amd 0.1.0 (gfx950)"; }
+
+int sd_context_create(int device, sd_context** out) {
+  return guarded([&]() -> int {
+    if (!out) return set_error(SD_EINVAL, "out is NULL");
+    int ndev = 0;
+    SD_HIP(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev)
+      return set_error(SD_EINVAL, "device %d not available (%d devices)", device, ndev);
+    SD_HIP(hipSetDevice(device));
+    auto* c = new sd_context();
+    c->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev_a);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev_b);
+    if (e != hipSuccess) {
+      delete c;
+      return set_error(SD_EDEVICE, "context setup failed: %s", hipGetErrorString(e));
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+      c->cu_count = prop.multiProcessorCount;
+    *out = c;
+    return SD_OK;
+  });
+}
+
+int sd_context_destroy(sd_context* ctx) {
+  if (!ctx) return SD_OK;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  for (auto& b : ctx->scratch) b.release();
+  if (ctx->ev_a) (void)hipEventDestroy(ctx->ev_a);
+  if (ctx->ev_b) (void)hipEventDestroy(ctx->ev_b);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return SD_OK;
+}
+
+int sd_cancel(sd_context* ctx) {
+  if (!ctx) return SD_EINVAL;
+  ctx->cancel.store(1);
+  return SD_OK;
+}
+
+int sd_context_stats(sd_context* ctx, sd_stats* out, int reset) {
+  if (!ctx || !out) return set_error(SD_EINVAL, "NULL argument");
+  *out = ctx->stats;
+  if (reset) ctx->stats = sd_stats{};
+  return SD_OK;
+}
+
+int sd_context_set_timing(sd_context* ctx, int on) {
+  if (!ctx) return set_error(SD_EINVAL, "NULL context");
+  ctx->timing = on != 0;
+  return SD_OK;
+}
+
+void* sd_context_stream(sd_context* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+int sd_matrix_upload_csc(sd_context* ctx, int64_t nrow, int64_t ncol, const int64_t* colptr,
+                         const int32_t* rowind, const double* values, sd_matrix** out) {
+  return guarded([&]() -> int {
+    if (!ctx || !out || !colptr) return set_error(SD_EINVAL, "NULL argument");
+    if (nrow < 0 || ncol < 0) return set_error(SD_EINVAL, "negative dimensions");
+    if (nrow >= ((int64_t)1 << 31)) return set_error(SD_EINVAL, "nrow must be below 2^31");
+    SD_HIP(hipSetDevice(ctx->device));
+    if (colptr[0] != 0) return set_error(SD_EINVAL, "colptr[0] must be 0");
+    const int64_t nnz_in = colptr[ncol];
+    // validate, drop explicit zeros, flags (one host pass)
+    std::vector<int64_t> p((size_t)ncol + 1);
+    std::vector<int32_t> ri;
+    std::vector<double> vx;
+    ri.reserve((size_t)nnz_in);
+    vx.reserve((size_t)nnz_in);
+    auto* m = new sd_matrix();
+    m->ctx = ctx;
+    m->nrow = nrow;
+    m->ncol = ncol;
+    m->hdeg.assign((size_t)ncol, 0);
+    p[0] = 0;
+    for (int64_t j = 0; j < ncol; ++j) {
+      if (colptr[j + 1] < colptr[j]) {
+        delete m;
+        return set_error(SD_EINVAL, "colptr must be non-decreasing");
+      }
+      int32_t prev = -1;
+      for (int64_t e = colptr[j]; e < colptr[j + 1]; ++e) {
+        const int32_t r = rowind[e];
+        if (r < 0 || r >= nrow || r <= prev) {
+          delete m;
+          return set_error(SD_EINVAL,
+                           "row indices must be in [0, nrow) and strictly ascending per column");
+        }
+        prev = r;
+        const double v = values[e];
+        if (v == 0.0) continue;
+        if (!std::isfinite(v)) m->any_nonfinite = true;
+        if (v < 0.0) m->any_negative = true;
+        ri.push_back(r);
+        vx.push_back(v);
+      }
+      p[(size_t)j + 1] = (int64_t)ri.size();
+      // colSums(abs(.)) == 0 <=> no stored non-zero value (NaN is not zero)
+      m->hdeg[(size_t)j] = (p[(size_t)j + 1] == p[(size_t)j]) ? 1 : 0;
+    }
+    m->nnz = (int64_t)ri.size();
+    int rc = SD_OK;
+    if ((rc = m->p.alloc(sizeof(int64_t) * (size_t)(ncol + 1))) != SD_OK ||
+        (rc = m->i.alloc(sizeof(int32_t) * (size_t)(m->nnz + 1))) != SD_OK ||
+        (rc = m->x.alloc(sizeof(double) * (size_t)(m->nnz + 1))) != SD_OK ||
+        (rc = m->deg.alloc((size_t)ncol + 16)) != SD_OK) {
+      delete m;
+      return rc;
+    }
+    hipError_t e = hipMemcpy(m->p.ptr, p.data(), sizeof(int64_t) * (size_t)(ncol + 1),
+                             hipMemcpyHostToDevice);
+    if (e == hipSuccess && m->nnz)
+      e = hipMemcpy(m->i.ptr, ri.data(), sizeof(int32_t) * (size_t)m->nnz, hipMemcpyHostToDevice);
+    if (e == hipSuccess && m->nnz)
+      e = hipMemcpy(m->x.ptr, vx.data(), sizeof(double) * (size_t)m->nnz, hipMemcpyHostToDevice);
+    if (e == hipSuccess && ncol)
+      e = hipMemcpy(m->deg.ptr, m->hdeg.data(), (size_t)ncol, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      delete m;
+      return set_error(SD_EDEVICE, "upload failed: %s", hipGetErrorString(e));
+    }
+    m->hp = std::move(p);
+    *out = m;
+    return SD_OK;
+  });
+}
+
+int sd_matrix_free(sd_matrix* m) {
+  if (!m) return SD_OK;
+  (void)hipSetDevice(m->ctx->device);
+  (void)hipStreamSynchronize(m->ctx->stream);
+  delete m;
+  return SD_OK;
+}
+
+int sd_matrix_drop_cache(sd_matrix* m) {
+  if (!m) return SD_OK;
+  (void)hipStreamSynchronize(m->ctx->stream);
+  m->unit.release();
+  m->scale.release();
+  m->unit_ready = false;
+  m->mean.release();
+  m->css.release();
+  m->centre_ready = false;
+  m->sqn.release();
+  m->sq_ready = false;
+  for (int t = 0; t < kDenseKinds; ++t) {
+    m->dense[t].release();
+    m->dense_kpad[t] = 0;
+  }
+  return SD_OK;
+}
+
+int sd_knn(sd_context* ctx, const sd_matrix* X, const sd_matrix* Y, const sd_knn_params* params,
+           int32_t* idx, double* dist, int32_t* k_eff, int32_t* warn) {
+  return guarded([&]() -> int {
+    if (!ctx || !X || !params || !idx || !dist || !k_eff || !warn)
+      return set_error(SD_EINVAL, "NULL argument");
+    SD_HIP(hipSetDevice(ctx->device));
+    ctx->cancel.store(0);
+    KnnPlan plan;
+    int64_t avail = 0;
+    SD_TRY(plan_knn(X, Y, params, &plan, &avail));
+    *warn = plan.warn;
+    *k_eff = plan.k;
+    return run_knn(ctx, const_cast<sd_matrix*>(X), const_cast<sd_matrix*>(Y), params, plan, idx,
+                   dist);
+  });
+}
+
+int sd_knn_cross(sd_context* ctx, const sd_matrix* X, const sd_matrix* Y,
+                 const sd_knn_params* params, int32_t* fwd_idx, double* fwd_dist, int32_t* k_fwd,
+                 int32_t* rev_idx, double* rev_dist, int32_t* k_rev, int32_t* warn) {
+  return guarded([&]() -> int {
+    if (!ctx || !X || !Y || !params || !fwd_idx || !fwd_dist || !rev_idx || !rev_dist || !k_fwd ||
+        !k_rev || !warn)
+      return set_error(SD_EINVAL, "NULL argument");
+    SD_HIP(hipSetDevice(ctx->device));
+    ctx->cancel.store(0);
+    if (params->k < 1)
+      return set_error(SD_EINVAL,
+                       "'k' must be a positive integer no larger than .Machine$integer.max.");
+    int dec = 0;
+    SD_TRY(method_direction(params->method, params->dist, &dec));
+    if (X->nrow != Y->nrow)
+      return set_error(SD_EINVAL, "'X' and 'Y' must have the same number of rows.");
+    if (X->ncol < 1) return set_error(SD_EINVAL, "'X' must have at least one column.");
+    if (Y->ncol < 1) return set_error(SD_EINVAL, "'Y' must have at least one column.");
+    // per-direction clamp (R/sparseKNNCross.R:117-127)
+    KnnPlan pf, pr;
+    int64_t av_f = 0, av_r = 0;
+    *warn = 0;
+    int rc = plan_knn(X, Y, params, &pf, &av_f);
+    if (rc == SD_ENOCAND)
+      return set_error(SD_ENOCAND, "No candidate neighbours available for the forward direction.");
+    SD_TRY(rc);
+    rc = plan_knn(Y, X, params, &pr, &av_r);
+    if (rc == SD_ENOCAND)
+      return set_error(SD_ENOCAND, "No candidate neighbours available for the reverse direction.");
+    SD_TRY(rc);
+    *warn = (pf.warn ? 1 : 0) | (pr.warn ? 2 : 0);
+    *k_fwd = pf.k;
+    *k_rev = pr.k;
+    SD_TRY(run_knn(ctx, const_cast<sd_matrix*>(X), const_cast<sd_matrix*>(Y), params, pf, fwd_idx,
+                   fwd_dist));
+    return run_knn(ctx, const_cast<sd_matrix*>(Y), const_cast<sd_matrix*>(X), params, pr, rev_idx,
+                   rev_dist);
+  });
+}
+
+int sd_slab(sd_context* ctx, int32_t method, const sd_matrix* X, const sd_matrix* Y, int32_t dist,
+            double* D) {
+  return guarded([&]() -> int {
+    if (!ctx || !X || !Y || !D) return set_error(SD_EINVAL, "NULL argument");
+    SD_HIP(hipSetDevice(ctx->device));
+    if (X->nrow != Y->nrow)
+      return set_error(SD_EINVAL, "Mismatched row dimensions of m (%lld) and m2 (%lld).",
+                       (long long)X->nrow, (long long)Y->nrow);
+    int dec;
+    if (method == SD_EUCLIDEAN || method == SD_MANHATTAN) dist = 1;
+    SD_TRY(method_direction(method, dist, &dec));
+    sd_matrix* m = const_cast<sd_matrix*>(X);
+    sd_matrix* m2 = const_cast<sd_matrix*>(Y);
+    SD_TRY(prep_method(ctx, m, method));
+    SD_TRY(prep_method(ctx, m2, method));
+    ExactArgs a = exact_args(m, m2, method, dist);
+    const int64_t nr = m->ncol, nc = m2->ncol;
+    if (nr == 0 || nc == 0) return SD_OK;
+    DevBuf dD, qc;
+    const int64_t nb = std::max<int64_t>(1, std::min<int64_t>(nc, ((int64_t)1 << 29) / std::max<int64_t>(nr, 1)));
+    SD_TRY(dD.alloc(sizeof(double) * (size_t)nr * (size_t)nb));
+    SD_TRY(qc.alloc(sizeof(int32_t) * (size_t)nc));
+    std::vector<int32_t> iota((size_t)nc);
+    for (int64_t j = 0; j < nc; ++j) iota[(size_t)j] = (int32_t)j;
+    SD_HIP(hipMemcpy(qc.ptr, iota.data(), sizeof(int32_t) * (size_t)nc, hipMemcpyHostToDevice));
+    for (int64_t c0 = 0; c0 < nc; c0 += nb) {
+      const int64_t n = std::min(nb, nc - c0);
+      SD_TRY(exact_slab(ctx, a, qc.as<int32_t>() + c0, n, dD.as<double>(), nr, 0));
+      SD_HIP(hipMemcpyAsync(D + c0 * nr, dD.ptr, sizeof(double) * (size_t)nr * (size_t)n,
+                            hipMemcpyDeviceToHost, ctx->stream));
+      SD_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    return SD_OK;
+  });
+}
+
+int sd_topk_block(sd_context* ctx, const double* D, int64_t nrow, int64_t ncol, int32_t k,
+                  int32_t decreasing, const int32_t* self_row, int32_t* idx, double* val) {
+  return guarded([&]() -> int {
+    if (!ctx || !D || !self_row || !idx || !val) return set_error(SD_EINVAL, "NULL argument");
+    SD_HIP(hipSetDevice(ctx->device));
+    if (k < 1) return set_error(SD_EINVAL, "k must be >= 1.");
+    std::vector<int32_t> skip((size_t)ncol);
+    for (int64_t b = 0; b < ncol; ++b) {
+      const int32_t sr = self_row[b];
+      if (sr == kNaInt || sr < 0 || (int64_t)sr > nrow)
+        return set_error(SD_EINVAL, "self_row entries must be 0 or valid 1-based rows of D.");
+      skip[(size_t)b] = sr - 1;
+    }
+    if (ncol == 0) return SD_OK;
+    DevBuf dD, ds, di, dv;
+    SD_TRY(dD.alloc(sizeof(double) * (size_t)nrow * (size_t)ncol));
+    SD_TRY(ds.alloc(sizeof(int32_t) * (size_t)ncol));
+    SD_TRY(di.alloc(sizeof(int32_t) * (size_t)ncol * (size_t)k));
+    SD_TRY(dv.alloc(sizeof(double) * (size_t)ncol * (size_t)k));
+    SD_HIP(hipMemcpy(dD.ptr, D, sizeof(double) * (size_t)nrow * (size_t)ncol, hipMemcpyHostToDevice));
+    SD_HIP(hipMemcpy(ds.ptr, skip.data(), sizeof(int32_t) * (size_t)ncol, hipMemcpyHostToDevice));
+    SD_TRY(topk_dense(ctx, dD.as<double>(), nrow, ncol, nrow, k, decreasing, ds.as<int32_t>(),
+                      di.as<int32_t>(), dv.as<double>(), ncol, nullptr));
+    SD_HIP(hipStreamSynchronize(ctx->stream));
+    SD_HIP(hipMemcpy(idx, di.ptr, sizeof(int32_t) * (size_t)ncol * (size_t)k, hipMemcpyDeviceToHost));
+    SD_HIP(hipMemcpy(val, dv.ptr, sizeof(double) * (size_t)ncol * (size_t)k, hipMemcpyDeviceToHost));
+    return SD_OK;
+  });
+}
+
+int sd_topk_rows_accum(sd_context* ctx, const double* D, int64_t nrow, int64_t ncol, int32_t k,
+                       int32_t decreasing, int32_t* best_idx, double* best_val,
+                       int64_t col_offset, const int32_t* self_col) {
+  return guarded([&]() -> int {
+    if (!ctx || !D || !best_idx || !best_val || !self_col)
+      return set_error(SD_EINVAL, "NULL argument");
+    SD_HIP(hipSetDevice(ctx->device));
+    if (k < 1) return set_error(SD_EINVAL, "k must be >= 1.");
+    if (col_offset < 0) return set_error(SD_EINVAL, "col_offset must be >= 0.");
+    for (int64_t i = 0; i < nrow; ++i)
+      if (self_col[i] == kNaInt || self_col[i] < 0)
+        return set_error(SD_EINVAL,
+                         "self_col entries must be 0 or a positive 1-based query index.");
+    if (nrow == 0) return SD_OK;
+    DevBuf dD, dsc, di, dv;
+    SD_TRY(dD.alloc(sizeof(double) * (size_t)nrow * (size_t)std::max<int64_t>(ncol, 1)));
+    SD_TRY(dsc.alloc(sizeof(int32_t) * (size_t)nrow));
+    SD_TRY(di.alloc(sizeof(int32_t) * (size_t)nrow * (size_t)k));
+    SD_TRY(dv.alloc(sizeof(double) * (size_t)nrow * (size_t)k));
+    if (ncol > 0)
+      SD_HIP(hipMemcpy(dD.ptr, D, sizeof(double) * (size_t)nrow * (size_t)ncol,
+                       hipMemcpyHostToDevice));
+    SD_HIP(hipMemcpy(dsc.ptr, self_col, sizeof(int32_t) * (size_t)nrow, hipMemcpyHostToDevice));
+    SD_HIP(hipMemcpy(di.ptr, best_idx, sizeof(int32_t) * (size_t)nrow * (size_t)k,
+                     hipMemcpyHostToDevice));
+    SD_HIP(hipMemcpy(dv.ptr, best_val, sizeof(double) * (size_t)nrow * (size_t)k,
+                     hipMemcpyHostToDevice));
+    SD_TRY(topk_rows(ctx, dD.as<double>(), nrow, ncol, k, decreasing, di.as<int32_t>(),
+                     dv.as<double>(), col_offset, dsc.as<int32_t>()));
+    SD_HIP(hipStreamSynchronize(ctx->stream));
+    SD_HIP(hipMemcpy(best_idx, di.ptr, sizeof(int32_t) * (size_t)nrow * (size_t)k,
+                     hipMemcpyDeviceToHost));
+    SD_HIP(hipMemcpy(best_val, dv.ptr, sizeof(double) * (size_t)nrow * (size_t)k,
+                     hipMemcpyDeviceToHost));
+    return SD_OK;
+  });
+}
+
+int sd_snn_jaccard(sd_context* ctx, const int32_t* idx, int32_t n, int32_t k,
+                   int32_t include_self, double* w) {
+  return guarded([&]() -> int {
+    if (!ctx || !idx || !w) return set_error(SD_EINVAL, "NULL argument");
+    if (n < 1 || k < 1) return set_error(SD_EINVAL, "'idx' must have at least one row and column.");
+    SD_HIP(hipSetDevice(ctx->device));
+    DevBuf di, dw;
+    const size_t total = (size_t)n * (size_t)k;
+    SD_TRY(di.alloc(sizeof(int32_t) * total));
+    SD_TRY(dw.alloc(sizeof(double) * total));
+    SD_HIP(hipMemcpy(di.ptr, idx, sizeof(int32_t) * total, hipMemcpyHostToDevice));
+    SD_TRY(snn_build(ctx, di.as<int32_t>(), n, k, include_self, 0.0, 0, 0, nullptr, dw.as<double>()));
+    SD_HIP(hipMemcpy(w, dw.ptr, sizeof(double) * total, hipMemcpyDeviceToHost));
+    return SD_OK;
+  });
+}
+
+int sd_snn(sd_context* ctx, const int32_t* idx, int32_t n, int32_t k, int32_t include_self,
+           double prune, int32_t symmetrise, int32_t edges, int32_t idx_on_device,
+           sd_graph** out) {
+  return guarded([&]() -> int {
+    if (!ctx || !idx || !out) return set_error(SD_EINVAL, "NULL argument");
+    if (n < 1 || k < 1) return set_error(SD_EINVAL, "'idx' must have at least one row and column.");
+    if (!(prune >= 0.0 && prune <= 1.0))
+      return set_error(SD_EINVAL, "'prune' must be a finite number between 0 and 1.");
+    if (edges != 0 && edges != 1) return set_error(SD_EINVAL, "edges must be 0 (knn) or 1 (shared)");
+    SD_HIP(hipSetDevice(ctx->device));
+    ctx->cancel.store(0);
+    DevBuf di;
+    const int32_t* d_idx = idx;
+    if (!idx_on_device) {
+      const size_t total = (size_t)n * (size_t)k;
+      SD_TRY(di.alloc(sizeof(int32_t) * total));
+      SD_HIP(hipMemcpyAsync(di.ptr, idx, sizeof(int32_t) * total, hipMemcpyHostToDevice,
+                            ctx->stream));
+      d_idx = di.as<int32_t>();
+    }
+    auto* g = new sd_graph();
+    int rc = snn_build(ctx, d_idx, n, k, include_self, prune, symmetrise, edges, g, nullptr);
+    if (rc != SD_OK) {
+      delete g;
+      return rc;
+    }
+    *out = g;
+    return SD_OK;
+  });
+}
+
+int sd_graph_info(const sd_graph* g, int64_t* n, int64_t* nnz) {
+  if (!g || !n || !nnz) return set_error(SD_EINVAL, "NULL argument");
+  *n = g->n;
+  *nnz = g->nnz;
+  return SD_OK;
+}
+
+int sd_graph_copy(const sd_graph* g, int64_t* colptr, int32_t* rowind, double* val) {
+  return guarded([&]() -> int {
+    if (!g || !colptr) return set_error(SD_EINVAL, "NULL argument");
+    SD_HIP(hipMemcpy(colptr, g->colptr.ptr, sizeof(int64_t) * (size_t)(g->n + 1),
+                     hipMemcpyDeviceToHost));
+    if (g->nnz > 0) {
+      if (!rowind || !val) return set_error(SD_EINVAL, "NULL argument");
+      SD_HIP(hipMemcpy(rowind, g->rowind.ptr, sizeof(int32_t) * (size_t)g->nnz,
+                       hipMemcpyDeviceToHost));
+      SD_HIP(hipMemcpy(val, g->val.ptr, sizeof(double) * (size_t)g->nnz, hipMemcpyDeviceToHost));
+    }
+    return SD_OK;
+  });
+}
+
+int sd_graph_free(sd_graph* g) {
+  delete g;
+  return SD_OK;
+}
+
+int sd_detect_cores(void) {
+  const unsigned n = std::thread::hardware_concurrency();
+  return (n < 2) ? 1 : (int)n - 1;
+}
+
+}  // extern "C"

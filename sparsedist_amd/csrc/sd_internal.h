@@ -1,0 +1,219 @@
+// sd_internal.h -- shared declarations of libsd_hip.so (host and device).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <cstdint>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/sd.h"
+
+namespace sd {
+
+constexpr int32_t kNaInt = INT32_MIN;
+constexpr uint64_t kNaRealBits = 0x7FF00000000007A2ULL;  // R's NA_real_
+
+// ---------------------------------------------------------------------------
+// error reporting (thread-local last error, as sd_last_error() documents)
+// ---------------------------------------------------------------------------
+int set_error(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+void clear_error();
+
+#define SD_HIP(expr)                                                                  \
+  do {                                                                                \
+    hipError_t e_ = (expr);                                                           \
+    if (e_ != hipSuccess)                                                             \
+      return ::sd::set_error(e_ == hipErrorOutOfMemory ? SD_ENOMEM : SD_EDEVICE,      \
+                             "HIP error %s at %s:%d", hipGetErrorString(e_), __FILE__, \
+                             __LINE__);                                               \
+  } while (0)
+
+#define SD_TRY(expr)          \
+  do {                        \
+    int rc_ = (expr);         \
+    if (rc_ != SD_OK) return rc_; \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// device buffers
+// ---------------------------------------------------------------------------
+struct DevBuf {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : ptr(o.ptr), bytes(o.bytes) { o.ptr = nullptr; o.bytes = 0; }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) { release(); ptr = o.ptr; bytes = o.bytes; o.ptr = nullptr; o.bytes = 0; }
+    return *this;
+  }
+  ~DevBuf() { release(); }
+  int alloc(size_t n);            // exact size, old contents dropped
+  int ensure(size_t n);           // grow-only
+  void release();
+  template <class T> T* as() const { return static_cast<T*>(ptr); }
+};
+
+// Device CSC view passed to kernels.
+struct Csc {
+  int64_t nrow = 0, ncol = 0;
+  const int64_t* p = nullptr;
+  const int32_t* i = nullptr;
+  const double* x = nullptr;
+};
+
+}  // namespace sd
+
+// ---------------------------------------------------------------------------
+// opaque handles
+// ---------------------------------------------------------------------------
+struct sd_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::atomic<int> cancel{0};
+  bool timing = false;
+  sd_stats stats{};
+  hipEvent_t ev_a = nullptr, ev_b = nullptr;
+  int cu_count = 256;
+  // scratch reused across calls (grow-only)
+  sd::DevBuf scratch[8];
+  std::vector<uint8_t> host_scratch;
+};
+
+struct sd_matrix {
+  sd_context* ctx = nullptr;
+  int64_t nrow = 0, ncol = 0, nnz = 0;
+  sd::DevBuf p, i, x;              // CSC (explicit zeros dropped)
+  std::vector<int64_t> hp;         // host copy of colptr (planning)
+  bool any_nonfinite = false;      // values contain NA/NaN/Inf
+  bool any_negative = false;
+  std::vector<uint8_t> hdeg;       // host copy of degenerate flags
+  sd::DevBuf deg;                  // uint8 per column: all stored values == 0
+  // cosine: unit-normalised values (nnz) and per-column max |v|
+  bool unit_ready = false;
+  sd::DevBuf unit, scale;
+  // pearson / covariance: per-column mean and centred sum of squares
+  bool centre_ready = false;
+  sd::DevBuf mean, css;
+  // euclidean: per-column sum of squares (FP64, merge order irrelevant)
+  bool sq_ready = false;
+  sd::DevBuf sqn;
+  // densified layouts, one per kind: [ncol][kpad] column-contiguous
+  int64_t dense_kpad[6] = {0, 0, 0, 0, 0, 0};
+  sd::DevBuf dense[6];
+  sd::Csc view() const {
+    sd::Csc c;
+    c.nrow = nrow; c.ncol = ncol;
+    c.p = p.as<int64_t>(); c.i = i.as<int32_t>(); c.x = x.as<double>();
+    return c;
+  }
+};
+
+struct sd_graph {
+  int64_t n = 0, nnz = 0;
+  sd::DevBuf colptr, rowind, val;
+};
+
+namespace sd {
+
+// densified layout kinds
+enum DenseKind { kDenseI8Pattern = 0, kDenseBf16Unit = 1, kDenseBf16Raw = 2, kDenseF32Unit = 3,
+                 kDenseF32Raw = 4, kDenseF32Pattern = 5, kDenseKinds = 6 };
+
+// ---- prep (k_prep.hip) ------------------------------------------------------
+int prep_flags(sd_context* ctx, sd_matrix* m);                 // deg, nonfinite, negative
+int prep_unit(sd_context* ctx, sd_matrix* m);                  // cosine unit values
+int prep_centre(sd_context* ctx, sd_matrix* m);                // pearson/covariance stats
+int prep_sq(sd_context* ctx, sd_matrix* m);                    // euclidean squared norms
+int prep_dense(sd_context* ctx, sd_matrix* m, int kind, int64_t kpad);
+
+// ---- exact FP64 path (k_exact.hip) -------------------------------------------
+struct ExactArgs {
+  int method = 0, dist = 1;
+  Csc ref, qry;                 // full matrices
+  const double* ref_unit = nullptr; const double* ref_scale = nullptr;
+  const double* qry_unit = nullptr; const double* qry_scale = nullptr;
+  const double* ref_mean = nullptr; const double* ref_css = nullptr;
+  const double* qry_mean = nullptr; const double* qry_css = nullptr;
+  const uint8_t* ref_deg = nullptr;   // nullptr: no masking
+  int self_mode = 0;            // query j is reference column j
+  int include_self = 0;
+};
+// D[i + b*ld] for refs i in [0, ref.ncol) and queries qcols[b] (b < nb).
+int exact_slab(sd_context* ctx, const ExactArgs& a, const int32_t* d_qcols, int64_t nb, double* D,
+               int64_t ld, int mask_self_and_deg);
+
+// Top-k of dense device columns D (nrow x ncol, leading dim ld): keys ranked
+// by (value, row) with the method's direction; rows listed in d_skip (per
+// column, 0-based, -1 none) and non-finite values are skipped.  Writes
+// 1-based idx / values into out (ncol x k, leading dim out_ld) at column
+// offsets given by d_outcol (nullptr: identity).
+int topk_dense(sd_context* ctx, const double* D, int64_t nrow, int64_t ncol, int64_t ld, int k,
+               int decreasing, const int32_t* d_skip, int32_t* out_idx, double* out_val,
+               int64_t out_ld, const int32_t* d_outcol);
+
+// ---- fused MFMA path (k_slab.hip) ------------------------------------------
+struct FusedArgs {
+  int method = 0;               // SD_BINARY / SD_COSINE / SD_EUCLIDEAN / SD_PEARSON
+  int prec = SD_PREC_BF16;
+  int64_t nref = 0, nqry = 0, kpad = 0;
+  const void* ref_dense = nullptr;
+  const void* qry_dense = nullptr;
+  const int64_t* ref_p = nullptr;   // column pointers (nnz per column)
+  const int64_t* qry_p = nullptr;
+  const float* ref_aux = nullptr;   // per-column float aux (euclid: ||x||^2)
+  const float* qry_aux = nullptr;
+  const uint8_t* ref_deg = nullptr;
+  int self_mode = 0, include_self = 0;
+  int L = 0;                        // list length kept per query
+  int splits = 1;
+};
+// Candidate lists: per (split, query) L entries of (key double, idx int32),
+// key larger = better, sorted best first; unfilled slots idx = -1.
+int fused_slab_topl(sd_context* ctx, const FusedArgs& a, double* cand_key, int32_t* cand_idx);
+
+struct RerankArgs {
+  ExactArgs ex;
+  const double* cand_key;      // merged candidate lists, nqry x L
+  const int32_t* cand_idx;
+  int64_t nqry;
+  int L, k, decreasing;
+  int nonneg;                  // both matrices non-negative
+  double in_round;             // relative rounding of each slab input
+  double acc_round;            // unit roundoff of the accumulator
+  const uint8_t* qdeg;
+  int32_t* out_idx;
+  double* out_val;
+  int32_t* fail_list;          // queries the certificate could not clear
+  int32_t* fail_count;
+};
+int fused_list_len(int want);
+int fused_effective_splits(const FusedArgs& a);
+int merge_lists(sd_context* ctx, const double* key, const int32_t* idx, int nlists, int64_t nqry,
+                int L, double* out_key, int32_t* out_idx);
+int finish_exact_lists(sd_context* ctx, const double* key, const int32_t* idx, int64_t nqry, int L,
+                       int k, int dist, const uint8_t* qdeg, int32_t* out_idx, double* out_val);
+int rerank_cosine(sd_context* ctx, const RerankArgs& R);
+int topk_rows(sd_context* ctx, const double* D, int64_t nrow, int64_t ncol, int k, int decreasing,
+              int32_t* best_idx, double* best_val, int64_t col_offset, const int32_t* self_col);
+
+// ---- SNN (k_snn.hip) ---------------------------------------------------------
+int snn_build(sd_context* ctx, const int32_t* d_idx, int n, int k, int include_self, double prune,
+              int symmetrise, int edges, sd_graph* g, double* d_w_out);
+
+// ---- timing helpers ------------------------------------------------------------
+struct KernelTimer {
+  sd_context* ctx;
+  double* acc;
+  int64_t* count;
+  bool on;
+  KernelTimer(sd_context* c, double* a, int64_t* n);
+  ~KernelTimer();
+};
+
+}  // namespace sd

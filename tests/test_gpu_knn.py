@@ -1,0 +1,249 @@
+"""GPU parity: libsd_hip.so against the CPU oracle on the same seeded inputs.
+
+Integer and index results must be identical; FP64 values identical bit for
+bit (the exact path and the FP64 re-rank use the reference's operation
+order).  Every call goes through the C ABI."""
+import json
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import sparsedist_amd as sd
+from conftest import bits
+from sparsedist_amd import synth
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+FIX = json.load(open(os.path.join(HERE, "golden", "fixtures.json")))
+NA_INT = np.iinfo(np.int32).min
+NT = min(16, os.cpu_count() or 1)
+METHODS = ["binary", "cosine", "euclidean", "manhattan", "pearson", "covariance"]
+
+
+def rand_sparse(nrow, ncol, density, seed, signed=False, levels=None, empty_cols=()):
+    rng = np.random.default_rng(seed)
+    M = sp.random(nrow, ncol, density=density, random_state=rng, format="csc")
+    if levels:
+        M.data = rng.integers(1, levels + 1, size=M.nnz).astype(float)
+    elif signed:
+        M.data = rng.standard_normal(M.nnz)
+    else:
+        M.data = np.abs(rng.standard_normal(M.nnz)) + 0.01
+    if empty_cols:
+        M = M.tolil()
+        for c in empty_cols:
+            M[:, c] = 0
+        M = M.tocsc()
+        M.eliminate_zeros()
+    M.sort_indices()
+    return M
+
+
+def same(gpu, ora, label=""):
+    gi, gd = gpu
+    oi, od = ora[0], ora[1]
+    assert gi.shape == oi.shape, label
+    bad = np.argwhere(gi != oi)
+    assert bad.size == 0, f"{label}: idx differs at {bad[:5].tolist()} gpu={gi[tuple(bad[0])]} " \
+                          f"oracle={oi[tuple(bad[0])]}"
+    db = np.argwhere(bits(gd) != bits(od))
+    assert db.size == 0, f"{label}: dist differs at {db[:5].tolist()} " \
+                         f"gpu={gd[tuple(db[0])]!r} oracle={od[tuple(db[0])]!r}"
+
+
+@pytest.mark.parametrize("method", METHODS)
+def test_slab_bitwise(oracle_lib, gpu_ctx, method):
+    signed = method in ("cosine", "pearson", "covariance")
+    A = rand_sparse(50, 23, 0.3, 1, signed=signed, empty_cols=(2,))
+    B = rand_sparse(50, 17, 0.3, 2, signed=signed, empty_cols=(5,))
+    for dist in ([True, False] if method in ("binary", "cosine", "pearson") else [True]):
+        g = sd.sparseDist2(A, B, method=method, dist=dist)
+        o = oracle_lib.slab(method, A, B, dist=dist)
+        assert np.array_equal(bits(g), bits(o)), (method, dist)
+
+
+@pytest.mark.parametrize("method", METHODS)
+@pytest.mark.parametrize("precision", ["auto", "exact", "fp32"])
+def test_knn_self_bitwise(oracle_lib, gpu_ctx, method, precision):
+    X = rand_sparse(70, 45, 0.25, 3, signed=method in ("pearson", "covariance"),
+                    empty_cols=(4, 9))
+    for dist in ([True, False] if method in ("binary", "cosine", "pearson") else [True]):
+        for include_self in (False, True):
+            for drop in (True, False):
+                lab = f"{method} {precision} dist={dist} self={include_self} drop={drop}"
+                g = sd.sparseKNN(X, k=6, method=method, dist=dist, include_self=include_self,
+                                 drop_degenerate=drop, precision=precision, verbose=False)
+                o = oracle_lib.knn(X, k=6, method=method, dist=dist, include_self=include_self,
+                                   drop_degenerate=drop, nthreads=NT)
+                same((g.idx, g.dist), o, lab)
+                assert g.self_search is True
+
+
+@pytest.mark.parametrize("method", ["binary", "cosine", "euclidean"])
+def test_knn_against_reference_matrix(oracle_lib, gpu_ctx, method):
+    X = rand_sparse(90, 140, 0.2, 5, empty_cols=(7,))
+    Y = rand_sparse(90, 300, 0.2, 6, empty_cols=(11, 12))
+    for k in (1, 5, 20, 40):
+        g = sd.sparseKNN(X, Y, k=k, method=method, verbose=False)
+        o = oracle_lib.knn(X, Y, k=k, method=method, nthreads=NT)
+        same((g.idx, g.dist), o, f"{method} k={k}")
+        assert g.self_search is False
+
+
+def test_large_k_uses_exact_path(oracle_lib, gpu_ctx):
+    X = rand_sparse(40, 300, 0.2, 7)
+    for method in ("binary", "cosine"):
+        g = sd.sparseKNN(X, k=120, method=method, verbose=False)
+        o = oracle_lib.knn(X, k=120, method=method, nthreads=NT)
+        same((g.idx, g.dist), o, method)
+
+
+@pytest.mark.parametrize("method", ["binary", "cosine"])
+def test_knn_c1_shape(oracle_lib, gpu_ctx, method):
+    """C1 geometry (2000 rows, d = 0.05, |N(0,1)| values), 1500 columns."""
+    X = synth.abs_normal(2000, 1500, 0.05, 11)
+    g = sd.sparseKNN(X, k=15, method=method, verbose=False)
+    o = oracle_lib.knn(X, k=15, method=method, nthreads=NT)
+    same((g.idx, g.dist), o, method)
+
+
+@pytest.mark.parametrize("method", ["binary", "cosine"])
+@pytest.mark.parametrize("dist", [True, False])
+def test_knn_count_matrix_shape(oracle_lib, gpu_ctx, method, dist):
+    """The count-matrix workload's rows and density (20000 x d = 0.06,
+    NB/log1p), 1200 columns so the oracle finishes in seconds."""
+    X = synth.nb_log1p(20000, 1200, 0.06, 12)
+    for prec in ("auto", "fp32"):
+        g = sd.sparseKNN(X, k=20, method=method, dist=dist, precision=prec, verbose=False)
+        o = oracle_lib.knn(X, k=20, method=method, dist=dist, nthreads=NT)
+        same((g.idx, g.dist), o, f"{method} {prec}")
+
+
+def test_certificate_counts(gpu_ctx):
+    X = synth.nb_log1p(20000, 1200, 0.06, 13)
+    gpu_ctx.stats(reset=True)
+    sd.sparseKNN(X, k=20, method="cosine", verbose=False)
+    st = gpu_ctx.stats(reset=True)
+    assert st["certified_queries"] + st["fallback_queries"] == 1200
+
+
+def test_cross_equivalence(oracle_lib, gpu_ctx):
+    X = rand_sparse(120, 61, 0.06, 8, levels=2, empty_cols=(0, 4, 60))
+    Y = rand_sparse(120, 37, 0.06, 9, levels=2, empty_cols=(1, 29))
+    for method in ("binary", "cosine", "euclidean", "manhattan"):
+        for dist in ([True, False] if method in ("binary", "cosine") else [True]):
+            cr = sd.sparseKNNCross(X, Y, k=7, method=method, dist=dist, verbose=False)
+            fw = sd.sparseKNN(X, Y, k=7, method=method, dist=dist, verbose=False)
+            rv = sd.sparseKNN(Y, X, k=7, method=method, dist=dist, verbose=False)
+            same((cr["forward"].idx, cr["forward"].dist), (fw.idx, fw.dist), "fwd")
+            same((cr["reverse"].idx, cr["reverse"].dist), (rv.idx, rv.dist), "rev")
+            (fi, fd), (ri, rd), _ = oracle_lib.knn_cross(X, Y, k=7, method=method, dist=dist,
+                                                          nthreads=NT)
+            same((cr["forward"].idx, cr["forward"].dist), (fi, fd), f"{method} oracle fwd")
+            same((cr["reverse"].idx, cr["reverse"].dist), (ri, rd), f"{method} oracle rev")
+
+
+def test_topk_block_parity(oracle_lib, gpu_ctx):
+    rng = np.random.default_rng(21)
+    D = rng.integers(0, 6, size=(300, 40)).astype(float)
+    D[rng.random(D.shape) < 0.05] = np.nan
+    D[3, 2] = np.inf
+    D[5, 2] = -0.0
+    self_row = rng.integers(0, 301, size=40).astype(np.int32)
+    for k in (1, 7, 64, 300):
+        for dec in (False, True):
+            g = sd.topKBlock(D, k, dec, self_row)
+            o = oracle_lib.topk_block(D, k, dec, self_row)
+            same(g, o, f"k={k} dec={dec}")
+
+
+def test_topk_rows_accum_parity(oracle_lib, gpu_ctx):
+    rng = np.random.default_rng(22)
+    D = rng.integers(0, 4, size=(50, 90)).astype(float)
+    k = 5
+    self_col = rng.integers(0, 91, size=50).astype(np.int32)
+    for dec in (False, True):
+        gi = np.full((50, k), NA_INT, np.int32, order="F")
+        gv = np.full((50, k), np.nan, order="F")
+        oi, ov = gi.copy(order="F"), gv.copy(order="F")
+        for c0 in range(0, 90, 13):
+            blk = np.asfortranarray(D[:, c0:c0 + 13])
+            sd.topKRowsAccum(blk, k, dec, gi, gv, c0, self_col)
+            oracle_lib.topk_rows_accum(blk, k, dec, oi, ov, c0, self_col)
+        same((gi, gv), (oi, ov), f"dec={dec}")
+
+
+def test_fixture_topk_na(gpu_ctx):
+    D = np.array([[1.0], [np.nan], [np.inf]])
+    idx, val = sd.topKBlock(D, 3, False, [0])
+    assert idx[0].tolist() == [1, NA_INT, NA_INT]
+    assert val[0, 0] == 1.0
+    assert sd.is_na_real(val[0, 1:]).all()
+    for case in FIX["topk_validation"]["cases"]:
+        with pytest.raises((ValueError, sd.SparseDistError), match=case["error"]):
+            sd.topKBlock(np.random.rand(3, 2), case["k"], False, case["self_row"])
+
+
+def test_fixture_ties_and_degenerate(gpu_ctx):
+    f = FIX["ties_lower_index"]
+    X = sp.csc_matrix(np.array(f["X"], dtype=float).T)
+    g = sd.sparseKNN(X, k=3, method="euclidean", verbose=False)
+    assert g.idx[f["row"] - 1].tolist() == f["expect_idx_row"]
+    f = FIX["massive_jaccard_ties"]
+    Xd = np.zeros((f["nrow"], f["ncol_x"]))
+    Xd[: f["ones_rows"]] = 1
+    Yd = np.zeros((f["nrow"], f["ncol_y"]))
+    Yd[: f["ones_rows"]] = 1
+    for bs in f["block_sizes"]:
+        cr = sd.sparseKNNCross(sp.csc_matrix(Xd), sp.csc_matrix(Yd), k=6, block_size=bs,
+                               verbose=False)
+        assert (cr["forward"].idx == np.arange(1, 7)).all()
+        assert (cr["reverse"].idx == np.arange(1, 7)).all()
+        assert (cr["forward"].dist == 0).all()
+    f = FIX["signed_cancel_not_empty"]
+    X = sp.csc_matrix(np.array(f["X"], dtype=float).T)
+    for m in f["methods"]:
+        g = sd.sparseKNN(X, k=2, method=m, verbose=False)
+        assert (g.idx[0] != NA_INT).all() and (g.idx[1] == NA_INT).all()
+        assert not (g.idx == 2).any() and (g.idx[1:] == 1).any()
+    f = FIX["cross_degenerate"]
+    clean = sp.csc_matrix(np.array(f["clean"], dtype=float).T)
+    holed = sp.csc_matrix(np.array(f["holed"], dtype=float).T)
+    a = sd.sparseKNN(holed, clean, k=2, verbose=False).idx
+    assert (a[1] == NA_INT).all() and (a[[0, 2]] != NA_INT).all()
+    b = sd.sparseKNN(clean, holed, k=2, verbose=False).idx
+    assert not (b == NA_INT).any() and not (b == 2).any()
+    with pytest.warns(sd.SparseDistWarning, match="exceeds"):
+        d = sd.sparseKNN(clean, holed, k=3, verbose=False)
+    assert d.idx.shape[1] == 2
+
+
+def test_fixture_pearson_and_cosine_conventions(gpu_ctx):
+    from test_oracle import check_cosine_conventions
+    check_cosine_conventions(lambda m, A, B, d: sd.sparseDist2(A, B, method=m, dist=d))
+    f = FIX["pearson_constant_self"]
+    X = sp.csc_matrix(np.array(f["X"], dtype=float).T)
+    s = sd.sparseKNN(X, k=2, method="pearson", dist=False, include_self=True, verbose=False)
+    at = list(s.idx[0]).index(1)
+    assert s.dist[0, at] == 1.0
+    d = sd.sparseKNN(X, k=2, method="pearson", dist=True, include_self=True, verbose=False)
+    assert d.idx[:, 0].tolist() == [1, 2] and d.dist[:, 0].tolist() == [0.0, 0.0]
+    X = sp.csc_matrix(np.array([[np.inf, 1.0], [0.0, 0.0]]))
+    with pytest.raises(sd.SparseDistError, match="finite"):
+        sd.sparseKNN(X, k=1, method="cosine", verbose=False)
+
+
+def test_block_size_and_batch_invariance(gpu_ctx):
+    X = synth.nb_log1p(3000, 700, 0.05, 14)
+    base = sd.sparseKNN(X, k=10, method="cosine", block_size=700, verbose=False)
+    for bs in (1, 61, 256, 10000):
+        g = sd.sparseKNN(X, k=10, method="cosine", block_size=bs, verbose=False)
+        same((g.idx, g.dist), (base.idx, base.dist), f"bs={bs}")
+    # a query's result does not depend on which other queries share the launch
+    Y = sd.DeviceMatrix(X)
+    sub = X[:, 100:110]
+    g = sd.sparseKNN(sub, Y, k=10, method="cosine", verbose=False)
+    full = sd.sparseKNN(X, Y, k=10, method="cosine", verbose=False)
+    same((g.idx, g.dist), (full.idx[100:110], full.dist[100:110]), "subset")
