@@ -396,7 +396,8 @@ namespace {
 
 __device__ inline uint64_t wave_shfl_u64(uint64_t v, int src) {
   const uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
-  return (uint64_t)__shfl((int)lo, src) | ((uint64_t)(uint32_t)__shfl((int)hi, src) << 32);
+  return (uint64_t)(uint32_t)__shfl((int)lo, src) |
+         ((uint64_t)(uint32_t)__shfl((int)hi, src) << 32);
 }
 
 // One wave per query: exact FP64 values of the L candidates (the reference's
