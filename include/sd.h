@@ -89,7 +89,8 @@ int sd_cancel(sd_context* ctx);
 
 /* Kernel statistics since the last reset: total device time in ms of the
  * fused slab kernel and its launch count, measured with HIP events on the
- * context's stream; uncertified query columns sent to the exact fallback. */
+ * context's stream; how many query columns the certificate cleared, how
+ * many needed the wider second pass, and how many went to the exact slab. */
 typedef struct {
   double slab_ms;
   int64_t slab_launches;
@@ -97,8 +98,9 @@ typedef struct {
   int64_t rerank_launches;
   double snn_ms;
   int64_t snn_launches;
-  int64_t fallback_queries;
-  int64_t certified_queries;
+  int64_t fallback_queries;   /* sent to the exact FP64 slab */
+  int64_t certified_queries;  /* cleared by the certificate (first or second pass) */
+  int64_t escalated_queries;  /* needed the second, wider candidate pass */
 } sd_stats;
 int sd_context_stats(sd_context* ctx, sd_stats* out, int reset);
 /* Enable/disable event timing of kernels (off by default). */

@@ -47,7 +47,8 @@ class Stats(C.Structure):
     _fields_ = [("slab_ms", C.c_double), ("slab_launches", C.c_int64),
                 ("rerank_ms", C.c_double), ("rerank_launches", C.c_int64),
                 ("snn_ms", C.c_double), ("snn_launches", C.c_int64),
-                ("fallback_queries", C.c_int64), ("certified_queries", C.c_int64)]
+                ("fallback_queries", C.c_int64), ("certified_queries", C.c_int64),
+                ("escalated_queries", C.c_int64)]
 
 
 _lib = None

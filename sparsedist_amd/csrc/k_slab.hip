@@ -104,6 +104,7 @@ struct Entry {
 struct FusedKernelArgs {
   const uint8_t* ref_dense;
   const uint8_t* qry_dense;
+  const int32_t* qlist;  // optional: position -> query column
   int64_t nref, nqry, kpad_bytes;
   const int64_t* ref_p;
   const int64_t* qry_p;
@@ -158,8 +159,10 @@ __global__ __launch_bounds__(WM * 256, 1) void k_fused(FusedKernelArgs P) {
   int qt, sp;
   map_block(blockIdx.x, P.qtiles, P.splits, qt, sp);
   const int64_t q_base = (int64_t)qt * TN;
-  const int64_t my_q = q_base + wn * 64 + h * 32 + n;  // the query this lane owns
-  const bool q_ok = my_q < P.nqry;
+  // position of the query this lane owns, and its column in the query matrix
+  const int64_t my_pos = q_base + wn * 64 + h * 32 + n;
+  const bool q_ok = my_pos < P.nqry;
+  const int64_t my_q = q_ok ? (P.qlist ? (int64_t)P.qlist[my_pos] : my_pos) : -1;
   uint32_t nj = 0;
   float qaux = 0.f;
   if (q_ok) {
@@ -182,6 +185,18 @@ __global__ __launch_bounds__(WM * 256, 1) void k_fused(FusedKernelArgs P) {
   constexpr int SEG_A = TM * 4 / NT;  // = 2
   constexpr int SEG_B = TN * 4 / NT;  // = 4 / WM
   uint4 ra[SEG_A], rb[SEG_B];
+  // per-thread source rows of the B (query) staging, fixed for the whole run
+  const uint8_t* pb[SEG_B];
+#pragma unroll
+  for (int u = 0; u < SEG_B; ++u) {
+    const int s = tid + u * NT;
+    const int64_t q = q_base + (s >> 2);
+    pb[u] = nullptr;
+    if (q < P.nqry) {
+      const int64_t gq = P.qlist ? (int64_t)P.qlist[q] : q;
+      pb[u] = P.qry_dense + gq * P.kpad_bytes + (s & 3) * 16;
+    }
+  }
 
   for (int64_t tile = t0; tile < t1; ++tile) {
     const int64_t r_base = tile * TM;
@@ -206,29 +221,22 @@ __global__ __launch_bounds__(WM * 256, 1) void k_fused(FusedKernelArgs P) {
 #pragma unroll
         for (int r = 0; r < 16; ++r) acc[b][j][r] = 0;
 
+    const uint8_t* pa[SEG_A];
+#pragma unroll
+    for (int u = 0; u < SEG_A; ++u) {
+      const int s = tid + u * NT;
+      const int64_t r = r_base + (s >> 2);
+      pa[u] = (r < P.nref) ? P.ref_dense + r * P.kpad_bytes + (s & 3) * 16 : nullptr;
+    }
     auto gload = [&](int64_t c) {
 #pragma unroll
-      for (int u = 0; u < SEG_A; ++u) {
-        const int s = tid + u * NT;
-        const int row = s >> 2, seg = s & 3;
-        const int64_t r = r_base + row;
-        if (r < P.nref)
-          ra[u] = *reinterpret_cast<const uint4*>(P.ref_dense + r * P.kpad_bytes + c * kRowBytes +
-                                                  seg * 16);
-        else
-          ra[u] = make_uint4(0, 0, 0, 0);
-      }
+      for (int u = 0; u < SEG_A; ++u)
+        ra[u] = pa[u] ? *reinterpret_cast<const uint4*>(pa[u] + c * kRowBytes)
+                      : make_uint4(0, 0, 0, 0);
 #pragma unroll
-      for (int u = 0; u < SEG_B; ++u) {
-        const int s = tid + u * NT;
-        const int row = s >> 2, seg = s & 3;
-        const int64_t q = q_base + row;
-        if (q < P.nqry)
-          rb[u] = *reinterpret_cast<const uint4*>(P.qry_dense + q * P.kpad_bytes + c * kRowBytes +
-                                                  seg * 16);
-        else
-          rb[u] = make_uint4(0, 0, 0, 0);
-      }
+      for (int u = 0; u < SEG_B; ++u)
+        rb[u] = pb[u] ? *reinterpret_cast<const uint4*>(pb[u] + c * kRowBytes)
+                      : make_uint4(0, 0, 0, 0);
     };
     auto lstore = [&](int buf) {
 #pragma unroll
@@ -334,7 +342,7 @@ __global__ __launch_bounds__(WM * 256, 1) void k_fused(FusedKernelArgs P) {
   }
 
   if (q_ok) {
-    const int64_t slot = ((int64_t)sp * WM + wm) * P.nqry + my_q;
+    const int64_t slot = ((int64_t)sp * WM + wm) * P.nqry + my_pos;
 #pragma unroll
     for (int t = 0; t < L; ++t) {
       const bool valid = list[t].idx != 0xFFFFFFFFu;
@@ -403,23 +411,94 @@ __device__ inline uint64_t wave_shfl_u64(uint64_t v, int src) {
 // One wave per query: exact FP64 values of the L candidates (the reference's
 // cosine arithmetic), exact order, certificate against the best value any
 // excluded column could have.
+//
+// Each wave first builds an LDS membership bitmap of its query's rows with
+// per-word prefix counts (and stages the query's unit values), so a lane
+// walks only its candidate column and looks each row up: the products of
+// the common rows are added in ascending row order, exactly the order of the
+// reference's merge walk (src/sparseDist.cpp:294-298).  When the bitmap does
+// not fit (very tall matrices) the lane runs the merge walk itself.
+constexpr int kRerankWaveBytes = 16384;  // LDS per wave
+
 __global__ __launch_bounds__(256) void k_rerank(RerankArgs R) {
-  const int lane = threadIdx.x & 63;
-  const int64_t j = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (j >= R.nqry) return;
+  extern __shared__ __align__(16) uint8_t rsm[];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t j = blockIdx.x * 4 + wave;  // position in the query list
+  const bool active = j < R.nqry;
   const int L = R.L;
+  const int64_t jq = active ? (R.qlist ? (int64_t)R.qlist[j] : j) : 0;  // query column
+  const int64_t q0 = R.ex.qry.p[jq], qn = active ? R.ex.qry.p[jq + 1] - q0 : 0;
+  const int nwords = R.map_words;
+  uint32_t* bm = reinterpret_cast<uint32_t*>(rsm + wave * kRerankWaveBytes);
+  uint16_t* pre = reinterpret_cast<uint16_t*>(bm + ((nwords + 3) & ~3));
+  double* qv_l = reinterpret_cast<double*>(pre + ((nwords + 7) & ~7));
+  const int64_t qcap = nwords > 0 ? (kRerankWaveBytes - ((nwords + 3) & ~3) * 4 -
+                                     ((nwords + 7) & ~7) * 2) / 8
+                                  : 0;
+  const bool use_map = nwords > 0 && active && qn < 65536;
+  const double* qv = (qn <= qcap) ? qv_l : R.ex.qry_unit + q0;
+  for (int w = lane; w < nwords; w += 64) bm[w] = 0u;
+  __syncthreads();
+  if (use_map) {
+    for (int64_t e = lane; e < qn; e += 64) {
+      const int32_t r = R.ex.qry.i[q0 + e];
+      atomicOr(&bm[r >> 5], 1u << (r & 31));
+      if (qn <= qcap) qv_l[e] = R.ex.qry_unit[q0 + e];
+    }
+  }
+  __syncthreads();
+  if (nwords > 0) {
+    // exclusive prefix of per-word popcounts: lane owns a contiguous run
+    const int per = (nwords + 63) / 64;
+    const int w0 = lane * per, w1 = (w0 + per < nwords) ? w0 + per : nwords;
+    int local = 0;
+    for (int w = w0; w < w1; ++w) local += __popc(bm[w]);
+    int incl = local;
+    for (int off = 1; off < 64; off <<= 1) {
+      const int y = __shfl_up(incl, off);
+      if (lane >= off) incl += y;
+    }
+    int run = incl - local;
+    for (int w = w0; w < w1; ++w) {
+      pre[w] = (uint16_t)run;
+      run += __popc(bm[w]);
+    }
+  }
+  __syncthreads();
+  if (!active) return;
   const int32_t ci = (lane < L) ? R.cand_idx[j * L + lane] : -1;
-  const bool deg_q = R.qdeg && R.qdeg[j];
-  const int64_t q0 = R.ex.qry.p[j], qn = R.ex.qry.p[j + 1] - q0;
+  const bool deg_q = R.qdeg && R.qdeg[jq];
   double c = -2.0, v = 0.0;
   const bool valid = ci >= 0 && !deg_q;
   if (valid) {
-    if (R.ex.self_mode && R.ex.include_self && ci == j) {
+    if (R.ex.self_mode && R.ex.include_self && ci == jq) {
       c = 1.0;  // restored self value (R/sparseKNN.R:270-279)
     } else {
       const int64_t a0 = R.ex.ref.p[ci], na = R.ex.ref.p[ci + 1] - a0;
-      c = cos_value(R.ex.ref.i + a0, R.ex.ref_unit + a0, na, R.ex.ref_scale[ci],
-                    R.ex.qry.i + q0, R.ex.qry_unit + q0, qn, R.ex.qry_scale[j]);
+      const double sa = R.ex.ref_scale[ci], sb = R.ex.qry_scale[jq];
+      if (use_map) {
+        const bool ea = (sa == 0.0), eb = (sb == 0.0);
+        if (ea || eb) {
+          c = (ea && eb) ? 1.0 : 0.0;
+        } else {
+          const int32_t* ra = R.ex.ref.i + a0;
+          const double* ua = R.ex.ref_unit + a0;
+          double acc = 0.0;
+          for (int64_t e = 0; e < na; ++e) {
+            const int32_t r = ra[e];
+            const uint32_t word = bm[r >> 5];
+            const uint32_t bit = 1u << (r & 31);
+            if (word & bit) acc += ua[e] * qv[pre[r >> 5] + __popc(word & (bit - 1u))];
+          }
+          c = acc;
+          if (!(c == c)) c = 0.0;
+          if (c > 1.0) c = 1.0;
+          if (c < -1.0) c = -1.0;
+        }
+      } else {
+        c = cos_value(R.ex.ref.i + a0, R.ex.ref_unit + a0, na, sa, R.ex.qry.i + q0,
+                      R.ex.qry_unit + q0, qn, sb);
+      }
     }
     v = R.ex.dist ? (1.0 - c) : c;
     if (v == 0.0) v = 0.0;
@@ -470,12 +549,12 @@ __global__ __launch_bounds__(256) void k_rerank(RerankArgs R) {
   if (!certified) {
     if (lane == 0) {
       const int at = atomicAdd(R.fail_count, 1);
-      R.fail_list[at] = (int32_t)j;
+      R.fail_list[at] = (int32_t)jq;
     }
     return;
   }
   if (lane < R.k) {
-    const int64_t e = (int64_t)lane * R.nqry + j;
+    const int64_t e = (int64_t)lane * R.out_ld + jq;
     if (lane < nvalid && !deg_q) {
       R.out_idx[e] = (int32_t)id + 1;
       R.out_val[e] = vv;
@@ -526,6 +605,7 @@ int fused_slab_topl(sd_context* ctx, const FusedArgs& a, double* cand_key, int32
   FusedKernelArgs P{};
   P.ref_dense = static_cast<const uint8_t*>(a.ref_dense);
   P.qry_dense = static_cast<const uint8_t*>(a.qry_dense);
+  P.qlist = a.qlist;
   P.nref = a.nref;
   P.nqry = a.nqry;
   const int elem = (a.method == SD_BINARY && a.prec != SD_PREC_FP32) ? 1
@@ -586,10 +666,17 @@ int finish_exact_lists(sd_context* ctx, const double* key, const int32_t* idx, i
   return SD_OK;
 }
 
+int rerank_map_words(int64_t nrow) {
+  const int64_t nwords = (nrow + 31) / 32;
+  const int64_t bytes = ((nwords + 3) & ~3LL) * 4 + ((nwords + 7) & ~7LL) * 2;
+  return (bytes + 64 * 8 <= kRerankWaveBytes) ? (int)nwords : 0;
+}
+
 int rerank_cosine(sd_context* ctx, const RerankArgs& R) {
   if (R.nqry <= 0) return SD_OK;
   KernelTimer timer(ctx, &ctx->stats.rerank_ms, &ctx->stats.rerank_launches);
-  hipLaunchKernelGGL(k_rerank, dim3((unsigned)((R.nqry + 3) / 4)), dim3(256), 0, ctx->stream, R);
+  hipLaunchKernelGGL(k_rerank, dim3((unsigned)((R.nqry + 3) / 4)), dim3(256),
+                     R.map_words > 0 ? 4 * kRerankWaveBytes : 0, ctx->stream, R);
   SD_HIP(hipGetLastError());
   return SD_OK;
 }

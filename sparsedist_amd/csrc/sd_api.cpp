@@ -283,7 +283,11 @@ int run_knn(sd_context* ctx, sd_matrix* X, sd_matrix* Y, const sd_knn_params* p,
                                 p->drop_degenerate ? X->deg.as<uint8_t>() : nullptr,
                                 out_idx.as<int32_t>(), out_val.as<double>()));
     } else {
+      // cosine: FP64 re-rank with certificate; queries it cannot clear get a
+      // second fused pass over just those columns with the widest list, and
+      // only what is still uncertified goes to the exact FP64 slab.
       DevBuf& fl = ctx->scratch[7];
+      DevBuf& fl2 = ctx->scratch[8];
       SD_TRY(fl.ensure(sizeof(int32_t) * (size_t)(nq + 16)));
       int32_t* fail_count = fl.as<int32_t>() + nq;
       SD_HIP(hipMemsetAsync(fail_count, 0, sizeof(int32_t), ctx->stream));
@@ -303,14 +307,62 @@ int run_knn(sd_context* ctx, sd_matrix* X, sd_matrix* Y, const sd_knn_params* p,
       R.out_val = out_val.as<double>();
       R.fail_list = fl.as<int32_t>();
       R.fail_count = fail_count;
+      R.qlist = nullptr;
+      R.out_ld = nq;
+      R.map_words = rerank_map_words(ref->nrow);
       SD_TRY(rerank_cosine(ctx, R));
       int32_t nfail = 0;
       SD_HIP(hipMemcpyAsync(&nfail, fail_count, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
       SD_HIP(hipStreamSynchronize(ctx->stream));
+      const int32_t* fail_ids = fl.as<int32_t>();
+      const int Lmax = fused_list_len(64);
+      if (nfail > 0 && L < Lmax) {
+        SD_TRY(check_cancel(ctx));
+        FusedArgs fb = fa;
+        fb.qlist = fl.as<int32_t>();
+        fb.nqry = nfail;
+        fb.L = Lmax;
+        const int64_t qt2 = (nfail + 255) / 256;
+        fb.splits = (int)std::min<int64_t>(round_up((2 * (int64_t)ctx->cu_count + qt2 - 1) / qt2, 8), 4096);
+        const int nl2 = fused_effective_splits(fb);
+        DevBuf& ck2 = ctx->scratch[9];
+        DevBuf& ci2 = ctx->scratch[10];
+        DevBuf& mk2 = ctx->scratch[11];
+        DevBuf& mi2 = ctx->scratch[12];
+        SD_TRY(ck2.ensure(sizeof(double) * (size_t)nl2 * (size_t)nfail * Lmax + 64));
+        SD_TRY(ci2.ensure(sizeof(int32_t) * (size_t)nl2 * (size_t)nfail * Lmax + 64));
+        SD_TRY(fused_slab_topl(ctx, fb, ck2.as<double>(), ci2.as<int32_t>()));
+        const double* k2 = ck2.as<double>();
+        const int32_t* i2 = ci2.as<int32_t>();
+        if (nl2 > 1) {
+          SD_TRY(mk2.ensure(sizeof(double) * (size_t)nfail * Lmax + 64));
+          SD_TRY(mi2.ensure(sizeof(int32_t) * (size_t)nfail * Lmax + 64));
+          SD_TRY(merge_lists(ctx, k2, i2, nl2, nfail, Lmax, mk2.as<double>(), mi2.as<int32_t>()));
+          k2 = mk2.as<double>();
+          i2 = mi2.as<int32_t>();
+        }
+        SD_TRY(fl2.ensure(sizeof(int32_t) * (size_t)(nfail + 16)));
+        int32_t* fail_count2 = fl2.as<int32_t>() + nfail;
+        SD_HIP(hipMemsetAsync(fail_count2, 0, sizeof(int32_t), ctx->stream));
+        RerankArgs R2 = R;
+        R2.cand_key = k2;
+        R2.cand_idx = i2;
+        R2.nqry = nfail;
+        R2.L = Lmax;
+        R2.qlist = fl.as<int32_t>();
+        R2.fail_list = fl2.as<int32_t>();
+        R2.fail_count = fail_count2;
+        SD_TRY(rerank_cosine(ctx, R2));
+        ctx->stats.escalated_queries += nfail;
+        SD_HIP(hipMemcpyAsync(&nfail, fail_count2, sizeof(int32_t), hipMemcpyDeviceToHost,
+                              ctx->stream));
+        SD_HIP(hipStreamSynchronize(ctx->stream));
+        fail_ids = fl2.as<int32_t>();
+      }
       ctx->stats.fallback_queries += nfail;
       ctx->stats.certified_queries += nq - nfail;
       if (nfail > 0)
-        SD_TRY(exact_knn_columns(ctx, ea, fl.as<int32_t>(), nfail, nq, k, plan.dec, 0,
+        SD_TRY(exact_knn_columns(ctx, ea, fail_ids, nfail, nq, k, plan.dec, 0,
                                  out_idx.as<int32_t>(), out_val.as<double>()));
     }
   } else {

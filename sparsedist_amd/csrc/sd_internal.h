@@ -81,7 +81,7 @@ struct sd_context {
   hipEvent_t ev_a = nullptr, ev_b = nullptr;
   int cu_count = 256;
   // scratch reused across calls (grow-only)
-  sd::DevBuf scratch[8];
+  sd::DevBuf scratch[16];
   std::vector<uint8_t> host_scratch;
 };
 
@@ -164,6 +164,7 @@ struct FusedArgs {
   int64_t nref = 0, nqry = 0, kpad = 0;
   const void* ref_dense = nullptr;
   const void* qry_dense = nullptr;
+  const int32_t* qlist = nullptr;   // optional: queries are these columns (positions 0..nqry)
   const int64_t* ref_p = nullptr;   // column pointers (nnz per column)
   const int64_t* qry_p = nullptr;
   const float* ref_aux = nullptr;   // per-column float aux (euclid: ||x||^2)
@@ -191,7 +192,11 @@ struct RerankArgs {
   double* out_val;
   int32_t* fail_list;          // queries the certificate could not clear
   int32_t* fail_count;
+  const int32_t* qlist = nullptr;  // optional: position -> query column
+  int64_t out_ld = 0;          // leading dimension of out_* (all query columns)
+  int map_words = 0;           // LDS row bitmap words (0: merge walk)
 };
+int rerank_map_words(int64_t nrow);
 int fused_list_len(int want);
 int fused_effective_splits(const FusedArgs& a);
 int merge_lists(sd_context* ctx, const double* key, const int32_t* idx, int nlists, int64_t nqry,

@@ -247,3 +247,27 @@ def test_block_size_and_batch_invariance(gpu_ctx):
     g = sd.sparseKNN(sub, Y, k=10, method="cosine", verbose=False)
     full = sd.sparseKNN(X, Y, k=10, method="cosine", verbose=False)
     same((g.idx, g.dist), (full.idx[100:110], full.dist[100:110]), "subset")
+
+
+def test_certificate_escalation_and_fallback(oracle_lib, gpu_ctx):
+    """A narrow first list (margin 1) forces the wider second pass, and
+    near-duplicate columns (cosines within BF16 rounding of each other) force
+    the exact FP64 fallback; every path must still match the oracle."""
+    X = synth.nb_log1p(3000, 600, 0.05, 31)
+    gpu_ctx.stats(reset=True)
+    g = sd.sparseKNN(X, k=6, method="cosine", margin=1, verbose=False)
+    st = gpu_ctx.stats(reset=True)
+    o = oracle_lib.knn(X, k=6, method="cosine", nthreads=NT)
+    same((g.idx, g.dist), o, "margin=1")
+    assert st["escalated_queries"] > 0
+    rng = np.random.default_rng(32)
+    base = synth.abs_normal(500, 1, 0.2, 33).toarray()
+    cols = [base * (1 + 1e-7 * rng.standard_normal(base.shape)) for _ in range(150)]
+    cols.append(synth.abs_normal(500, 100, 0.2, 34).toarray())
+    Xd = sp.csc_matrix(np.hstack(cols))
+    gpu_ctx.stats(reset=True)
+    g = sd.sparseKNN(Xd, k=10, method="cosine", verbose=False)
+    st = gpu_ctx.stats(reset=True)
+    o = oracle_lib.knn(Xd, k=10, method="cosine", nthreads=NT)
+    same((g.idx, g.dist), o, "near duplicates")
+    assert st["fallback_queries"] > 0
