@@ -27,7 +27,6 @@ namespace {
 
 constexpr int kTileRefs = 128;  // reference columns per wave row
 constexpr int kRowBytes = 64;   // K bytes per row per chunk
-constexpr int kLdsStride = 80;  // padded LDS row stride (bytes)
 
 // ---- MFMA operand types ------------------------------------------------------
 struct OpI8 {  // 0/1 patterns, exact int32 counts
@@ -132,22 +131,35 @@ __device__ inline void map_block(int b, int qtiles, int splits, int& qt, int& sp
   sp = lin % splits;
 }
 
+// Workgroup barrier for LDS only: wait for this wave's LDS operations, then
+// s_barrier.  Unlike __syncthreads() it does not wait for outstanding global
+// loads, which is what lets direct-to-LDS loads stay in flight across it.
+__device__ inline void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
 template <class Op, class K, int L, int WM>
 __global__ __launch_bounds__(WM * 256, 1) void k_fused(FusedKernelArgs P) {
   constexpr int WAVES = WM * 4;
   constexpr int NT = WAVES * 64;
   constexpr int TM = kTileRefs * WM;  // refs per workgroup tile
   constexpr int TN = 256;             // queries per workgroup
-  // A/B staging (double-buffered) shares its LDS with the epilogue's lane
-  // exchange area (NT lanes x 65 words, padded against bank conflicts).
-  constexpr int AB_BYTES = 2 * TM * kLdsStride + 2 * TN * kLdsStride;
-  constexpr int XCH_BYTES = NT * 65 * 4;
-  constexpr int UNION_BYTES = AB_BYTES > XCH_BYTES ? AB_BYTES : XCH_BYTES;
-  __shared__ __align__(16) uint8_t lds_u[UNION_BYTES];
-  uint8_t (*lds_a)[TM * kLdsStride] = reinterpret_cast<uint8_t (*)[TM * kLdsStride]>(lds_u);
-  uint8_t (*lds_b)[TN * kLdsStride] =
-      reinterpret_cast<uint8_t (*)[TN * kLdsStride]>(lds_u + 2 * TM * kLdsStride);
-  uint32_t* xch = reinterpret_cast<uint32_t*>(lds_u);
+  // K streams through a ring of four LDS stages filled by direct
+  // global->LDS loads (global_load_lds_dwordx4): each wave instruction writes
+  // 64 lanes x 16 B = 1 KB contiguously, i.e. 16 rows x 64 B.  Rows are not
+  // padded; the 16-byte slot of a segment is XOR-swizzled by (row >> 2) & 3
+  // so that 16 consecutive lanes reading one row each hit all 64 banks once.
+  // Each stage is its own LDS object and the loop is unrolled by four, so the
+  // compiler sees that the loads in flight never alias the stage being read
+  // and waits only for the chunk it needs.
+  static_assert(WM == 1, "the staging layout assumes one wave row");
+  constexpr int A_BYTES = TM * kRowBytes;  // 8 KB
+  constexpr int B_BYTES = TN * kRowBytes;  // 16 KB
+  constexpr int STAGE_BYTES = A_BYTES + B_BYTES;
+  __shared__ __align__(16) uint8_t st0[STAGE_BYTES];
+  __shared__ __align__(16) uint8_t st1[STAGE_BYTES];
+  __shared__ __align__(16) uint8_t st2[STAGE_BYTES];
+  __shared__ __align__(16) uint8_t st3[STAGE_BYTES];
+  // epilogue exchange: NT lanes x 33 words (32 values + 1 pad)
+  __shared__ uint32_t xch[NT * 33];
   __shared__ int32_t s_rnnz[TM];
   __shared__ float s_raux[TM];
   __shared__ uint8_t s_rdeg[TM];
@@ -181,22 +193,22 @@ __global__ __launch_bounds__(WM * 256, 1) void k_fused(FusedKernelArgs P) {
   const int64_t t1 = (t0 + P.tiles_per_split < P.ntiles) ? t0 + P.tiles_per_split : P.ntiles;
   const int64_t nchunks = P.kpad_bytes / kRowBytes;
 
-  // global -> register staging: A has TM*4 16-byte segments, B has TN*4
-  constexpr int SEG_A = TM * 4 / NT;  // = 2
-  constexpr int SEG_B = TN * 4 / NT;  // = 4 / WM
-  uint4 ra[SEG_A], rb[SEG_B];
-  // per-thread source rows of the B (query) staging, fixed for the whole run
-  const uint8_t* pb[SEG_B];
+  // Direct-load sources.  Wave w issues A instructions 2w, 2w+1 (of 8) and
+  // B instructions 4w..4w+3 (of 16); lane l of instruction i carries row
+  // 16i + l/4, physical slot l%4, i.e. logical segment (l%4) ^ ((l>>4)&3).
+  constexpr int A_INST = A_BYTES / 1024 / WAVES;  // 2
+  constexpr int B_INST = B_BYTES / 1024 / WAVES;  // 4
+  const int seg_l = (lane & 3) ^ ((lane >> 4) & 3);
+  const uint8_t* pb[B_INST];
 #pragma unroll
-  for (int u = 0; u < SEG_B; ++u) {
-    const int s = tid + u * NT;
-    const int64_t q = q_base + (s >> 2);
-    pb[u] = nullptr;
-    if (q < P.nqry) {
-      const int64_t gq = P.qlist ? (int64_t)P.qlist[q] : q;
-      pb[u] = P.qry_dense + gq * P.kpad_bytes + (s & 3) * 16;
-    }
+  for (int u = 0; u < B_INST; ++u) {
+    const int i = wave * B_INST + u;
+    int64_t q = q_base + 16 * i + (lane >> 2);
+    if (q >= P.nqry) q = P.nqry - 1;  // clamped rows feed only unused lanes
+    const int64_t gq = P.qlist ? (int64_t)P.qlist[q] : q;
+    pb[u] = P.qry_dense + gq * P.kpad_bytes + seg_l * 16;
   }
+  typedef __attribute__((address_space(3))) void lds_void;
 
   for (int64_t tile = t0; tile < t1; ++tile) {
     const int64_t r_base = tile * TM;
@@ -221,95 +233,110 @@ __global__ __launch_bounds__(WM * 256, 1) void k_fused(FusedKernelArgs P) {
 #pragma unroll
         for (int r = 0; r < 16; ++r) acc[b][j][r] = 0;
 
-    const uint8_t* pa[SEG_A];
+    const uint8_t* pa[A_INST];
 #pragma unroll
-    for (int u = 0; u < SEG_A; ++u) {
-      const int s = tid + u * NT;
-      const int64_t r = r_base + (s >> 2);
-      pa[u] = (r < P.nref) ? P.ref_dense + r * P.kpad_bytes + (s & 3) * 16 : nullptr;
+    for (int u = 0; u < A_INST; ++u) {
+      const int i = wave * A_INST + u;
+      int64_t r = r_base + 16 * i + (lane >> 2);
+      if (r >= P.nref) r = P.nref - 1;  // clamped rows are masked by s_rdeg
+      pa[u] = P.ref_dense + r * P.kpad_bytes + seg_l * 16;
     }
-    auto gload = [&](int64_t c) {
+    auto issue = [&](uint8_t* st, int64_t c) {
 #pragma unroll
-      for (int u = 0; u < SEG_A; ++u)
-        ra[u] = pa[u] ? *reinterpret_cast<const uint4*>(pa[u] + c * kRowBytes)
-                      : make_uint4(0, 0, 0, 0);
+      for (int u = 0; u < A_INST; ++u)
+        __builtin_amdgcn_global_load_lds((const void*)(pa[u] + c * kRowBytes),
+                                         (lds_void*)(st + (wave * A_INST + u) * 1024), 16, 0, 0);
 #pragma unroll
-      for (int u = 0; u < SEG_B; ++u)
-        rb[u] = pb[u] ? *reinterpret_cast<const uint4*>(pb[u] + c * kRowBytes)
-                      : make_uint4(0, 0, 0, 0);
+      for (int u = 0; u < B_INST; ++u)
+        __builtin_amdgcn_global_load_lds((const void*)(pb[u] + c * kRowBytes),
+                                         (lds_void*)(st + A_BYTES + (wave * B_INST + u) * 1024),
+                                         16, 0, 0);
     };
-    auto lstore = [&](int buf) {
-#pragma unroll
-      for (int u = 0; u < SEG_A; ++u) {
-        const int s = tid + u * NT;
-        *reinterpret_cast<uint4*>(&lds_a[buf][(s >> 2) * kLdsStride + (s & 3) * 16]) = ra[u];
-      }
-#pragma unroll
-      for (int u = 0; u < SEG_B; ++u) {
-        const int s = tid + u * NT;
-        *reinterpret_cast<uint4*>(&lds_b[buf][(s >> 2) * kLdsStride + (s & 3) * 16]) = rb[u];
-      }
-    };
-
-    gload(0);
-    __syncthreads();  // previous tile's readers of lds_* and s_r* are done
-    lstore(0);
-    __syncthreads();
-    for (int64_t c = 0; c < nchunks; ++c) {
-      const int buf = (int)(c & 1);
-      if (c + 1 < nchunks) gload(c + 1);
+    const int swz = (n >> 2) & 3;
+    const int s0 = ((2 * h) ^ swz) * 16, s1 = ((2 * h + 1) ^ swz) * 16;
+    auto compute = [&](const uint8_t* st) {
       uint4 fa[4][2], fb[2][2];
 #pragma unroll
       for (int b = 0; b < 4; ++b) {
-        const uint8_t* p = &lds_a[buf][(wm * kTileRefs + b * 32 + n) * kLdsStride + h * 32];
-        fa[b][0] = *reinterpret_cast<const uint4*>(p);
-        fa[b][1] = *reinterpret_cast<const uint4*>(p + 16);
+        const uint8_t* p = st + (b * 32 + n) * kRowBytes;
+        fa[b][0] = *reinterpret_cast<const uint4*>(p + s0);
+        fa[b][1] = *reinterpret_cast<const uint4*>(p + s1);
       }
 #pragma unroll
       for (int j = 0; j < 2; ++j) {
-        const uint8_t* p = &lds_b[buf][(wn * 64 + j * 32 + n) * kLdsStride + h * 32];
-        fb[j][0] = *reinterpret_cast<const uint4*>(p);
-        fb[j][1] = *reinterpret_cast<const uint4*>(p + 16);
+        const uint8_t* p = st + A_BYTES + (wn * 64 + j * 32 + n) * kRowBytes;
+        fb[j][0] = *reinterpret_cast<const uint4*>(p + s0);
+        fb[j][1] = *reinterpret_cast<const uint4*>(p + s1);
       }
 #pragma unroll
       for (int b = 0; b < 4; ++b)
 #pragma unroll
         for (int j = 0; j < 2; ++j) Op::mma(acc[b][j], fa[b], fb[j]);
-      if (c + 1 < nchunks) lstore(buf ^ 1);
-      __syncthreads();
+    };
+
+    lds_barrier();  // the previous tile's epilogue is done with the stages
+    if (nchunks > 0) issue(st0, 0);
+    if (nchunks > 1) issue(st1, 1);
+    if (nchunks > 2) issue(st2, 2);
+    // One step: wait until this wave's loads of chunk c have landed (loads
+    // retire in order, so at most (A_INST + B_INST) x (chunks issued after c)
+    // may still be outstanding), make every wave's part visible, refill the
+    // stage freed by the previous step with chunk c + 3, compute chunk c.
+#define SD_FUSED_STEP(CUR, NXT)                                               \
+    if (c < nchunks) {                                                        \
+      const int64_t ahead = nchunks - 1 - c;                                  \
+      if (ahead >= 2)                                                         \
+        __builtin_amdgcn_s_waitcnt(0xF70 | (2 * (A_INST + B_INST)));          \
+      else if (ahead == 1)                                                    \
+        __builtin_amdgcn_s_waitcnt(0xF70 | (A_INST + B_INST));                \
+      else                                                                    \
+        __builtin_amdgcn_s_waitcnt(0xF70);                                    \
+      lds_barrier();                                                          \
+      if (c + 3 < nchunks) issue(NXT, c + 3);                                 \
+      compute(CUR);                                                           \
+      ++c;                                                                    \
     }
+    for (int64_t c = 0; c < nchunks;) {
+      SD_FUSED_STEP(st0, st3)
+      SD_FUSED_STEP(st1, st0)
+      SD_FUSED_STEP(st2, st1)
+      SD_FUSED_STEP(st3, st2)
+    }
+#undef SD_FUSED_STEP
+    lds_barrier();  // every wave is done reading the stages
 
     // ---- epilogue: this lane owns query my_q and must see its 128 values.
     // acc[b][j][r] belongs to ref 32b + 8(r/4) + 4h + (r%4), query block j.
-    // Phase 0 scans the lane's own block-h values, phase 1 the block-h
-    // values of lane l ^ 32 (the other four rows of every group of eight).
-    // Values pass through the (now idle) staging LDS so the scan is a plain
-    // loop; rows are padded to 65 words against bank conflicts.
+    // Sub-phases 0-1 scan the lane's own block-h values (reference blocks
+    // 0-1, then 2-3), sub-phases 2-3 the block-h values held by lane l ^ 32
+    // (the other four rows of every group of eight).  Values pass through
+    // LDS (rows of 33 words against bank conflicts) so the scan is a loop.
     const int rloc_base = wm * kTileRefs;
 #pragma unroll
-    for (int phase = 0; phase < 2; ++phase) {
+    for (int sub = 0; sub < 4; ++sub) {
+      const int phase = sub >> 1, b0 = (sub & 1) * 2;
 #pragma unroll
-      for (int b = 0; b < 4; ++b)
+      for (int bb = 0; bb < 2; ++bb)
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
-          const auto v0 = acc[b][0][r], v1 = acc[b][1][r];
+          const auto v0 = acc[b0 + bb][0][r], v1 = acc[b0 + bb][1][r];
           const auto x = (phase == 0) ? (h ? v1 : v0) : (h ? v0 : v1);
           uint32_t bits;
           if constexpr (sizeof(K) == 8)
             bits = (uint32_t)x;  // exact integer count (int or float accumulator)
           else
             __builtin_memcpy(&bits, &x, 4);
-          xch[((phase == 0) ? tid : (tid ^ 32)) * 65 + b * 16 + r] = bits;
+          xch[((phase == 0) ? tid : (tid ^ 32)) * 33 + bb * 16 + r] = bits;
         }
-      __syncthreads();
+      lds_barrier();
       const int hh = (phase == 0) ? h : 1 - h;
       if (q_ok) {
-        for (int v = 0; v < 64; ++v) {
-          const int b = v >> 4, rr = v & 15;
+        for (int v = 0; v < 32; ++v) {
+          const int b = b0 + (v >> 4), rr = v & 15;
           const int rloc = rloc_base + b * 32 + 8 * (rr >> 2) + 4 * hh + (rr & 3);
           const int64_t rg = r_base + rloc;
           if (s_rdeg[rloc] || (P.self_mode && rg == my_q && !P.include_self)) continue;
-          const uint32_t bits = xch[tid * 65 + v];
+          const uint32_t bits = xch[tid * 33 + v];
           K key;
           if constexpr (sizeof(K) == 8) {
             const uint32_t c = bits;
@@ -337,7 +364,7 @@ __global__ __launch_bounds__(WM * 256, 1) void k_fused(FusedKernelArgs P) {
           }
         }
       }
-      __syncthreads();
+      lds_barrier();
     }
   }
 
