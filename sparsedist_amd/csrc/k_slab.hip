@@ -26,46 +26,41 @@ typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 namespace {
 
 constexpr int kTileRefs = 128;  // reference columns per wave row
-constexpr int kRowBytes = 64;   // K bytes per row per chunk
+constexpr int kRowBytes = 128;  // K bytes per row per chunk
 
 // ---- MFMA operand types ------------------------------------------------------
+// mma() consumes one 16-byte K segment of each operand per lane.  Within one
+// MFMA, lane (m, h) supplies K elements of its own choosing; A and B lanes
+// with the same h always supply the same logical segment, so the product is
+// the full dot product whatever the hardware's K order inside a lane.
 struct OpI8 {  // 0/1 patterns, exact int32 counts
   using Acc = v16i;
-  static constexpr int kElem = 1;
-  __device__ static void mma(Acc& c, const uint4 a[2], const uint4 b[2]) {
-#pragma unroll
-    for (int s = 0; s < 2; ++s) {
-      v4i av, bv;
-      __builtin_memcpy(&av, &a[s], 16);
-      __builtin_memcpy(&bv, &b[s], 16);
-      c = __builtin_amdgcn_mfma_i32_32x32x32_i8(av, bv, c, 0, 0, 0);
-    }
+  __device__ static void mma(Acc& c, const uint4& a, const uint4& b) {
+    v4i av, bv;
+    __builtin_memcpy(&av, &a, 16);
+    __builtin_memcpy(&bv, &b, 16);
+    c = __builtin_amdgcn_mfma_i32_32x32x32_i8(av, bv, c, 0, 0, 0);
   }
 };
 
 struct OpBf16 {  // BF16 values, FP32 accumulation
   using Acc = v16f;
-  static constexpr int kElem = 2;
-  __device__ static void mma(Acc& c, const uint4 a[2], const uint4 b[2]) {
-#pragma unroll
-    for (int s = 0; s < 2; ++s) {
-      v8bf av, bv;
-      __builtin_memcpy(&av, &a[s], 16);
-      __builtin_memcpy(&bv, &b[s], 16);
-      c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, c, 0, 0, 0);
-    }
+  __device__ static void mma(Acc& c, const uint4& a, const uint4& b) {
+    v8bf av, bv;
+    __builtin_memcpy(&av, &a, 16);
+    __builtin_memcpy(&bv, &b, 16);
+    c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, c, 0, 0, 0);
   }
 };
 
 struct OpF32 {  // plain FP32 values and products
   using Acc = v16f;
-  static constexpr int kElem = 4;
-  __device__ static void mma(Acc& c, const uint4 a[2], const uint4 b[2]) {
-    float av[8], bv[8];
-    __builtin_memcpy(av, a, 32);
-    __builtin_memcpy(bv, b, 32);
+  __device__ static void mma(Acc& c, const uint4& a, const uint4& b) {
+    float av[4], bv[4];
+    __builtin_memcpy(av, &a, 16);
+    __builtin_memcpy(bv, &b, 16);
 #pragma unroll
-    for (int e = 0; e < 8; ++e) c = __builtin_amdgcn_mfma_f32_32x32x2f32(av[e], bv[e], c, 0, 0, 0);
+    for (int e = 0; e < 4; ++e) c = __builtin_amdgcn_mfma_f32_32x32x2f32(av[e], bv[e], c, 0, 0, 0);
   }
 };
 
@@ -142,24 +137,24 @@ __global__ __launch_bounds__(WM * 256, 1) void k_fused(FusedKernelArgs P) {
   constexpr int NT = WAVES * 64;
   constexpr int TM = kTileRefs * WM;  // refs per workgroup tile
   constexpr int TN = 256;             // queries per workgroup
-  // K streams through a ring of four LDS stages filled by direct
+  // K streams through a ring of three LDS stages filled by direct
   // global->LDS loads (global_load_lds_dwordx4): each wave instruction writes
-  // 64 lanes x 16 B = 1 KB contiguously, i.e. 16 rows x 64 B.  Rows are not
-  // padded; the 16-byte slot of a segment is XOR-swizzled by (row >> 2) & 3
+  // 64 lanes x 16 B = 1 KB contiguously, i.e. 8 rows x 128 B.  Rows are not
+  // padded; the 16-byte slot of a segment is XOR-swizzled by (row >> 1) & 7
   // so that 16 consecutive lanes reading one row each hit all 64 banks once.
-  // Each stage is its own LDS object and the loop is unrolled by four, so the
-  // compiler sees that the loads in flight never alias the stage being read
-  // and waits only for the chunk it needs.
+  // Each stage is its own LDS object and the loop is unrolled by three, so
+  // the compiler sees that the loads in flight never alias the stage being
+  // read and waits only for the chunk it needs.  The epilogue's lane
+  // exchange (NT lanes x 33 words) reuses stage 0 once the ring is drained.
   static_assert(WM == 1, "the staging layout assumes one wave row");
-  constexpr int A_BYTES = TM * kRowBytes;  // 8 KB
-  constexpr int B_BYTES = TN * kRowBytes;  // 16 KB
+  constexpr int A_BYTES = TM * kRowBytes;  // 16 KB
+  constexpr int B_BYTES = TN * kRowBytes;  // 32 KB
   constexpr int STAGE_BYTES = A_BYTES + B_BYTES;
+  static_assert(NT * 33 * 4 <= STAGE_BYTES, "exchange area must fit in one stage");
   __shared__ __align__(16) uint8_t st0[STAGE_BYTES];
   __shared__ __align__(16) uint8_t st1[STAGE_BYTES];
   __shared__ __align__(16) uint8_t st2[STAGE_BYTES];
-  __shared__ __align__(16) uint8_t st3[STAGE_BYTES];
-  // epilogue exchange: NT lanes x 33 words (32 values + 1 pad)
-  __shared__ uint32_t xch[NT * 33];
+  uint32_t* xch = reinterpret_cast<uint32_t*>(st0);
   __shared__ int32_t s_rnnz[TM];
   __shared__ float s_raux[TM];
   __shared__ uint8_t s_rdeg[TM];
@@ -193,20 +188,21 @@ __global__ __launch_bounds__(WM * 256, 1) void k_fused(FusedKernelArgs P) {
   const int64_t t1 = (t0 + P.tiles_per_split < P.ntiles) ? t0 + P.tiles_per_split : P.ntiles;
   const int64_t nchunks = P.kpad_bytes / kRowBytes;
 
-  // Direct-load sources.  Wave w issues A instructions 2w, 2w+1 (of 8) and
-  // B instructions 4w..4w+3 (of 16); lane l of instruction i carries row
-  // 16i + l/4, physical slot l%4, i.e. logical segment (l%4) ^ ((l>>4)&3).
-  constexpr int A_INST = A_BYTES / 1024 / WAVES;  // 2
-  constexpr int B_INST = B_BYTES / 1024 / WAVES;  // 4
-  const int seg_l = (lane & 3) ^ ((lane >> 4) & 3);
+  // Direct-load sources.  Wave w issues A instructions 4w..4w+3 (of 16) and
+  // B instructions 8w..8w+7 (of 32); lane l of instruction i carries row
+  // 8i + l/8 and physical slot l%8, i.e. logical segment
+  // (l%8) ^ ((row >> 1) & 7) = (l%8) ^ ((4i + l/16) & 7).
+  constexpr int A_INST = A_BYTES / 1024 / WAVES;  // 4
+  constexpr int B_INST = B_BYTES / 1024 / WAVES;  // 8
+  auto seg_of = [&](int i) { return (lane & 7) ^ ((4 * i + (lane >> 4)) & 7); };
   const uint8_t* pb[B_INST];
 #pragma unroll
   for (int u = 0; u < B_INST; ++u) {
     const int i = wave * B_INST + u;
-    int64_t q = q_base + 16 * i + (lane >> 2);
+    int64_t q = q_base + 8 * i + (lane >> 3);
     if (q >= P.nqry) q = P.nqry - 1;  // clamped rows feed only unused lanes
     const int64_t gq = P.qlist ? (int64_t)P.qlist[q] : q;
-    pb[u] = P.qry_dense + gq * P.kpad_bytes + seg_l * 16;
+    pb[u] = P.qry_dense + gq * P.kpad_bytes + seg_of(i) * 16;
   }
   typedef __attribute__((address_space(3))) void lds_void;
 
@@ -237,9 +233,9 @@ __global__ __launch_bounds__(WM * 256, 1) void k_fused(FusedKernelArgs P) {
 #pragma unroll
     for (int u = 0; u < A_INST; ++u) {
       const int i = wave * A_INST + u;
-      int64_t r = r_base + 16 * i + (lane >> 2);
+      int64_t r = r_base + 8 * i + (lane >> 3);
       if (r >= P.nref) r = P.nref - 1;  // clamped rows are masked by s_rdeg
-      pa[u] = P.ref_dense + r * P.kpad_bytes + seg_l * 16;
+      pa[u] = P.ref_dense + r * P.kpad_bytes + seg_of(i) * 16;
     }
     auto issue = [&](uint8_t* st, int64_t c) {
 #pragma unroll
@@ -252,55 +248,49 @@ __global__ __launch_bounds__(WM * 256, 1) void k_fused(FusedKernelArgs P) {
                                          (lds_void*)(st + A_BYTES + (wave * B_INST + u) * 1024),
                                          16, 0, 0);
     };
-    const int swz = (n >> 2) & 3;
-    const int s0 = ((2 * h) ^ swz) * 16, s1 = ((2 * h + 1) ^ swz) * 16;
+    const int swz = (n >> 1) & 7;
     auto compute = [&](const uint8_t* st) {
-      uint4 fa[4][2], fb[2][2];
+      // MFMA step t: lanes with h supply logical segment 2t + h
 #pragma unroll
-      for (int b = 0; b < 4; ++b) {
-        const uint8_t* p = st + (b * 32 + n) * kRowBytes;
-        fa[b][0] = *reinterpret_cast<const uint4*>(p + s0);
-        fa[b][1] = *reinterpret_cast<const uint4*>(p + s1);
+      for (int t = 0; t < 4; ++t) {
+        const int slot = ((2 * t + h) ^ swz) * 16;
+        uint4 fa[4], fb[2];
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+          fa[b] = *reinterpret_cast<const uint4*>(st + (b * 32 + n) * kRowBytes + slot);
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+          fb[j] = *reinterpret_cast<const uint4*>(st + A_BYTES + (wn * 64 + j * 32 + n) * kRowBytes +
+                                                  slot);
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+#pragma unroll
+          for (int j = 0; j < 2; ++j) Op::mma(acc[b][j], fa[b], fb[j]);
       }
-#pragma unroll
-      for (int j = 0; j < 2; ++j) {
-        const uint8_t* p = st + A_BYTES + (wn * 64 + j * 32 + n) * kRowBytes;
-        fb[j][0] = *reinterpret_cast<const uint4*>(p + s0);
-        fb[j][1] = *reinterpret_cast<const uint4*>(p + s1);
-      }
-#pragma unroll
-      for (int b = 0; b < 4; ++b)
-#pragma unroll
-        for (int j = 0; j < 2; ++j) Op::mma(acc[b][j], fa[b], fb[j]);
     };
 
     lds_barrier();  // the previous tile's epilogue is done with the stages
     if (nchunks > 0) issue(st0, 0);
     if (nchunks > 1) issue(st1, 1);
-    if (nchunks > 2) issue(st2, 2);
     // One step: wait until this wave's loads of chunk c have landed (loads
-    // retire in order, so at most (A_INST + B_INST) x (chunks issued after c)
-    // may still be outstanding), make every wave's part visible, refill the
-    // stage freed by the previous step with chunk c + 3, compute chunk c.
+    // retire in order, so (A_INST + B_INST) loads of chunk c + 1 may still be
+    // outstanding), make every wave's part visible, refill the stage freed by
+    // the previous step with chunk c + 2, compute chunk c.
 #define SD_FUSED_STEP(CUR, NXT)                                               \
     if (c < nchunks) {                                                        \
-      const int64_t ahead = nchunks - 1 - c;                                  \
-      if (ahead >= 2)                                                         \
-        __builtin_amdgcn_s_waitcnt(0xF70 | (2 * (A_INST + B_INST)));          \
-      else if (ahead == 1)                                                    \
+      if (c + 1 < nchunks)                                                    \
         __builtin_amdgcn_s_waitcnt(0xF70 | (A_INST + B_INST));                \
       else                                                                    \
         __builtin_amdgcn_s_waitcnt(0xF70);                                    \
       lds_barrier();                                                          \
-      if (c + 3 < nchunks) issue(NXT, c + 3);                                 \
+      if (c + 2 < nchunks) issue(NXT, c + 2);                                 \
       compute(CUR);                                                           \
       ++c;                                                                    \
     }
     for (int64_t c = 0; c < nchunks;) {
-      SD_FUSED_STEP(st0, st3)
+      SD_FUSED_STEP(st0, st2)
       SD_FUSED_STEP(st1, st0)
       SD_FUSED_STEP(st2, st1)
-      SD_FUSED_STEP(st3, st2)
     }
 #undef SD_FUSED_STEP
     lds_barrier();  // every wave is done reading the stages

@@ -232,7 +232,7 @@ int run_knn(sd_context* ctx, sd_matrix* X, sd_matrix* Y, const sd_knn_params* p,
 
   if (fused_ok_binary || fused_ok_cosine) {
     const bool use_f32 = (prec == SD_PREC_FP32);
-    const int64_t kpad = round_up(std::max<int64_t>(ref->nrow, 1), 64);
+    const int64_t kpad = round_up(std::max<int64_t>(ref->nrow, 1), 128);
     int kind;
     if (method == SD_BINARY)
       kind = use_f32 ? kDenseF32Pattern : kDenseI8Pattern;
