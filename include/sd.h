@@ -101,6 +101,7 @@ typedef struct {
   int64_t fallback_queries;   /* sent to the exact FP64 slab */
   int64_t certified_queries;  /* cleared by the certificate (first or second pass) */
   int64_t escalated_queries;  /* needed the second, wider candidate pass */
+  int64_t overflow_queries;   /* candidate buffer overflowed: full per-query pass */
 } sd_stats;
 int sd_context_stats(sd_context* ctx, sd_stats* out, int reset);
 /* Enable/disable event timing of kernels (off by default). */

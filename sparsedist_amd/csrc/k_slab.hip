@@ -107,9 +107,17 @@ struct FusedKernelArgs {
   const uint8_t* ref_deg;
   int self_mode, include_self, method;
   int splits, qtiles, tiles_per_split;
-  int64_t ntiles;
+  int64_t ntiles;       // list mode: virtual tiles (every tile_stride-th reference tile)
+  int tile_stride;
   double* cand_key;
   int32_t* cand_idx;
+  // filter mode: one (query tile, reference tile) per workgroup
+  int rtiles, qgroups;
+  const double* thr;    // per query position: keep candidates with key >= thr
+  int cap;
+  int32_t* cnt;         // per query position: candidates appended
+  double* fkey;         // [nqry][cap]
+  int32_t* fidx;
 };
 
 namespace {
@@ -126,12 +134,26 @@ __device__ inline void map_block(int b, int qtiles, int splits, int& qt, int& sp
   sp = lin % splits;
 }
 
+// Filter-mode rasterisation: tiles are ordered (group of 4 query tiles,
+// reference tile, query tile in group) and each XCD takes a contiguous range
+// of that order, so the 32 workgroups resident on one XCD cover 4 query tiles
+// x 8 reference tiles and share those A and B tiles in the XCD's L2.
+__device__ inline bool map_tile(int64_t b, int64_t nblk, int qtiles, int rtiles, int& qt,
+                                int64_t& rt) {
+  const int64_t lin = (b & 7) * (nblk >> 3) + (b >> 3);
+  const int64_t per = 4 * (int64_t)rtiles;
+  const int64_t qg = lin / per, rem = lin % per;
+  rt = rem / 4;
+  qt = (int)(qg * 4 + rem % 4);
+  return qt < qtiles;
+}
+
 // Workgroup barrier for LDS only: wait for this wave's LDS operations, then
 // s_barrier.  Unlike __syncthreads() it does not wait for outstanding global
 // loads, which is what lets direct-to-LDS loads stay in flight across it.
 __device__ inline void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
 
-template <class Op, class K, int L, int WM>
+template <class Op, class K, int L, int WM, bool FILTER>
 __global__ __launch_bounds__(WM * 256, 1) void k_fused(FusedKernelArgs P) {
   constexpr int WAVES = WM * 4;
   constexpr int NT = WAVES * 64;
@@ -163,8 +185,13 @@ __global__ __launch_bounds__(WM * 256, 1) void k_fused(FusedKernelArgs P) {
   const int wave = tid >> 6, lane = tid & 63;
   const int wm = wave / 4, wn = wave % 4;
   const int h = lane >> 5, n = lane & 31;
-  int qt, sp;
-  map_block(blockIdx.x, P.qtiles, P.splits, qt, sp);
+  int qt, sp = 0;
+  int64_t rt = 0;
+  if constexpr (FILTER) {
+    if (!map_tile(blockIdx.x, gridDim.x, P.qtiles, P.rtiles, qt, rt)) return;
+  } else {
+    map_block(blockIdx.x, P.qtiles, P.splits, qt, sp);
+  }
   const int64_t q_base = (int64_t)qt * TN;
   // position of the query this lane owns, and its column in the query matrix
   const int64_t my_pos = q_base + wn * 64 + h * 32 + n;
@@ -184,8 +211,16 @@ __global__ __launch_bounds__(WM * 256, 1) void k_fused(FusedKernelArgs P) {
     list[t].idx = 0xFFFFFFFFu;
   }
 
-  const int64_t t0 = (int64_t)sp * P.tiles_per_split;
-  const int64_t t1 = (t0 + P.tiles_per_split < P.ntiles) ? t0 + P.tiles_per_split : P.ntiles;
+  int64_t t0, t1;
+  double thr_q = 0.0;
+  if constexpr (FILTER) {
+    t0 = rt;
+    t1 = rt + 1;
+    if (q_ok) thr_q = P.thr[my_pos];
+  } else {
+    t0 = (int64_t)sp * P.tiles_per_split;
+    t1 = (t0 + P.tiles_per_split < P.ntiles) ? t0 + P.tiles_per_split : P.ntiles;
+  }
   const int64_t nchunks = P.kpad_bytes / kRowBytes;
 
   // Direct-load sources.  Wave w issues A instructions 4w..4w+3 (of 16) and
@@ -206,7 +241,8 @@ __global__ __launch_bounds__(WM * 256, 1) void k_fused(FusedKernelArgs P) {
   }
   typedef __attribute__((address_space(3))) void lds_void;
 
-  for (int64_t tile = t0; tile < t1; ++tile) {
+  for (int64_t vt = t0; vt < t1; ++vt) {
+    const int64_t tile = FILTER ? vt : vt * P.tile_stride;
     const int64_t r_base = tile * TM;
     for (int t = tid; t < TM; t += NT) {
       const int64_t r = r_base + t;
@@ -341,7 +377,30 @@ __global__ __launch_bounds__(WM * 256, 1) void k_fused(FusedKernelArgs P) {
               key = sv;
             if (P.self_mode && rg == my_q) key = 2.0f;  // restored self similarity
           }
-          if (better(key, (uint32_t)rg, list[L - 1].key, list[L - 1].idx)) {
+          if constexpr (FILTER) {
+            double kd;
+            bool pass;
+            if constexpr (sizeof(K) == 8) {
+              // cheap float screen (errors far below 2^-20), then the exact rational
+              const float f = (float)key.c / (float)key.d;
+              pass = f >= (float)thr_q * (1.0f - 9.5367431640625e-07f);
+              kd = 0.0;
+              if (pass) {
+                kd = key_to_double(key);
+                pass = kd >= thr_q;
+              }
+            } else {
+              kd = (double)key;
+              pass = kd >= thr_q;
+            }
+            if (pass) {
+              const int at = atomicAdd(&P.cnt[my_pos], 1);
+              if (at < P.cap) {
+                P.fkey[my_pos * P.cap + at] = kd;
+                P.fidx[my_pos * P.cap + at] = (int32_t)rg;
+              }
+            }
+          } else if (better(key, (uint32_t)rg, list[L - 1].key, list[L - 1].idx)) {
             Entry<K> carry{key, (uint32_t)rg};
 #pragma unroll
             for (int p = 0; p < L; ++p) {
@@ -358,7 +417,7 @@ __global__ __launch_bounds__(WM * 256, 1) void k_fused(FusedKernelArgs P) {
     }
   }
 
-  if (q_ok) {
+  if (!FILTER && q_ok) {
     const int64_t slot = ((int64_t)sp * WM + wm) * P.nqry + my_pos;
 #pragma unroll
     for (int t = 0; t < L; ++t) {
@@ -411,6 +470,57 @@ __global__ void k_finish_exact(const double* __restrict__ key, const int32_t* __
     out_idx[e] = i + 1;
     out_val[e] = dist ? (1.0 - sim) : sim;
   }
+}
+
+
+// Top-L of each query's appended candidates (key desc, idx asc); queries
+// whose buffer overflowed are listed for a full list-mode pass instead.
+__global__ __launch_bounds__(256) void k_select(const double* __restrict__ fkey,
+                                                const int32_t* __restrict__ fidx,
+                                                const int32_t* __restrict__ cnt, int cap,
+                                                int64_t nqry, int L, double* __restrict__ out_key,
+                                                int32_t* __restrict__ out_idx,
+                                                int32_t* __restrict__ over_list,
+                                                int32_t* __restrict__ over_count) {
+  __shared__ BlockTopKStorage<512> st;
+  const int64_t q = blockIdx.x;
+  const int n = cnt[q];
+  if (n > cap) {
+    if (threadIdx.x == 0) over_list[atomicAdd(over_count, 1)] = (int32_t)q;
+    for (int t = threadIdx.x; t < L; t += blockDim.x) out_idx[q * L + t] = -1;
+    return;
+  }
+  BlockTopK<512> tk(st, L, 1);
+  tk.run((int64_t)n, [&](int64_t t, double& v, int64_t& id) {
+    v = fkey[q * cap + t];
+    id = fidx[q * cap + t];
+    return true;
+  });
+  __syncthreads();
+  const int c = st.cnt;
+  for (int t = threadIdx.x; t < L; t += blockDim.x) {
+    out_key[q * L + t] = (t < c) ? st.val[t] : 0.0;
+    out_idx[q * L + t] = (t < c) ? (int32_t)st.idx[t] : -1;
+  }
+}
+
+// Threshold per query from a list: its L-th key, or -inf when not full.
+__global__ void k_threshold(const double* __restrict__ key, const int32_t* __restrict__ idx,
+                            int64_t nqry, int L, double* __restrict__ thr) {
+  const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (q >= nqry) return;
+  thr[q] = (idx[q * L + L - 1] >= 0) ? key[q * L + L - 1] : -INFINITY;
+}
+
+// Copy list rows j into rows rows[j] of the main lists.
+__global__ void k_scatter_lists(const double* __restrict__ key, const int32_t* __restrict__ idx,
+                                const int32_t* __restrict__ rows, int64_t n, int L,
+                                double* __restrict__ out_key, int32_t* __restrict__ out_idx) {
+  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (e >= n * L) return;
+  const int64_t j = e / L, t = e % L;
+  out_key[(int64_t)rows[j] * L + t] = key[e];
+  out_idx[(int64_t)rows[j] * L + t] = idx[e];
 }
 
 }  // namespace
@@ -591,7 +701,19 @@ __global__ __launch_bounds__(256) void k_rerank(RerankArgs R) {
 template <class Op, class K, int L, int WM>
 static int launch_fused_t(sd_context* ctx, FusedKernelArgs& P) {
   const int nblk = P.qtiles * P.splits;
-  hipLaunchKernelGGL((k_fused<Op, K, L, WM>), dim3((unsigned)nblk), dim3(WM * 256), 0, ctx->stream,
+  hipLaunchKernelGGL((k_fused<Op, K, L, WM, false>), dim3((unsigned)nblk), dim3(WM * 256), 0,
+                     ctx->stream, P);
+  SD_HIP(hipGetLastError());
+  return SD_OK;
+}
+
+template <class Op, class K>
+static int launch_filter_t(sd_context* ctx, FusedKernelArgs& P) {
+  P.qgroups = (P.qtiles + 3) / 4;
+  int64_t nblk = (int64_t)P.qgroups * 4 * P.rtiles;
+  nblk = (nblk + 7) / 8 * 8;
+  if (nblk >= ((int64_t)1 << 31)) return set_error(SD_EUNSUPPORTED, "slab grid too large");
+  hipLaunchKernelGGL((k_fused<Op, K, 8, 1, true>), dim3((unsigned)nblk), dim3(256), 0, ctx->stream,
                      P);
   SD_HIP(hipGetLastError());
   return SD_OK;
@@ -639,7 +761,8 @@ int fused_slab_topl(sd_context* ctx, const FusedArgs& a, double* cand_key, int32
   P.method = a.method;
   constexpr int WM = 1;
   const int64_t TM = (int64_t)kTileRefs * WM;
-  P.ntiles = (a.nref + TM - 1) / TM;
+  P.tile_stride = a.tile_stride < 1 ? 1 : a.tile_stride;
+  P.ntiles = ((a.nref + TM - 1) / TM + P.tile_stride - 1) / P.tile_stride;
   P.qtiles = (int)((a.nqry + 255) / 256);
   int splits = a.splits < 1 ? 1 : a.splits;
   if (splits > P.ntiles) splits = (int)P.ntiles;
@@ -657,11 +780,78 @@ int fused_slab_topl(sd_context* ctx, const FusedArgs& a, double* cand_key, int32
 }
 
 int fused_effective_splits(const FusedArgs& a) {
-  const int64_t ntiles = (a.nref + kTileRefs - 1) / kTileRefs;
+  const int64_t stride = a.tile_stride < 1 ? 1 : a.tile_stride;
+  const int64_t ntiles = ((a.nref + kTileRefs - 1) / kTileRefs + stride - 1) / stride;
   int splits = a.splits < 1 ? 1 : a.splits;
   if (splits > ntiles) splits = (int)ntiles;
   const int64_t tps = (ntiles + splits - 1) / splits;
   return (int)((ntiles + tps - 1) / tps);
+}
+
+int fused_filter_pass(sd_context* ctx, const FusedArgs& a, const double* thr, int cap,
+                      int32_t* cnt, double* fkey, int32_t* fidx) {
+  FusedKernelArgs P{};
+  P.ref_dense = static_cast<const uint8_t*>(a.ref_dense);
+  P.qry_dense = static_cast<const uint8_t*>(a.qry_dense);
+  P.qlist = a.qlist;
+  P.nref = a.nref;
+  P.nqry = a.nqry;
+  const int elem = (a.method == SD_BINARY && a.prec != SD_PREC_FP32) ? 1
+                   : (a.prec == SD_PREC_FP32)                         ? 4
+                                                                      : 2;
+  P.kpad_bytes = a.kpad * elem;
+  P.ref_p = a.ref_p;
+  P.qry_p = a.qry_p;
+  P.ref_aux = a.ref_aux;
+  P.qry_aux = a.qry_aux;
+  P.ref_deg = a.ref_deg;
+  P.self_mode = a.self_mode;
+  P.include_self = a.include_self;
+  P.method = a.method;
+  P.qtiles = (int)((a.nqry + 255) / 256);
+  P.rtiles = (int)((a.nref + kTileRefs - 1) / kTileRefs);
+  P.tile_stride = 1;
+  P.thr = thr;
+  P.cap = cap;
+  P.cnt = cnt;
+  P.fkey = fkey;
+  P.fidx = fidx;
+  SD_HIP(hipMemsetAsync(cnt, 0, sizeof(int32_t) * (size_t)a.nqry, ctx->stream));
+  KernelTimer timer(ctx, &ctx->stats.slab_ms, &ctx->stats.slab_launches);
+  if (a.method == SD_BINARY) {
+    if (a.prec == SD_PREC_FP32) return launch_filter_t<OpF32, KeyJ>(ctx, P);
+    return launch_filter_t<OpI8, KeyJ>(ctx, P);
+  }
+  if (a.prec == SD_PREC_FP32) return launch_filter_t<OpF32, float>(ctx, P);
+  return launch_filter_t<OpBf16, float>(ctx, P);
+}
+
+int select_lists(sd_context* ctx, const double* fkey, const int32_t* fidx, const int32_t* cnt,
+                 int cap, int64_t nqry, int L, double* out_key, int32_t* out_idx,
+                 int32_t* over_list, int32_t* over_count) {
+  if (nqry <= 0) return SD_OK;
+  hipLaunchKernelGGL(k_select, dim3((unsigned)nqry), dim3(256), 0, ctx->stream, fkey, fidx, cnt, cap,
+                     nqry, L, out_key, out_idx, over_list, over_count);
+  SD_HIP(hipGetLastError());
+  return SD_OK;
+}
+
+int list_thresholds(sd_context* ctx, const double* key, const int32_t* idx, int64_t nqry, int L,
+                    double* thr) {
+  if (nqry <= 0) return SD_OK;
+  hipLaunchKernelGGL(k_threshold, dim3((unsigned)((nqry + 255) / 256)), dim3(256), 0, ctx->stream,
+                     key, idx, nqry, L, thr);
+  SD_HIP(hipGetLastError());
+  return SD_OK;
+}
+
+int scatter_lists(sd_context* ctx, const double* key, const int32_t* idx, const int32_t* rows,
+                  int64_t n, int L, double* out_key, int32_t* out_idx) {
+  if (n <= 0) return SD_OK;
+  hipLaunchKernelGGL(k_scatter_lists, dim3((unsigned)((n * L + 255) / 256)), dim3(256), 0,
+                     ctx->stream, key, idx, rows, n, L, out_key, out_idx);
+  SD_HIP(hipGetLastError());
+  return SD_OK;
 }
 
 int merge_lists(sd_context* ctx, const double* key, const int32_t* idx, int nlists, int64_t nqry,

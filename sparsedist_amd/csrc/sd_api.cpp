@@ -201,6 +201,102 @@ int plan_knn(const sd_matrix* X, const sd_matrix* Y, const sd_knn_params* p, Knn
   return SD_OK;
 }
 
+// Per-query candidate lists (top-L by the slab key) for every query of fa.
+//
+// Large searches run in two passes.  A list-mode pass over every stride-th
+// reference tile gives each query a threshold (its L-th key there), a valid
+// lower bound for its L-th key over all references.  The filter-mode pass
+// then computes every (query tile, reference tile) exactly once in a
+// rasterised grid -- concurrent workgroups share their tiles in L2 -- and
+// appends only candidates at or above the threshold; a per-query top-L of
+// those is the result.  Queries whose buffer overflows (many ties at the
+// threshold) get a full list-mode pass instead.  Small searches use the
+// list mode directly.
+int fused_candidate_lists(sd_context* ctx, FusedArgs fa, const double** out_key,
+                          const int32_t** out_idx) {
+  const int64_t nq = fa.nqry;
+  const int L = fa.L;
+  const int64_t rtiles = (fa.nref + 127) / 128;
+  auto list_pass = [&](FusedArgs a, DevBuf& kbuf, DevBuf& ibuf, DevBuf& mkbuf, DevBuf& mibuf,
+                       const double** rk, const int32_t** ri) -> int {
+    const int64_t qtiles = (a.nqry + 255) / 256;
+    int64_t splits = (2 * (int64_t)ctx->cu_count + qtiles - 1) / qtiles;
+    a.splits = (int)std::min<int64_t>(round_up(std::max<int64_t>(splits, 1), 8), 4096);
+    const int nlists = fused_effective_splits(a);
+    SD_TRY(kbuf.ensure(sizeof(double) * (size_t)nlists * (size_t)a.nqry * (size_t)L + 64));
+    SD_TRY(ibuf.ensure(sizeof(int32_t) * (size_t)nlists * (size_t)a.nqry * (size_t)L + 64));
+    SD_TRY(fused_slab_topl(ctx, a, kbuf.as<double>(), ibuf.as<int32_t>()));
+    *rk = kbuf.as<double>();
+    *ri = ibuf.as<int32_t>();
+    if (nlists > 1) {
+      SD_TRY(mkbuf.ensure(sizeof(double) * (size_t)a.nqry * (size_t)L + 64));
+      SD_TRY(mibuf.ensure(sizeof(int32_t) * (size_t)a.nqry * (size_t)L + 64));
+      SD_TRY(merge_lists(ctx, *rk, *ri, nlists, a.nqry, L, mkbuf.as<double>(), mibuf.as<int32_t>()));
+      *rk = mkbuf.as<double>();
+      *ri = mibuf.as<int32_t>();
+    }
+    return SD_OK;
+  };
+  const int64_t stride = 16;
+  if (rtiles < 4 * stride) {
+    fa.tile_stride = 1;
+    return list_pass(fa, ctx->scratch[3], ctx->scratch[4], ctx->scratch[5], ctx->scratch[6],
+                     out_key, out_idx);
+  }
+  // 1. thresholds from every stride-th reference tile
+  const double* pk = nullptr;
+  const int32_t* pi = nullptr;
+  FusedArgs pre = fa;
+  pre.tile_stride = (int)stride;
+  SD_TRY(list_pass(pre, ctx->scratch[3], ctx->scratch[4], ctx->scratch[5], ctx->scratch[6], &pk,
+                   &pi));
+  DevBuf& thr = ctx->scratch[13];
+  SD_TRY(thr.ensure(sizeof(double) * (size_t)nq + 64));
+  SD_TRY(list_thresholds(ctx, pk, pi, nq, L, thr.as<double>()));
+  // 2. filtered full pass
+  const int cap = (int)std::min<int64_t>(4096, 2 * L * stride + 256);
+  DevBuf& cnt = ctx->scratch[14];
+  DevBuf& fk = ctx->scratch[9];
+  DevBuf& fi = ctx->scratch[10];
+  SD_TRY(cnt.ensure(sizeof(int32_t) * (size_t)(nq + 16)));
+  SD_TRY(fk.ensure(sizeof(double) * (size_t)nq * (size_t)cap + 64));
+  SD_TRY(fi.ensure(sizeof(int32_t) * (size_t)nq * (size_t)cap + 64));
+  SD_TRY(check_cancel(ctx));
+  SD_TRY(fused_filter_pass(ctx, fa, thr.as<double>(), cap, cnt.as<int32_t>(), fk.as<double>(),
+                           fi.as<int32_t>()));
+  // 3. per-query top-L of the appended candidates
+  DevBuf& mk = ctx->scratch[5];
+  DevBuf& mi = ctx->scratch[6];
+  DevBuf& ov = ctx->scratch[15];
+  SD_TRY(mk.ensure(sizeof(double) * (size_t)nq * (size_t)L + 64));
+  SD_TRY(mi.ensure(sizeof(int32_t) * (size_t)nq * (size_t)L + 64));
+  SD_TRY(ov.ensure(sizeof(int32_t) * (size_t)(nq + 16)));
+  int32_t* over_count = ov.as<int32_t>() + nq;
+  SD_HIP(hipMemsetAsync(over_count, 0, sizeof(int32_t), ctx->stream));
+  SD_TRY(select_lists(ctx, fk.as<double>(), fi.as<int32_t>(), cnt.as<int32_t>(), cap, nq, L,
+                      mk.as<double>(), mi.as<int32_t>(), ov.as<int32_t>(), over_count));
+  int32_t nover = 0;
+  SD_HIP(hipMemcpyAsync(&nover, over_count, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+  SD_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->stats.overflow_queries += nover;
+  if (nover > 0) {
+    // 4. overflowed queries: full list-mode pass over all reference tiles
+    FusedArgs ofa = fa;
+    ofa.tile_stride = 1;
+    ofa.qlist = ov.as<int32_t>();
+    ofa.nqry = nover;
+    const double* ok = nullptr;
+    const int32_t* oi = nullptr;
+    SD_TRY(list_pass(ofa, ctx->scratch[3], ctx->scratch[4], ctx->scratch[11], ctx->scratch[12], &ok,
+                     &oi));
+    SD_TRY(scatter_lists(ctx, ok, oi, ov.as<int32_t>(), nover, L, mk.as<double>(),
+                         mi.as<int32_t>()));
+  }
+  *out_key = mk.as<double>();
+  *out_idx = mi.as<int32_t>();
+  return SD_OK;
+}
+
 int run_knn(sd_context* ctx, sd_matrix* X, sd_matrix* Y, const sd_knn_params* p, const KnnPlan& plan,
             int32_t* h_idx, double* h_val) {
   sd_matrix* ref = plan.selfmode ? X : Y;
@@ -256,28 +352,10 @@ int run_knn(sd_context* ctx, sd_matrix* X, sd_matrix* Y, const sd_knn_params* p,
     fa.self_mode = plan.selfmode;
     fa.include_self = p->include_self;
     fa.L = fused_list_len(method == SD_BINARY ? k : k + margin);
-    const int64_t qtiles = (nq + 255) / 256;
-    int64_t splits = (2 * (int64_t)ctx->cu_count + qtiles - 1) / qtiles;
-    splits = round_up(std::max<int64_t>(splits, 1), 8);
-    fa.splits = (int)std::min<int64_t>(splits, 4096);
-    const int nlists = fused_effective_splits(fa);
     const int L = fa.L;
-    DevBuf& ck = ctx->scratch[3];
-    DevBuf& ci = ctx->scratch[4];
-    SD_TRY(ck.ensure(sizeof(double) * (size_t)nlists * (size_t)nq * (size_t)L + 64));
-    SD_TRY(ci.ensure(sizeof(int32_t) * (size_t)nlists * (size_t)nq * (size_t)L + 64));
-    SD_TRY(fused_slab_topl(ctx, fa, ck.as<double>(), ci.as<int32_t>()));
-    const double* lkey = ck.as<double>();
-    const int32_t* lidx = ci.as<int32_t>();
-    DevBuf& mk = ctx->scratch[5];
-    DevBuf& mi = ctx->scratch[6];
-    if (nlists > 1) {
-      SD_TRY(mk.ensure(sizeof(double) * (size_t)nq * (size_t)L + 64));
-      SD_TRY(mi.ensure(sizeof(int32_t) * (size_t)nq * (size_t)L + 64));
-      SD_TRY(merge_lists(ctx, lkey, lidx, nlists, nq, L, mk.as<double>(), mi.as<int32_t>()));
-      lkey = mk.as<double>();
-      lidx = mi.as<int32_t>();
-    }
+    const double* lkey = nullptr;
+    const int32_t* lidx = nullptr;
+    SD_TRY(fused_candidate_lists(ctx, fa, &lkey, &lidx));
     if (method == SD_BINARY) {
       SD_TRY(finish_exact_lists(ctx, lkey, lidx, nq, L, k, p->dist,
                                 p->drop_degenerate ? X->deg.as<uint8_t>() : nullptr,

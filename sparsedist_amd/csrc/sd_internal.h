@@ -81,7 +81,7 @@ struct sd_context {
   hipEvent_t ev_a = nullptr, ev_b = nullptr;
   int cu_count = 256;
   // scratch reused across calls (grow-only)
-  sd::DevBuf scratch[16];
+  sd::DevBuf scratch[24];
   std::vector<uint8_t> host_scratch;
 };
 
@@ -173,6 +173,7 @@ struct FusedArgs {
   int self_mode = 0, include_self = 0;
   int L = 0;                        // list length kept per query
   int splits = 1;
+  int tile_stride = 1;              // list mode: only every tile_stride-th reference tile
 };
 // Candidate lists: per (split, query) L entries of (key double, idx int32),
 // key larger = better, sorted best first; unfilled slots idx = -1.
@@ -198,6 +199,17 @@ struct RerankArgs {
 };
 int rerank_map_words(int64_t nrow);
 int fused_list_len(int want);
+// Filter mode: every (query tile, reference tile) once; candidates with key
+// >= thr[query] are appended to fkey/fidx ([nqry][cap]), counted in cnt.
+int fused_filter_pass(sd_context* ctx, const FusedArgs& a, const double* thr, int cap,
+                      int32_t* cnt, double* fkey, int32_t* fidx);
+int select_lists(sd_context* ctx, const double* fkey, const int32_t* fidx, const int32_t* cnt,
+                 int cap, int64_t nqry, int L, double* out_key, int32_t* out_idx,
+                 int32_t* over_list, int32_t* over_count);
+int list_thresholds(sd_context* ctx, const double* key, const int32_t* idx, int64_t nqry, int L,
+                    double* thr);
+int scatter_lists(sd_context* ctx, const double* key, const int32_t* idx, const int32_t* rows,
+                  int64_t n, int L, double* out_key, int32_t* out_idx);
 int fused_effective_splits(const FusedArgs& a);
 int merge_lists(sd_context* ctx, const double* key, const int32_t* idx, int nlists, int64_t nqry,
                 int L, double* out_key, int32_t* out_idx);

@@ -271,3 +271,23 @@ def test_certificate_escalation_and_fallback(oracle_lib, gpu_ctx):
     o = oracle_lib.knn(Xd, k=10, method="cosine", nthreads=NT)
     same((g.idx, g.dist), o, "near duplicates")
     assert st["fallback_queries"] > 0
+
+
+@pytest.mark.parametrize("method", ["binary", "cosine"])
+def test_two_pass_filter_path(oracle_lib, gpu_ctx, method):
+    """Enough reference columns (>= 64 tiles of 128) for the threshold
+    pre-pass + filtered full pass; queries are a subset so the oracle stays
+    fast, and a heavily tied block exercises the buffer-overflow path."""
+    Y = synth.nb_log1p(600, 9000, 0.05, 41)
+    Yd = Y.tolil()
+    # 2200 copies of column 0: its candidate buffer overflows
+    for c in range(3000, 5200):
+        Yd[:, c] = Y[:, 0]
+    Y = Yd.tocsc()
+    X = Y[:, :300]
+    gpu_ctx.stats(reset=True)
+    g = sd.sparseKNN(X, Y, k=20, method=method, verbose=False)
+    st = gpu_ctx.stats(reset=True)
+    o = oracle_lib.knn(X, Y, k=20, method=method, nthreads=NT)
+    same((g.idx, g.dist), o, method)
+    assert st["overflow_queries"] >= 1
