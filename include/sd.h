@@ -87,10 +87,13 @@ int sd_context_destroy(sd_context* ctx);
  * SD_ECANCELLED and writes no partial output.  Callable from any thread. */
 int sd_cancel(sd_context* ctx);
 
-/* Kernel statistics since the last reset: total device time in ms of the
- * fused slab kernel and its launch count, measured with HIP events on the
- * context's stream; how many query columns the certificate cleared, how
- * many needed the wider second pass, and how many went to the exact slab. */
+/* Kernel statistics since the last reset, measured with HIP events on the
+ * context's stream while timing is on: device time in ms and launch count of
+ * the fused slab kernel in list mode (threshold pre-pass, small searches and
+ * the wider second pass), of its filtered full pass over every reference
+ * tile, of the FP64 re-rank and of the SNN builder; how many query columns
+ * the certificate cleared, how many needed the wider second pass, how many
+ * went to the exact slab and how many overflowed the candidate buffer. */
 typedef struct {
   double slab_ms;
   int64_t slab_launches;
@@ -102,6 +105,8 @@ typedef struct {
   int64_t certified_queries;  /* cleared by the certificate (first or second pass) */
   int64_t escalated_queries;  /* needed the second, wider candidate pass */
   int64_t overflow_queries;   /* candidate buffer overflowed: full per-query pass */
+  double filter_ms;           /* filtered full pass (not included in slab_ms) */
+  int64_t filter_launches;
 } sd_stats;
 int sd_context_stats(sd_context* ctx, sd_stats* out, int reset);
 /* Enable/disable event timing of kernels (off by default). */

@@ -48,7 +48,8 @@ class Stats(C.Structure):
                 ("rerank_ms", C.c_double), ("rerank_launches", C.c_int64),
                 ("snn_ms", C.c_double), ("snn_launches", C.c_int64),
                 ("fallback_queries", C.c_int64), ("certified_queries", C.c_int64),
-                ("escalated_queries", C.c_int64), ("overflow_queries", C.c_int64)]
+                ("escalated_queries", C.c_int64), ("overflow_queries", C.c_int64),
+                ("filter_ms", C.c_double), ("filter_launches", C.c_int64)]
 
 
 _lib = None

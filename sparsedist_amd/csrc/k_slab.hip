@@ -27,6 +27,9 @@ namespace {
 
 constexpr int kTileRefs = 128;  // reference columns per wave row
 constexpr int kRowBytes = 128;  // K bytes per row per chunk
+// Word 3 of a raw buffer resource on gfx9-family parts: 32-bit data format,
+// no swizzle, range checking against num_records in bytes.
+constexpr int kRsrcWord3 = 0x00020000;
 
 // ---- MFMA operand types ------------------------------------------------------
 // mma() consumes one 16-byte K segment of each operand per lane.  Within one
@@ -151,7 +154,20 @@ __device__ inline bool map_tile(int64_t b, int64_t nblk, int qtiles, int rtiles,
 // Workgroup barrier for LDS only: wait for this wave's LDS operations, then
 // s_barrier.  Unlike __syncthreads() it does not wait for outstanding global
 // loads, which is what lets direct-to-LDS loads stay in flight across it.
-__device__ inline void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+// The builtin wait comes first so that the compiler's own wait accounting
+// knows that every LDS and scalar load has completed here.
+__device__ inline void lds_barrier() {
+  __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0), other counters untouched
+  asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
+// s_waitcnt vmcnt(N) with the other counters left alone (gfx9 encoding:
+// vmcnt bits 3:0 and 15:14, expcnt 6:4, lgkmcnt 11:8).
+template <int N>
+__device__ inline void wait_vm() {
+  static_assert(N >= 0 && N < 64, "vmcnt is six bits");
+  __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | 0x0F70);
+}
 
 template <class Op, class K, int L, int WM, bool FILTER>
 __global__ __launch_bounds__(WM * 256, 1) void k_fused(FusedKernelArgs P) {
@@ -230,14 +246,24 @@ __global__ __launch_bounds__(WM * 256, 1) void k_fused(FusedKernelArgs P) {
   constexpr int A_INST = A_BYTES / 1024 / WAVES;  // 4
   constexpr int B_INST = B_BYTES / 1024 / WAVES;  // 8
   auto seg_of = [&](int i) { return (lane & 7) ^ ((4 * i + (lane >> 4)) & 7); };
-  const uint8_t* pb[B_INST];
+  // The loads go through buffer resources (buffer_load_dwordx4 ... lds) rather
+  // than global_load_lds: the compiler treats an LDS-writing FLAT load as
+  // pending on both counters and would then round every LDS wait in the loop
+  // down to lgkmcnt(0), serialising the fragment reads with the MFMAs.  A
+  // resource covers one tile (at most 256 rows x kpad_bytes < 2^31 bytes, which
+  // the host checks); lane offsets are 32-bit and rows past the end are
+  // clamped to the last row (they feed only masked or unused lanes).  Rows
+  // are addressed by query position: the host gathers listed queries first.
+  const int64_t b_rows = (P.nqry - q_base < TN) ? P.nqry - q_base : TN;
+  const __amdgpu_buffer_rsrc_t rb = __builtin_amdgcn_make_buffer_rsrc(
+      (void*)(P.qry_dense + q_base * P.kpad_bytes), 0, (int)(b_rows * P.kpad_bytes), kRsrcWord3);
+  uint32_t vb[B_INST];
 #pragma unroll
   for (int u = 0; u < B_INST; ++u) {
     const int i = wave * B_INST + u;
-    int64_t q = q_base + 8 * i + (lane >> 3);
-    if (q >= P.nqry) q = P.nqry - 1;  // clamped rows feed only unused lanes
-    const int64_t gq = P.qlist ? (int64_t)P.qlist[q] : q;
-    pb[u] = P.qry_dense + gq * P.kpad_bytes + seg_of(i) * 16;
+    int row = 8 * i + (lane >> 3);
+    if (row >= b_rows) row = (int)b_rows - 1;
+    vb[u] = (uint32_t)(row * P.kpad_bytes) + seg_of(i) * 16;
   }
   typedef __attribute__((address_space(3))) void lds_void;
 
@@ -265,70 +291,126 @@ __global__ __launch_bounds__(WM * 256, 1) void k_fused(FusedKernelArgs P) {
 #pragma unroll
         for (int r = 0; r < 16; ++r) acc[b][j][r] = 0;
 
-    const uint8_t* pa[A_INST];
+    const int64_t a_rows = (P.nref - r_base < TM) ? P.nref - r_base : TM;
+    const __amdgpu_buffer_rsrc_t ra = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)(P.ref_dense + r_base * P.kpad_bytes), 0, (int)(a_rows * P.kpad_bytes), kRsrcWord3);
+    uint32_t va[A_INST];
 #pragma unroll
     for (int u = 0; u < A_INST; ++u) {
       const int i = wave * A_INST + u;
-      int64_t r = r_base + 8 * i + (lane >> 3);
-      if (r >= P.nref) r = P.nref - 1;  // clamped rows are masked by s_rdeg
-      pa[u] = P.ref_dense + r * P.kpad_bytes + seg_of(i) * 16;
+      int row = 8 * i + (lane >> 3);
+      if (row >= a_rows) row = (int)a_rows - 1;  // clamped rows are masked by s_rdeg
+      va[u] = (uint32_t)(row * P.kpad_bytes) + seg_of(i) * 16;
     }
     auto issue = [&](uint8_t* st, int64_t c) {
+      const int soff = (int)(c * kRowBytes);
 #pragma unroll
       for (int u = 0; u < A_INST; ++u)
-        __builtin_amdgcn_global_load_lds((const void*)(pa[u] + c * kRowBytes),
-                                         (lds_void*)(st + (wave * A_INST + u) * 1024), 16, 0, 0);
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(
+            ra, (lds_void*)(st + (wave * A_INST + u) * 1024), 16, (int)va[u], soff, 0, 0);
 #pragma unroll
       for (int u = 0; u < B_INST; ++u)
-        __builtin_amdgcn_global_load_lds((const void*)(pb[u] + c * kRowBytes),
-                                         (lds_void*)(st + A_BYTES + (wave * B_INST + u) * 1024),
-                                         16, 0, 0);
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(
+            rb, (lds_void*)(st + A_BYTES + (wave * B_INST + u) * 1024), 16, (int)vb[u], soff, 0, 0);
     };
     const int swz = (n >> 1) & 7;
-    auto compute = [&](const uint8_t* st) {
-      // MFMA step t: lanes with h supply logical segment 2t + h
-#pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        const int slot = ((2 * t + h) ^ swz) * 16;
-        uint4 fa[4], fb[2];
-#pragma unroll
-        for (int b = 0; b < 4; ++b)
-          fa[b] = *reinterpret_cast<const uint4*>(st + (b * 32 + n) * kRowBytes + slot);
-#pragma unroll
-        for (int j = 0; j < 2; ++j)
-          fb[j] = *reinterpret_cast<const uint4*>(st + A_BYTES + (wn * 64 + j * 32 + n) * kRowBytes +
-                                                  slot);
-#pragma unroll
-        for (int b = 0; b < 4; ++b)
-#pragma unroll
-          for (int j = 0; j < 2; ++j) Op::mma(acc[b][j], fa[b], fb[j]);
-      }
+    // Operand fragments of one MFMA step (16 K bytes per lane): lanes with h
+    // supply logical segment 2t + h.  Two register sets alternate so that the
+    // LDS reads of step t + 1 are in flight while step t's MFMAs run; with one
+    // wave per SIMD nothing else hides their latency.
+    struct Frag {
+      uint4 a[4], b[2];
     };
+    auto load_frag = [&](Frag& f, const uint8_t* st, int t) {
+      const int slot = ((2 * t + h) ^ swz) * 16;
+#pragma unroll
+      for (int b = 0; b < 4; ++b)
+        f.a[b] = *reinterpret_cast<const uint4*>(st + (b * 32 + n) * kRowBytes + slot);
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+        f.b[j] = *reinterpret_cast<const uint4*>(st + A_BYTES + (wn * 64 + j * 32 + n) * kRowBytes +
+                                                 slot);
+    };
+    auto mma_frag = [&](const Frag& f) {
+#pragma unroll
+      for (int b = 0; b < 4; ++b)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) Op::mma(acc[b][j], f.a[b], f.b[j]);
+    };
+    Frag f0, f1;
 
     lds_barrier();  // the previous tile's epilogue is done with the stages
-    if (nchunks > 0) issue(st0, 0);
-    if (nchunks > 1) issue(st1, 1);
-    // One step: wait until this wave's loads of chunk c have landed (loads
-    // retire in order, so (A_INST + B_INST) loads of chunk c + 1 may still be
-    // outstanding), make every wave's part visible, refill the stage freed by
-    // the previous step with chunk c + 2, compute chunk c.
-#define SD_FUSED_STEP(CUR, NXT)                                               \
-    if (c < nchunks) {                                                        \
-      if (c + 1 < nchunks)                                                    \
-        __builtin_amdgcn_s_waitcnt(0xF70 | (A_INST + B_INST));                \
-      else                                                                    \
-        __builtin_amdgcn_s_waitcnt(0xF70);                                    \
+    // kpad is a positive multiple of 128 rows, so there is at least one
+    // chunk.  Refills past the last chunk reload the last chunk into a stage
+    // nobody reads again: every step then issues the same loads and waits for
+    // the same count, and no LDS read sits behind a branch (at a join the
+    // compiler would wait for all loads in flight before reading LDS).
+    const int64_t last = nchunks - 1;
+    issue(st0, 0);
+    issue(st1, last < 1 ? last : 1);
+    issue(st2, last < 2 ? last : 2);
+    wait_vm<2 * (A_INST + B_INST)>();
+    lds_barrier();
+    load_frag(f0, st0, 0);
+    // One chunk c in stage CUR (step 0's fragments already requested): steps
+    // 0-2 alternate fragment sets; before step 3's MFMAs, wait until this
+    // wave's loads of chunk c + 1 have landed (loads retire in order, so the
+    // loads of chunk c + 2 may still be outstanding), barrier -- after which
+    // no wave reads CUR any more and every wave's part of chunk c + 1 is
+    // visible -- refill CUR with chunk c + 3 and request chunk c + 1's first
+    // fragments, all while step 3's MFMAs keep the matrix core busy.
+    // Scheduling barriers keep the compiler from sinking the reads next to
+    // the MFMAs that consume them.
+#define SD_STEPS_0_2(CUR)                                                     \
+    load_frag(f1, CUR, 1);                                                    \
+    __builtin_amdgcn_sched_barrier(0);                                        \
+    mma_frag(f0);                                                             \
+    __builtin_amdgcn_sched_barrier(0);                                        \
+    load_frag(f0, CUR, 2);                                                    \
+    __builtin_amdgcn_sched_barrier(0);                                        \
+    mma_frag(f1);                                                             \
+    __builtin_amdgcn_sched_barrier(0);                                        \
+    load_frag(f1, CUR, 3);                                                    \
+    __builtin_amdgcn_sched_barrier(0);                                        \
+    mma_frag(f0);                                                             \
+    __builtin_amdgcn_sched_barrier(0);
+#define SD_CHUNK(CUR, NXT, C)                                                 \
+    {                                                                         \
+      SD_STEPS_0_2(CUR)                                                       \
+      wait_vm<A_INST + B_INST>();                                             \
       lds_barrier();                                                          \
-      if (c + 2 < nchunks) issue(NXT, c + 2);                                 \
-      compute(CUR);                                                           \
-      ++c;                                                                    \
+      issue(CUR, (C) + 3 < last ? (C) + 3 : last);                            \
+      load_frag(f0, NXT, 0);                                                  \
+      __builtin_amdgcn_sched_barrier(0);                                      \
+      mma_frag(f1);                                                           \
+      __builtin_amdgcn_sched_barrier(0);                                      \
     }
-    for (int64_t c = 0; c < nchunks;) {
-      SD_FUSED_STEP(st0, st2)
-      SD_FUSED_STEP(st1, st0)
-      SD_FUSED_STEP(st2, st1)
+#define SD_LAST_CHUNK(CUR)                                                    \
+    {                                                                         \
+      SD_STEPS_0_2(CUR)                                                       \
+      mma_frag(f1);                                                           \
     }
-#undef SD_FUSED_STEP
+    int64_t c = 0;
+    for (; c + 3 < nchunks; c += 3) {
+      SD_CHUNK(st0, st1, c)
+      SD_CHUNK(st1, st2, c + 1)
+      SD_CHUNK(st2, st0, c + 2)
+    }
+    // the last one to three chunks (chunk c sits in st0)
+    if (nchunks - c == 3) {
+      SD_CHUNK(st0, st1, c)
+      SD_CHUNK(st1, st2, c + 1)
+      SD_LAST_CHUNK(st2)
+    } else if (nchunks - c == 2) {
+      SD_CHUNK(st0, st1, c)
+      SD_LAST_CHUNK(st1)
+    } else {
+      SD_LAST_CHUNK(st0)
+    }
+#undef SD_LAST_CHUNK
+#undef SD_CHUNK
+#undef SD_STEPS_0_2
+    wait_vm<0>();  // the trailing refills land before stage 0 is reused
     lds_barrier();  // every wave is done reading the stages
 
     // ---- epilogue: this lane owns query my_q and must see its 128 values.
@@ -733,6 +815,43 @@ static int launch_fused_L(sd_context* ctx, FusedKernelArgs& P, int L) {
   }
 }
 
+// Copy the dense rows of listed query columns into a position-indexed block,
+// so the slab kernel's 32-bit buffer offsets stay within one compact tile.
+__global__ void k_gather_rows(const uint4* __restrict__ src, const int32_t* __restrict__ qlist,
+                              int64_t n, int64_t row_vecs, uint4* __restrict__ dst) {
+  const int64_t row = blockIdx.y + (int64_t)blockIdx.z * 65535;
+  if (row >= n) return;
+  const uint4* s = src + (int64_t)qlist[row] * row_vecs;
+  uint4* d = dst + row * row_vecs;
+  for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < row_vecs;
+       v += (int64_t)gridDim.x * blockDim.x)
+    d[v] = s[v];
+}
+
+// Gathers a.qlist's query rows when a list is given; returns the rows to read.
+static int query_rows(sd_context* ctx, const FusedArgs& a, int64_t kpad_bytes,
+                      const uint8_t** rows) {
+  *rows = static_cast<const uint8_t*>(a.qry_dense);
+  if (!a.qlist || a.nqry <= 0) return SD_OK;
+  DevBuf& g = ctx->scratch[16];
+  SD_TRY(g.ensure((size_t)a.nqry * (size_t)kpad_bytes + 64));
+  const int64_t vecs = kpad_bytes / 16;
+  const unsigned gx = (unsigned)std::min<int64_t>((vecs + 255) / 256, 64);
+  const unsigned gy = (unsigned)std::min<int64_t>(a.nqry, 65535);
+  const unsigned gz = (unsigned)((a.nqry + 65534) / 65535);
+  hipLaunchKernelGGL(k_gather_rows, dim3(gx, gy, gz), dim3(256), 0, ctx->stream,
+                     static_cast<const uint4*>(a.qry_dense), a.qlist, a.nqry, vecs,
+                     g.as<uint4>());
+  SD_HIP(hipGetLastError());
+  *rows = g.as<uint8_t>();
+  return SD_OK;
+}
+
+bool fused_fits(int64_t kpad, int elem_bytes) {
+  // one 256-row tile must stay below 2^31 bytes of buffer offset
+  return kpad > 0 && kpad * elem_bytes * 256 < ((int64_t)1 << 31);
+}
+
 int fused_list_len(int want) {
   static const int sizes[] = {8, 16, 24, 32, 40, 48, 64};
   for (int s : sizes)
@@ -743,7 +862,6 @@ int fused_list_len(int want) {
 int fused_slab_topl(sd_context* ctx, const FusedArgs& a, double* cand_key, int32_t* cand_idx) {
   FusedKernelArgs P{};
   P.ref_dense = static_cast<const uint8_t*>(a.ref_dense);
-  P.qry_dense = static_cast<const uint8_t*>(a.qry_dense);
   P.qlist = a.qlist;
   P.nref = a.nref;
   P.nqry = a.nqry;
@@ -751,6 +869,10 @@ int fused_slab_topl(sd_context* ctx, const FusedArgs& a, double* cand_key, int32
                    : (a.prec == SD_PREC_FP32)                         ? 4
                                                                       : 2;
   P.kpad_bytes = a.kpad * elem;
+  if (!fused_fits(a.kpad, elem))
+    return set_error(SD_EINVAL, "internal: %lld padded rows exceed the slab kernel's tile range",
+                     (long long)a.kpad);
+  SD_TRY(query_rows(ctx, a, P.kpad_bytes, &P.qry_dense));
   P.ref_p = a.ref_p;
   P.qry_p = a.qry_p;
   P.ref_aux = a.ref_aux;
@@ -792,7 +914,6 @@ int fused_filter_pass(sd_context* ctx, const FusedArgs& a, const double* thr, in
                       int32_t* cnt, double* fkey, int32_t* fidx) {
   FusedKernelArgs P{};
   P.ref_dense = static_cast<const uint8_t*>(a.ref_dense);
-  P.qry_dense = static_cast<const uint8_t*>(a.qry_dense);
   P.qlist = a.qlist;
   P.nref = a.nref;
   P.nqry = a.nqry;
@@ -800,6 +921,10 @@ int fused_filter_pass(sd_context* ctx, const FusedArgs& a, const double* thr, in
                    : (a.prec == SD_PREC_FP32)                         ? 4
                                                                       : 2;
   P.kpad_bytes = a.kpad * elem;
+  if (!fused_fits(a.kpad, elem))
+    return set_error(SD_EINVAL, "internal: %lld padded rows exceed the slab kernel's tile range",
+                     (long long)a.kpad);
+  SD_TRY(query_rows(ctx, a, P.kpad_bytes, &P.qry_dense));
   P.ref_p = a.ref_p;
   P.qry_p = a.qry_p;
   P.ref_aux = a.ref_aux;
@@ -817,7 +942,7 @@ int fused_filter_pass(sd_context* ctx, const FusedArgs& a, const double* thr, in
   P.fkey = fkey;
   P.fidx = fidx;
   SD_HIP(hipMemsetAsync(cnt, 0, sizeof(int32_t) * (size_t)a.nqry, ctx->stream));
-  KernelTimer timer(ctx, &ctx->stats.slab_ms, &ctx->stats.slab_launches);
+  KernelTimer timer(ctx, &ctx->stats.filter_ms, &ctx->stats.filter_launches);
   if (a.method == SD_BINARY) {
     if (a.prec == SD_PREC_FP32) return launch_filter_t<OpF32, KeyJ>(ctx, P);
     return launch_filter_t<OpI8, KeyJ>(ctx, P);

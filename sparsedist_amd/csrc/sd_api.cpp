@@ -317,10 +317,13 @@ int run_knn(sd_context* ctx, sd_matrix* X, sd_matrix* Y, const sd_knn_params* p,
 
   int prec = p->precision;
   int margin = p->margin > 0 ? p->margin : 16;
+  const int64_t kpad_rows = round_up(std::max<int64_t>(ref->nrow, 1), 128);
   const bool fused_ok_binary = (method == SD_BINARY) && prec != SD_PREC_EXACT &&
-                               fused_list_len(k) > 0 && ref->nrow < (1 << 25);
+                               fused_list_len(k) > 0 &&
+                               fused_fits(kpad_rows, prec == SD_PREC_FP32 ? 4 : 1);
   const bool fused_ok_cosine = (method == SD_COSINE) && prec != SD_PREC_EXACT &&
-                               fused_list_len(k + margin) > 0 && nref >= 1;
+                               fused_list_len(k + margin) > 0 && nref >= 1 &&
+                               fused_fits(kpad_rows, prec == SD_PREC_FP32 ? 4 : 2);
   ExactArgs ea = exact_args(ref, X, method, p->dist);
   ea.ref_deg = ref_deg;
   ea.self_mode = plan.selfmode;

@@ -199,6 +199,9 @@ struct RerankArgs {
 };
 int rerank_map_words(int64_t nrow);
 int fused_list_len(int want);
+// Whether a slab tile of 256 rows of kpad elements of elem_bytes stays within
+// the 32-bit buffer offsets of the slab kernel.
+bool fused_fits(int64_t kpad, int elem_bytes);
 // Filter mode: every (query tile, reference tile) once; candidates with key
 // >= thr[query] are appended to fkey/fidx ([nqry][cap]), counted in cnt.
 int fused_filter_pass(sd_context* ctx, const FusedArgs& a, const double* thr, int cap,
