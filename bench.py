@@ -111,11 +111,11 @@ def main():
     cos_ms = kern("cosine")
     roof_bin = dict(bound="mfma", achieved=flop / (bin_ms / 1e3) / 1e12, peak=5000.0,
                     unit="TOPS", frac=(flop / (bin_ms / 1e3)) / MI355X_PEAK["int8"],
-                    traffic=None, kernel="k_fused<OpI8, KeyJ, 8, 1, true> (int8 Jaccard filter pass)",
+                    traffic=None, kernel="k_filter_wide<OpI8, KeyJ> (int8 Jaccard filtered pass)",
                     avg_launch_ms=bin_ms)
     roof_cos = dict(bound="mfma", achieved=flop / (cos_ms / 1e3) / 1e12, peak=2500.0,
                     unit="TFLOP/s", frac=(flop / (cos_ms / 1e3)) / MI355X_PEAK["bf16"],
-                    traffic=None, kernel="k_fused<OpBf16, float, 8, 1, true> (BF16 cosine filter pass)",
+                    traffic=None, kernel="k_filter_wide<OpBf16, float> (BF16 cosine filtered pass)",
                     avg_launch_ms=cos_ms)
     dominant = roof_bin if bin_ms >= cos_ms else roof_cos
     edges = 2 * nq * k

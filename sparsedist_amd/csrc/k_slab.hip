@@ -439,6 +439,7 @@ __global__ __launch_bounds__(WM * 256, 1) void k_fused(FusedKernelArgs P) {
       lds_barrier();
       const int hh = (phase == 0) ? h : 1 - h;
       if (q_ok) {
+#pragma unroll 1
         for (int v = 0; v < 32; ++v) {
           const int b = b0 + (v >> 4), rr = v & 15;
           const int rloc = rloc_base + b * 32 + 8 * (rr >> 2) + 4 * hh + (rr & 3);
@@ -507,6 +508,271 @@ __global__ __launch_bounds__(WM * 256, 1) void k_fused(FusedKernelArgs P) {
       P.cand_key[slot * L + t] = valid ? key_to_double(list[t].key) : 0.0;
       P.cand_idx[slot * L + t] = valid ? (int32_t)list[t].idx : -1;
     }
+  }
+}
+
+// ---- filtered full pass: 256 x 256 tiles, eight waves ------------------------
+// One workgroup per (reference tile of 256, query tile of 256); every value
+// that reaches its query's threshold is appended to that query's candidate
+// buffer.  Wave (wm, wn) computes references 128 wm .. +128 x queries
+// 64 wn .. +64, as in k_fused, but a workgroup of eight waves (two per SIMD)
+// shares each staged byte among 1.5x the MFMA work of a 128 x 256 tile, and
+// the second wave on each SIMD covers the first one's barrier and LDS waits.
+// K streams through four LDS stages of 64-byte row chunks (A and B 16 KB
+// each); a stage is refilled with the chunk four ahead as soon as every wave
+// has read it, so up to three chunks (96 KB) are in flight per CU.
+constexpr int kWideTile = 256;     // references and queries per workgroup
+constexpr int kWideRowBytes = 64;  // K bytes per row per chunk
+
+template <class Op, class K>
+__global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
+  constexpr int NT = 512;
+  constexpr int TM = kWideTile, TN = kWideTile, RB = kWideRowBytes;
+  constexpr int A_BYTES = TM * RB;  // 16 KB
+  constexpr int B_BYTES = TN * RB;  // 16 KB
+  constexpr int STAGE_BYTES = A_BYTES + B_BYTES;
+  // Each wave instruction writes 1 KB = 16 rows x 64 B.  The 16-byte slot of
+  // logical segment s in row R is s ^ ((R >> 2) & 3): 16 lanes reading one
+  // segment of 16 consecutive rows then cover all 64 banks once.
+  constexpr int A_INST = A_BYTES / 1024 / 8;  // 2
+  constexpr int B_INST = B_BYTES / 1024 / 8;  // 2
+  constexpr int INST = A_INST + B_INST;
+  // Exchange rows of 32 words per lane, column v stored at v ^ (row >> 1):
+  // the 64 rows of a wave fall on 64 distinct banks for every v.
+  static_assert((NT / 2) * 32 * 4 <= STAGE_BYTES, "exchange area must fit in one stage");
+  __shared__ __align__(16) uint8_t st0[STAGE_BYTES];
+  __shared__ __align__(16) uint8_t st1[STAGE_BYTES];
+  __shared__ __align__(16) uint8_t st2[STAGE_BYTES];
+  __shared__ __align__(16) uint8_t st3[STAGE_BYTES];
+  __shared__ int32_t s_rnnz[TM];
+  __shared__ float s_raux[TM];
+  __shared__ uint8_t s_rdeg[TM];
+
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6, lane = tid & 63;
+  const int wm = wave >> 2, wn = wave & 3;
+  const int h = lane >> 5, n = lane & 31;
+  int qt;
+  int64_t rt;
+  if (!map_tile(blockIdx.x, gridDim.x, P.qtiles, P.rtiles, qt, rt)) return;
+  const int64_t q_base = (int64_t)qt * TN;
+  const int64_t r_base = rt * TM;
+  const int64_t my_pos = q_base + wn * 64 + h * 32 + n;
+  const bool q_ok = my_pos < P.nqry;
+  const int64_t my_q = q_ok ? (P.qlist ? (int64_t)P.qlist[my_pos] : my_pos) : -1;
+  uint32_t nj = 0;
+  float qaux = 0.f;
+  double thr_q = 0.0;
+  if (q_ok) {
+    nj = (uint32_t)(P.qry_p[my_q + 1] - P.qry_p[my_q]);
+    if (P.qry_aux) qaux = P.qry_aux[my_q];
+    thr_q = P.thr[my_pos];
+  }
+  for (int t = tid; t < TM; t += NT) {
+    const int64_t r = r_base + t;
+    if (r < P.nref) {
+      s_rnnz[t] = (int32_t)(P.ref_p[r + 1] - P.ref_p[r]);
+      s_raux[t] = P.ref_aux ? P.ref_aux[r] : 0.f;
+      s_rdeg[t] = P.ref_deg ? P.ref_deg[r] : 0;
+    } else {
+      s_rnnz[t] = 0;
+      s_raux[t] = 0.f;
+      s_rdeg[t] = 1;
+    }
+  }
+
+  // Buffer resources over the two tiles (rows past the end are clamped to the
+  // last row and feed only masked or unused lanes); lane l of a load carries
+  // row 16i + l/4 and physical slot l%4, i.e. logical segment
+  // (l%4) ^ (((16i + l/4) >> 2) & 3) = (l%4) ^ ((l/16) & 3).
+  const int seg = (lane & 3) ^ ((lane >> 4) & 3);
+  const int64_t a_rows = (P.nref - r_base < TM) ? P.nref - r_base : TM;
+  const int64_t b_rows = (P.nqry - q_base < TN) ? P.nqry - q_base : TN;
+  const __amdgpu_buffer_rsrc_t ra = __builtin_amdgcn_make_buffer_rsrc(
+      (void*)(P.ref_dense + r_base * P.kpad_bytes), 0, (int)(a_rows * P.kpad_bytes), kRsrcWord3);
+  const __amdgpu_buffer_rsrc_t rb = __builtin_amdgcn_make_buffer_rsrc(
+      (void*)(P.qry_dense + q_base * P.kpad_bytes), 0, (int)(b_rows * P.kpad_bytes), kRsrcWord3);
+  uint32_t va[A_INST], vb[B_INST];
+#pragma unroll
+  for (int u = 0; u < A_INST; ++u) {
+    int row = 16 * (wave * A_INST + u) + (lane >> 2);
+    if (row >= a_rows) row = (int)a_rows - 1;
+    va[u] = (uint32_t)(row * P.kpad_bytes) + seg * 16;
+  }
+#pragma unroll
+  for (int u = 0; u < B_INST; ++u) {
+    int row = 16 * (wave * B_INST + u) + (lane >> 2);
+    if (row >= b_rows) row = (int)b_rows - 1;
+    vb[u] = (uint32_t)(row * P.kpad_bytes) + seg * 16;
+  }
+  typedef __attribute__((address_space(3))) void lds_void;
+  auto issue = [&](uint8_t* st, int64_t c) {
+    const int soff = (int)(c * RB);
+#pragma unroll
+    for (int u = 0; u < A_INST; ++u)
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(
+          ra, (lds_void*)(st + (wave * A_INST + u) * 1024), 16, (int)va[u], soff, 0, 0);
+#pragma unroll
+    for (int u = 0; u < B_INST; ++u)
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(
+          rb, (lds_void*)(st + A_BYTES + (wave * B_INST + u) * 1024), 16, (int)vb[u], soff, 0, 0);
+  };
+
+  typename Op::Acc acc[4][2];
+#pragma unroll
+  for (int b = 0; b < 4; ++b)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[b][j][r] = 0;
+
+  // MFMA step t (of two per chunk): lanes with h supply logical segment 2t + h
+  // of their row; every A and B row read here has (row >> 2) & 3 == (n >> 2) & 3.
+  const int swz = (n >> 2) & 3;
+  struct Frag {
+    uint4 a[4], b[2];
+  };
+  auto load_frag = [&](Frag& f, const uint8_t* st, int t) {
+    const int slot = ((2 * t + h) ^ swz) * 16;
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+      f.a[b] = *reinterpret_cast<const uint4*>(st + (wm * 128 + b * 32 + n) * RB + slot);
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+      f.b[j] = *reinterpret_cast<const uint4*>(st + A_BYTES + (wn * 64 + j * 32 + n) * RB + slot);
+  };
+  auto mma_frag = [&](const Frag& f) {
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) Op::mma(acc[b][j], f.a[b], f.b[j]);
+  };
+  Frag f0, f1;
+
+  // kpad is a positive multiple of kDenseRowAlign (256) rows, so the chunks
+  // come in whole groups of four.  Refills past the last chunk reload the
+  // last chunk into a stage nobody reads again, so every step issues and
+  // waits alike and no LDS read sits behind a branch.
+  const int64_t nchunks = P.kpad_bytes / RB;
+  const int64_t last = nchunks - 1;
+  lds_barrier();  // s_rnnz and friends are visible
+  issue(st0, 0);
+  issue(st1, last < 1 ? last : 1);
+  issue(st2, last < 2 ? last : 2);
+  issue(st3, last < 3 ? last : 3);
+  wait_vm<3 * INST>();
+  lds_barrier();
+  load_frag(f0, st0, 0);
+  // Chunk c in stage CUR (its step-0 fragments already requested): request
+  // step 1, run step 0; wait for this wave's loads of chunk c + 1 (the loads
+  // of chunks c + 2 and c + 3 may still be in flight); barrier, after which
+  // every wave has read CUR and chunk c + 1 is visible; refill CUR with chunk
+  // c + 4 and request chunk c + 1's step 0 while step 1's MFMAs run.
+#define SD_WIDE_CHUNK(CUR, NXT, C)                                            \
+  {                                                                           \
+    load_frag(f1, CUR, 1);                                                    \
+    __builtin_amdgcn_sched_barrier(0);                                        \
+    mma_frag(f0);                                                             \
+    __builtin_amdgcn_sched_barrier(0);                                        \
+    wait_vm<2 * INST>();                                                      \
+    lds_barrier();                                                            \
+    issue(CUR, (C) + 4 < last ? (C) + 4 : last);                              \
+    load_frag(f0, NXT, 0);                                                    \
+    __builtin_amdgcn_sched_barrier(0);                                        \
+    mma_frag(f1);                                                             \
+    __builtin_amdgcn_sched_barrier(0);                                        \
+  }
+#define SD_WIDE_LAST(CUR)                                                     \
+  {                                                                           \
+    load_frag(f1, CUR, 1);                                                    \
+    __builtin_amdgcn_sched_barrier(0);                                        \
+    mma_frag(f0);                                                             \
+    mma_frag(f1);                                                             \
+  }
+  int64_t c = 0;
+  for (; c + 4 < nchunks; c += 4) {
+    SD_WIDE_CHUNK(st0, st1, c)
+    SD_WIDE_CHUNK(st1, st2, c + 1)
+    SD_WIDE_CHUNK(st2, st3, c + 2)
+    SD_WIDE_CHUNK(st3, st0, c + 3)
+  }
+  // the last group of four (chunk c sits in st0)
+  SD_WIDE_CHUNK(st0, st1, c)
+  SD_WIDE_CHUNK(st1, st2, c + 1)
+  SD_WIDE_CHUNK(st2, st3, c + 2)
+  SD_WIDE_LAST(st3)
+#undef SD_WIDE_LAST
+#undef SD_WIDE_CHUNK
+  wait_vm<0>();  // the trailing refills land before the stages are reused
+  lds_barrier();
+
+  // ---- epilogue: this lane owns query my_pos and must see its 128 values.
+  // acc[b][j][r] belongs to ref 128 wm + 32b + 8(r/4) + 4h + (r%4), query
+  // block j.  Sub-phases 0-1 scan the lane's own block-h values (reference
+  // blocks 0-1, then 2-3), sub-phases 2-3 the block-h values held by lane
+  // l ^ 32.  Wave row wm exchanges through stage wm.
+  uint32_t* xch = reinterpret_cast<uint32_t*>(wm ? st1 : st0);
+  const int lt = tid & 255;
+  auto xcol = [](int row, int v) { return row * 32 + ((v ^ (row >> 1)) & 31); };
+#pragma unroll
+  for (int sub = 0; sub < 4; ++sub) {
+    const int phase = sub >> 1, b0 = (sub & 1) * 2;
+    const int wrow = (phase == 0) ? lt : (lt ^ 32);
+#pragma unroll
+    for (int bb = 0; bb < 2; ++bb)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const auto v0 = acc[b0 + bb][0][r], v1 = acc[b0 + bb][1][r];
+        const auto x = (phase == 0) ? (h ? v1 : v0) : (h ? v0 : v1);
+        uint32_t bits;
+        if constexpr (sizeof(K) == 8)
+          bits = (uint32_t)x;  // exact integer count (int or float accumulator)
+        else
+          __builtin_memcpy(&bits, &x, 4);
+        xch[xcol(wrow, bb * 16 + r)] = bits;
+      }
+    lds_barrier();
+    const int hh = (phase == 0) ? h : 1 - h;
+    if (q_ok) {
+#pragma unroll 1
+      for (int v = 0; v < 32; ++v) {
+        const int b = b0 + (v >> 4), rr = v & 15;
+        const int rloc = wm * 128 + b * 32 + 8 * (rr >> 2) + 4 * hh + (rr & 3);
+        const int64_t rg = r_base + rloc;
+        if (s_rdeg[rloc] || (P.self_mode && rg == my_q && !P.include_self)) continue;
+        const uint32_t bits = xch[xcol(lt, v)];
+        double kd;
+        bool pass;
+        if constexpr (sizeof(K) == 8) {
+          const uint32_t cc = bits;
+          const uint32_t d = (uint32_t)s_rnnz[rloc] + nj - cc;
+          const KeyJ key = (d == 0) ? KeyJ{1u, 1u} : KeyJ{cc, d};
+          // cheap float screen (errors far below 2^-20), then the exact rational
+          const float f = (float)key.c / (float)key.d;
+          pass = f >= (float)thr_q * (1.0f - 9.5367431640625e-07f);
+          kd = 0.0;
+          if (pass) {
+            kd = key_to_double(key);
+            pass = kd >= thr_q;
+          }
+        } else {
+          float sv;
+          __builtin_memcpy(&sv, &bits, 4);
+          float key = (P.method == SD_EUCLIDEAN) ? -(qaux + s_raux[rloc] - 2.f * sv) : sv;
+          if (P.self_mode && rg == my_q) key = 2.0f;  // restored self similarity
+          kd = (double)key;
+          pass = kd >= thr_q;
+        }
+        if (pass) {
+          const int at = atomicAdd(&P.cnt[my_pos], 1);
+          if (at < P.cap) {
+            P.fkey[my_pos * P.cap + at] = kd;
+            P.fidx[my_pos * P.cap + at] = (int32_t)rg;
+          }
+        }
+      }
+    }
+    lds_barrier();
   }
 }
 
@@ -791,12 +1057,16 @@ static int launch_fused_t(sd_context* ctx, FusedKernelArgs& P) {
 
 template <class Op, class K>
 static int launch_filter_t(sd_context* ctx, FusedKernelArgs& P) {
+  if (P.kpad_bytes <= 0 || P.kpad_bytes % (4 * kWideRowBytes) != 0)
+    return set_error(SD_EINVAL, "internal: padded row bytes %lld are not whole chunk groups",
+                     (long long)P.kpad_bytes);
+  P.qtiles = (int)((P.nqry + kWideTile - 1) / kWideTile);
+  P.rtiles = (int)((P.nref + kWideTile - 1) / kWideTile);
   P.qgroups = (P.qtiles + 3) / 4;
   int64_t nblk = (int64_t)P.qgroups * 4 * P.rtiles;
   nblk = (nblk + 7) / 8 * 8;
   if (nblk >= ((int64_t)1 << 31)) return set_error(SD_EUNSUPPORTED, "slab grid too large");
-  hipLaunchKernelGGL((k_fused<Op, K, 8, 1, true>), dim3((unsigned)nblk), dim3(256), 0, ctx->stream,
-                     P);
+  hipLaunchKernelGGL((k_filter_wide<Op, K>), dim3((unsigned)nblk), dim3(512), 0, ctx->stream, P);
   SD_HIP(hipGetLastError());
   return SD_OK;
 }

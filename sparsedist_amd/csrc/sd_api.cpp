@@ -317,7 +317,7 @@ int run_knn(sd_context* ctx, sd_matrix* X, sd_matrix* Y, const sd_knn_params* p,
 
   int prec = p->precision;
   int margin = p->margin > 0 ? p->margin : 16;
-  const int64_t kpad_rows = round_up(std::max<int64_t>(ref->nrow, 1), 128);
+  const int64_t kpad_rows = round_up(std::max<int64_t>(ref->nrow, 1), kDenseRowAlign);
   const bool fused_ok_binary = (method == SD_BINARY) && prec != SD_PREC_EXACT &&
                                fused_list_len(k) > 0 &&
                                fused_fits(kpad_rows, prec == SD_PREC_FP32 ? 4 : 1);
@@ -331,7 +331,7 @@ int run_knn(sd_context* ctx, sd_matrix* X, sd_matrix* Y, const sd_knn_params* p,
 
   if (fused_ok_binary || fused_ok_cosine) {
     const bool use_f32 = (prec == SD_PREC_FP32);
-    const int64_t kpad = round_up(std::max<int64_t>(ref->nrow, 1), 128);
+    const int64_t kpad = round_up(std::max<int64_t>(ref->nrow, 1), kDenseRowAlign);
     int kind;
     if (method == SD_BINARY)
       kind = use_f32 ? kDenseF32Pattern : kDenseI8Pattern;

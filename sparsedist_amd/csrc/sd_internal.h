@@ -198,6 +198,10 @@ struct RerankArgs {
   int map_words = 0;           // LDS row bitmap words (0: merge walk)
 };
 int rerank_map_words(int64_t nrow);
+// Densified columns are padded with zero rows to a multiple of this many
+// rows, so that every slab kernel's K loop runs whole groups of chunks (the
+// filtered pass streams 64-byte chunks in groups of four).
+constexpr int64_t kDenseRowAlign = 256;
 int fused_list_len(int want);
 // Whether a slab tile of 256 rows of kpad elements of elem_bytes stays within
 // the 32-bit buffer offsets of the slab kernel.
