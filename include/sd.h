@@ -207,6 +207,84 @@ int sd_graph_info(const sd_graph* g, int64_t* n, int64_t* nnz);
 int sd_graph_copy(const sd_graph* g, int64_t* colptr, int32_t* rowind, double* val);
 int sd_graph_free(sd_graph* g);
 
+/* ---- serving ---------------------------------------------------------------
+ * A request loop over one reference matrix Y kept resident in device memory
+ * with its norms and densified layouts.  The reference has no counterpart:
+ * its closest analogue is the block loop of R/sparseKNN.R:248-294, which
+ * re-converts the whole reference for every block.  Each request is one
+ * sd_knn(X_request, Y) call (same parameters, same results bit for bit),
+ * optionally followed by the shared-neighbour graph of its query columns.
+ *
+ *  - Admission: a request reserves an estimate of the device memory its work
+ *    needs (one chunk's densified columns and candidate buffers, its results
+ *    and SNN workspace).  Requests are admitted first come first served while
+ *    the reservations fit in budget_bytes; a request that cannot fit even
+ *    alone, or that arrives while max_pending requests wait, is refused with
+ *    SD_EBUDGET.
+ *  - Continuous batching: `slots` workers, each with its own context and
+ *    stream, take work items of at most chunk_cols query columns from the
+ *    admitted requests, always from the request with the fewest unscheduled
+ *    columns, so small requests overtake large ones and the work of
+ *    different requests overlaps on the device.
+ *  - Results do not depend on chunking or on the other requests in flight.
+ * All functions may be called from any host thread. */
+typedef struct sd_server sd_server;
+
+typedef struct {
+  int32_t slots;         /* concurrent workers (1..8; 0 = 4) */
+  int32_t chunk_cols;    /* query columns per work item (0 = 4096) */
+  int64_t budget_bytes;  /* device memory admitted requests may reserve (0 = 16 GiB) */
+  int64_t max_pending;   /* requests allowed to wait for admission (0 = unlimited) */
+} sd_server_params;
+
+typedef struct {
+  int64_t nrow, ncol;     /* query matrix X (CSC, host memory; copied by submit) */
+  const int64_t* colptr;
+  const int32_t* rowind;
+  const double* values;
+  sd_knn_params knn;      /* as for sd_knn(ctx, X, Y, ...); include_self is ignored */
+  int32_t snn;            /* 1: also build the shared-neighbour graph of the query columns */
+  int32_t reserved;
+  double prune;           /* SNN: keep edges with weight > prune */
+} sd_request;
+
+typedef struct {
+  int32_t done;      /* 1 once the request has finished (successfully or not) */
+  int32_t status;    /* SD_OK or the failure code (message: sd_server_error) */
+  int32_t k_eff;     /* columns of the result */
+  int32_t warn;      /* k was clamped (as sd_knn) */
+  double queued_ms;  /* submit -> admission */
+  double run_ms;     /* admission -> finish */
+  double total_ms;   /* submit -> finish */
+} sd_request_info;
+
+typedef struct {
+  int64_t submitted, admitted, completed, refused;
+  int64_t pending, active;  /* waiting for admission / admitted and unfinished */
+  int64_t reserved_bytes;   /* current reservations of admitted requests */
+  int64_t query_cols;       /* query columns of completed requests */
+} sd_server_stats;
+
+/* Y must stay alive until sd_server_destroy; ctx gives the device. */
+int sd_server_create(sd_context* ctx, const sd_matrix* Y, const sd_server_params* params,
+                     sd_server** out);
+/* Queue a request; *ticket identifies it.  Validation errors of the kNN call
+ * itself are reported through sd_server_wait, like any other failure. */
+int sd_server_submit(sd_server* s, const sd_request* req, int64_t* ticket);
+/* Wait up to timeout_s seconds (< 0: forever) for a request; fills *info. */
+int sd_server_wait(sd_server* s, int64_t ticket, double timeout_s, sd_request_info* info);
+/* Message of a failed request (empty if none). */
+const char* sd_server_error(sd_server* s, int64_t ticket);
+/* Copy a finished request's result: idx/dist hold ncol * k_eff entries in the
+ * sd_knn layout; *snn (may be NULL) receives the SNN graph, whose ownership
+ * passes to the caller (sd_graph_free), or NULL when none was asked for. */
+int sd_server_result(sd_server* s, int64_t ticket, int32_t* idx, double* dist, sd_graph** snn);
+/* Forget a finished request. */
+int sd_server_release(sd_server* s, int64_t ticket);
+int sd_server_stats_get(sd_server* s, sd_server_stats* out);
+/* Stop the workers (in-flight work is cancelled) and free the server. */
+int sd_server_destroy(sd_server* s);
+
 /* Number of usable host cores minus one, at least 1 (R/sparseDist.R:221-224
  * over detectCoresCpp, src/detectCores.cpp:29-34).  The GPU paths ignore
  * ncores; it is kept for the R signatures. */

@@ -11,6 +11,7 @@ from .api import (  # noqa: F401
     Context,
     DeviceMatrix,
     KNNResult,
+    Server,
     SparseDistWarning,
     default_context,
     detect_cores,

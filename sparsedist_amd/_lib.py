@@ -24,7 +24,9 @@ EXPORTS = (
     "sd_context_stats", "sd_context_set_timing", "sd_context_stream", "sd_matrix_upload_csc",
     "sd_matrix_free", "sd_matrix_drop_cache", "sd_knn", "sd_knn_cross", "sd_slab",
     "sd_topk_block", "sd_topk_rows_accum", "sd_snn_jaccard", "sd_snn", "sd_graph_info",
-    "sd_graph_copy", "sd_graph_free", "sd_detect_cores",
+    "sd_graph_copy", "sd_graph_free", "sd_detect_cores", "sd_server_create", "sd_server_submit",
+    "sd_server_wait", "sd_server_error", "sd_server_result", "sd_server_release",
+    "sd_server_stats_get", "sd_server_destroy",
 )
 
 
@@ -50,6 +52,29 @@ class Stats(C.Structure):
                 ("fallback_queries", C.c_int64), ("certified_queries", C.c_int64),
                 ("escalated_queries", C.c_int64), ("overflow_queries", C.c_int64),
                 ("filter_ms", C.c_double), ("filter_launches", C.c_int64)]
+
+
+class ServerParams(C.Structure):
+    _fields_ = [("slots", C.c_int32), ("chunk_cols", C.c_int32), ("budget_bytes", C.c_int64),
+                ("max_pending", C.c_int64)]
+
+
+class Request(C.Structure):
+    _fields_ = [("nrow", C.c_int64), ("ncol", C.c_int64), ("colptr", C.c_void_p),
+                ("rowind", C.c_void_p), ("values", C.c_void_p), ("knn", KnnParams),
+                ("snn", C.c_int32), ("reserved", C.c_int32), ("prune", C.c_double)]
+
+
+class RequestInfo(C.Structure):
+    _fields_ = [("done", C.c_int32), ("status", C.c_int32), ("k_eff", C.c_int32),
+                ("warn", C.c_int32), ("queued_ms", C.c_double), ("run_ms", C.c_double),
+                ("total_ms", C.c_double)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_int64), ("admitted", C.c_int64), ("completed", C.c_int64),
+                ("refused", C.c_int64), ("pending", C.c_int64), ("active", C.c_int64),
+                ("reserved_bytes", C.c_int64), ("query_cols", C.c_int64)]
 
 
 _lib = None
@@ -91,6 +116,15 @@ def load(path: str = LIB_PATH):
         L.sd_graph_copy.argtypes = [vp, vp, vp, vp]
         L.sd_graph_free.argtypes = [vp]
         L.sd_detect_cores.argtypes = []
+        L.sd_server_create.argtypes = [vp, vp, C.POINTER(ServerParams), C.POINTER(vp)]
+        L.sd_server_submit.argtypes = [vp, C.POINTER(Request), C.POINTER(i64)]
+        L.sd_server_wait.argtypes = [vp, i64, dbl, C.POINTER(RequestInfo)]
+        L.sd_server_error.argtypes = [vp, i64]
+        L.sd_server_error.restype = C.c_char_p
+        L.sd_server_result.argtypes = [vp, i64, vp, vp, C.POINTER(vp)]
+        L.sd_server_release.argtypes = [vp, i64]
+        L.sd_server_stats_get.argtypes = [vp, C.POINTER(ServerStats)]
+        L.sd_server_destroy.argtypes = [vp]
         _lib = L
         return L
 

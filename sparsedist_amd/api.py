@@ -456,6 +456,12 @@ def sparseSNN(idx, include_self=True, prune=0, symmetrise=_MISSING, edges="knn",
     check(_lib.load().sd_snn(ctx.handle, idx32.ctypes.data, n, k, int(include_self),
                              float(prune), int(symmetrise), 0 if edges == "knn" else 1, 0,
                              C.byref(g)))
+    return _graph_to_csc(g, n)
+
+
+def _graph_to_csc(g, n_rows_cols):
+    """Copy an sd_graph to a scipy CSC matrix and free it."""
+    n = n_rows_cols
     try:
         nn, nnz = C.c_int64(0), C.c_int64(0)
         check(_lib.load().sd_graph_info(g, C.byref(nn), C.byref(nnz)))
@@ -468,6 +474,117 @@ def sparseSNN(idx, include_self=True, prune=0, symmetrise=_MISSING, edges="knn",
         _lib.load().sd_graph_free(g)
     m = nnz.value
     return sp.csc_matrix((vals[:m], rows[:m], colptr), shape=(n, n))
+
+
+# ---------------------------------------------------------------------------
+# serving: a request loop over a resident reference (sd_server)
+# ---------------------------------------------------------------------------
+class Server:
+    """Serves kNN (+ optional SNN) requests against a resident reference Y.
+
+    Each request equals ``sparseKNN(X, Y, k, method, dist, drop_degenerate)``
+    bit for bit; with ``snn=True`` it also carries the shared-neighbour graph
+    of its query columns (queries joined when their neighbour sets in Y
+    intersect, Jaccard-weighted, kept when the weight exceeds ``prune``).
+    Requests are admitted against ``budget_bytes`` of device memory and cut
+    into work items of ``chunk_cols`` columns that ``slots`` workers, each on
+    its own stream, take smallest-request-first.
+    """
+
+    def __init__(self, Y, slots=4, chunk_cols=4096, budget_bytes=0, max_pending=0, *,
+                 context: Context | None = None):
+        self.ctx = context or default_context()
+        self.Y = Y if isinstance(Y, DeviceMatrix) else DeviceMatrix(Y, self.ctx)
+        prm = _lib.ServerParams(int(slots), int(chunk_cols), int(budget_bytes), int(max_pending))
+        h = C.c_void_p()
+        check(_lib.load().sd_server_create(self.ctx.handle, self.Y.handle, C.byref(prm),
+                                           C.byref(h)))
+        self._h = h
+        self._keep = {}
+
+    def submit(self, X, k=10, method="binary", dist=True, drop_degenerate=True, snn=False,
+               prune=0.0, precision="auto", margin=0) -> int:
+        """Queue a request; returns its ticket.  Raises SparseDistError with
+        code SD_EBUDGET when the server refuses it."""
+        _chk_flag(dist, "dist")
+        _chk_flag(drop_degenerate, "drop_degenerate")
+        k = _chk_count(k, "k")
+        method = _match_method(method)
+        if method == "js":
+            raise SparseDistError(_lib.SD_EUNSUPPORTED,
+                                  "method \"js\" (dense Jensen-Shannon) is not provided by this library.")
+        M = _as_csc(X)
+        p = np.ascontiguousarray(M.indptr, dtype=np.int64)
+        i = np.ascontiguousarray(M.indices, dtype=np.int32)
+        x = np.ascontiguousarray(M.data, dtype=np.float64)
+        r = _lib.Request()
+        r.nrow, r.ncol = M.shape
+        r.colptr, r.rowind, r.values = p.ctypes.data, i.ctypes.data, x.ctypes.data
+        r.knn = _params(k, method, dist, False, drop_degenerate, 256, precision, margin)
+        r.snn = int(bool(snn))
+        r.prune = float(prune)
+        t = C.c_int64(0)
+        check(_lib.load().sd_server_submit(self._h, C.byref(r), C.byref(t)))
+        self._keep[t.value] = (M.shape[1], bool(snn))
+        return t.value
+
+    def wait(self, ticket, timeout=None) -> dict:
+        """Wait for a request (timeout in seconds, None = forever); returns
+        its info: done, status, k_eff, warn, queued_ms, run_ms, total_ms."""
+        info = _lib.RequestInfo()
+        check(_lib.load().sd_server_wait(self._h, ticket, -1.0 if timeout is None else
+                                         float(timeout), C.byref(info)))
+        return {f: getattr(info, f) for f, _ in info._fields_}
+
+    def result(self, ticket, release=True):
+        """The finished request's KNNResult (and SNN graph as ``["snn"]``)."""
+        info = self.wait(ticket)
+        if info["status"] != _lib.SD_OK:
+            msg = _lib.load().sd_server_error(self._h, ticket).decode()
+            if release:
+                self.release(ticket)
+            raise SparseDistError(info["status"], msg)
+        nq, want_snn = self._keep[ticket]
+        ke = info["k_eff"]
+        idx = np.empty(max(nq * ke, 1), dtype=np.int32)
+        dst = np.empty(max(nq * ke, 1), dtype=np.float64)
+        g = C.c_void_p()
+        check(_lib.load().sd_server_result(self._h, ticket, idx.ctypes.data, dst.ctypes.data,
+                                           C.byref(g)))
+        res = KNNResult(idx[: nq * ke].reshape((nq, ke), order="F"),
+                        dst[: nq * ke].reshape((nq, ke), order="F"), False)
+        if want_snn and g.value:
+            res["snn"] = _graph_to_csc(g, nq)
+        res["info"] = info
+        if release:
+            self.release(ticket)
+        return res
+
+    def release(self, ticket):
+        check(_lib.load().sd_server_release(self._h, ticket))
+        self._keep.pop(ticket, None)
+
+    def stats(self) -> dict:
+        s = _lib.ServerStats()
+        check(_lib.load().sd_server_stats_get(self._h, C.byref(s)))
+        return {f: getattr(s, f) for f, _ in s._fields_}
+
+    def close(self):
+        if self._h:
+            _lib.load().sd_server_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 # ---------------------------------------------------------------------------

@@ -1,15 +1,20 @@
-// k_slab.hip -- the fused slab kernel: densified reference x query tiles on
-// MFMA with a per-query running top-L kept in registers, so the
-// n_ref x n_query slab (the reference's dense block, R/sparseKNN.R:256-258)
-// never exists in memory.  Plus the kernels that finish a search: merging
-// the per-split candidate lists, the FP64 re-rank with its certificate, and
+// k_slab.hip -- the slab kernels: densified reference x query tiles on MFMA,
+// so the n_ref x n_query slab (the reference's dense block,
+// R/sparseKNN.R:256-258) never exists in memory.
+//   k_fused        list mode: each lane keeps its query's L best candidates
+//                  sorted in registers over a range of reference tiles (small
+//                  searches, the threshold pre-pass, the wider second pass).
+//   k_filter_wide  filtered full pass: every value at or above its query's
+//                  threshold is appended to a per-query candidate buffer.
+// Plus the kernels that finish a search: merging per-split lists, selecting
+// from the candidate buffers, the FP64 re-rank with its certificate, and
 // writing FNN-layout results.
 //
-// Workgroup = WM x 4 waves.  Wave (wm, wn) computes 128 reference columns x
-// 64 query columns with 32x32 MFMA blocks (4 along refs, 2 along queries).
-// After each reference tile the two lanes holding the same query exchange
-// values (lane l owns query block 0, lane l+32 query block 1), so one lane
-// owns one query and keeps its L best candidates sorted in registers.
+// k_fused: workgroup = WM x 4 waves.  Wave (wm, wn) computes 128 reference
+// columns x 64 query columns with 32x32 MFMA blocks (4 along refs, 2 along
+// queries).  After each reference tile the two lanes holding the same query
+// exchange values (lane l owns query block 0, lane l+32 query block 1), so
+// one lane owns one query and keeps its L best candidates sorted in registers.
 #include <hip/hip_bf16.h>
 
 #include "pair_math.h"
@@ -169,7 +174,7 @@ __device__ inline void wait_vm() {
   __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | 0x0F70);
 }
 
-template <class Op, class K, int L, int WM, bool FILTER>
+template <class Op, class K, int L, int WM>
 __global__ __launch_bounds__(WM * 256, 1) void k_fused(FusedKernelArgs P) {
   constexpr int WAVES = WM * 4;
   constexpr int NT = WAVES * 64;
@@ -201,13 +206,8 @@ __global__ __launch_bounds__(WM * 256, 1) void k_fused(FusedKernelArgs P) {
   const int wave = tid >> 6, lane = tid & 63;
   const int wm = wave / 4, wn = wave % 4;
   const int h = lane >> 5, n = lane & 31;
-  int qt, sp = 0;
-  int64_t rt = 0;
-  if constexpr (FILTER) {
-    if (!map_tile(blockIdx.x, gridDim.x, P.qtiles, P.rtiles, qt, rt)) return;
-  } else {
-    map_block(blockIdx.x, P.qtiles, P.splits, qt, sp);
-  }
+  int qt, sp;
+  map_block(blockIdx.x, P.qtiles, P.splits, qt, sp);
   const int64_t q_base = (int64_t)qt * TN;
   // position of the query this lane owns, and its column in the query matrix
   const int64_t my_pos = q_base + wn * 64 + h * 32 + n;
@@ -227,16 +227,8 @@ __global__ __launch_bounds__(WM * 256, 1) void k_fused(FusedKernelArgs P) {
     list[t].idx = 0xFFFFFFFFu;
   }
 
-  int64_t t0, t1;
-  double thr_q = 0.0;
-  if constexpr (FILTER) {
-    t0 = rt;
-    t1 = rt + 1;
-    if (q_ok) thr_q = P.thr[my_pos];
-  } else {
-    t0 = (int64_t)sp * P.tiles_per_split;
-    t1 = (t0 + P.tiles_per_split < P.ntiles) ? t0 + P.tiles_per_split : P.ntiles;
-  }
+  const int64_t t0 = (int64_t)sp * P.tiles_per_split;
+  const int64_t t1 = (t0 + P.tiles_per_split < P.ntiles) ? t0 + P.tiles_per_split : P.ntiles;
   const int64_t nchunks = P.kpad_bytes / kRowBytes;
 
   // Direct-load sources.  Wave w issues A instructions 4w..4w+3 (of 16) and
@@ -268,7 +260,7 @@ __global__ __launch_bounds__(WM * 256, 1) void k_fused(FusedKernelArgs P) {
   typedef __attribute__((address_space(3))) void lds_void;
 
   for (int64_t vt = t0; vt < t1; ++vt) {
-    const int64_t tile = FILTER ? vt : vt * P.tile_stride;
+    const int64_t tile = vt * P.tile_stride;
     const int64_t r_base = tile * TM;
     for (int t = tid; t < TM; t += NT) {
       const int64_t r = r_base + t;
@@ -460,30 +452,7 @@ __global__ __launch_bounds__(WM * 256, 1) void k_fused(FusedKernelArgs P) {
               key = sv;
             if (P.self_mode && rg == my_q) key = 2.0f;  // restored self similarity
           }
-          if constexpr (FILTER) {
-            double kd;
-            bool pass;
-            if constexpr (sizeof(K) == 8) {
-              // cheap float screen (errors far below 2^-20), then the exact rational
-              const float f = (float)key.c / (float)key.d;
-              pass = f >= (float)thr_q * (1.0f - 9.5367431640625e-07f);
-              kd = 0.0;
-              if (pass) {
-                kd = key_to_double(key);
-                pass = kd >= thr_q;
-              }
-            } else {
-              kd = (double)key;
-              pass = kd >= thr_q;
-            }
-            if (pass) {
-              const int at = atomicAdd(&P.cnt[my_pos], 1);
-              if (at < P.cap) {
-                P.fkey[my_pos * P.cap + at] = kd;
-                P.fidx[my_pos * P.cap + at] = (int32_t)rg;
-              }
-            }
-          } else if (better(key, (uint32_t)rg, list[L - 1].key, list[L - 1].idx)) {
+          if (better(key, (uint32_t)rg, list[L - 1].key, list[L - 1].idx)) {
             Entry<K> carry{key, (uint32_t)rg};
 #pragma unroll
             for (int p = 0; p < L; ++p) {
@@ -500,7 +469,7 @@ __global__ __launch_bounds__(WM * 256, 1) void k_fused(FusedKernelArgs P) {
     }
   }
 
-  if (!FILTER && q_ok) {
+  if (q_ok) {
     const int64_t slot = ((int64_t)sp * WM + wm) * P.nqry + my_pos;
 #pragma unroll
     for (int t = 0; t < L; ++t) {
@@ -1049,7 +1018,7 @@ __global__ __launch_bounds__(256) void k_rerank(RerankArgs R) {
 template <class Op, class K, int L, int WM>
 static int launch_fused_t(sd_context* ctx, FusedKernelArgs& P) {
   const int nblk = P.qtiles * P.splits;
-  hipLaunchKernelGGL((k_fused<Op, K, L, WM, false>), dim3((unsigned)nblk), dim3(WM * 256), 0,
+  hipLaunchKernelGGL((k_fused<Op, K, L, WM>), dim3((unsigned)nblk), dim3(WM * 256), 0,
                      ctx->stream, P);
   SD_HIP(hipGetLastError());
   return SD_OK;

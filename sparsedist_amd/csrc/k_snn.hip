@@ -15,27 +15,30 @@
 namespace sd {
 namespace {
 
-__global__ void k_validate(const int32_t* __restrict__ idx, int64_t total, int n,
+__global__ void k_validate(const int32_t* __restrict__ idx, int64_t total, int nids,
                            int* __restrict__ bad) {
   const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (e >= total) return;
   const int32_t v = idx[e];
-  if (v != kNaInt && (v < 1 || v > n)) atomicOr(bad, 1);
+  if (v != kNaInt && (v < 1 || v > nids)) atomicOr(bad, 1);
 }
 
-// One thread per node: gather, insertion-sort, de-duplicate.
+// One thread per node: gather, insertion-sort, de-duplicate.  self_space:
+// neighbour ids name the nodes themselves (a node's own id is dropped, and
+// added back when include_self); otherwise they name another column space.
 __global__ void k_build_sets(const int32_t* __restrict__ idx, int n, int k, int include_self,
-                             int32_t* __restrict__ sets, int32_t* __restrict__ len) {
+                             int self_space, int32_t* __restrict__ sets,
+                             int32_t* __restrict__ len) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   int32_t* S = sets + (int64_t)i * (k + 1);
   int m = 0;
-  if (include_self) S[m++] = i;
+  if (include_self && self_space) S[m++] = i;
   for (int t = 0; t < k; ++t) {
     const int32_t v = idx[(int64_t)t * n + i];
     if (v == kNaInt) continue;
     const int32_t nb = v - 1;
-    if (nb == i) continue;
+    if (self_space && nb == i) continue;
     int p = m++;
     while (p > 0 && S[p - 1] > nb) {
       S[p] = S[p - 1];
@@ -287,20 +290,25 @@ int assemble_csc(sd_context* ctx, DevBuf& keys, DevBuf& vals, int64_t m, int64_t
 }  // namespace
 
 int snn_build(sd_context* ctx, const int32_t* d_idx, int n, int k, int include_self, double prune,
-              int symmetrise, int edges, sd_graph* g, double* d_w_out) {
+              int symmetrise, int edges, sd_graph* g, double* d_w_out, int nids) {
   if (n < 1 || k < 1) return set_error(SD_EINVAL, "'idx' must have at least one row and column.");
+  const bool self_space = (nids <= 0);
+  if (self_space) nids = n;
+  if (!self_space && edges == 0)
+    return set_error(SD_EINVAL, "internal: kNN-edge weights need neighbour ids in the node space");
   const int64_t total = (int64_t)n * k;
   DevBuf bad;
   SD_TRY(bad.alloc(sizeof(int)));
   SD_HIP(hipMemsetAsync(bad.ptr, 0, sizeof(int), ctx->stream));
   hipLaunchKernelGGL(k_validate, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, ctx->stream,
-                     d_idx, total, n, bad.as<int>());
+                     d_idx, total, nids, bad.as<int>());
   int hbad = 0;
   SD_HIP(hipMemcpyAsync(&hbad, bad.ptr, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
   SD_HIP(hipStreamSynchronize(ctx->stream));
   if (hbad)
-    return set_error(SD_EINVAL,
-                     "'idx' entries must be NA or 1-based row indices in [1, nrow(idx)].");
+    return set_error(SD_EINVAL, self_space
+                                    ? "'idx' entries must be NA or 1-based row indices in [1, nrow(idx)]."
+                                    : "neighbour ids must be NA or 1-based column indices of the reference.");
 
   KernelTimer timer(ctx, &ctx->stats.snn_ms, &ctx->stats.snn_launches);
   DevBuf sets, len;
@@ -308,7 +316,7 @@ int snn_build(sd_context* ctx, const int32_t* d_idx, int n, int k, int include_s
   SD_TRY(len.alloc(sizeof(int32_t) * (size_t)n));
   const unsigned gn = (unsigned)((n + 255) / 256);
   hipLaunchKernelGGL(k_build_sets, dim3(gn), dim3(256), 0, ctx->stream, d_idx, n, k, include_self,
-                     sets.as<int32_t>(), len.as<int32_t>());
+                     (int)self_space, sets.as<int32_t>(), len.as<int32_t>());
   SD_HIP(hipGetLastError());
 
   const int end_bit = 32 + bits_for(n);
@@ -349,41 +357,43 @@ int snn_build(sd_context* ctx, const int32_t* d_idx, int n, int k, int include_s
 
   // ---- edges = "shared" ------------------------------------------------------
   DevBuf rc, rp, fill, rv, pc, poff;
-  SD_TRY(rc.alloc(sizeof(int64_t) * (size_t)(n + 1)));
-  SD_TRY(rp.alloc(sizeof(int64_t) * (size_t)(n + 1)));
-  SD_TRY(fill.alloc(sizeof(int64_t) * (size_t)(n + 1)));
-  SD_HIP(hipMemsetAsync(rc.ptr, 0, sizeof(int64_t) * (size_t)(n + 1), ctx->stream));
-  SD_HIP(hipMemsetAsync(fill.ptr, 0, sizeof(int64_t) * (size_t)(n + 1), ctx->stream));
+  // inverted index over the neighbour ids: R(m) = nodes whose set holds m
+  const unsigned gm = (unsigned)((nids + 255) / 256);
+  SD_TRY(rc.alloc(sizeof(int64_t) * (size_t)(nids + 1)));
+  SD_TRY(rp.alloc(sizeof(int64_t) * (size_t)(nids + 1)));
+  SD_TRY(fill.alloc(sizeof(int64_t) * (size_t)(nids + 1)));
+  SD_HIP(hipMemsetAsync(rc.ptr, 0, sizeof(int64_t) * (size_t)(nids + 1), ctx->stream));
+  SD_HIP(hipMemsetAsync(fill.ptr, 0, sizeof(int64_t) * (size_t)(nids + 1), ctx->stream));
   hipLaunchKernelGGL(k_rev_count, dim3(gn), dim3(256), 0, ctx->stream, sets.as<int32_t>(),
                      len.as<int32_t>(), n, k, rc.as<int64_t>());
   size_t bytes = 0;
   SD_HIP(rocprim::exclusive_scan(nullptr, bytes, rc.as<int64_t>(), rp.as<int64_t>(), (int64_t)0,
-                                 (size_t)(n + 1), rocprim::plus<int64_t>(), ctx->stream));
+                                 (size_t)(nids + 1), rocprim::plus<int64_t>(), ctx->stream));
   DevBuf tmp;
   SD_TRY(tmp.alloc(bytes + 16));
   SD_HIP(rocprim::exclusive_scan(tmp.ptr, bytes, rc.as<int64_t>(), rp.as<int64_t>(), (int64_t)0,
-                                 (size_t)(n + 1), rocprim::plus<int64_t>(), ctx->stream));
+                                 (size_t)(nids + 1), rocprim::plus<int64_t>(), ctx->stream));
   int64_t nrev = 0;
-  SD_HIP(hipMemcpyAsync(&nrev, rp.as<int64_t>() + n, sizeof(int64_t), hipMemcpyDeviceToHost,
+  SD_HIP(hipMemcpyAsync(&nrev, rp.as<int64_t>() + nids, sizeof(int64_t), hipMemcpyDeviceToHost,
                         ctx->stream));
   SD_HIP(hipStreamSynchronize(ctx->stream));
   SD_TRY(rv.alloc(sizeof(int32_t) * (size_t)nrev + 16));
   hipLaunchKernelGGL(k_rev_fill, dim3(gn), dim3(256), 0, ctx->stream, sets.as<int32_t>(),
                      len.as<int32_t>(), n, k, rp.as<int64_t>(), fill.as<int64_t>(),
                      rv.as<int32_t>());
-  SD_TRY(pc.alloc(sizeof(int64_t) * (size_t)(n + 1)));
-  SD_TRY(poff.alloc(sizeof(int64_t) * (size_t)(n + 1)));
-  SD_HIP(hipMemsetAsync(pc.ptr, 0, sizeof(int64_t) * (size_t)(n + 1), ctx->stream));
-  hipLaunchKernelGGL(k_pair_count, dim3(gn), dim3(256), 0, ctx->stream, rp.as<int64_t>(), n,
+  SD_TRY(pc.alloc(sizeof(int64_t) * (size_t)(nids + 1)));
+  SD_TRY(poff.alloc(sizeof(int64_t) * (size_t)(nids + 1)));
+  SD_HIP(hipMemsetAsync(pc.ptr, 0, sizeof(int64_t) * (size_t)(nids + 1), ctx->stream));
+  hipLaunchKernelGGL(k_pair_count, dim3(gm), dim3(256), 0, ctx->stream, rp.as<int64_t>(), nids,
                      pc.as<int64_t>());
   bytes = 0;
   SD_HIP(rocprim::exclusive_scan(nullptr, bytes, pc.as<int64_t>(), poff.as<int64_t>(), (int64_t)0,
-                                 (size_t)(n + 1), rocprim::plus<int64_t>(), ctx->stream));
+                                 (size_t)(nids + 1), rocprim::plus<int64_t>(), ctx->stream));
   SD_TRY(tmp.alloc(bytes + 16));
   SD_HIP(rocprim::exclusive_scan(tmp.ptr, bytes, pc.as<int64_t>(), poff.as<int64_t>(), (int64_t)0,
-                                 (size_t)(n + 1), rocprim::plus<int64_t>(), ctx->stream));
+                                 (size_t)(nids + 1), rocprim::plus<int64_t>(), ctx->stream));
   int64_t npairs = 0;
-  SD_HIP(hipMemcpyAsync(&npairs, poff.as<int64_t>() + n, sizeof(int64_t), hipMemcpyDeviceToHost,
+  SD_HIP(hipMemcpyAsync(&npairs, poff.as<int64_t>() + nids, sizeof(int64_t), hipMemcpyDeviceToHost,
                         ctx->stream));
   SD_HIP(hipStreamSynchronize(ctx->stream));
   if (npairs == 0) {
@@ -393,8 +403,8 @@ int snn_build(sd_context* ctx, const int32_t* d_idx, int n, int k, int include_s
   DevBuf pk, pk2;
   SD_TRY(pk.alloc(sizeof(uint64_t) * (size_t)npairs + 16));
   SD_TRY(pk2.alloc(sizeof(uint64_t) * (size_t)npairs + 16));
-  hipLaunchKernelGGL(k_pair_emit, dim3((unsigned)n), dim3(256), 0, ctx->stream, rp.as<int64_t>(),
-                     rv.as<int32_t>(), n, poff.as<int64_t>(), pk.as<uint64_t>());
+  hipLaunchKernelGGL(k_pair_emit, dim3((unsigned)nids), dim3(256), 0, ctx->stream, rp.as<int64_t>(),
+                     rv.as<int32_t>(), nids, poff.as<int64_t>(), pk.as<uint64_t>());
   SD_HIP(hipGetLastError());
   bytes = 0;
   SD_HIP(rocprim::radix_sort_keys(nullptr, bytes, pk.as<uint64_t>(), pk2.as<uint64_t>(),

@@ -237,7 +237,7 @@ int fused_candidate_lists(sd_context* ctx, FusedArgs fa, const double** out_key,
     }
     return SD_OK;
   };
-  const int64_t stride = 16;
+  const int64_t stride = 32;  // pre-pass samples 1/32 of the reference tiles
   if (rtiles < 4 * stride) {
     fa.tile_stride = 1;
     return list_pass(fa, ctx->scratch[3], ctx->scratch[4], ctx->scratch[5], ctx->scratch[6],
@@ -303,7 +303,11 @@ int run_knn(sd_context* ctx, sd_matrix* X, sd_matrix* Y, const sd_knn_params* p,
   const int method = p->method;
   const int k = plan.k;
   const int64_t nq = X->ncol, nref = ref->ncol;
-  SD_TRY(prep_method(ctx, ref, method));
+  {
+    std::lock_guard<std::mutex> lk(ref->prep_mu);
+    SD_TRY(prep_method(ctx, ref, method));
+    SD_HIP(hipStreamSynchronize(ctx->stream));
+  }
   if (!plan.selfmode) SD_TRY(prep_method(ctx, X, method));
   SD_TRY(check_cancel(ctx));
 
@@ -337,7 +341,11 @@ int run_knn(sd_context* ctx, sd_matrix* X, sd_matrix* Y, const sd_knn_params* p,
       kind = use_f32 ? kDenseF32Pattern : kDenseI8Pattern;
     else
       kind = use_f32 ? kDenseF32Unit : kDenseBf16Unit;
-    SD_TRY(prep_dense(ctx, ref, kind, kpad));
+    {
+      std::lock_guard<std::mutex> lk(ref->prep_mu);
+      SD_TRY(prep_dense(ctx, ref, kind, kpad));
+      SD_HIP(hipStreamSynchronize(ctx->stream));
+    }
     if (!plan.selfmode) SD_TRY(prep_dense(ctx, X, kind, kpad));
     SD_TRY(check_cancel(ctx));
 

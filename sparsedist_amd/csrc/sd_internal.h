@@ -87,6 +87,10 @@ struct sd_context {
 
 struct sd_matrix {
   sd_context* ctx = nullptr;
+  // Held while derived layouts are built and until they are complete on the
+  // building stream, so contexts on other streams (server workers) that
+  // share the matrix never read a half-built layout.
+  std::mutex prep_mu;
   int64_t nrow = 0, ncol = 0, nnz = 0;
   sd::DevBuf p, i, x;              // CSC (explicit zeros dropped)
   std::vector<int64_t> hp;         // host copy of colptr (planning)
@@ -227,8 +231,12 @@ int topk_rows(sd_context* ctx, const double* D, int64_t nrow, int64_t ncol, int 
               int32_t* best_idx, double* best_val, int64_t col_offset, const int32_t* self_col);
 
 // ---- SNN (k_snn.hip) ---------------------------------------------------------
+// nids = 0: neighbour ids are 1-based node indices (the reference's sparseSNN
+// input).  nids > 0: they are 1-based columns of another matrix with nids
+// columns (a query batch's neighbours in a reference); only edges = 1
+// ("shared") applies and a node is never its own neighbour.
 int snn_build(sd_context* ctx, const int32_t* d_idx, int n, int k, int include_self, double prune,
-              int symmetrise, int edges, sd_graph* g, double* d_w_out);
+              int symmetrise, int edges, sd_graph* g, double* d_w_out, int nids = 0);
 
 // ---- timing helpers ------------------------------------------------------------
 struct KernelTimer {

@@ -1,0 +1,457 @@
+// sd_server.cpp -- request loop over a resident reference matrix.
+//
+// Requests are sd_knn(X_request, Y) calls (plus, on request, the
+// shared-neighbour graph of the query columns) served by a fixed set of
+// worker slots.  Each slot owns an sd_context, i.e. its own HIP stream and
+// scratch memory, so the work of different requests overlaps on the device.
+// A request is cut into work items of at most chunk_cols query columns;
+// a free slot always takes the next item of the admitted request with the
+// fewest unscheduled columns, so small requests overtake large ones.
+// Per-query results are independent of the chunking and of the requests
+// sharing the device, so each result equals the one-shot sd_knn call.
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <memory>
+#include <thread>
+
+#include "sd_internal.h"
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+double ms_between(Clock::time_point a, Clock::time_point b) {
+  return std::chrono::duration<double, std::milli>(b - a).count();
+}
+
+template <class F>
+int guarded_call(F&& f) {
+  try {
+    sd::clear_error();
+    return f();
+  } catch (const std::bad_alloc&) {
+    return sd::set_error(SD_ENOMEM, "host allocation failed");
+  } catch (...) {
+    return sd::set_error(SD_EDEVICE, "unexpected internal error");
+  }
+}
+
+struct Request {
+  int64_t ticket = 0;
+  // inputs (host copies)
+  int64_t nrow = 0, ncol = 0;
+  std::vector<int64_t> colptr;
+  std::vector<int32_t> rowind;
+  std::vector<double> values;
+  sd_knn_params knn{};
+  int snn = 0;
+  double prune = 0.0;
+  int64_t reserve = 0;  // bytes reserved while admitted
+  // progress (guarded by the server mutex)
+  int64_t next_col = 0;   // first unscheduled column
+  int64_t done_cols = 0;  // columns whose results are in place
+  int inflight = 0;       // items taken and not yet returned
+  bool admitted = false, finished = false;
+  int status = SD_OK;
+  std::string error;
+  int32_t k_eff = 0, warn = 0;
+  // results: ncol x k column-major, compacted to k_eff columns at the end
+  std::vector<int32_t> idx;
+  std::vector<double> dist;
+  sd_graph* graph = nullptr;
+  Clock::time_point t_submit, t_admit, t_done;
+};
+
+}  // namespace
+
+struct sd_server {
+  int device = 0;
+  const sd_matrix* Y = nullptr;
+  int slots = 4;
+  int64_t chunk = 4096;
+  int64_t budget = 0;
+  int64_t max_pending = 0;
+  std::vector<sd_context*> ctx;
+  std::vector<std::thread> workers;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  bool stop = false;
+  int64_t next_ticket = 1;
+  std::map<int64_t, std::unique_ptr<Request>> all;
+  std::deque<Request*> pending;
+  std::vector<Request*> active;
+  int64_t reserved = 0;
+  sd_server_stats st{};
+  std::string last_msg;  // storage for sd_server_error's return value
+};
+
+namespace {
+
+// Device bytes one request may hold while admitted: one work item's
+// densified query columns, candidate buffers and staging (the slot scratch
+// grows to this), its results, and the SNN workspace (sets, inverted index
+// and up to n k^2 candidate pairs, sorted with keys and values).
+int64_t reserve_estimate(const sd_server* s, const Request& r) {
+  const int64_t cols = std::min<int64_t>(r.ncol, s->chunk);
+  const int64_t kpad =
+      (std::max<int64_t>(r.nrow, 1) + sd::kDenseRowAlign - 1) / sd::kDenseRowAlign *
+      sd::kDenseRowAlign;
+  const int64_t k = std::max(r.knn.k, 1);
+  const int64_t L = 64;
+  const int64_t cap = 4096;
+  const int64_t nnz = r.colptr.empty() ? 0 : r.colptr.back();
+  const int64_t chunk_nnz = r.ncol > 0 ? (nnz + r.ncol - 1) / r.ncol * cols : 0;
+  int64_t b = cols * kpad * 2              // densified query columns (BF16)
+              + cols * cap * 12            // candidate buffer (key + index)
+              + cols * L * 12 * 4          // candidate lists and merges
+              + chunk_nnz * 28 + cols * 64  // CSC copy, unit values, flags
+              + cols * k * 12;             // results of one item
+  if (r.snn) b += r.ncol * k * (k + 1) * 4 + r.ncol * k * k * 40 + r.ncol * 64;
+  return b + (1 << 20);
+}
+
+// Admit waiting requests in arrival order while their reservations fit.
+void admit_locked(sd_server* s) {
+  while (!s->pending.empty()) {
+    Request* r = s->pending.front();
+    if (s->reserved + r->reserve > s->budget) break;
+    s->pending.pop_front();
+    s->reserved += r->reserve;
+    r->admitted = true;
+    r->t_admit = Clock::now();
+    s->active.push_back(r);
+    ++s->st.admitted;
+  }
+}
+
+void finish_locked(sd_server* s, Request* r) {
+  r->finished = true;
+  r->t_done = Clock::now();
+  s->reserved -= r->reserve;
+  s->active.erase(std::remove(s->active.begin(), s->active.end(), r), s->active.end());
+  ++s->st.completed;
+  s->st.query_cols += r->ncol;
+  admit_locked(s);
+  s->cv_done.notify_all();
+  s->cv_work.notify_all();
+}
+
+// The admitted request with the fewest unscheduled columns (ties: earliest).
+Request* pick_locked(sd_server* s) {
+  Request* best = nullptr;
+  for (Request* r : s->active) {
+    if (r->next_col >= r->ncol) continue;
+    const int64_t left = r->ncol - r->next_col;
+    if (!best || left < best->ncol - best->next_col ||
+        (left == best->ncol - best->next_col && r->ticket < best->ticket))
+      best = r;
+  }
+  return best;
+}
+
+// One work item: query columns [c0, c1) of r against Y on the slot's context.
+int run_item(sd_server* s, sd_context* ctx, Request* r, int64_t c0, int64_t c1) {
+  const int64_t nc = c1 - c0;
+  const int64_t base = r->colptr[c0];
+  std::vector<int64_t> p((size_t)nc + 1);
+  for (int64_t j = 0; j <= nc; ++j) p[j] = r->colptr[c0 + j] - base;
+  sd_matrix* X = nullptr;
+  SD_TRY(sd_matrix_upload_csc(ctx, r->nrow, nc, p.data(), r->rowind.data() + base,
+                              r->values.data() + base, &X));
+  const int32_t k = r->knn.k;
+  std::vector<int32_t> idx((size_t)nc * k);
+  std::vector<double> dist((size_t)nc * k);
+  int32_t keff = 0, warn = 0;
+  sd_knn_params prm = r->knn;
+  prm.include_self = 0;
+  int rc = sd_knn(ctx, X, s->Y, &prm, idx.data(), dist.data(), &keff, &warn);
+  sd_matrix_free(X);
+  if (rc != SD_OK) return rc;
+  // columns [c0, c1) belong to this item alone, so no lock is needed here;
+  // k_eff is a property of (Y, k) and equal for every item.
+  for (int32_t t = 0; t < keff; ++t) {
+    std::copy(idx.begin() + (size_t)t * nc, idx.begin() + (size_t)(t + 1) * nc,
+              r->idx.begin() + (size_t)t * r->ncol + c0);
+    std::copy(dist.begin() + (size_t)t * nc, dist.begin() + (size_t)(t + 1) * nc,
+              r->dist.begin() + (size_t)t * r->ncol + c0);
+  }
+  std::lock_guard<std::mutex> lk(s->mu);
+  r->k_eff = keff;
+  r->warn = warn;
+  return SD_OK;
+}
+
+// Shared-neighbour graph of the query columns: two queries are joined when
+// their neighbour sets in Y intersect, weighted by the Jaccard index of the
+// sets and kept when the weight exceeds prune (the "shared" edge rule of
+// R/sparseSNN.R:206-218 with neighbour ids in Y's column space).
+int run_snn(sd_server* s, sd_context* ctx, Request* r) {
+  const int64_t n = r->ncol;
+  const int32_t k = r->k_eff;
+  if (n >= ((int64_t)1 << 31) || s->Y->ncol >= ((int64_t)1 << 31) || k < 1)
+    return sd::set_error(SD_EINVAL, "SNN needs fewer than 2^31 columns and k >= 1");
+  sd::DevBuf di;
+  SD_TRY(di.alloc(sizeof(int32_t) * (size_t)n * (size_t)k));
+  SD_HIP(hipMemcpyAsync(di.ptr, r->idx.data(), sizeof(int32_t) * (size_t)n * (size_t)k,
+                        hipMemcpyHostToDevice, ctx->stream));
+  auto* g = new sd_graph();
+  const int rc = sd::snn_build(ctx, di.as<int32_t>(), (int)n, k, 0, r->prune, 0, 1, g, nullptr,
+                               (int)s->Y->ncol);
+  if (rc != SD_OK) {
+    delete g;
+    return rc;
+  }
+  r->graph = g;
+  return SD_OK;
+}
+
+void worker(sd_server* s, int slot) {
+  sd_context* ctx = s->ctx[slot];
+  (void)hipSetDevice(s->device);
+  for (;;) {
+    Request* r = nullptr;
+    int64_t c0 = 0, c1 = 0;
+    {
+      std::unique_lock<std::mutex> lk(s->mu);
+      s->cv_work.wait(lk, [&] { return s->stop || pick_locked(s) != nullptr; });
+      if (s->stop) return;
+      r = pick_locked(s);
+      c0 = r->next_col;
+      c1 = std::min(r->ncol, c0 + s->chunk);
+      r->next_col = c1;
+      ++r->inflight;
+    }
+    int rc = guarded_call([&] { return run_item(s, ctx, r, c0, c1); });
+    const std::string msg = rc == SD_OK ? std::string() : std::string(sd_last_error());
+    bool complete = false, need_snn = false;
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      --r->inflight;
+      if (rc == SD_OK) {
+        r->done_cols += c1 - c0;
+      } else if (r->status == SD_OK) {
+        r->status = rc;
+        r->error = msg;
+        r->next_col = r->ncol;  // schedule nothing more of it
+      }
+      // the item that returns last, with nothing left to schedule, completes it
+      complete = r->inflight == 0 && r->next_col == r->ncol;
+      need_snn = complete && r->status == SD_OK && r->snn;
+      if (complete && !need_snn) finish_locked(s, r);
+    }
+    if (need_snn) {
+      rc = guarded_call([&] { return run_snn(s, ctx, r); });
+      std::lock_guard<std::mutex> lk(s->mu);
+      if (rc != SD_OK) {
+        r->status = rc;
+        r->error = sd_last_error();
+      }
+      finish_locked(s, r);
+    }
+  }
+}
+
+Request* find_locked(sd_server* s, int64_t ticket) {
+  auto it = s->all.find(ticket);
+  return it == s->all.end() ? nullptr : it->second.get();
+}
+
+}  // namespace
+
+extern "C" {
+
+int sd_server_create(sd_context* ctx, const sd_matrix* Y, const sd_server_params* params,
+                     sd_server** out) {
+  return guarded_call([&]() -> int {
+    if (!ctx || !Y || !out) return sd::set_error(SD_EINVAL, "NULL argument");
+    sd_server_params p{};
+    if (params) p = *params;
+    if (p.slots < 0 || p.slots > 8) return sd::set_error(SD_EINVAL, "slots must be in 0..8");
+    if (p.chunk_cols < 0) return sd::set_error(SD_EINVAL, "chunk_cols must be >= 0");
+    if (p.budget_bytes < 0 || p.max_pending < 0)
+      return sd::set_error(SD_EINVAL, "budget_bytes and max_pending must be >= 0");
+    if (Y->ncol < 1) return sd::set_error(SD_EINVAL, "'Y' must have at least one column.");
+    auto s = std::make_unique<sd_server>();
+    s->device = ctx->device;
+    s->Y = Y;
+    s->slots = p.slots ? p.slots : 4;
+    s->chunk = p.chunk_cols ? p.chunk_cols : 4096;
+    s->budget = p.budget_bytes ? p.budget_bytes : ((int64_t)16 << 30);
+    s->max_pending = p.max_pending;
+    for (int i = 0; i < s->slots; ++i) {
+      sd_context* c = nullptr;
+      const int rc = sd_context_create(s->device, &c);
+      if (rc != SD_OK) {
+        for (sd_context* d : s->ctx) sd_context_destroy(d);
+        return rc;
+      }
+      s->ctx.push_back(c);
+    }
+    for (int i = 0; i < s->slots; ++i) s->workers.emplace_back(worker, s.get(), i);
+    *out = s.release();
+    return SD_OK;
+  });
+}
+
+int sd_server_submit(sd_server* s, const sd_request* req, int64_t* ticket) {
+  return guarded_call([&]() -> int {
+    if (!s || !req || !ticket) return sd::set_error(SD_EINVAL, "NULL argument");
+    if (req->ncol < 1) return sd::set_error(SD_EINVAL, "'X' must have at least one column.");
+    if (req->nrow != s->Y->nrow)
+      return sd::set_error(SD_EINVAL, "'X' and 'Y' must have the same number of rows.");
+    if (!req->colptr || (req->colptr[req->ncol] > 0 && (!req->rowind || !req->values)))
+      return sd::set_error(SD_EINVAL, "NULL argument");
+    if (req->colptr[0] != 0) return sd::set_error(SD_EINVAL, "colptr[0] must be 0");
+    if (req->knn.k < 1)
+      return sd::set_error(SD_EINVAL,
+                           "'k' must be a positive integer no larger than .Machine$integer.max.");
+    if (req->snn && !(req->prune >= 0.0 && req->prune <= 1.0))
+      return sd::set_error(SD_EINVAL, "'prune' must be a finite number between 0 and 1.");
+    auto r = std::make_unique<Request>();
+    r->nrow = req->nrow;
+    r->ncol = req->ncol;
+    const int64_t nnz = req->colptr[req->ncol];
+    r->colptr.assign(req->colptr, req->colptr + req->ncol + 1);
+    if (nnz > 0) {
+      r->rowind.assign(req->rowind, req->rowind + nnz);
+      r->values.assign(req->values, req->values + nnz);
+    }
+    r->knn = req->knn;
+    r->snn = req->snn;
+    r->prune = req->prune;
+    r->reserve = reserve_estimate(s, *r);
+    if (r->reserve > s->budget)
+      return sd::set_error(SD_EBUDGET,
+                           "request needs an estimated %lld bytes of device memory, more than "
+                           "the server budget of %lld",
+                           (long long)r->reserve, (long long)s->budget);
+    r->idx.assign((size_t)r->ncol * (size_t)r->knn.k, sd::kNaInt);
+    r->dist.assign((size_t)r->ncol * (size_t)r->knn.k, 0.0);
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->max_pending > 0 && (int64_t)s->pending.size() >= s->max_pending) {
+      ++s->st.refused;
+      return sd::set_error(SD_EBUDGET, "server queue full (%lld requests waiting)",
+                           (long long)s->pending.size());
+    }
+    r->ticket = s->next_ticket++;
+    r->t_submit = Clock::now();
+    *ticket = r->ticket;
+    s->pending.push_back(r.get());
+    s->all.emplace(r->ticket, std::move(r));
+    ++s->st.submitted;
+    admit_locked(s);
+    s->cv_work.notify_all();
+    return SD_OK;
+  });
+}
+
+int sd_server_wait(sd_server* s, int64_t ticket, double timeout_s, sd_request_info* info) {
+  return guarded_call([&]() -> int {
+    if (!s || !info) return sd::set_error(SD_EINVAL, "NULL argument");
+    std::unique_lock<std::mutex> lk(s->mu);
+    Request* r = find_locked(s, ticket);
+    if (!r) return sd::set_error(SD_EINVAL, "unknown ticket %lld", (long long)ticket);
+    auto ready = [&] { return r->finished; };
+    if (timeout_s < 0)
+      s->cv_done.wait(lk, ready);
+    else
+      s->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), ready);
+    *info = sd_request_info{};
+    info->done = r->finished ? 1 : 0;
+    info->status = r->status;
+    info->k_eff = r->k_eff;
+    info->warn = r->warn;
+    if (r->admitted) info->queued_ms = ms_between(r->t_submit, r->t_admit);
+    if (r->finished) {
+      info->run_ms = ms_between(r->t_admit, r->t_done);
+      info->total_ms = ms_between(r->t_submit, r->t_done);
+    }
+    return SD_OK;
+  });
+}
+
+const char* sd_server_error(sd_server* s, int64_t ticket) {
+  if (!s) return "";
+  std::lock_guard<std::mutex> lk(s->mu);
+  Request* r = find_locked(s, ticket);
+  s->last_msg = r ? r->error : std::string();
+  return s->last_msg.c_str();
+}
+
+int sd_server_result(sd_server* s, int64_t ticket, int32_t* idx, double* dist, sd_graph** snn) {
+  return guarded_call([&]() -> int {
+    if (!s || !idx || !dist) return sd::set_error(SD_EINVAL, "NULL argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    Request* r = find_locked(s, ticket);
+    if (!r) return sd::set_error(SD_EINVAL, "unknown ticket %lld", (long long)ticket);
+    if (!r->finished) return sd::set_error(SD_EINVAL, "request %lld has not finished",
+                                           (long long)ticket);
+    if (r->status != SD_OK) return sd::set_error(r->status, "%s", r->error.c_str());
+    const size_t m = (size_t)r->ncol * (size_t)r->k_eff;
+    std::copy(r->idx.begin(), r->idx.begin() + m, idx);
+    std::copy(r->dist.begin(), r->dist.begin() + m, dist);
+    if (snn) {
+      *snn = r->graph;
+      r->graph = nullptr;
+    }
+    return SD_OK;
+  });
+}
+
+int sd_server_release(sd_server* s, int64_t ticket) {
+  return guarded_call([&]() -> int {
+    if (!s) return sd::set_error(SD_EINVAL, "NULL argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    Request* r = find_locked(s, ticket);
+    if (!r) return sd::set_error(SD_EINVAL, "unknown ticket %lld", (long long)ticket);
+    if (!r->finished) return sd::set_error(SD_EINVAL, "request %lld has not finished",
+                                           (long long)ticket);
+    if (r->graph) sd_graph_free(r->graph);
+    s->all.erase(ticket);
+    return SD_OK;
+  });
+}
+
+int sd_server_stats_get(sd_server* s, sd_server_stats* out) {
+  if (!s || !out) return sd::set_error(SD_EINVAL, "NULL argument");
+  std::lock_guard<std::mutex> lk(s->mu);
+  *out = s->st;
+  out->pending = (int64_t)s->pending.size();
+  out->active = (int64_t)s->active.size();
+  out->reserved_bytes = s->reserved;
+  return SD_OK;
+}
+
+int sd_server_destroy(sd_server* s) {
+  if (!s) return SD_OK;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stop = true;
+  }
+  for (sd_context* c : s->ctx) sd_cancel(c);
+  s->cv_work.notify_all();
+  for (auto& t : s->workers) t.join();
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    for (auto& kv : s->all) {
+      Request* r = kv.second.get();
+      if (!r->finished) {
+        r->finished = true;
+        if (r->status == SD_OK) {
+          r->status = SD_ECANCELLED;
+          r->error = "Computation interrupted by the user.";
+        }
+      }
+      if (r->graph) sd_graph_free(r->graph);
+    }
+    s->cv_done.notify_all();
+  }
+  for (sd_context* c : s->ctx) sd_context_destroy(c);
+  delete s;
+  return SD_OK;
+}
+
+}  // extern "C"

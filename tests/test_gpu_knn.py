@@ -275,13 +275,13 @@ def test_certificate_escalation_and_fallback(oracle_lib, gpu_ctx):
 
 @pytest.mark.parametrize("method", ["binary", "cosine"])
 def test_two_pass_filter_path(oracle_lib, gpu_ctx, method):
-    """Enough reference columns (>= 64 tiles of 128) for the threshold
+    """Enough reference columns (>= 128 tiles of 128) for the threshold
     pre-pass + filtered full pass; queries are a subset so the oracle stays
     fast, and a heavily tied block exercises the buffer-overflow path."""
-    Y = synth.nb_log1p(600, 9000, 0.05, 41)
+    Y = synth.nb_log1p(600, 20000, 0.05, 41)
     Yd = Y.tolil()
-    # 2200 copies of column 0: its candidate buffer overflows
-    for c in range(3000, 5200):
+    # 4700 copies of column 0: more than any candidate buffer holds (4096)
+    for c in range(3000, 7700):
         Yd[:, c] = Y[:, 0]
     Y = Yd.tocsc()
     X = Y[:, :300]

@@ -1,0 +1,145 @@
+"""The request loop (sd_server): each result is the same alone and in the mix,
+small requests overtake large ones, admission honours the memory budget.
+
+kNN results are compared bit for bit with one-shot sparseKNN(X, Y) calls (the
+path the parity tests pin against the oracle).  The SNN graph of a request is
+checked against a brute force over all query pairs: queries i and j are
+joined with weight |S_i & S_j| / |S_i | S_j| when that exceeds prune, where
+S_i is the set of i's neighbours in Y.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import sparsedist_amd as sd
+from sparsedist_amd import _lib, synth
+from conftest import bits
+
+pytestmark = pytest.mark.gpu
+
+PRUNE = 1 / 15
+
+
+def brute_shared_snn(idx, prune):
+    n, k = idx.shape
+    sets = [set(int(v) for v in idx[i] if v != sd.NA_INTEGER) for i in range(n)]
+    rows, cols, vals = [], [], []
+    for i in range(n):
+        for j in range(i + 1, n):
+            u = len(sets[i] & sets[j])
+            if u == 0:
+                continue
+            w = u / (len(sets[i]) + len(sets[j]) - u)
+            if w > prune:
+                rows += [i, j]
+                cols += [j, i]
+                vals += [w, w]
+    return sp.csc_matrix((vals, (rows, cols)), shape=(n, n))
+
+
+def same_graph(a, b):
+    a = a.tocsc()
+    b = b.tocsc()
+    a.sort_indices()
+    b.sort_indices()
+    assert a.shape == b.shape
+    assert np.array_equal(a.indptr, b.indptr)
+    assert np.array_equal(a.indices, b.indices)
+    assert np.array_equal(bits(a.data), bits(b.data))
+
+
+@pytest.fixture(scope="module")
+def ref_and_requests():
+    Y = synth.nb_log1p(400, 6000, 0.05, 71)
+    sizes = [1, 3, 17, 250, 900, 2500, 40, 7, 1200, 64, 333, 2]
+    reqs = []
+    for t, n in enumerate(sizes):
+        X = synth.nb_log1p(400, n, 0.05, 100 + t)
+        reqs.append(dict(X=X, k=(10, 20, 30)[t % 3], method=("cosine", "binary")[t % 2],
+                         snn=(t % 2 == 0) and n <= 900))
+    return Y, reqs
+
+
+def test_server_same_alone_and_in_mix(ref_and_requests, gpu_ctx):
+    Y, reqs = ref_and_requests
+    dY = sd.DeviceMatrix(Y)
+    with sd.Server(dY, slots=4, chunk_cols=256) as srv:
+        tickets = [srv.submit(r["X"], k=r["k"], method=r["method"], snn=r["snn"], prune=PRUNE)
+                   for r in reqs]
+        mixed = [srv.result(t) for t in tickets]
+        st = srv.stats()
+    assert st["completed"] == len(reqs) and st["pending"] == 0 and st["active"] == 0
+    assert st["reserved_bytes"] == 0
+    for r, m in zip(reqs, mixed):
+        # one-shot call on the user's context
+        alone = sd.sparseKNN(r["X"], dY, k=r["k"], method=r["method"], verbose=False)
+        assert np.array_equal(m.idx, alone.idx)
+        assert np.array_equal(bits(m.dist), bits(alone.dist))
+        if r["snn"]:
+            same_graph(m["snn"], brute_shared_snn(m.idx, PRUNE))
+    # the same requests alone on a one-slot server, one work item each
+    with sd.Server(dY, slots=1, chunk_cols=100000) as srv1:
+        for r, m in zip(reqs, mixed):
+            if not r["snn"]:
+                continue
+            a = srv1.result(srv1.submit(r["X"], k=r["k"], method=r["method"], snn=True,
+                                        prune=PRUNE))
+            assert np.array_equal(a.idx, m.idx)
+            same_graph(a["snn"], m["snn"])
+
+
+def test_small_request_overtakes_large(gpu_ctx):
+    Y = synth.nb_log1p(300, 5000, 0.05, 72)
+    big = synth.nb_log1p(300, 6000, 0.05, 73)
+    small = synth.nb_log1p(300, 5, 0.05, 74)
+    with sd.Server(Y, slots=1, chunk_cols=128) as srv:
+        tb = srv.submit(big, k=10, method="cosine")
+        ts = srv.submit(small, k=10, method="cosine")
+        info_s = srv.wait(ts)
+        info_b_then = srv.wait(tb, timeout=0)
+        rb = srv.result(tb)
+        rs = srv.result(ts)
+    assert info_s["done"] == 1 and info_s["status"] == 0
+    assert info_b_then["done"] == 0  # the large one was still running
+    alone = sd.sparseKNN(small, Y, k=10, method="cosine", verbose=False)
+    assert np.array_equal(rs.idx, alone.idx)
+    assert rb.idx.shape == (6000, 10)
+
+
+def test_budget_admission(gpu_ctx):
+    Y = synth.nb_log1p(200, 3000, 0.05, 75)
+    X = synth.nb_log1p(200, 50, 0.05, 76)
+    with sd.Server(Y, slots=2, chunk_cols=64, budget_bytes=1 << 20) as srv:
+        with pytest.raises(sd.SparseDistError) as e:
+            srv.submit(X, k=5, method="binary")
+        assert e.value.code == _lib.SD_EBUDGET
+    # a budget that fits one request at a time (about 3.2 MB reserved each)
+    # still serves all of them, in order
+    with sd.Server(Y, slots=2, chunk_cols=64, budget_bytes=4 << 20) as srv:
+        ts = [srv.submit(synth.nb_log1p(200, 40 + i, 0.05, 80 + i), k=5, method="binary")
+              for i in range(4)]
+        res = [srv.result(t) for t in ts]
+        st = srv.stats()
+    assert st["completed"] == 4 and st["admitted"] == 4
+    for i, r in enumerate(res):
+        alone = sd.sparseKNN(synth.nb_log1p(200, 40 + i, 0.05, 80 + i), Y, k=5,
+                             method="binary", verbose=False)
+        assert np.array_equal(r.idx, alone.idx)
+
+
+def test_request_errors_are_reported(gpu_ctx):
+    Y = synth.nb_log1p(200, 1000, 0.05, 77)
+    with sd.Server(Y, slots=2) as srv:
+        with pytest.raises(sd.SparseDistError) as e:
+            srv.submit(synth.nb_log1p(150, 10, 0.05, 78), k=5)
+        assert "same number of rows" in str(e.value)
+        # non-finite cosine input fails inside the worker and is reported by result()
+        X = synth.nb_log1p(200, 10, 0.05, 79).tolil()
+        X[3, 2] = np.inf
+        t = srv.submit(X.tocsc(), k=5, method="cosine")
+        with pytest.raises(sd.SparseDistError) as e:
+            srv.result(t)
+        assert e.value.code == _lib.SD_ENONFINITE
+        # the server keeps serving after a failed request
+        ok = srv.result(srv.submit(synth.nb_log1p(200, 10, 0.05, 81), k=5, method="cosine"))
+        assert ok.idx.shape == (10, 5)
