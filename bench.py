@@ -22,11 +22,80 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-MI355X_PEAK = {"int8": 5.0e15, "bf16": 2.5e15, "fp32": 157.3e12, "hbm": 8.0e12}
+MI355X_PEAK = {"fp4": 10.1e15, "int8": 5.0e15, "bf16": 2.5e15, "fp32": 157.3e12, "hbm": 8.0e12}
 
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+SERVE_RATE = 200.0        # requests per second (Poisson)
+SERVE_MAX_COLS = 50_000   # request sizes log-uniform on [1, 50000] columns
+
+
+def serve_arrivals(window_s, seed, pool_cols):
+    """The request stream: Poisson arrivals, log-uniform sizes, method uniform
+    over {cosine, binary}, k uniform over {10, 20, 30}, SNN on half."""
+    rng = np.random.default_rng(seed)
+    t, out = 0.0, []
+    while True:
+        t += rng.exponential(1.0 / SERVE_RATE)
+        if t >= window_s:
+            return out
+        n = int(np.exp(rng.uniform(0.0, np.log(SERVE_MAX_COLS + 1))))
+        n = max(1, min(n, SERVE_MAX_COLS, pool_cols))
+        out.append(dict(t=t, n=n, method=("cosine", "binary")[int(rng.integers(2))],
+                        k=int((10, 20, 30)[int(rng.integers(3))]), snn=bool(rng.random() < 0.5),
+                        start=int(rng.integers(0, pool_cols - n + 1))))
+
+
+def serve_run(sd, dY, pool, arrivals, precision, slots=4, chunk_cols=4096, max_pending=16):
+    """Replays the request stream against a server over dY; each request is a
+    contiguous slice of the query pool.  Returns latency and throughput."""
+    srv = sd.Server(dY, slots=slots, chunk_cols=chunk_cols, max_pending=max_pending)
+    try:
+        # build Y's derived layouts for both methods before the clock starts
+        for m in ("binary", "cosine"):
+            srv.result(srv.submit_csc(pool.shape[0], pool.indptr[:2] - pool.indptr[0],
+                                      pool.indices[: pool.indptr[1]], pool.data[: pool.indptr[1]],
+                                      k=10, method=m, precision=precision))
+        live, refused = [], 0
+        t0 = time.perf_counter()
+        for a in arrivals:
+            now = time.perf_counter() - t0
+            if now < a["t"]:
+                time.sleep(a["t"] - now)
+            c0, c1 = a["start"], a["start"] + a["n"]
+            p0, p1 = int(pool.indptr[c0]), int(pool.indptr[c1])
+            try:
+                tk = srv.submit_csc(pool.shape[0], pool.indptr[c0:c1 + 1] - p0,
+                                    pool.indices[p0:p1], pool.data[p0:p1], k=a["k"],
+                                    method=a["method"], snn=a["snn"], prune=1 / 15,
+                                    precision=precision, borrow=True)
+                live.append((tk, a))
+            except sd.SparseDistError as e:
+                if e.code != sd._lib.SD_EBUDGET:
+                    raise
+                refused += 1
+        infos = []
+        for tk, a in live:
+            info = srv.wait(tk)
+            if info["status"] != 0:
+                raise RuntimeError(f"request failed: {srv.result(tk)}")
+            infos.append((info, a))
+            srv.release(tk)
+        t_end = time.perf_counter() - t0
+    finally:
+        srv.close()
+    lat = np.array([i["total_ms"] for i, _ in infos])
+    cols = sum(a["n"] for _, a in infos)
+    small = np.array([i["total_ms"] for i, a in infos if a["n"] <= 1000])
+    return dict(precision=precision, offered=len(arrivals), served=len(infos), refused=refused,
+                p50_ms=float(np.percentile(lat, 50)) if lat.size else None,
+                p99_ms=float(np.percentile(lat, 99)) if lat.size else None,
+                p99_small_ms=float(np.percentile(small, 99)) if small.size else None,
+                served_cols_per_s=cols / t_end, wall_s=t_end,
+                snn_requests=sum(1 for _, a in infos if a["snn"]))
 
 
 def main():
@@ -41,7 +110,23 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=48)
+    ap.add_argument("--serve-window", type=float, default=4.0,
+                    help="seconds of request arrivals in the serving run (0: skip it)")
+    ap.add_argument("--serve", action="store_true",
+                    help="serving benchmark only, on the 20000 x 500000 resident reference")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run one plain 1 GiB device copy (calibrates rocprofv3's "
+                         "FETCH_SIZE / WRITE_SIZE counters against known bytes)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        a = torch.ones(1 << 28, dtype=torch.float32, device="cuda")  # 1 GiB
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": a.numel() * 4, "copy_bytes_written": b.numel() * 4}))
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
               "(the query-sharded multi-GPU design is described but not run).", file=sys.stderr)
@@ -50,9 +135,16 @@ def main():
     import sparsedist_amd as sd
     from sparsedist_amd import synth
 
+    workers = max(1, min(16, len(os.sched_getaffinity(0))))
+    if args.serve:
+        return serve_main(args, sd, synth, workers)
     t0 = time.time()
     X = synth.nb_log1p(args.nrow, args.ncol, args.density, seed=20)
     log(f"generated X {X.shape} nnz={X.nnz} in {time.time() - t0:.1f}s")
+    pool = None
+    if args.serve_window > 0:
+        # query pool for the serving run, generated before the GPU is touched
+        pool = synth.nb_log1p_parallel(args.nrow, SERVE_MAX_COLS, args.density, 21, workers)
     ctx = sd.default_context()
     dX = sd.DeviceMatrix(X, ctx)
     nq = args.ncol
@@ -109,9 +201,10 @@ def main():
     flop = 2.0 * args.nrow * nq * nq  # 2 r nR nQ: dense-equivalent operations of one full slab
     bin_ms = kern("binary")
     cos_ms = kern("cosine")
-    roof_bin = dict(bound="mfma", achieved=flop / (bin_ms / 1e3) / 1e12, peak=5000.0,
-                    unit="TOPS", frac=(flop / (bin_ms / 1e3)) / MI355X_PEAK["int8"],
-                    traffic=None, kernel="k_filter_wide<OpI8, KeyJ> (int8 Jaccard filtered pass)",
+    roof_bin = dict(bound="mfma", achieved=flop / (bin_ms / 1e3) / 1e12, peak=10100.0,
+                    unit="TFLOP/s", frac=(flop / (bin_ms / 1e3)) / MI355X_PEAK["fp4"],
+                    traffic=None,
+                    kernel="k_filter_wide<OpF4, KeyJ> (FP4-pattern Jaccard filtered pass)",
                     avg_launch_ms=bin_ms)
     roof_cos = dict(bound="mfma", achieved=flop / (cos_ms / 1e3) / 1e12, peak=2500.0,
                     unit="TFLOP/s", frac=(flop / (cos_ms / 1e3)) / MI355X_PEAK["bf16"],
@@ -145,6 +238,22 @@ def main():
                           f"and cosine k={k}, oracle/sd_oracle.c with OpenMP ({tc:.1f} s)")
         log(f"cpu baseline: {cpu['value']:.2f} cols/s on {nt} threads")
 
+    serving = None
+    if pool is not None:
+        arr = serve_arrivals(args.serve_window, 22, pool.shape[1])
+        runs = [serve_run(sd, dX, pool, arr, "auto")]
+        if not args.no_plain:
+            runs.append(serve_run(sd, dX, pool, arr, "fp32"))
+        serving = dict(reference=f"the count matrix X ({args.nrow} x {nq}) resident",
+                       stream=f"Poisson {SERVE_RATE:.0f} req/s for {args.serve_window:.0f} s, "
+                              "sizes log-uniform on [1, 50000] columns, cosine/binary, "
+                              "k in {10, 20, 30}, SNN on half; 4 slots, 4096-column work "
+                              "items, 16 waiting requests at most",
+                       runs=runs)
+        for r in runs:
+            log(f"serving {r['precision']}: served {r['served']}/{r['offered']} p50 "
+                f"{r['p50_ms']:.0f} ms p99 {r['p99_ms']:.0f} ms, {r['served_cols_per_s']:.0f} cols/s")
+
     out = {
         "metric": "exact-kNN query cols/s (k=20) + SNN edges/s",
         "value": value,
@@ -156,7 +265,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "int8+bf16 slab, fp64 re-rank",
+        "dtype": "fp4 0/1 patterns (Jaccard, exact counts) + bf16 (cosine) slab, fp64 re-rank",
         "data": "synthetic (seeded NB/log1p count matrix)",
         "config": {"workload": f"count-matrix kNN {args.nrow}x{nq} d={args.density} "
                                f"binary+cosine k={k} + SNN prune=1/15",
@@ -172,6 +281,41 @@ def main():
         "roofline": dominant,
         "roofline_other": roof_cos if dominant is roof_bin else roof_bin,
         "cpu_baseline": cpu,
+        "serving": serving,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def serve_main(args, sd, synth, workers):
+    """Serving on the 20000 x 500000 reference (density 0.05, NB/log1p)."""
+    t0 = time.time()
+    Y = synth.nb_log1p_parallel(20_000, 500_000, 0.05, 50, workers)
+    pool = synth.nb_log1p_parallel(20_000, SERVE_MAX_COLS, 0.05, 51, workers)
+    log(f"generated Y {Y.shape} nnz={Y.nnz} and the query pool in {time.time() - t0:.1f}s")
+    ctx = sd.default_context()
+    dY = sd.DeviceMatrix(Y, ctx)
+    del Y
+    window = args.serve_window if args.serve_window > 0 else 10.0
+    arr = serve_arrivals(window, 52, pool.shape[1])
+    runs = [serve_run(sd, dY, pool, arr, "auto")]
+    if not args.no_plain:
+        runs.append(serve_run(sd, dY, pool, arr, "fp32"))
+    for r in runs:
+        log(f"serving {r['precision']}: served {r['served']}/{r['offered']} p50 "
+            f"{r['p50_ms']:.0f} ms p99 {r['p99_ms']:.0f} ms, {r['served_cols_per_s']:.0f} cols/s")
+    a = runs[0]
+    out = {
+        "metric": "serving p50/p99 latency (ms) and served query cols/s",
+        "value": a["served_cols_per_s"], "unit": "served query cols/s", "n_gpus": 1,
+        "higher_is_better": True, "p50_ms": a["p50_ms"], "p99_ms": a["p99_ms"],
+        "dtype": "fp4 0/1 patterns (Jaccard, exact counts) + bf16 (cosine) slab, fp64 re-rank", "data": "synthetic (seeded NB/log1p)",
+        "config": {"workload": "serving: Y 20000 x 500000 d=0.05 NB/log1p resident; Poisson "
+                               f"{SERVE_RATE:.0f} req/s for {window:.0f} s; sizes log-uniform "
+                               "[1, 50000]; cosine/binary; k in {10, 20, 30}; SNN on half",
+                   "slots": 4, "chunk_cols": 4096, "max_pending": 16},
+        "runs": runs,
+        "vs_plain_fp32": (a["served_cols_per_s"] / runs[1]["served_cols_per_s"])
+        if len(runs) > 1 else None,
     }
     print(json.dumps(out), flush=True)
 

@@ -244,7 +244,8 @@ typedef struct {
   const double* values;
   sd_knn_params knn;      /* as for sd_knn(ctx, X, Y, ...); include_self is ignored */
   int32_t snn;            /* 1: also build the shared-neighbour graph of the query columns */
-  int32_t reserved;
+  int32_t borrow;         /* 1: colptr/rowind/values stay valid and unchanged until the
+                             request finishes, so submit does not copy them */
   double prune;           /* SNN: keep edges with weight > prune */
 } sd_request;
 

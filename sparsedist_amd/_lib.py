@@ -62,7 +62,7 @@ class ServerParams(C.Structure):
 class Request(C.Structure):
     _fields_ = [("nrow", C.c_int64), ("ncol", C.c_int64), ("colptr", C.c_void_p),
                 ("rowind", C.c_void_p), ("values", C.c_void_p), ("knn", KnnParams),
-                ("snn", C.c_int32), ("reserved", C.c_int32), ("prune", C.c_double)]
+                ("snn", C.c_int32), ("borrow", C.c_int32), ("prune", C.c_double)]
 
 
 class RequestInfo(C.Structure):

@@ -503,9 +503,11 @@ class Server:
         self._keep = {}
 
     def submit(self, X, k=10, method="binary", dist=True, drop_degenerate=True, snn=False,
-               prune=0.0, precision="auto", margin=0) -> int:
+               prune=0.0, precision="auto", margin=0, borrow=False) -> int:
         """Queue a request; returns its ticket.  Raises SparseDistError with
-        code SD_EBUDGET when the server refuses it."""
+        code SD_EBUDGET when the server refuses it.  borrow=True lends X's
+        arrays to the server instead of copying them (the server keeps a
+        reference until the request is released)."""
         _chk_flag(dist, "dist")
         _chk_flag(drop_degenerate, "drop_degenerate")
         k = _chk_count(k, "k")
@@ -514,18 +516,30 @@ class Server:
             raise SparseDistError(_lib.SD_EUNSUPPORTED,
                                   "method \"js\" (dense Jensen-Shannon) is not provided by this library.")
         M = _as_csc(X)
-        p = np.ascontiguousarray(M.indptr, dtype=np.int64)
-        i = np.ascontiguousarray(M.indices, dtype=np.int32)
-        x = np.ascontiguousarray(M.data, dtype=np.float64)
+        return self.submit_csc(M.shape[0], M.indptr, M.indices, M.data, k=k, method=method,
+                               dist=dist, drop_degenerate=drop_degenerate, snn=snn, prune=prune,
+                               precision=precision, margin=margin, borrow=borrow)
+
+    def submit_csc(self, nrow, indptr, indices, data, k=10, method="binary", dist=True,
+                   drop_degenerate=True, snn=False, prune=0.0, precision="auto", margin=0,
+                   borrow=False) -> int:
+        """submit() for a CSC matrix given as arrays (indptr[0] == 0, rows
+        ascending within each column)."""
+        k = _chk_count(k, "k")
+        method = _match_method(method)
+        p = np.ascontiguousarray(indptr, dtype=np.int64)
+        i = np.ascontiguousarray(indices, dtype=np.int32)
+        x = np.ascontiguousarray(data, dtype=np.float64)
         r = _lib.Request()
-        r.nrow, r.ncol = M.shape
+        r.nrow, r.ncol = int(nrow), p.size - 1
         r.colptr, r.rowind, r.values = p.ctypes.data, i.ctypes.data, x.ctypes.data
         r.knn = _params(k, method, dist, False, drop_degenerate, 256, precision, margin)
         r.snn = int(bool(snn))
+        r.borrow = int(bool(borrow))
         r.prune = float(prune)
         t = C.c_int64(0)
         check(_lib.load().sd_server_submit(self._h, C.byref(r), C.byref(t)))
-        self._keep[t.value] = (M.shape[1], bool(snn))
+        self._keep[t.value] = (p.size - 1, bool(snn), (p, i, x) if borrow else None)
         return t.value
 
     def wait(self, ticket, timeout=None) -> dict:
@@ -544,7 +558,7 @@ class Server:
             if release:
                 self.release(ticket)
             raise SparseDistError(info["status"], msg)
-        nq, want_snn = self._keep[ticket]
+        nq, want_snn, _ = self._keep[ticket]
         ke = info["k_eff"]
         idx = np.empty(max(nq * ke, 1), dtype=np.int32)
         dst = np.empty(max(nq * ke, 1), dtype=np.float64)

@@ -89,7 +89,10 @@ __device__ inline uint16_t f32_to_bf16_rne(float f) {
 // Densify column chunks into [ncol][kpad] column-contiguous storage.  One
 // workgroup per (column, K chunk): the chunk is zeroed in LDS, the column's
 // stored entries that fall into it are scattered, and the chunk is written
-// out with 16-byte stores.  ELEM bytes per element.
+// out with 16-byte stores.  ELEM bytes per element; ELEM == 0 is the FP4
+// pattern layout: two rows per byte, row 2i in the low nibble, stored
+// entries as e2m1 1.0 (0x2), set with LDS atomics since neighbouring rows
+// share a byte.
 template <int KIND, int ELEM>
 __global__ __launch_bounds__(256) void k_densify(const int64_t* __restrict__ p,
                                                  const int32_t* __restrict__ ri,
@@ -103,7 +106,7 @@ __global__ __launch_bounds__(256) void k_densify(const int64_t* __restrict__ p,
   if (j >= ncol) return;
   const int64_t k0 = c * chunk;
   const int64_t len = (kpad - k0 < chunk) ? (kpad - k0) : chunk;  // multiple of 64
-  const int64_t bytes = len * ELEM;
+  const int64_t bytes = ELEM ? len * ELEM : len / 2;
   uint4* l4 = reinterpret_cast<uint4*>(lds);
   for (int64_t t = threadIdx.x; t < bytes / 16; t += blockDim.x) l4[t] = make_uint4(0, 0, 0, 0);
   __syncthreads();
@@ -111,8 +114,8 @@ __global__ __launch_bounds__(256) void k_densify(const int64_t* __restrict__ p,
     const int64_t r = ri[e];
     if (r < k0 || r >= k0 + len) continue;
     const int64_t o = r - k0;
-    if (KIND == kDenseI8Pattern) {
-      lds[o] = 1;
+    if (KIND == kDenseF4Pattern) {
+      atomicOr(reinterpret_cast<uint32_t*>(lds) + (o >> 3), 0x2u << ((o & 7) * 4));
     } else if (KIND == kDenseBf16Unit || KIND == kDenseBf16Raw) {
       reinterpret_cast<uint16_t*>(lds)[o] = f32_to_bf16_rne((float)vals[e]);
     } else if (KIND == kDenseF32Pattern) {
@@ -122,7 +125,7 @@ __global__ __launch_bounds__(256) void k_densify(const int64_t* __restrict__ p,
     }
   }
   __syncthreads();
-  uint4* dst = reinterpret_cast<uint4*>(out + (j * kpad + k0) * ELEM);
+  uint4* dst = reinterpret_cast<uint4*>(out + (ELEM ? (j * kpad + k0) * ELEM : (j * kpad + k0) / 2));
   for (int64_t t = threadIdx.x; t < bytes / 16; t += blockDim.x) dst[t] = l4[t];
 }
 
@@ -175,24 +178,29 @@ int prep_sq(sd_context* ctx, sd_matrix* m) {
 int prep_dense(sd_context* ctx, sd_matrix* m, int kind, int64_t kpad) {
   if (kind < 0 || kind >= kDenseKinds) return set_error(SD_EINVAL, "internal: dense kind %d", kind);
   if (m->dense_kpad[kind] == kpad && m->dense[kind].ptr) return SD_OK;
-  const int elem = (kind == kDenseI8Pattern) ? 1 : (kind == kDenseBf16Unit || kind == kDenseBf16Raw) ? 2 : 4;
+  if (kpad % dense_row_align(kind) != 0)
+    return set_error(SD_EINVAL, "internal: kpad %lld is not aligned for dense kind %d",
+                     (long long)kpad, kind);
+  const int64_t row_bytes = dense_row_bytes(kind, kpad);
   m->dense_kpad[kind] = 0;
-  SD_TRY(m->dense[kind].alloc((size_t)m->ncol * (size_t)kpad * (size_t)elem + 256));
+  SD_TRY(m->dense[kind].alloc((size_t)m->ncol * (size_t)row_bytes + 256));
   uint8_t* dst = m->dense[kind].as<uint8_t>();
   const double* vals = m->x.as<double>();
   if (kind == kDenseBf16Unit || kind == kDenseF32Unit) {
     SD_TRY(prep_unit(ctx, m));
     vals = m->unit.as<double>();
   }
+  // rows per chunk: 64 KB of LDS, a multiple of 128 rows (whole 16-byte
+  // stores, and whole 32-bit words of eight FP4 rows)
   const int64_t chunk_bytes = 64 * 1024;
-  const int64_t chunk = chunk_bytes / elem;  // multiple of 64 elements
+  const int64_t chunk = kind == kDenseF4Pattern ? chunk_bytes * 2 : chunk_bytes / (row_bytes / kpad);
   const int64_t nchunks = (kpad + chunk - 1) / chunk;
   const int64_t grid = m->ncol * nchunks;
-  const size_t lds = (size_t)((kpad < chunk ? kpad : chunk) * elem);
+  const size_t lds = (size_t)dense_row_bytes(kind, kpad < chunk ? kpad : chunk);
   if (grid > 0) {
     switch (kind) {
-      case kDenseI8Pattern:
-        hipLaunchKernelGGL((k_densify<kDenseI8Pattern, 1>), dim3((unsigned)grid), dim3(256), lds,
+      case kDenseF4Pattern:
+        hipLaunchKernelGGL((k_densify<kDenseF4Pattern, 0>), dim3((unsigned)grid), dim3(256), lds,
                            ctx->stream, m->p.as<int64_t>(), m->i.as<int32_t>(), vals, m->ncol, kpad,
                            chunk, dst);
         break;

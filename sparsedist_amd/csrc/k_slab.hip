@@ -23,8 +23,7 @@
 
 namespace sd {
 
-typedef int v4i __attribute__((ext_vector_type(4)));
-typedef int v16i __attribute__((ext_vector_type(16)));
+typedef int v8i __attribute__((ext_vector_type(8)));
 typedef float v16f __attribute__((ext_vector_type(16)));
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 
@@ -41,13 +40,14 @@ constexpr int kRsrcWord3 = 0x00020000;
 // MFMA, lane (m, h) supplies K elements of its own choosing; A and B lanes
 // with the same h always supply the same logical segment, so the product is
 // the full dot product whatever the hardware's K order inside a lane.
-struct OpI8 {  // 0/1 patterns, exact int32 counts
-  using Acc = v16i;
+struct OpF4 {  // 0/1 patterns as FP4 (e2m1) nibbles: 1.0 products, exact FP32 counts
+  using Acc = v16f;  // counts stay exact below 2^24 (rows are far fewer)
   __device__ static void mma(Acc& c, const uint4& a, const uint4& b) {
-    v4i av, bv;
-    __builtin_memcpy(&av, &a, 16);
-    __builtin_memcpy(&bv, &b, 16);
-    c = __builtin_amdgcn_mfma_i32_32x32x32_i8(av, bv, c, 0, 0, 0);
+    // 32 K elements per lane; only the low four dwords of an FP4 operand
+    // are read.  Both block scales are 127 (E8M0 for 2^0).
+    const v8i av = {(int)a.x, (int)a.y, (int)a.z, (int)a.w, 0, 0, 0, 0};
+    const v8i bv = {(int)b.x, (int)b.y, (int)b.z, (int)b.w, 0, 0, 0, 0};
+    c = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(av, bv, c, 4, 4, 0, 127, 0, 127);
   }
 };
 
@@ -1086,9 +1086,9 @@ static int query_rows(sd_context* ctx, const FusedArgs& a, int64_t kpad_bytes,
   return SD_OK;
 }
 
-bool fused_fits(int64_t kpad, int elem_bytes) {
-  // one 256-row tile must stay below 2^31 bytes of buffer offset
-  return kpad > 0 && kpad * elem_bytes * 256 < ((int64_t)1 << 31);
+bool fused_fits(int64_t row_bytes) {
+  // one tile of 256 columns must stay below 2^31 bytes of buffer offset
+  return row_bytes > 0 && row_bytes * 256 < ((int64_t)1 << 31);
 }
 
 int fused_list_len(int want) {
@@ -1104,11 +1104,8 @@ int fused_slab_topl(sd_context* ctx, const FusedArgs& a, double* cand_key, int32
   P.qlist = a.qlist;
   P.nref = a.nref;
   P.nqry = a.nqry;
-  const int elem = (a.method == SD_BINARY && a.prec != SD_PREC_FP32) ? 1
-                   : (a.prec == SD_PREC_FP32)                         ? 4
-                                                                      : 2;
-  P.kpad_bytes = a.kpad * elem;
-  if (!fused_fits(a.kpad, elem))
+  P.kpad_bytes = dense_row_bytes(a.kind, a.kpad);
+  if (!fused_fits(P.kpad_bytes))
     return set_error(SD_EINVAL, "internal: %lld padded rows exceed the slab kernel's tile range",
                      (long long)a.kpad);
   SD_TRY(query_rows(ctx, a, P.kpad_bytes, &P.qry_dense));
@@ -1133,8 +1130,8 @@ int fused_slab_topl(sd_context* ctx, const FusedArgs& a, double* cand_key, int32
   P.cand_idx = cand_idx;
   KernelTimer timer(ctx, &ctx->stats.slab_ms, &ctx->stats.slab_launches);
   if (a.method == SD_BINARY) {
-    if (a.prec == SD_PREC_FP32) return launch_fused_L<OpF32, KeyJ, WM>(ctx, P, a.L);
-    return launch_fused_L<OpI8, KeyJ, WM>(ctx, P, a.L);
+    if (a.kind == kDenseF32Pattern) return launch_fused_L<OpF32, KeyJ, WM>(ctx, P, a.L);
+    return launch_fused_L<OpF4, KeyJ, WM>(ctx, P, a.L);
   }
   if (a.prec == SD_PREC_FP32) return launch_fused_L<OpF32, float, WM>(ctx, P, a.L);
   return launch_fused_L<OpBf16, float, WM>(ctx, P, a.L);
@@ -1156,11 +1153,8 @@ int fused_filter_pass(sd_context* ctx, const FusedArgs& a, const double* thr, in
   P.qlist = a.qlist;
   P.nref = a.nref;
   P.nqry = a.nqry;
-  const int elem = (a.method == SD_BINARY && a.prec != SD_PREC_FP32) ? 1
-                   : (a.prec == SD_PREC_FP32)                         ? 4
-                                                                      : 2;
-  P.kpad_bytes = a.kpad * elem;
-  if (!fused_fits(a.kpad, elem))
+  P.kpad_bytes = dense_row_bytes(a.kind, a.kpad);
+  if (!fused_fits(P.kpad_bytes))
     return set_error(SD_EINVAL, "internal: %lld padded rows exceed the slab kernel's tile range",
                      (long long)a.kpad);
   SD_TRY(query_rows(ctx, a, P.kpad_bytes, &P.qry_dense));
@@ -1183,8 +1177,8 @@ int fused_filter_pass(sd_context* ctx, const FusedArgs& a, const double* thr, in
   SD_HIP(hipMemsetAsync(cnt, 0, sizeof(int32_t) * (size_t)a.nqry, ctx->stream));
   KernelTimer timer(ctx, &ctx->stats.filter_ms, &ctx->stats.filter_launches);
   if (a.method == SD_BINARY) {
-    if (a.prec == SD_PREC_FP32) return launch_filter_t<OpF32, KeyJ>(ctx, P);
-    return launch_filter_t<OpI8, KeyJ>(ctx, P);
+    if (a.kind == kDenseF32Pattern) return launch_filter_t<OpF32, KeyJ>(ctx, P);
+    return launch_filter_t<OpF4, KeyJ>(ctx, P);
   }
   if (a.prec == SD_PREC_FP32) return launch_filter_t<OpF32, float>(ctx, P);
   return launch_filter_t<OpBf16, float>(ctx, P);

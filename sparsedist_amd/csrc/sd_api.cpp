@@ -321,26 +321,23 @@ int run_knn(sd_context* ctx, sd_matrix* X, sd_matrix* Y, const sd_knn_params* p,
 
   int prec = p->precision;
   int margin = p->margin > 0 ? p->margin : 16;
-  const int64_t kpad_rows = round_up(std::max<int64_t>(ref->nrow, 1), kDenseRowAlign);
+  // densified layout: binary -> 0/1 patterns (FP4 nibbles, or FP32 on the
+  // plain path); cosine -> unit-normalised values (BF16, or FP32)
+  const bool use_f32 = (prec == SD_PREC_FP32);
+  const int kind = method == SD_BINARY ? (use_f32 ? kDenseF32Pattern : kDenseF4Pattern)
+                                       : (use_f32 ? kDenseF32Unit : kDenseBf16Unit);
+  const int64_t kpad = round_up(std::max<int64_t>(ref->nrow, 1), dense_row_align(kind));
+  const bool fits = fused_fits(dense_row_bytes(kind, kpad));
   const bool fused_ok_binary = (method == SD_BINARY) && prec != SD_PREC_EXACT &&
-                               fused_list_len(k) > 0 &&
-                               fused_fits(kpad_rows, prec == SD_PREC_FP32 ? 4 : 1);
+                               fused_list_len(k) > 0 && fits;
   const bool fused_ok_cosine = (method == SD_COSINE) && prec != SD_PREC_EXACT &&
-                               fused_list_len(k + margin) > 0 && nref >= 1 &&
-                               fused_fits(kpad_rows, prec == SD_PREC_FP32 ? 4 : 2);
+                               fused_list_len(k + margin) > 0 && nref >= 1 && fits;
   ExactArgs ea = exact_args(ref, X, method, p->dist);
   ea.ref_deg = ref_deg;
   ea.self_mode = plan.selfmode;
   ea.include_self = p->include_self;
 
   if (fused_ok_binary || fused_ok_cosine) {
-    const bool use_f32 = (prec == SD_PREC_FP32);
-    const int64_t kpad = round_up(std::max<int64_t>(ref->nrow, 1), kDenseRowAlign);
-    int kind;
-    if (method == SD_BINARY)
-      kind = use_f32 ? kDenseF32Pattern : kDenseI8Pattern;
-    else
-      kind = use_f32 ? kDenseF32Unit : kDenseBf16Unit;
     {
       std::lock_guard<std::mutex> lk(ref->prep_mu);
       SD_TRY(prep_dense(ctx, ref, kind, kpad));
@@ -352,6 +349,7 @@ int run_knn(sd_context* ctx, sd_matrix* X, sd_matrix* Y, const sd_knn_params* p,
     FusedArgs fa;
     fa.method = method;
     fa.prec = use_f32 ? SD_PREC_FP32 : SD_PREC_BF16;
+    fa.kind = kind;
     fa.nref = nref;
     fa.nqry = nq;
     fa.kpad = kpad;

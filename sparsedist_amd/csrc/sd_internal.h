@@ -126,7 +126,7 @@ struct sd_graph {
 namespace sd {
 
 // densified layout kinds
-enum DenseKind { kDenseI8Pattern = 0, kDenseBf16Unit = 1, kDenseBf16Raw = 2, kDenseF32Unit = 3,
+enum DenseKind { kDenseF4Pattern = 0, kDenseBf16Unit = 1, kDenseBf16Raw = 2, kDenseF32Unit = 3,
                  kDenseF32Raw = 4, kDenseF32Pattern = 5, kDenseKinds = 6 };
 
 // ---- prep (k_prep.hip) ------------------------------------------------------
@@ -165,6 +165,7 @@ int topk_dense(sd_context* ctx, const double* D, int64_t nrow, int64_t ncol, int
 struct FusedArgs {
   int method = 0;               // SD_BINARY / SD_COSINE / SD_EUCLIDEAN / SD_PEARSON
   int prec = SD_PREC_BF16;
+  int kind = kDenseBf16Unit;    // densified layout of ref_dense / qry_dense
   int64_t nref = 0, nqry = 0, kpad = 0;
   const void* ref_dense = nullptr;
   const void* qry_dense = nullptr;
@@ -202,14 +203,20 @@ struct RerankArgs {
   int map_words = 0;           // LDS row bitmap words (0: merge walk)
 };
 int rerank_map_words(int64_t nrow);
-// Densified columns are padded with zero rows to a multiple of this many
-// rows, so that every slab kernel's K loop runs whole groups of chunks (the
-// filtered pass streams 64-byte chunks in groups of four).
-constexpr int64_t kDenseRowAlign = 256;
+// Densified columns are padded with zero rows so that every slab kernel's
+// K loop runs whole groups of chunks: a padded column is a multiple of 256
+// bytes (the filtered pass streams 64-byte chunks in groups of four) and of
+// 256 rows.  The FP4 pattern layout packs two rows per byte.
+inline int64_t dense_row_align(int kind) { return kind == kDenseF4Pattern ? 512 : 256; }
+inline int64_t dense_row_bytes(int kind, int64_t kpad) {
+  return kind == kDenseF4Pattern ? kpad / 2
+         : (kind == kDenseBf16Unit || kind == kDenseBf16Raw) ? kpad * 2
+                                                             : kpad * 4;
+}
 int fused_list_len(int want);
-// Whether a slab tile of 256 rows of kpad elements of elem_bytes stays within
-// the 32-bit buffer offsets of the slab kernel.
-bool fused_fits(int64_t kpad, int elem_bytes);
+// Whether a slab tile of 256 columns of row_bytes each stays within the
+// 32-bit buffer offsets of the slab kernels.
+bool fused_fits(int64_t row_bytes);
 // Filter mode: every (query tile, reference tile) once; candidates with key
 // >= thr[query] are appended to fkey/fidx ([nqry][cap]), counted in cnt.
 int fused_filter_pass(sd_context* ctx, const FusedArgs& a, const double* thr, int cap,

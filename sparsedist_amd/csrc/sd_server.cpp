@@ -41,11 +41,14 @@ int guarded_call(F&& f) {
 
 struct Request {
   int64_t ticket = 0;
-  // inputs (host copies)
-  int64_t nrow = 0, ncol = 0;
-  std::vector<int64_t> colptr;
-  std::vector<int32_t> rowind;
-  std::vector<double> values;
+  // inputs: host copies, or the caller's arrays when borrowed
+  int64_t nrow = 0, ncol = 0, nnz = 0;
+  std::vector<int64_t> own_p;
+  std::vector<int32_t> own_i;
+  std::vector<double> own_x;
+  const int64_t* colptr = nullptr;
+  const int32_t* rowind = nullptr;
+  const double* values = nullptr;
   sd_knn_params knn{};
   int snn = 0;
   double prune = 0.0;
@@ -96,14 +99,11 @@ namespace {
 // and up to n k^2 candidate pairs, sorted with keys and values).
 int64_t reserve_estimate(const sd_server* s, const Request& r) {
   const int64_t cols = std::min<int64_t>(r.ncol, s->chunk);
-  const int64_t kpad =
-      (std::max<int64_t>(r.nrow, 1) + sd::kDenseRowAlign - 1) / sd::kDenseRowAlign *
-      sd::kDenseRowAlign;
+  const int64_t kpad = (std::max<int64_t>(r.nrow, 1) + 511) / 512 * 512;  // widest alignment
   const int64_t k = std::max(r.knn.k, 1);
   const int64_t L = 64;
   const int64_t cap = 4096;
-  const int64_t nnz = r.colptr.empty() ? 0 : r.colptr.back();
-  const int64_t chunk_nnz = r.ncol > 0 ? (nnz + r.ncol - 1) / r.ncol * cols : 0;
+  const int64_t chunk_nnz = r.ncol > 0 ? (r.nnz + r.ncol - 1) / r.ncol * cols : 0;
   int64_t b = cols * kpad * 2              // densified query columns (BF16)
               + cols * cap * 12            // candidate buffer (key + index)
               + cols * L * 12 * 4          // candidate lists and merges
@@ -159,8 +159,8 @@ int run_item(sd_server* s, sd_context* ctx, Request* r, int64_t c0, int64_t c1) 
   std::vector<int64_t> p((size_t)nc + 1);
   for (int64_t j = 0; j <= nc; ++j) p[j] = r->colptr[c0 + j] - base;
   sd_matrix* X = nullptr;
-  SD_TRY(sd_matrix_upload_csc(ctx, r->nrow, nc, p.data(), r->rowind.data() + base,
-                              r->values.data() + base, &X));
+  SD_TRY(sd_matrix_upload_csc(ctx, r->nrow, nc, p.data(), r->rowind + base, r->values + base,
+                              &X));
   const int32_t k = r->knn.k;
   std::vector<int32_t> idx((size_t)nc * k);
   std::vector<double> dist((size_t)nc * k);
@@ -310,14 +310,32 @@ int sd_server_submit(sd_server* s, const sd_request* req, int64_t* ticket) {
                            "'k' must be a positive integer no larger than .Machine$integer.max.");
     if (req->snn && !(req->prune >= 0.0 && req->prune <= 1.0))
       return sd::set_error(SD_EINVAL, "'prune' must be a finite number between 0 and 1.");
+    {
+      // refuse before copying anything when the queue is full
+      std::lock_guard<std::mutex> lk(s->mu);
+      if (s->max_pending > 0 && (int64_t)s->pending.size() >= s->max_pending) {
+        ++s->st.refused;
+        return sd::set_error(SD_EBUDGET, "server queue full (%lld requests waiting)",
+                             (long long)s->pending.size());
+      }
+    }
     auto r = std::make_unique<Request>();
     r->nrow = req->nrow;
     r->ncol = req->ncol;
-    const int64_t nnz = req->colptr[req->ncol];
-    r->colptr.assign(req->colptr, req->colptr + req->ncol + 1);
-    if (nnz > 0) {
-      r->rowind.assign(req->rowind, req->rowind + nnz);
-      r->values.assign(req->values, req->values + nnz);
+    r->nnz = req->colptr[req->ncol];
+    if (req->borrow) {
+      r->colptr = req->colptr;
+      r->rowind = req->rowind;
+      r->values = req->values;
+    } else {
+      r->own_p.assign(req->colptr, req->colptr + req->ncol + 1);
+      if (r->nnz > 0) {
+        r->own_i.assign(req->rowind, req->rowind + r->nnz);
+        r->own_x.assign(req->values, req->values + r->nnz);
+      }
+      r->colptr = r->own_p.data();
+      r->rowind = r->own_i.data();
+      r->values = r->own_x.data();
     }
     r->knn = req->knn;
     r->snn = req->snn;

@@ -6,6 +6,8 @@
   explicit zero is stored), values NB(mu=2, size=0.5) truncated at zero, then
   log1p(1e4 * x / colsum) per column (the workload described by the
   reference's bench generator, bench/R/03-datasets.R:186-235).
+- ``nb_log1p_parallel``: the same distribution generated by worker processes
+  (for the 20000 x 500000 serving reference).
 - ``abs_normal``: the same placement with |N(0, 1)| values (the reference
   tests' ``abs(rsparsematrix(...))`` shape, tests/testthat/test-sparseKNN.R:7-12).
 - ``unit_normal_dense``: dense N(0, 1) columns normalised to unit length in FP32.
@@ -50,6 +52,77 @@ def nb_log1p(nrow, ncol, density, seed):
     colsum = np.where(np.diff(indptr) > 0, colsum, 1.0)
     scale = np.repeat(1e4 / colsum, np.diff(indptr))
     vals = np.log1p(x * scale)
+    return sp.csc_matrix((vals, rows, indptr), shape=(nrow, ncol))
+
+
+def _nb_block(args):
+    """Worker of nb_log1p_parallel: columns [c0, c1) written into shared memory."""
+    from multiprocessing import shared_memory
+    (nrow, c0, c1, counts, seed, shm_rows, shm_vals, nnz_total, off) = args
+    rng = np.random.default_rng(seed)
+    n = c1 - c0
+    indptr = np.zeros(n + 1, dtype=np.int64)
+    np.cumsum(counts, out=indptr[1:])
+    rows = np.empty(int(indptr[-1]), dtype=np.int32)
+    for b0 in range(0, n, 1024):
+        b1 = min(n, b0 + 1024)
+        need = counts[b0:b1].astype(np.int64)
+        keys = np.empty(0, dtype=np.int64)
+        while True:
+            cols = np.repeat(np.arange(b0, b1, dtype=np.int64), need)
+            r = rng.integers(0, nrow, size=cols.size, dtype=np.int64)
+            keys = np.unique(np.concatenate((keys, cols * nrow + r)))
+            got = np.bincount(keys // nrow - b0, minlength=b1 - b0)
+            need = counts[b0:b1] - got
+            if not need.any():
+                break
+        rows[indptr[b0]:indptr[b1]] = (keys % nrow).astype(np.int32)
+    x = _nb_truncated(rng, rows.size)
+    colsum = np.add.reduceat(x, indptr[:-1]) if x.size else np.zeros(n)
+    colsum = np.where(np.diff(indptr) > 0, colsum, 1.0)
+    vals = np.log1p(x * np.repeat(1e4 / colsum, np.diff(indptr)))
+    a = shared_memory.SharedMemory(name=shm_rows)
+    b = shared_memory.SharedMemory(name=shm_vals)
+    np.ndarray(nnz_total, dtype=np.int32, buffer=a.buf)[off:off + rows.size] = rows
+    np.ndarray(nnz_total, dtype=np.float64, buffer=b.buf)[off:off + rows.size] = vals
+    a.close()
+    b.close()
+    return rows.size
+
+
+def nb_log1p_parallel(nrow, ncol, density, seed, workers=16):
+    """nb_log1p's distribution for very large matrices, generated by worker
+    processes over column ranges (within a column: rows drawn with
+    replacement, duplicates dropped and redrawn, which leaves a uniformly
+    random subset of the column's count).  Deterministic for a given
+    (seed, workers).  Call it before the process initialises the GPU: the
+    workers are forked."""
+    import multiprocessing as mp
+    from multiprocessing import shared_memory
+    ss = np.random.SeedSequence(seed)
+    rng = np.random.default_rng(ss.spawn(1)[0])
+    nnz = int(round(density * nrow * ncol))
+    counts = np.minimum(rng.multinomial(nnz, np.full(ncol, 1.0 / ncol)), nrow).astype(np.int64)
+    indptr = np.zeros(ncol + 1, dtype=np.int64)
+    np.cumsum(counts, out=indptr[1:])
+    total = int(indptr[-1])
+    a = shared_memory.SharedMemory(create=True, size=max(total * 4, 8))
+    b = shared_memory.SharedMemory(create=True, size=max(total * 8, 8))
+    try:
+        bounds = np.linspace(0, ncol, workers + 1).astype(np.int64)
+        seeds = ss.spawn(workers + 1)[1:]
+        jobs = [(nrow, int(bounds[w]), int(bounds[w + 1]), counts[bounds[w]:bounds[w + 1]],
+                 seeds[w], a.name, b.name, total, int(indptr[bounds[w]]))
+                for w in range(workers) if bounds[w + 1] > bounds[w]]
+        with mp.get_context("fork").Pool(len(jobs)) as pool:
+            pool.map(_nb_block, jobs)
+        rows = np.ndarray(total, dtype=np.int32, buffer=a.buf).copy()
+        vals = np.ndarray(total, dtype=np.float64, buffer=b.buf).copy()
+    finally:
+        a.close()
+        a.unlink()
+        b.close()
+        b.unlink()
     return sp.csc_matrix((vals, rows, indptr), shape=(nrow, ncol))
 
 
