@@ -210,6 +210,17 @@ def main():
                     unit="TFLOP/s", frac=(flop / (cos_ms / 1e3)) / MI355X_PEAK["bf16"],
                     traffic=None, kernel="k_filter_wide<OpBf16, float> (BF16 cosine filtered pass)",
                     avg_launch_ms=cos_ms)
+    # HBM bytes per launch (reads + writes) from the committed rocprofv3 PMC
+    # runs of this workload (FETCH_SIZE / WRITE_SIZE, copy-calibrated)
+    tpath = os.path.join(ROOT, "profiles", "traffic_count_matrix_20000x100000.json")
+    if (args.nrow, args.ncol, args.density, k) == (20_000, 100_000, 0.06, 20) and \
+            os.path.exists(tpath):
+        tr = json.load(open(tpath))
+        for roof, key in ((roof_bin, "void k_filter_wide<OpF4, KeyJ>"),
+                          (roof_cos, "void k_filter_wide<OpBf16, float>")):
+            if key in tr:
+                roof["traffic"] = tr[key]["fetch_bytes"] + tr[key]["write_bytes"]
+                roof["traffic_unit"] = "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE)"
     dominant = roof_bin if bin_ms >= cos_ms else roof_cos
     edges = 2 * nq * k
     snn_s = med("binary", "t_snn") + med("cosine", "t_snn")
