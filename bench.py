@@ -114,6 +114,8 @@ def main():
                     help="seconds of request arrivals in the serving run (0: skip it)")
     ap.add_argument("--serve", action="store_true",
                     help="serving benchmark only, on the 20000 x 500000 resident reference")
+    ap.add_argument("--cross", action="store_true",
+                    help="dense-embedding cross kNN only (E 256 x 200000, Y 256 x 1000000)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one plain 1 GiB device copy (calibrates rocprofv3's "
                          "FETCH_SIZE / WRITE_SIZE counters against known bytes)")
@@ -138,6 +140,8 @@ def main():
     workers = max(1, min(16, len(os.sched_getaffinity(0))))
     if args.serve:
         return serve_main(args, sd, synth, workers)
+    if args.cross:
+        return cross_main(args, sd, synth)
     t0 = time.time()
     X = synth.nb_log1p(args.nrow, args.ncol, args.density, seed=20)
     log(f"generated X {X.shape} nnz={X.nnz} in {time.time() - t0:.1f}s")
@@ -327,6 +331,78 @@ def serve_main(args, sd, synth, workers):
         "runs": runs,
         "vs_plain_fp32": (a["served_cols_per_s"] / runs[1]["served_cols_per_s"])
         if len(runs) > 1 else None,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def cross_main(args, sd, synth):
+    """sparseKNNCross(E, Y, k=10, method="cosine", dist=False) on dense embeddings:
+    E 256 x 200000 and Y 256 x 1000000, N(0,1) FP32 columns unit-normalised."""
+    t0 = time.time()
+    E = synth.unit_normal_dense(256, 200_000, 60)
+    Y = synth.unit_normal_dense(256, 1_000_000, 61)
+    log(f"generated E {E.shape} and Y {Y.shape} in {time.time() - t0:.1f}s")
+    ctx = sd.default_context()
+    dE = sd.DeviceMatrix(E.astype(np.float64), ctx)
+    dY = sd.DeviceMatrix(Y.astype(np.float64), ctx)
+    k = 10
+
+    def run(precision):
+        dE.drop_cache()
+        dY.drop_cache()
+        ctx.stats(reset=True)
+        ctx.set_timing(True)
+        t = time.perf_counter()
+        cr = sd.sparseKNNCross(dE, dY, k=k, method="cosine", dist=False, precision=precision,
+                               verbose=False, context=ctx)
+        dt = time.perf_counter() - t
+        st = ctx.stats(reset=True)
+        ctx.set_timing(False)
+        return cr, dt, st
+
+    run("auto")  # warm-up (code objects, allocations)
+    cr, dt, st = run("auto")
+    nq = 200_000 + 1_000_000
+    log(f"cross auto: {dt * 1e3:.0f} ms for both directions ({nq / dt:.0f} query cols/s); "
+        f"certified {st['certified_queries']}, escalated {st['escalated_queries']}, "
+        f"fallback {st['fallback_queries']}")
+    plain = None
+    if not args.no_plain:
+        cp, dtp, _ = run("fp32")
+        same = (np.array_equal(cp["forward"].idx, cr["forward"].idx)
+                and np.array_equal(cp["reverse"].idx, cr["reverse"].idx))
+        plain = dict(seconds=dtp, cols_per_s=nq / dtp, idx_identical=bool(same))
+        log(f"cross plain fp32: {dtp * 1e3:.0f} ms")
+    flop = 2.0 * 256 * 200_000 * 1_000_000 * 2  # both directions
+    fms = st["filter_ms"] / max(st["filter_launches"], 1)
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as O
+        import scipy.sparse as sp
+        O.build()
+        nt = min(16, os.cpu_count() or 1)
+        ns = 16
+        t = time.perf_counter()
+        O.knn(sp.csc_matrix(E[:, :ns].astype(np.float64)), sp.csc_matrix(Y.astype(np.float64)),
+              k=k, method="cosine", dist=False, nthreads=nt)
+        tc = time.perf_counter() - t
+        cpu = dict(value=ns / tc, unit="query cols/s", cores=nt, kind="port",
+                   sample=f"{ns} E columns x all 1,000,000 Y columns (forward direction), "
+                          f"oracle/sd_oracle.c with OpenMP ({tc:.1f} s)")
+    out = {
+        "metric": "exact cross-kNN query cols/s (k=10, both directions)",
+        "value": nq / dt, "unit": "query cols/s", "n_gpus": 1, "higher_is_better": True,
+        "ms": dt * 1e3, "dtype": "bf16 slab, fp64 re-rank", "data": "synthetic N(0,1) unit columns",
+        "config": {"workload": "dense-embedding cross kNN: E 256 x 200000, Y 256 x 1000000, "
+                               "cosine, dist=FALSE, k=10, forward + reverse"},
+        "roofline": dict(bound="mfma", achieved=(flop / 2) / (fms / 1e3) / 1e12, peak=2500.0,
+                         unit="TFLOP/s", frac=(flop / 2) / (fms / 1e3) / MI355X_PEAK["bf16"],
+                         traffic=None, kernel="k_filter_wide<OpBf16, float> (per direction)",
+                         avg_launch_ms=fms),
+        "certified": st["certified_queries"], "escalated": st["escalated_queries"],
+        "fallback": st["fallback_queries"],
+        "plain_fp32": plain, "vs_plain_fp32": (dtp / dt) if plain else None,
+        "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
 

@@ -506,14 +506,11 @@ __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
   constexpr int A_INST = A_BYTES / 1024 / 8;  // 2
   constexpr int B_INST = B_BYTES / 1024 / 8;  // 2
   constexpr int INST = A_INST + B_INST;
-  // Exchange rows of 32 words per lane, column v stored at v ^ (row >> 1):
-  // the 64 rows of a wave fall on 64 distinct banks for every v.
-  static_assert((NT / 2) * 32 * 4 <= STAGE_BYTES, "exchange area must fit in one stage");
   __shared__ __align__(16) uint8_t st0[STAGE_BYTES];
   __shared__ __align__(16) uint8_t st1[STAGE_BYTES];
   __shared__ __align__(16) uint8_t st2[STAGE_BYTES];
   __shared__ __align__(16) uint8_t st3[STAGE_BYTES];
-  __shared__ int32_t s_rnnz[TM];
+  __shared__ __align__(16) int32_t s_rnnz[TM];
   __shared__ float s_raux[TM];
   __shared__ uint8_t s_rdeg[TM];
 
@@ -526,17 +523,6 @@ __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
   if (!map_tile(blockIdx.x, gridDim.x, P.qtiles, P.rtiles, qt, rt)) return;
   const int64_t q_base = (int64_t)qt * TN;
   const int64_t r_base = rt * TM;
-  const int64_t my_pos = q_base + wn * 64 + h * 32 + n;
-  const bool q_ok = my_pos < P.nqry;
-  const int64_t my_q = q_ok ? (P.qlist ? (int64_t)P.qlist[my_pos] : my_pos) : -1;
-  uint32_t nj = 0;
-  float qaux = 0.f;
-  double thr_q = 0.0;
-  if (q_ok) {
-    nj = (uint32_t)(P.qry_p[my_q + 1] - P.qry_p[my_q]);
-    if (P.qry_aux) qaux = P.qry_aux[my_q];
-    thr_q = P.thr[my_pos];
-  }
   for (int t = tid; t < TM; t += NT) {
     const int64_t r = r_base + t;
     if (r < P.nref) {
@@ -672,77 +658,115 @@ __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
   SD_WIDE_LAST(st3)
 #undef SD_WIDE_LAST
 #undef SD_WIDE_CHUNK
-  wait_vm<0>();  // the trailing refills land before the stages are reused
-  lds_barrier();
+  wait_vm<0>();  // no refill is left in flight when the workgroup ends
 
-  // ---- epilogue: this lane owns query my_pos and must see its 128 values.
-  // acc[b][j][r] belongs to ref 128 wm + 32b + 8(r/4) + 4h + (r%4), query
-  // block j.  Sub-phases 0-1 scan the lane's own block-h values (reference
-  // blocks 0-1, then 2-3), sub-phases 2-3 the block-h values held by lane
-  // l ^ 32.  Wave row wm exchanges through stage wm.
-  uint32_t* xch = reinterpret_cast<uint32_t*>(wm ? st1 : st0);
-  const int lt = tid & 255;
-  auto xcol = [](int row, int v) { return row * 32 + ((v ^ (row >> 1)) & 31); };
+  // ---- epilogue: straight from the accumulators, no exchange.  acc[b][j][r]
+  // is the value of reference 128 wm + 32b + 8(r/4) + 4h + (r%4) for query
+  // 64 wn + 32j + n: each lane tests its 128 values against the thresholds of
+  // its two queries with plain compares (most fail), runs the exact checks
+  // only on those that pass, and appends them with one atomic per (lane,
+  // query) after counting, so a wave never waits on a memory round trip per
+  // candidate.
+  // the lane's two queries: 64 wn + 32 j + n
+  int64_t qp[2], qc[2];
+  bool qv[2];
+  uint32_t qn[2];
+  float qa[2], thr_f[2];
+  double thr_d[2];
 #pragma unroll
-  for (int sub = 0; sub < 4; ++sub) {
-    const int phase = sub >> 1, b0 = (sub & 1) * 2;
-    const int wrow = (phase == 0) ? lt : (lt ^ 32);
+  for (int j = 0; j < 2; ++j) {
+    qp[j] = q_base + wn * 64 + j * 32 + n;
+    qv[j] = qp[j] < P.nqry;
+    qc[j] = qv[j] ? (P.qlist ? (int64_t)P.qlist[qp[j]] : qp[j]) : -1;
+    qn[j] = qv[j] ? (uint32_t)(P.qry_p[qc[j] + 1] - P.qry_p[qc[j]]) : 0u;
+    qa[j] = (qv[j] && P.qry_aux) ? P.qry_aux[qc[j]] : 0.f;
+    thr_d[j] = qv[j] ? P.thr[qp[j]] : INFINITY;
+    // float screen a little below the threshold; the exact test follows
+    const float t = (float)thr_d[j];
+    thr_f[j] = t - fabsf(t) * 9.5367431640625e-07f - 1e-30f;
+  }
+  const int64_t rg0 = r_base + wm * 128 + 4 * h;
+  // exact test of one value: reference rg (tile row rloc, ni stored entries)
+  // for query j; lanes of invalid queries have an infinite threshold
+  auto exact = [&](const float x, int j, int rloc, int ni, int64_t rg, double& kd) -> bool {
+    if (s_rdeg[rloc]) return false;
+    const bool self = P.self_mode && rg == qc[j];
+    if (self && !P.include_self) return false;
+    if constexpr (sizeof(K) == 8) {
+      const uint32_t cc = (uint32_t)x;
+      const uint32_t d = (uint32_t)ni + qn[j] - cc;
+      kd = (d == 0) ? 1.0 : key_to_double(KeyJ{cc, d});
+    } else {
+      float key = (P.method == SD_EUCLIDEAN) ? -(qa[j] + s_raux[rloc] - 2.f * x) : x;
+      if (self) key = 2.0f;  // restored self similarity
+      kd = (double)key;
+    }
+    return kd >= thr_d[j];
+  };
+  // cheap screen (may keep too much, never too little)
+  auto screen = [&](const float x, int j, int ni, int64_t rg) -> bool {
+    if (P.self_mode && rg == qc[j]) return true;
+    if constexpr (sizeof(K) == 8) {
+      // c / (ni + nj - c) >= thr  <=>  c >= thr (ni + nj - c)
+      const float d = (float)((uint32_t)ni + qn[j]) - x;
+      return x >= thr_f[j] * d || d <= 0.f;
+    } else if (P.method == SD_EUCLIDEAN) {
+      return true;
+    } else {
+      return x >= thr_f[j];
+    }
+  };
+  // pass 1: screen and test every value; remember the passing ones
+  uint64_t pm[2] = {0ull, 0ull};  // bit 16b + r
+  int cnt[2] = {0, 0};
 #pragma unroll
-    for (int bb = 0; bb < 2; ++bb)
+  for (int b = 0; b < 4; ++b)
 #pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        const auto v0 = acc[b0 + bb][0][r], v1 = acc[b0 + bb][1][r];
-        const auto x = (phase == 0) ? (h ? v1 : v0) : (h ? v0 : v1);
-        uint32_t bits;
-        if constexpr (sizeof(K) == 8)
-          bits = (uint32_t)x;  // exact integer count (int or float accumulator)
-        else
-          __builtin_memcpy(&bits, &x, 4);
-        xch[xcol(wrow, bb * 16 + r)] = bits;
-      }
-    lds_barrier();
-    const int hh = (phase == 0) ? h : 1 - h;
-    if (q_ok) {
-#pragma unroll 1
-      for (int v = 0; v < 32; ++v) {
-        const int b = b0 + (v >> 4), rr = v & 15;
-        const int rloc = wm * 128 + b * 32 + 8 * (rr >> 2) + 4 * hh + (rr & 3);
-        const int64_t rg = r_base + rloc;
-        if (s_rdeg[rloc] || (P.self_mode && rg == my_q && !P.include_self)) continue;
-        const uint32_t bits = xch[xcol(lt, v)];
-        double kd;
-        bool pass;
-        if constexpr (sizeof(K) == 8) {
-          const uint32_t cc = bits;
-          const uint32_t d = (uint32_t)s_rnnz[rloc] + nj - cc;
-          const KeyJ key = (d == 0) ? KeyJ{1u, 1u} : KeyJ{cc, d};
-          // cheap float screen (errors far below 2^-20), then the exact rational
-          const float f = (float)key.c / (float)key.d;
-          pass = f >= (float)thr_q * (1.0f - 9.5367431640625e-07f);
-          kd = 0.0;
-          if (pass) {
-            kd = key_to_double(key);
-            pass = kd >= thr_q;
-          }
-        } else {
-          float sv;
-          __builtin_memcpy(&sv, &bits, 4);
-          float key = (P.method == SD_EUCLIDEAN) ? -(qaux + s_raux[rloc] - 2.f * sv) : sv;
-          if (P.self_mode && rg == my_q) key = 2.0f;  // restored self similarity
-          kd = (double)key;
-          pass = kd >= thr_q;
-        }
-        if (pass) {
-          const int at = atomicAdd(&P.cnt[my_pos], 1);
-          if (at < P.cap) {
-            P.fkey[my_pos * P.cap + at] = kd;
-            P.fidx[my_pos * P.cap + at] = (int32_t)rg;
+    for (int q4 = 0; q4 < 4; ++q4) {
+      // the four references 8 q4 + 4h + 0..3 of block b are consecutive
+      const int rl = wm * 128 + b * 32 + 8 * q4 + 4 * h;
+      int4 nn = make_int4(0, 0, 0, 0);
+      if constexpr (sizeof(K) == 8) nn = *reinterpret_cast<const int4*>(&s_rnnz[rl]);
+      const int nis[4] = {nn.x, nn.y, nn.z, nn.w};
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int r = 4 * q4 + e;
+        const int64_t rg = rg0 + b * 32 + 8 * q4 + e;
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          const float x = acc[b][j][r];
+          if (screen(x, j, nis[e], rg)) {
+            double kd;
+            if (exact(x, j, rl + e, nis[e], rg, kd)) {
+              pm[j] |= 1ull << (16 * b + r);
+              ++cnt[j];
+            }
           }
         }
       }
     }
-    lds_barrier();
-  }
+  if (cnt[0] + cnt[1] == 0) return;
+  // pass 2: reserve with one atomic per query, write the passing values
+  int at[2];
+#pragma unroll
+  for (int j = 0; j < 2; ++j) at[j] = cnt[j] ? atomicAdd(&P.cnt[qp[j]], cnt[j]) : 0;
+#pragma unroll
+  for (int b = 0; b < 4; ++b)
+#pragma unroll
+    for (int r = 0; r < 16; ++r)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        if (!((pm[j] >> (16 * b + r)) & 1ull)) continue;
+        const int rloc = wm * 128 + b * 32 + 8 * (r >> 2) + 4 * h + (r & 3);
+        const int64_t rg = rg0 + b * 32 + 8 * (r >> 2) + (r & 3);
+        double kd;
+        exact(acc[b][j][r], j, rloc, s_rnnz[rloc], rg, kd);
+        if (at[j] < P.cap) {
+          P.fkey[qp[j] * P.cap + at[j]] = kd;
+          P.fidx[qp[j] * P.cap + at[j]] = (int32_t)rg;
+        }
+        ++at[j];
+      }
 }
 
 // ---- merge per-split lists: top-L by (key desc, idx asc) --------------------

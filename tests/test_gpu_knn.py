@@ -145,6 +145,27 @@ def test_cross_equivalence(oracle_lib, gpu_ctx):
             same((cr["reverse"].idx, cr["reverse"].dist), (ri, rd), f"{method} oracle rev")
 
 
+def test_dense_embedding_cross(oracle_lib, gpu_ctx):
+    """The dense-embedding cross shape: unit-normalised N(0,1) FP32 columns
+    (signed values, so the re-rank certificate uses its general bound), cosine
+    similarity, both directions; the forward direction is large enough for the
+    threshold pre-pass + filtered pass."""
+    E = synth.unit_normal_dense(64, 600, 90).astype(np.float64)
+    Y = synth.unit_normal_dense(64, 20000, 91).astype(np.float64)
+    gpu_ctx.stats(reset=True)
+    gpu_ctx.set_timing(True)  # launches are counted while timing is on
+    try:
+        cr = sd.sparseKNNCross(E, Y, k=10, method="cosine", dist=False, verbose=False)
+    finally:
+        gpu_ctx.set_timing(False)
+    st = gpu_ctx.stats(reset=True)
+    (fi, fd), (ri, rd), _ = oracle_lib.knn_cross(sp.csc_matrix(E), sp.csc_matrix(Y), k=10,
+                                                  method="cosine", dist=False, nthreads=NT)
+    same((cr["forward"].idx, cr["forward"].dist), (fi, fd), "dense fwd")
+    same((cr["reverse"].idx, cr["reverse"].dist), (ri, rd), "dense rev")
+    assert st["filter_launches"] >= 1
+
+
 def test_topk_block_parity(oracle_lib, gpu_ctx):
     rng = np.random.default_rng(21)
     D = rng.integers(0, 6, size=(300, 40)).astype(float)
