@@ -197,6 +197,8 @@ int prep_dense(sd_context* ctx, sd_matrix* m, int kind, int64_t kpad) {
   const int64_t nchunks = (kpad + chunk - 1) / chunk;
   const int64_t grid = m->ncol * nchunks;
   const size_t lds = (size_t)dense_row_bytes(kind, kpad < chunk ? kpad : chunk);
+  if (grid * 256 >= ((int64_t)1 << 32))
+    return set_error(SD_EUNSUPPORTED, "densify grid too large (%lld workgroups)", (long long)grid);
   if (grid > 0) {
     switch (kind) {
       case kDenseF4Pattern:

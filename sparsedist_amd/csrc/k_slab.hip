@@ -520,7 +520,10 @@ __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
   const int h = lane >> 5, n = lane & 31;
   int qt;
   int64_t rt;
-  if (!map_tile(blockIdx.x, gridDim.x, P.qtiles, P.rtiles, qt, rt)) return;
+  // the grid is two-dimensional when one dimension cannot hold every tile
+  // (a dispatch counts work-items in 32 bits); dispatch order is x-major
+  const int64_t blk = (int64_t)blockIdx.y * gridDim.x + blockIdx.x;
+  if (!map_tile(blk, (int64_t)gridDim.x * gridDim.y, P.qtiles, P.rtiles, qt, rt)) return;
   const int64_t q_base = (int64_t)qt * TN;
   const int64_t r_base = rt * TM;
   for (int t = tid; t < TM; t += NT) {
@@ -1041,7 +1044,9 @@ __global__ __launch_bounds__(256) void k_rerank(RerankArgs R) {
 
 template <class Op, class K, int L, int WM>
 static int launch_fused_t(sd_context* ctx, FusedKernelArgs& P) {
-  const int nblk = P.qtiles * P.splits;
+  const int64_t nblk = (int64_t)P.qtiles * P.splits;
+  if (nblk * WM * 256 >= ((int64_t)1 << 32))
+    return set_error(SD_EUNSUPPORTED, "slab grid too large (%lld workgroups)", (long long)nblk);
   hipLaunchKernelGGL((k_fused<Op, K, L, WM>), dim3((unsigned)nblk), dim3(WM * 256), 0,
                      ctx->stream, P);
   SD_HIP(hipGetLastError());
@@ -1056,10 +1061,14 @@ static int launch_filter_t(sd_context* ctx, FusedKernelArgs& P) {
   P.qtiles = (int)((P.nqry + kWideTile - 1) / kWideTile);
   P.rtiles = (int)((P.nref + kWideTile - 1) / kWideTile);
   P.qgroups = (P.qtiles + 3) / 4;
-  int64_t nblk = (int64_t)P.qgroups * 4 * P.rtiles;
-  nblk = (nblk + 7) / 8 * 8;
-  if (nblk >= ((int64_t)1 << 31)) return set_error(SD_EUNSUPPORTED, "slab grid too large");
-  hipLaunchKernelGGL((k_filter_wide<Op, K>), dim3((unsigned)nblk), dim3(512), 0, ctx->stream, P);
+  const int64_t need = (int64_t)P.qgroups * 4 * P.rtiles;
+  // x: a multiple of 8 (whole rounds of the XCD round-robin) small enough that
+  // x * 512 work-items fit a dispatch's 32-bit grid size; y: the rest
+  const int64_t gx = std::min<int64_t>((need + 7) / 8 * 8, (int64_t)1 << 20);
+  const int64_t gy = (need + gx - 1) / gx;
+  if (gy > 65535) return set_error(SD_EUNSUPPORTED, "slab grid too large (%lld tiles)", (long long)need);
+  hipLaunchKernelGGL((k_filter_wide<Op, K>), dim3((unsigned)gx, (unsigned)gy), dim3(512), 0,
+                     ctx->stream, P);
   SD_HIP(hipGetLastError());
   return SD_OK;
 }

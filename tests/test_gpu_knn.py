@@ -166,6 +166,22 @@ def test_dense_embedding_cross(oracle_lib, gpu_ctx):
     assert st["filter_launches"] >= 1
 
 
+def test_filtered_pass_two_dimensional_grid(oracle_lib, gpu_ctx):
+    """More (query tile, reference tile) pairs than one grid dimension holds
+    (1100 x 1100 tiles of 256 > 2^20): the filtered pass runs on a 2-D grid
+    and must still visit every tile.  Per-query results do not depend on the
+    other queries, so a seeded sample of queries is checked against the oracle."""
+    n = 256 * 1100
+    Q = synth.unit_normal_dense(32, n, 92).astype(np.float64)
+    R = synth.unit_normal_dense(32, n, 93).astype(np.float64)
+    g = sd.sparseKNN(Q, R, k=5, method="cosine", verbose=False)
+    rng = np.random.default_rng(94)
+    sample = np.sort(rng.choice(n, size=160, replace=False))
+    oi, od, _ = oracle_lib.knn(sp.csc_matrix(Q[:, sample]), sp.csc_matrix(R), k=5,
+                               method="cosine", nthreads=NT)
+    same((g.idx[sample], g.dist[sample]), (oi, od), "2-D grid sample")
+
+
 def test_topk_block_parity(oracle_lib, gpu_ctx):
     rng = np.random.default_rng(21)
     D = rng.integers(0, 6, size=(300, 40)).astype(float)
