@@ -107,12 +107,41 @@ typedef struct {
   int64_t overflow_queries;   /* candidate buffer overflowed: full per-query pass */
   double filter_ms;           /* filtered full pass (not included in slab_ms) */
   int64_t filter_launches;
+  int64_t fused_calls;        /* searches run on the densified MFMA path */
+  int64_t exact_calls;        /* searches run on the FP64 merge walk */
+  int64_t last_margin;        /* cosine: re-rank margin m of the last fused search */
 } sd_stats;
 int sd_context_stats(sd_context* ctx, sd_stats* out, int reset);
 /* Enable/disable event timing of kernels (off by default). */
 int sd_context_set_timing(sd_context* ctx, int on);
 /* The context's hipStream_t, for callers that order their own work. */
 void* sd_context_stream(sd_context* ctx);
+
+/* ---- autotuning ------------------------------------------------------------ */
+/* With SD_PREC_AUTO each search picks its path from a cost model: the
+ * densified MFMA path costs a fixed overhead plus 2 r n_ref per query column
+ * (whole query tiles of 256) at the filtered pass's rate; the FP64 merge
+ * walk costs a smaller overhead plus n_ref (nnz_q + nnz_ref) merge steps per
+ * query column.  Both paths give identical results, so the choice never
+ * changes an answer.  For cosine the re-rank margin m adapts per reference
+ * matrix: a search whose certificate sends more than 1% of its queries to the
+ * wider second pass doubles m for the next search on that reference (up to
+ * 48), and a search with none lowers it by 8 (down to 16).  The reference
+ * has no counterpart; its bench chooses between dense and sparse inputs by a
+ * density sweep (bench/R/20-design.R:115-160).
+ * sd_autotune measures the model's rates on this device (about a second of
+ * work on synthetic inputs) and keeps them in the context; until it is
+ * called, measured MI355X defaults are used. */
+typedef struct {
+  double bf16_ops;   /* filtered-pass dense-equivalent ops/s, BF16 values */
+  double fp4_ops;    /* same, FP4 0/1 patterns */
+  double merge_steps; /* FP64 merge-walk steps/s of the exact slab */
+  double fused_fixed_ms;  /* per-search overhead of the densified path */
+  double exact_fixed_ms;  /* per-search overhead of the merge walk */
+  int32_t calibrated;     /* 1 once sd_autotune has measured them */
+  int32_t reserved;
+} sd_tune_info;
+int sd_autotune(sd_context* ctx, int32_t measure, sd_tune_info* out);
 
 /* ---- matrices ------------------------------------------------------------- */
 /* Copy a CSC matrix to device memory.  Replaces the per-call dgCMatrix ->

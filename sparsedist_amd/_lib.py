@@ -26,7 +26,7 @@ EXPORTS = (
     "sd_topk_block", "sd_topk_rows_accum", "sd_snn_jaccard", "sd_snn", "sd_graph_info",
     "sd_graph_copy", "sd_graph_free", "sd_detect_cores", "sd_server_create", "sd_server_submit",
     "sd_server_wait", "sd_server_error", "sd_server_result", "sd_server_release",
-    "sd_server_stats_get", "sd_server_destroy",
+    "sd_server_stats_get", "sd_server_destroy", "sd_autotune",
 )
 
 
@@ -51,7 +51,15 @@ class Stats(C.Structure):
                 ("snn_ms", C.c_double), ("snn_launches", C.c_int64),
                 ("fallback_queries", C.c_int64), ("certified_queries", C.c_int64),
                 ("escalated_queries", C.c_int64), ("overflow_queries", C.c_int64),
-                ("filter_ms", C.c_double), ("filter_launches", C.c_int64)]
+                ("filter_ms", C.c_double), ("filter_launches", C.c_int64),
+                ("fused_calls", C.c_int64), ("exact_calls", C.c_int64),
+                ("last_margin", C.c_int64)]
+
+
+class TuneInfo(C.Structure):
+    _fields_ = [("bf16_ops", C.c_double), ("fp4_ops", C.c_double), ("merge_steps", C.c_double),
+                ("fused_fixed_ms", C.c_double), ("exact_fixed_ms", C.c_double),
+                ("calibrated", C.c_int32), ("reserved", C.c_int32)]
 
 
 class ServerParams(C.Structure):
@@ -125,6 +133,7 @@ def load(path: str = LIB_PATH):
         L.sd_server_release.argtypes = [vp, i64]
         L.sd_server_stats_get.argtypes = [vp, C.POINTER(ServerStats)]
         L.sd_server_destroy.argtypes = [vp]
+        L.sd_autotune.argtypes = [vp, i32, C.POINTER(TuneInfo)]
         _lib = L
         return L
 

@@ -82,6 +82,13 @@ class Context:
     def stream(self) -> int:
         return _lib.load().sd_context_stream(self._h) or 0
 
+    def autotune(self, measure: bool = True) -> dict:
+        """Measure (or, with measure=False, report) the rates of the path
+        chooser's cost model on this device; see sd_autotune."""
+        t = _lib.TuneInfo()
+        check(_lib.load().sd_autotune(self._h, int(bool(measure)), C.byref(t)))
+        return {f: getattr(t, f) for f, _ in t._fields_ if f != "reserved"}
+
     def __del__(self):
         try:
             self.close()

@@ -320,7 +320,12 @@ int run_knn(sd_context* ctx, sd_matrix* X, sd_matrix* Y, const sd_knn_params* p,
   SD_TRY(out_val.ensure(sizeof(double) * (size_t)nq * (size_t)k + 64));
 
   int prec = p->precision;
+  // cosine re-rank margin: the caller's, else the one tuned on this reference
   int margin = p->margin > 0 ? p->margin : 16;
+  if (p->margin <= 0 && prec == SD_PREC_AUTO) {
+    std::lock_guard<std::mutex> lk(ref->prep_mu);
+    if (ref->tuned_margin > 0) margin = ref->tuned_margin;
+  }
   // densified layout: binary -> 0/1 patterns (FP4 nibbles, or FP32 on the
   // plain path); cosine -> unit-normalised values (BF16, or FP32)
   const bool use_f32 = (prec == SD_PREC_FP32);
@@ -328,16 +333,28 @@ int run_knn(sd_context* ctx, sd_matrix* X, sd_matrix* Y, const sd_knn_params* p,
                                        : (use_f32 ? kDenseF32Unit : kDenseBf16Unit);
   const int64_t kpad = round_up(std::max<int64_t>(ref->nrow, 1), dense_row_align(kind));
   const bool fits = fused_fits(dense_row_bytes(kind, kpad));
-  const bool fused_ok_binary = (method == SD_BINARY) && prec != SD_PREC_EXACT &&
-                               fused_list_len(k) > 0 && fits;
-  const bool fused_ok_cosine = (method == SD_COSINE) && prec != SD_PREC_EXACT &&
-                               fused_list_len(k + margin) > 0 && nref >= 1 && fits;
+  bool fused_ok_binary = (method == SD_BINARY) && prec != SD_PREC_EXACT &&
+                         fused_list_len(k) > 0 && fits;
+  bool fused_ok_cosine = (method == SD_COSINE) && prec != SD_PREC_EXACT &&
+                         fused_list_len(k + margin) > 0 && nref >= 1 && fits;
+  if (fused_ok_binary || fused_ok_cosine) {
+    // SD_PREC_AUTO: the cost model may prefer the merge walk (tiny batches
+    // against very sparse references); both paths give identical results
+    if (prec == SD_PREC_AUTO &&
+        tune_prefer_exact(ctx, kind, nq, nref, kpad,
+                          (double)X->nnz / (double)std::max<int64_t>(X->ncol, 1),
+                          (double)ref->nnz / (double)std::max<int64_t>(nref, 1))) {
+      fused_ok_binary = false;
+      fused_ok_cosine = false;
+    }
+  }
   ExactArgs ea = exact_args(ref, X, method, p->dist);
   ea.ref_deg = ref_deg;
   ea.self_mode = plan.selfmode;
   ea.include_self = p->include_self;
 
   if (fused_ok_binary || fused_ok_cosine) {
+    ++ctx->stats.fused_calls;
     {
       std::lock_guard<std::mutex> lk(ref->prep_mu);
       SD_TRY(prep_dense(ctx, ref, kind, kpad));
@@ -402,6 +419,16 @@ int run_knn(sd_context* ctx, sd_matrix* X, sd_matrix* Y, const sd_knn_params* p,
       SD_HIP(hipMemcpyAsync(&nfail, fail_count, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
       SD_HIP(hipStreamSynchronize(ctx->stream));
       const int32_t* fail_ids = fl.as<int32_t>();
+      ctx->stats.last_margin = margin;
+      if (prec == SD_PREC_AUTO && p->margin <= 0) {
+        // adapt this reference's margin: widen when the certificate sent more
+        // than 1% of the queries to the second pass, narrow after a clean search
+        std::lock_guard<std::mutex> lk(ref->prep_mu);
+        if ((int64_t)nfail * 100 > nq)
+          ref->tuned_margin = std::min(2 * margin, 48);
+        else if (nfail == 0)
+          ref->tuned_margin = std::max(margin - 8, 16);
+      }
       const int Lmax = fused_list_len(64);
       if (nfail > 0 && L < Lmax) {
         SD_TRY(check_cancel(ctx));
@@ -454,6 +481,7 @@ int run_knn(sd_context* ctx, sd_matrix* X, sd_matrix* Y, const sd_knn_params* p,
     }
   } else {
     // exact FP64 path over every query column
+    ++ctx->stats.exact_calls;
     DevBuf& qc = ctx->scratch[7];
     SD_TRY(qc.ensure(sizeof(int32_t) * (size_t)(nq + 16)));
     std::vector<int32_t> iota((size_t)nq);

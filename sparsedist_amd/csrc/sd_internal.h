@@ -72,8 +72,23 @@ struct Csc {
 // ---------------------------------------------------------------------------
 // opaque handles
 // ---------------------------------------------------------------------------
+namespace sd {
+// Rates of the path chooser's cost model (sd_autotune measures them; the
+// defaults were measured on an MI355X with the count-matrix workload).
+struct TuneRates {
+  double bf16_ops = 1.2e15;    // filtered pass, dense-equivalent ops/s
+  double fp4_ops = 3.5e15;
+  double fp32_ops = 1.2e14;
+  double merge_steps = 8.0e10; // exact slab merge-walk steps/s
+  double fused_fixed_s = 3.0e-3;
+  double exact_fixed_s = 0.4e-3;
+  int calibrated = 0;
+};
+}  // namespace sd
+
 struct sd_context {
   int device = 0;
+  sd::TuneRates tune;
   hipStream_t stream = nullptr;
   std::atomic<int> cancel{0};
   bool timing = false;
@@ -107,6 +122,8 @@ struct sd_matrix {
   // euclidean: per-column sum of squares (FP64, merge order irrelevant)
   bool sq_ready = false;
   sd::DevBuf sqn;
+  // cosine re-rank margin tuned on this matrix as a reference (0 = default)
+  int tuned_margin = 0;
   // densified layouts, one per kind: [ncol][kpad] column-contiguous
   int64_t dense_kpad[6] = {0, 0, 0, 0, 0, 0};
   sd::DevBuf dense[6];
@@ -213,6 +230,10 @@ inline int64_t dense_row_bytes(int kind, int64_t kpad) {
          : (kind == kDenseBf16Unit || kind == kDenseBf16Raw) ? kpad * 2
                                                              : kpad * 4;
 }
+// Cost-model choice between the densified path and the FP64 merge walk for
+// a search of nq query columns against nref reference columns.
+bool tune_prefer_exact(const sd_context* ctx, int kind, int64_t nq, int64_t nref, int64_t kpad,
+                       double nnz_q, double nnz_ref);
 int fused_list_len(int want);
 // Whether a slab tile of 256 columns of row_bytes each stays within the
 // 32-bit buffer offsets of the slab kernels.

@@ -1,0 +1,189 @@
+// sd_tune.cpp -- the path chooser's cost model and its on-device calibration.
+//
+// A search of nq query columns against nref reference columns with r rows
+// costs, on the densified MFMA path, a fixed overhead plus whole query tiles
+// of 256 columns times 2 r nref dense-equivalent operations at the filtered
+// pass's rate; on the FP64 merge walk it costs a smaller overhead plus
+// nq nref (nnz_q + nnz_ref) merge steps.  Both paths are exact, so the choice
+// changes only the time.  sd_autotune replaces the default rates by ones
+// measured on synthetic inputs of known size.
+#include <algorithm>
+#include <chrono>
+#include <cstdint>
+#include <vector>
+
+#include "sd_internal.h"
+
+namespace sd {
+
+bool tune_prefer_exact(const sd_context* ctx, int kind, int64_t nq, int64_t nref, int64_t kpad,
+                       double nnz_q, double nnz_ref) {
+  const TuneRates& t = ctx->tune;
+  const double rate = kind == kDenseF4Pattern   ? t.fp4_ops
+                      : kind == kDenseBf16Unit ? t.bf16_ops
+                                               : t.fp32_ops;
+  const double qcols = (double)((nq + 255) / 256 * 256);
+  const double fused = t.fused_fixed_s + qcols * (double)nref * 2.0 * (double)kpad / rate;
+  const double exact =
+      t.exact_fixed_s + (double)nq * (double)nref * (nnz_q + nnz_ref) / t.merge_steps;
+  return exact < fused;
+}
+
+namespace {
+
+// CSC with `per` distinct random rows per column (partial Fisher-Yates over a
+// running permutation, then sorted) and values in [0.5, 1.5).
+struct SynthCsc {
+  std::vector<int64_t> p;
+  std::vector<int32_t> i;
+  std::vector<double> x;
+};
+
+SynthCsc synth_csc(int64_t nrow, int64_t ncol, int per, uint64_t seed) {
+  SynthCsc m;
+  m.p.resize((size_t)ncol + 1);
+  m.i.resize((size_t)ncol * per);
+  m.x.resize((size_t)ncol * per);
+  std::vector<int32_t> perm((size_t)nrow);
+  for (int64_t r = 0; r < nrow; ++r) perm[(size_t)r] = (int32_t)r;
+  uint64_t s = seed * 0x9E3779B97F4A7C15ull + 1;
+  auto next = [&]() {
+    s ^= s << 13;
+    s ^= s >> 7;
+    s ^= s << 17;
+    return s;
+  };
+  for (int64_t j = 0; j < ncol; ++j) {
+    m.p[(size_t)j] = j * per;
+    for (int t = 0; t < per; ++t) {
+      const int64_t u = t + (int64_t)(next() % (uint64_t)(nrow - t));
+      std::swap(perm[(size_t)t], perm[(size_t)u]);
+    }
+    int32_t* rows = m.i.data() + j * per;
+    std::copy(perm.begin(), perm.begin() + per, rows);
+    std::sort(rows, rows + per);
+    for (int t = 0; t < per; ++t)
+      m.x[(size_t)(j * per + t)] = 0.5 + (double)(next() >> 11) * (1.0 / 9007199254740992.0);
+  }
+  m.p[(size_t)ncol] = ncol * per;
+  return m;
+}
+
+int upload(sd_context* ctx, int64_t nrow, const SynthCsc& m, sd_matrix** out) {
+  return sd_matrix_upload_csc(ctx, nrow, (int64_t)m.p.size() - 1, m.p.data(), m.i.data(),
+                              m.x.data(), out);
+}
+
+// Wall time of one sd_knn call (synchronous: results are copied to the host).
+int time_knn(sd_context* ctx, sd_matrix* Q, sd_matrix* Y, int method, int prec, double* sec) {
+  sd_knn_params p{};
+  p.k = 10;
+  p.method = method;
+  p.dist = 1;
+  p.drop_degenerate = 1;
+  p.precision = prec;
+  p.block_size = 4096;
+  std::vector<int32_t> idx((size_t)Q->ncol * p.k);
+  std::vector<double> dst((size_t)Q->ncol * p.k);
+  int32_t keff = 0, warn = 0;
+  const auto t0 = std::chrono::steady_clock::now();
+  SD_TRY(sd_knn(ctx, Q, Y, &p, idx.data(), dst.data(), &keff, &warn));
+  *sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return SD_OK;
+}
+
+int best_of(int reps, sd_context* ctx, sd_matrix* Q, sd_matrix* Y, int method, int prec,
+            double* sec) {
+  double best = 1e30;
+  for (int r = 0; r < reps; ++r) {
+    double t = 0;
+    SD_TRY(time_knn(ctx, Q, Y, method, prec, &t));
+    best = std::min(best, t);
+  }
+  *sec = best;
+  return SD_OK;
+}
+
+bool plausible(double v, double lo, double hi) { return v == v && v >= lo && v <= hi; }
+
+int calibrate(sd_context* ctx) {
+  constexpr int64_t kRows = 16384, kRef = 32768;
+  constexpr int kPer = 256;
+  const SynthCsc Yh = synth_csc(kRows, kRef, kPer, 1);
+  const SynthCsc Q1h = synth_csc(kRows, 256, kPer, 2);
+  const SynthCsc Q2h = synth_csc(kRows, 8192, kPer, 3);
+  const SynthCsc E1h = synth_csc(kRows, 8, kPer, 4);
+  const SynthCsc E2h = synth_csc(kRows, 40, kPer, 5);
+  sd_matrix *Y = nullptr, *Q1 = nullptr, *Q2 = nullptr, *E1 = nullptr, *E2 = nullptr;
+  auto release = [&]() {
+    for (sd_matrix* m : {Y, Q1, Q2, E1, E2})
+      if (m) sd_matrix_free(m);
+  };
+  int rc = SD_OK;
+  TuneRates t = ctx->tune;
+  do {
+    if ((rc = upload(ctx, kRows, Yh, &Y)) != SD_OK) break;
+    if ((rc = upload(ctx, kRows, Q1h, &Q1)) != SD_OK) break;
+    if ((rc = upload(ctx, kRows, Q2h, &Q2)) != SD_OK) break;
+    if ((rc = upload(ctx, kRows, E1h, &E1)) != SD_OK) break;
+    if ((rc = upload(ctx, kRows, E2h, &E2)) != SD_OK) break;
+    // densified path: two query batch sizes per kind; slope = rate, intercept = overhead
+    const int methods[2] = {SD_COSINE, SD_BINARY};
+    const int kinds[2] = {kDenseBf16Unit, kDenseF4Pattern};
+    for (int m = 0; m < 2 && rc == SD_OK; ++m) {
+      double w = 0, t1 = 0, t2 = 0;
+      if ((rc = time_knn(ctx, Q1, Y, methods[m], SD_PREC_BF16, &w)) != SD_OK) break;  // warm-up
+      if ((rc = best_of(3, ctx, Q1, Y, methods[m], SD_PREC_BF16, &t1)) != SD_OK) break;
+      if ((rc = best_of(3, ctx, Q2, Y, methods[m], SD_PREC_BF16, &t2)) != SD_OK) break;
+      const double kpad = (double)((kRows + dense_row_align(kinds[m]) - 1) /
+                                   dense_row_align(kinds[m]) * dense_row_align(kinds[m]));
+      const double w1 = 256.0 * kRef * 2.0 * kpad, w2 = 8192.0 * kRef * 2.0 * kpad;
+      const double rate = (w2 - w1) / std::max(t2 - t1, 1e-6);
+      if (plausible(rate, 1e12, 2e16)) {
+        (m == 0 ? t.bf16_ops : t.fp4_ops) = rate;
+        if (m == 0) t.fused_fixed_s = std::max(t1 - w1 / rate, 1e-4);
+      }
+    }
+    if (rc != SD_OK) break;
+    // merge walk: two small query batches on the exact path
+    double w = 0, e1 = 0, e2 = 0;
+    if ((rc = time_knn(ctx, E1, Y, SD_COSINE, SD_PREC_EXACT, &w)) != SD_OK) break;
+    if ((rc = best_of(3, ctx, E1, Y, SD_COSINE, SD_PREC_EXACT, &e1)) != SD_OK) break;
+    if ((rc = best_of(3, ctx, E2, Y, SD_COSINE, SD_PREC_EXACT, &e2)) != SD_OK) break;
+    const double s1 = 8.0 * kRef * 2.0 * kPer, s2 = 40.0 * kRef * 2.0 * kPer;
+    const double steps = (s2 - s1) / std::max(e2 - e1, 1e-6);
+    if (plausible(steps, 1e8, 1e14)) {
+      t.merge_steps = steps;
+      t.exact_fixed_s = std::max(e1 - s1 / steps, 5e-5);
+    }
+    t.calibrated = 1;
+  } while (false);
+  release();
+  if (rc == SD_OK) ctx->tune = t;
+  return rc;
+}
+
+}  // namespace
+}  // namespace sd
+
+extern "C" int sd_autotune(sd_context* ctx, int32_t measure, sd_tune_info* out) {
+  if (!ctx || !out) return sd::set_error(SD_EINVAL, "NULL argument");
+  if (measure) {
+    const sd_stats keep = ctx->stats;  // calibration runs are not the caller's searches
+    int rc;
+    try {
+      sd::clear_error();
+      rc = sd::calibrate(ctx);
+    } catch (const std::bad_alloc&) {
+      rc = sd::set_error(SD_ENOMEM, "host allocation failed");
+    } catch (...) {
+      rc = sd::set_error(SD_EDEVICE, "unexpected internal error");
+    }
+    ctx->stats = keep;
+    if (rc != SD_OK) return rc;
+  }
+  const sd::TuneRates& t = ctx->tune;
+  *out = sd_tune_info{t.bf16_ops, t.fp4_ops, t.merge_steps, t.fused_fixed_s * 1e3,
+                      t.exact_fixed_s * 1e3, t.calibrated, 0};
+  return SD_OK;
+}

@@ -65,7 +65,7 @@ def test_slab_bitwise(oracle_lib, gpu_ctx, method):
 
 
 @pytest.mark.parametrize("method", METHODS)
-@pytest.mark.parametrize("precision", ["auto", "exact", "fp32"])
+@pytest.mark.parametrize("precision", ["auto", "bf16", "exact", "fp32"])
 def test_knn_self_bitwise(oracle_lib, gpu_ctx, method, precision):
     X = rand_sparse(70, 45, 0.25, 3, signed=method in ("pearson", "covariance"),
                     empty_cols=(4, 9))
@@ -124,7 +124,7 @@ def test_knn_count_matrix_shape(oracle_lib, gpu_ctx, method, dist):
 def test_certificate_counts(gpu_ctx):
     X = synth.nb_log1p(20000, 1200, 0.06, 13)
     gpu_ctx.stats(reset=True)
-    sd.sparseKNN(X, k=20, method="cosine", verbose=False)
+    sd.sparseKNN(X, k=20, method="cosine", precision="bf16", verbose=False)
     st = gpu_ctx.stats(reset=True)
     assert st["certified_queries"] + st["fallback_queries"] == 1200
 
@@ -155,7 +155,8 @@ def test_dense_embedding_cross(oracle_lib, gpu_ctx):
     gpu_ctx.stats(reset=True)
     gpu_ctx.set_timing(True)  # launches are counted while timing is on
     try:
-        cr = sd.sparseKNNCross(E, Y, k=10, method="cosine", dist=False, verbose=False)
+        cr = sd.sparseKNNCross(E, Y, k=10, method="cosine", dist=False, precision="bf16",
+                               verbose=False)
     finally:
         gpu_ctx.set_timing(False)
     st = gpu_ctx.stats(reset=True)
@@ -180,6 +181,32 @@ def test_filtered_pass_two_dimensional_grid(oracle_lib, gpu_ctx):
     oi, od, _ = oracle_lib.knn(sp.csc_matrix(Q[:, sample]), sp.csc_matrix(R), k=5,
                                method="cosine", nthreads=NT)
     same((g.idx[sample], g.dist[sample]), (oi, od), "2-D grid sample")
+
+
+def test_autotune_path_choice(oracle_lib, gpu_ctx):
+    """sd_autotune measures plausible rates; with precision="auto" a single
+    query against a very sparse reference takes the FP64 merge walk and a
+    large batch the densified path, and both give the oracle's answer."""
+    rates = gpu_ctx.autotune()
+    assert rates["calibrated"] == 1
+    assert 1e13 < rates["bf16_ops"] < 5e15 and 1e13 < rates["fp4_ops"] < 2e16
+    assert 1e9 < rates["merge_steps"] < 1e13
+    Y = synth.nb_log1p(30000, 3000, 0.0003, 95)     # about 9 stored rows per column
+    X1 = synth.nb_log1p(30000, 1, 0.0003, 96)
+    Xb = synth.nb_log1p(30000, 3000, 0.02, 97)
+    Yb = synth.nb_log1p(30000, 3000, 0.02, 98)
+    for method in ("binary", "cosine"):
+        gpu_ctx.stats(reset=True)
+        g = sd.sparseKNN(X1, Y, k=5, method=method, verbose=False)
+        st = gpu_ctx.stats(reset=True)
+        assert st["exact_calls"] == 1 and st["fused_calls"] == 0, method
+        oi, od, _ = oracle_lib.knn(X1, Y, k=5, method=method, nthreads=NT)
+        same((g.idx, g.dist), (oi, od), f"{method} tiny batch")
+        g = sd.sparseKNN(Xb, Yb, k=5, method=method, verbose=False)
+        st = gpu_ctx.stats(reset=True)
+        assert st["fused_calls"] == 1 and st["exact_calls"] == 0, method
+        e = sd.sparseKNN(Xb, Yb, k=5, method=method, precision="exact", verbose=False)
+        same((g.idx, g.dist), (e.idx, e.dist), f"{method} batch vs exact path")
 
 
 def test_topk_block_parity(oracle_lib, gpu_ctx):
@@ -292,7 +319,7 @@ def test_certificate_escalation_and_fallback(oracle_lib, gpu_ctx):
     the exact FP64 fallback; every path must still match the oracle."""
     X = synth.nb_log1p(3000, 600, 0.05, 31)
     gpu_ctx.stats(reset=True)
-    g = sd.sparseKNN(X, k=6, method="cosine", margin=1, verbose=False)
+    g = sd.sparseKNN(X, k=6, method="cosine", margin=1, precision="bf16", verbose=False)
     st = gpu_ctx.stats(reset=True)
     o = oracle_lib.knn(X, k=6, method="cosine", nthreads=NT)
     same((g.idx, g.dist), o, "margin=1")
@@ -303,7 +330,7 @@ def test_certificate_escalation_and_fallback(oracle_lib, gpu_ctx):
     cols.append(synth.abs_normal(500, 100, 0.2, 34).toarray())
     Xd = sp.csc_matrix(np.hstack(cols))
     gpu_ctx.stats(reset=True)
-    g = sd.sparseKNN(Xd, k=10, method="cosine", verbose=False)
+    g = sd.sparseKNN(Xd, k=10, method="cosine", precision="bf16", verbose=False)
     st = gpu_ctx.stats(reset=True)
     o = oracle_lib.knn(Xd, k=10, method="cosine", nthreads=NT)
     same((g.idx, g.dist), o, "near duplicates")
@@ -323,7 +350,7 @@ def test_two_pass_filter_path(oracle_lib, gpu_ctx, method):
     Y = Yd.tocsc()
     X = Y[:, :300]
     gpu_ctx.stats(reset=True)
-    g = sd.sparseKNN(X, Y, k=20, method=method, verbose=False)
+    g = sd.sparseKNN(X, Y, k=20, method=method, precision="bf16", verbose=False)
     st = gpu_ctx.stats(reset=True)
     o = oracle_lib.knn(X, Y, k=20, method=method, nthreads=NT)
     same((g.idx, g.dist), o, method)
