@@ -114,6 +114,9 @@ def main():
                     help="seconds of request arrivals in the serving run (0: skip it)")
     ap.add_argument("--serve", action="store_true",
                     help="serving benchmark only, on the 20000 x 500000 resident reference")
+    ap.add_argument("--sweep", action="store_true",
+                    help="path-chooser sweep: densified vs merge walk vs auto over density "
+                         "and batch size (20000-row references of 20000 columns)")
     ap.add_argument("--large", action="store_true",
                     help="large sparse self-search only (30000 x 1000000, density 0.03)")
     ap.add_argument("--cross", action="store_true",
@@ -146,6 +149,8 @@ def main():
         return cross_main(args, sd, synth)
     if args.large:
         return large_main(args, sd, synth, workers)
+    if args.sweep:
+        return sweep_main(args, sd, synth, workers)
     t0 = time.time()
     X = synth.nb_log1p(args.nrow, args.ncol, args.density, seed=20)
     log(f"generated X {X.shape} nnz={X.nnz} in {time.time() - t0:.1f}s")
@@ -462,6 +467,54 @@ def large_main(args, sd, synth, workers):
         "methods": res,
     }
     print(json.dumps(out), flush=True)
+
+
+def sweep_main(args, sd, synth, workers):
+    """Times each search three ways (forced densified path, forced FP64 merge
+    walk, precision="auto") over reference density and query batch size, and
+    reports which path the cost model chose and whether it was the faster."""
+    dens = (0.0005, 0.002, 0.01, 0.06)
+    sizes = (1, 32, 2048)
+    refs = {d: synth.nb_log1p_parallel(20_000, 20_000, d, 80 + i, workers)
+            for i, d in enumerate(dens)}
+    qrys = {d: synth.nb_log1p_parallel(20_000, max(sizes), d, 90 + i, workers)
+            for i, d in enumerate(dens)}
+    ctx = sd.default_context()
+    rates = ctx.autotune()
+    log(f"calibrated rates: {rates}")
+    rows = []
+    for d in dens:
+        dY = sd.DeviceMatrix(refs[d], ctx)
+        for n in sizes:
+            X = qrys[d][:, :n]
+            for method in ("binary", "cosine"):
+                t = {}
+                for prec in ("bf16", "exact", "auto"):
+                    best = 1e30
+                    for _ in range(3):
+                        ctx.stats(reset=True)
+                        t0 = time.perf_counter()
+                        sd.sparseKNN(X, dY, k=10, method=method, precision=prec, verbose=False,
+                                     context=ctx)
+                        best = min(best, time.perf_counter() - t0)
+                        st = ctx.stats(reset=True)
+                    t[prec] = best
+                    if prec == "auto":
+                        chose = "exact" if st["exact_calls"] else "densified"
+                faster = "exact" if t["exact"] < t["bf16"] else "densified"
+                rows.append(dict(density=d, query_cols=n, method=method,
+                                 densified_ms=t["bf16"] * 1e3, merge_walk_ms=t["exact"] * 1e3,
+                                 auto_ms=t["auto"] * 1e3, auto_chose=chose,
+                                 faster=faster, right=(chose == faster)))
+                log(f"d={d} n={n} {method}: densified {t['bf16'] * 1e3:.2f} ms, merge walk "
+                    f"{t['exact'] * 1e3:.2f} ms, auto {t['auto'] * 1e3:.2f} ms ({chose})")
+        dY.free()
+    print(json.dumps({"metric": "path chooser sweep (ms per search)", "rates": rates,
+                      "config": {"workload": "references 20000 x 20000 NB/log1p at densities "
+                                             f"{list(dens)}; query batches {list(sizes)}; k=10"},
+                      "rows": rows,
+                      "right_choices": sum(r["right"] for r in rows), "cases": len(rows)}),
+          flush=True)
 
 
 if __name__ == "__main__":

@@ -136,7 +136,8 @@ typedef struct {
   double bf16_ops;   /* filtered-pass dense-equivalent ops/s, BF16 values */
   double fp4_ops;    /* same, FP4 0/1 patterns */
   double merge_steps; /* FP64 merge-walk steps/s of the exact slab */
-  double fused_fixed_ms;  /* per-search overhead of the densified path */
+  double bf16_fixed_ms;   /* per-search overhead of the densified cosine path */
+  double fp4_fixed_ms;    /* per-search overhead of the densified Jaccard path */
   double exact_fixed_ms;  /* per-search overhead of the merge walk */
   int32_t calibrated;     /* 1 once sd_autotune has measured them */
   int32_t reserved;

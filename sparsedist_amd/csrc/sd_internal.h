@@ -80,8 +80,10 @@ struct TuneRates {
   double fp4_ops = 3.5e15;
   double fp32_ops = 1.2e14;
   double merge_steps = 8.0e10; // exact slab merge-walk steps/s
-  double fused_fixed_s = 3.0e-3;
-  double exact_fixed_s = 0.4e-3;
+  double bf16_fixed_s = 1.5e-3;  // per-search overheads
+  double fp4_fixed_s = 0.5e-3;
+  double fp32_fixed_s = 1.5e-3;
+  double exact_fixed_s = 0.1e-3;
   int calibrated = 0;
 };
 }  // namespace sd

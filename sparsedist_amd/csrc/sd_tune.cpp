@@ -22,8 +22,11 @@ bool tune_prefer_exact(const sd_context* ctx, int kind, int64_t nq, int64_t nref
   const double rate = kind == kDenseF4Pattern   ? t.fp4_ops
                       : kind == kDenseBf16Unit ? t.bf16_ops
                                                : t.fp32_ops;
+  const double fixed = kind == kDenseF4Pattern   ? t.fp4_fixed_s
+                       : kind == kDenseBf16Unit ? t.bf16_fixed_s
+                                                : t.fp32_fixed_s;
   const double qcols = (double)((nq + 255) / 256 * 256);
-  const double fused = t.fused_fixed_s + qcols * (double)nref * 2.0 * (double)kpad / rate;
+  const double fused = fixed + qcols * (double)nref * 2.0 * (double)kpad / rate;
   const double exact =
       t.exact_fixed_s + (double)nq * (double)nref * (nnz_q + nnz_ref) / t.merge_steps;
   return exact < fused;
@@ -141,7 +144,7 @@ int calibrate(sd_context* ctx) {
       const double rate = (w2 - w1) / std::max(t2 - t1, 1e-6);
       if (plausible(rate, 1e12, 2e16)) {
         (m == 0 ? t.bf16_ops : t.fp4_ops) = rate;
-        if (m == 0) t.fused_fixed_s = std::max(t1 - w1 / rate, 1e-4);
+        (m == 0 ? t.bf16_fixed_s : t.fp4_fixed_s) = std::max(t1 - w1 / rate, 1e-4);
       }
     }
     if (rc != SD_OK) break;
@@ -183,7 +186,8 @@ extern "C" int sd_autotune(sd_context* ctx, int32_t measure, sd_tune_info* out) 
     if (rc != SD_OK) return rc;
   }
   const sd::TuneRates& t = ctx->tune;
-  *out = sd_tune_info{t.bf16_ops, t.fp4_ops, t.merge_steps, t.fused_fixed_s * 1e3,
-                      t.exact_fixed_s * 1e3, t.calibrated, 0};
+  *out = sd_tune_info{t.bf16_ops,          t.fp4_ops,     t.merge_steps,
+                      t.bf16_fixed_s * 1e3, t.fp4_fixed_s * 1e3, t.exact_fixed_s * 1e3,
+                      t.calibrated,         0};
   return SD_OK;
 }
