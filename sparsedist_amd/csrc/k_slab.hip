@@ -490,26 +490,40 @@ __global__ __launch_bounds__(WM * 256, 1) void k_fused(FusedKernelArgs P) {
 // K streams through four LDS stages of 64-byte row chunks (A and B 16 KB
 // each); a stage is refilled with the chunk four ahead as soon as every wave
 // has read it, so up to three chunks (96 KB) are in flight per CU.
-constexpr int kWideTile = 256;     // references and queries per workgroup
-constexpr int kWideRowBytes = 64;  // K bytes per row per chunk
+constexpr int kWideTile = 256;  // references and queries per workgroup
 
-template <class Op, class K>
+// Staging layouts of the filtered pass by K bytes per row per chunk (RB):
+// RB = 64 runs four 32 KB stages, RB = 128 two 64 KB stages.  A wave
+// instruction writes 1 KB = (1024 / RB) rows x RB bytes; the 16-byte slot of
+// logical segment s in row R is s ^ ((R / (256 / RB)) % (RB / 16)), so 16
+// lanes reading one segment of 16 consecutive rows hit all 64 banks once.
+template <int RB>
+struct WideLayout {
+  static_assert(RB == 64 || RB == 128, "64- or 128-byte chunks");
+  static constexpr int kStages = RB == 64 ? 4 : 2;
+  static constexpr int kSlots = RB / 16;          // 16-byte segments per row chunk
+  static constexpr int kRowsPerInst = 64 / kSlots;
+  static constexpr int kSteps = RB / 32;          // MFMA steps per chunk
+  static constexpr int kRowShift = RB == 64 ? 2 : 1;  // rows per 256 bytes = 1 << shift
+  __device__ static int swz(int row) { return (row >> kRowShift) & (kSlots - 1); }
+};
+
+template <class Op, class K, int RB>
 __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
+  using Lay = WideLayout<RB>;
   constexpr int NT = 512;
-  constexpr int TM = kWideTile, TN = kWideTile, RB = kWideRowBytes;
-  constexpr int A_BYTES = TM * RB;  // 16 KB
-  constexpr int B_BYTES = TN * RB;  // 16 KB
-  constexpr int STAGE_BYTES = A_BYTES + B_BYTES;
-  // Each wave instruction writes 1 KB = 16 rows x 64 B.  The 16-byte slot of
-  // logical segment s in row R is s ^ ((R >> 2) & 3): 16 lanes reading one
-  // segment of 16 consecutive rows then cover all 64 banks once.
-  constexpr int A_INST = A_BYTES / 1024 / 8;  // 2
-  constexpr int B_INST = B_BYTES / 1024 / 8;  // 2
+  constexpr int NS = Lay::kStages, STEPS = Lay::kSteps;
+  constexpr int TM = kWideTile, TN = kWideTile;
+  constexpr int A_BYTES = TM * RB;
+  constexpr int B_BYTES = TN * RB;
+  constexpr int STAGE_BYTES = A_BYTES + B_BYTES;  // 32 KB or 64 KB
+  constexpr int A_INST = A_BYTES / 1024 / 8;      // per wave: 2 or 4
+  constexpr int B_INST = B_BYTES / 1024 / 8;
   constexpr int INST = A_INST + B_INST;
   __shared__ __align__(16) uint8_t st0[STAGE_BYTES];
   __shared__ __align__(16) uint8_t st1[STAGE_BYTES];
-  __shared__ __align__(16) uint8_t st2[STAGE_BYTES];
-  __shared__ __align__(16) uint8_t st3[STAGE_BYTES];
+  __shared__ __align__(16) uint8_t st2[NS > 2 ? STAGE_BYTES : 16];
+  __shared__ __align__(16) uint8_t st3[NS > 2 ? STAGE_BYTES : 16];
   __shared__ __align__(16) int32_t s_rnnz[TM];
   __shared__ float s_raux[TM];
   __shared__ uint8_t s_rdeg[TM];
@@ -540,29 +554,26 @@ __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
   }
 
   // Buffer resources over the two tiles (rows past the end are clamped to the
-  // last row and feed only masked or unused lanes); lane l of a load carries
-  // row 16i + l/4 and physical slot l%4, i.e. logical segment
-  // (l%4) ^ (((16i + l/4) >> 2) & 3) = (l%4) ^ ((l/16) & 3).
-  const int seg = (lane & 3) ^ ((lane >> 4) & 3);
+  // last row and feed only masked or unused lanes).  Lane l of load i carries
+  // row kRowsPerInst i + l / kSlots in physical slot l % kSlots, which holds
+  // logical segment (l % kSlots) ^ swz(row).
   const int64_t a_rows = (P.nref - r_base < TM) ? P.nref - r_base : TM;
   const int64_t b_rows = (P.nqry - q_base < TN) ? P.nqry - q_base : TN;
   const __amdgpu_buffer_rsrc_t ra = __builtin_amdgcn_make_buffer_rsrc(
       (void*)(P.ref_dense + r_base * P.kpad_bytes), 0, (int)(a_rows * P.kpad_bytes), kRsrcWord3);
   const __amdgpu_buffer_rsrc_t rb = __builtin_amdgcn_make_buffer_rsrc(
       (void*)(P.qry_dense + q_base * P.kpad_bytes), 0, (int)(b_rows * P.kpad_bytes), kRsrcWord3);
+  auto lane_offset = [&](int i, int64_t rows) {
+    const int row = Lay::kRowsPerInst * i + lane / Lay::kSlots;
+    const int seg = (lane % Lay::kSlots) ^ Lay::swz(row);
+    const int r = row < rows ? row : (int)rows - 1;
+    return (uint32_t)(r * P.kpad_bytes) + seg * 16;
+  };
   uint32_t va[A_INST], vb[B_INST];
 #pragma unroll
-  for (int u = 0; u < A_INST; ++u) {
-    int row = 16 * (wave * A_INST + u) + (lane >> 2);
-    if (row >= a_rows) row = (int)a_rows - 1;
-    va[u] = (uint32_t)(row * P.kpad_bytes) + seg * 16;
-  }
+  for (int u = 0; u < A_INST; ++u) va[u] = lane_offset(wave * A_INST + u, a_rows);
 #pragma unroll
-  for (int u = 0; u < B_INST; ++u) {
-    int row = 16 * (wave * B_INST + u) + (lane >> 2);
-    if (row >= b_rows) row = (int)b_rows - 1;
-    vb[u] = (uint32_t)(row * P.kpad_bytes) + seg * 16;
-  }
+  for (int u = 0; u < B_INST; ++u) vb[u] = lane_offset(wave * B_INST + u, b_rows);
   typedef __attribute__((address_space(3))) void lds_void;
   auto issue = [&](uint8_t* st, int64_t c) {
     const int soff = (int)(c * RB);
@@ -584,9 +595,10 @@ __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
 #pragma unroll
       for (int r = 0; r < 16; ++r) acc[b][j][r] = 0;
 
-  // MFMA step t (of two per chunk): lanes with h supply logical segment 2t + h
-  // of their row; every A and B row read here has (row >> 2) & 3 == (n >> 2) & 3.
-  const int swz = (n >> 2) & 3;
+  // MFMA step t: lanes with h supply logical segment 2t + h of their row;
+  // every A and B row read here has swz(row) == swz(n) (the row offsets are
+  // multiples of 32).
+  const int swz = Lay::swz(n);
   struct Frag {
     uint4 a[4], b[2];
   };
@@ -607,60 +619,57 @@ __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
   };
   Frag f0, f1;
 
-  // kpad is a positive multiple of kDenseRowAlign (256) rows, so the chunks
-  // come in whole groups of four.  Refills past the last chunk reload the
-  // last chunk into a stage nobody reads again, so every step issues and
-  // waits alike and no LDS read sits behind a branch.
+  // A padded column is a multiple of 256 bytes, so the chunks come in whole
+  // groups of NS.  Refills past the last chunk reload the last chunk into a
+  // stage nobody reads again, so every step issues and waits alike and no
+  // LDS read sits behind a branch.
   const int64_t nchunks = P.kpad_bytes / RB;
   const int64_t last = nchunks - 1;
   lds_barrier();  // s_rnnz and friends are visible
   issue(st0, 0);
   issue(st1, last < 1 ? last : 1);
-  issue(st2, last < 2 ? last : 2);
-  issue(st3, last < 3 ? last : 3);
-  wait_vm<3 * INST>();
+  if constexpr (NS > 2) {
+    issue(st2, last < 2 ? last : 2);
+    issue(st3, last < 3 ? last : 3);
+  }
+  wait_vm<(NS - 1) * INST>();
   lds_barrier();
   load_frag(f0, st0, 0);
-  // Chunk c in stage CUR (its step-0 fragments already requested): request
-  // step 1, run step 0; wait for this wave's loads of chunk c + 1 (the loads
-  // of chunks c + 2 and c + 3 may still be in flight); barrier, after which
-  // every wave has read CUR and chunk c + 1 is visible; refill CUR with chunk
-  // c + 4 and request chunk c + 1's step 0 while step 1's MFMAs run.
-#define SD_WIDE_CHUNK(CUR, NXT, C)                                            \
-  {                                                                           \
-    load_frag(f1, CUR, 1);                                                    \
-    __builtin_amdgcn_sched_barrier(0);                                        \
-    mma_frag(f0);                                                             \
-    __builtin_amdgcn_sched_barrier(0);                                        \
-    wait_vm<2 * INST>();                                                      \
-    lds_barrier();                                                            \
-    issue(CUR, (C) + 4 < last ? (C) + 4 : last);                              \
-    load_frag(f0, NXT, 0);                                                    \
-    __builtin_amdgcn_sched_barrier(0);                                        \
-    mma_frag(f1);                                                             \
-    __builtin_amdgcn_sched_barrier(0);                                        \
-  }
-#define SD_WIDE_LAST(CUR)                                                     \
-  {                                                                           \
-    load_frag(f1, CUR, 1);                                                    \
-    __builtin_amdgcn_sched_barrier(0);                                        \
-    mma_frag(f0);                                                             \
-    mma_frag(f1);                                                             \
-  }
+  // Chunk c in stage CUR (its step-0 fragments already requested): the steps
+  // alternate fragment sets, each requesting the next step's fragments before
+  // running its MFMAs.  Before the last step: wait for this wave's loads of
+  // chunk c + 1 (those of chunks c + 2 .. c + NS - 1 may still be in flight);
+  // barrier, after which every wave has read CUR and chunk c + 1 is visible;
+  // refill CUR with chunk c + NS and request chunk c + 1's step 0 while the
+  // last step's MFMAs run.
+#define SD_WIDE_STEPS(CUR)                                                      load_frag(f1, CUR, 1);                                                        __builtin_amdgcn_sched_barrier(0);                                            mma_frag(f0);                                                                 __builtin_amdgcn_sched_barrier(0);                                            if constexpr (STEPS == 4) {                                                     load_frag(f0, CUR, 2);                                                        __builtin_amdgcn_sched_barrier(0);                                            mma_frag(f1);                                                                 __builtin_amdgcn_sched_barrier(0);                                            load_frag(f1, CUR, 3);                                                        __builtin_amdgcn_sched_barrier(0);                                            mma_frag(f0);                                                                 __builtin_amdgcn_sched_barrier(0);                                          }
+#define SD_WIDE_CHUNK(CUR, NXT, C)                                              {                                                                               SD_WIDE_STEPS(CUR)                                                            wait_vm<(NS - 2) * INST>();                                                   lds_barrier();                                                                issue(CUR, (C) + NS < last ? (C) + NS : last);                                load_frag(f0, NXT, 0);                                                        __builtin_amdgcn_sched_barrier(0);                                            mma_frag(f1);                                                                 __builtin_amdgcn_sched_barrier(0);                                          }
+#define SD_WIDE_LAST(CUR)                                                       {                                                                               SD_WIDE_STEPS(CUR)                                                            mma_frag(f1);                                                               }
   int64_t c = 0;
-  for (; c + 4 < nchunks; c += 4) {
+  if constexpr (NS == 4) {
+    for (; c + 4 < nchunks; c += 4) {
+      SD_WIDE_CHUNK(st0, st1, c)
+      SD_WIDE_CHUNK(st1, st2, c + 1)
+      SD_WIDE_CHUNK(st2, st3, c + 2)
+      SD_WIDE_CHUNK(st3, st0, c + 3)
+    }
+    // the last group of four (chunk c sits in st0)
     SD_WIDE_CHUNK(st0, st1, c)
     SD_WIDE_CHUNK(st1, st2, c + 1)
     SD_WIDE_CHUNK(st2, st3, c + 2)
-    SD_WIDE_CHUNK(st3, st0, c + 3)
+    SD_WIDE_LAST(st3)
+  } else {
+    for (; c + 2 < nchunks; c += 2) {
+      SD_WIDE_CHUNK(st0, st1, c)
+      SD_WIDE_CHUNK(st1, st0, c + 1)
+    }
+    // the last pair (chunk c sits in st0)
+    SD_WIDE_CHUNK(st0, st1, c)
+    SD_WIDE_LAST(st1)
   }
-  // the last group of four (chunk c sits in st0)
-  SD_WIDE_CHUNK(st0, st1, c)
-  SD_WIDE_CHUNK(st1, st2, c + 1)
-  SD_WIDE_CHUNK(st2, st3, c + 2)
-  SD_WIDE_LAST(st3)
 #undef SD_WIDE_LAST
 #undef SD_WIDE_CHUNK
+#undef SD_WIDE_STEPS
   wait_vm<0>();  // no refill is left in flight when the workgroup ends
 
   // ---- epilogue: straight from the accumulators, no exchange.  acc[b][j][r]
@@ -1053,9 +1062,12 @@ static int launch_fused_t(sd_context* ctx, FusedKernelArgs& P) {
   return SD_OK;
 }
 
-template <class Op, class K>
+// K bytes per row per chunk of the filtered pass (64 or 128)
+constexpr int kFilterRowBytes = 128;
+
+template <class Op, class K, int RB>
 static int launch_filter_t(sd_context* ctx, FusedKernelArgs& P) {
-  if (P.kpad_bytes <= 0 || P.kpad_bytes % (4 * kWideRowBytes) != 0)
+  if (P.kpad_bytes <= 0 || P.kpad_bytes % (WideLayout<RB>::kStages * RB) != 0)
     return set_error(SD_EINVAL, "internal: padded row bytes %lld are not whole chunk groups",
                      (long long)P.kpad_bytes);
   P.qtiles = (int)((P.nqry + kWideTile - 1) / kWideTile);
@@ -1067,7 +1079,7 @@ static int launch_filter_t(sd_context* ctx, FusedKernelArgs& P) {
   const int64_t gx = std::min<int64_t>((need + 7) / 8 * 8, (int64_t)1 << 20);
   const int64_t gy = (need + gx - 1) / gx;
   if (gy > 65535) return set_error(SD_EUNSUPPORTED, "slab grid too large (%lld tiles)", (long long)need);
-  hipLaunchKernelGGL((k_filter_wide<Op, K>), dim3((unsigned)gx, (unsigned)gy), dim3(512), 0,
+  hipLaunchKernelGGL((k_filter_wide<Op, K, RB>), dim3((unsigned)gx, (unsigned)gy), dim3(512), 0,
                      ctx->stream, P);
   SD_HIP(hipGetLastError());
   return SD_OK;
@@ -1210,11 +1222,11 @@ int fused_filter_pass(sd_context* ctx, const FusedArgs& a, const double* thr, in
   SD_HIP(hipMemsetAsync(cnt, 0, sizeof(int32_t) * (size_t)a.nqry, ctx->stream));
   KernelTimer timer(ctx, &ctx->stats.filter_ms, &ctx->stats.filter_launches);
   if (a.method == SD_BINARY) {
-    if (a.kind == kDenseF32Pattern) return launch_filter_t<OpF32, KeyJ>(ctx, P);
-    return launch_filter_t<OpF4, KeyJ>(ctx, P);
+    if (a.kind == kDenseF32Pattern) return launch_filter_t<OpF32, KeyJ, kFilterRowBytes>(ctx, P);
+    return launch_filter_t<OpF4, KeyJ, kFilterRowBytes>(ctx, P);
   }
-  if (a.prec == SD_PREC_FP32) return launch_filter_t<OpF32, float>(ctx, P);
-  return launch_filter_t<OpBf16, float>(ctx, P);
+  if (a.prec == SD_PREC_FP32) return launch_filter_t<OpF32, float, kFilterRowBytes>(ctx, P);
+  return launch_filter_t<OpBf16, float, kFilterRowBytes>(ctx, P);
 }
 
 int select_lists(sd_context* ctx, const double* fkey, const int32_t* fidx, const int32_t* cnt,
