@@ -418,8 +418,10 @@ def cross_main(args, sd, synth):
 
 def large_main(args, sd, synth, workers):
     """Self-search kNN (k=20) on X = 30000 x 1000000 CSC, density 0.03, NB/log1p:
-    binary and cosine over every column; euclidean (exact FP64 path) on a
-    sample of query columns against all 1,000,000 columns."""
+    binary and cosine over every column; euclidean over every column in
+    blocks of 125,000 query columns against the resident matrix (a cross
+    search per block, so each query also finds itself at distance 0; the
+    blocks keep a progress line per block and bound the query-side memory)."""
     t0 = time.time()
     X = synth.nb_log1p_parallel(30_000, 1_000_000, 0.03, 70, workers)
     log(f"generated X {X.shape} nnz={X.nnz} in {time.time() - t0:.1f}s")
@@ -429,41 +431,49 @@ def large_main(args, sd, synth, workers):
     log(f"uploaded in {time.time() - t0:.1f}s")
     k, nq = 20, X.shape[1]
     res = {}
-    for method in ("binary", "cosine"):
+    for method in ("binary", "cosine", "euclidean"):
         dX.drop_cache()
         ctx.stats(reset=True)
         ctx.set_timing(True)
         t = time.perf_counter()
-        nn = sd.sparseKNN(dX, k=k, method=method, block_size=8192, verbose=False, context=ctx)
+        if method == "euclidean":
+            blk = 125_000
+            for b0 in range(0, nq, blk):
+                nn = sd.sparseKNN(X[:, b0:b0 + blk], dX, k=k, method=method, verbose=False,
+                                  context=ctx)
+                del nn
+                log(f"euclidean block {b0 // blk + 1}/{(nq + blk - 1) // blk}: "
+                    f"{time.perf_counter() - t:.1f} s")
+        else:
+            nn = sd.sparseKNN(dX, k=k, method=method, block_size=8192, verbose=False,
+                              context=ctx)
+            del nn
         dt = time.perf_counter() - t
         st = ctx.stats(reset=True)
         ctx.set_timing(False)
         fms = st["filter_ms"] / max(st["filter_launches"], 1)
         peak = MI355X_PEAK["fp4" if method == "binary" else "bf16"]
-        flop = 2.0 * X.shape[0] * nq * nq
+        # dense-equivalent operations of one filtered pass (the split layout
+        # runs three products per row)
+        rows = X.shape[0] * (3 if method == "euclidean" else 1)
+        qpass = nq / max(st["filter_launches"], 1)
+        flop = 2.0 * rows * nq * qpass
         res[method] = dict(seconds=dt, cols_per_s=nq / dt, filter_ms=fms,
                            filter_frac_of_peak=flop / (fms / 1e3) / peak,
                            certified=st["certified_queries"], escalated=st["escalated_queries"],
                            fallback=st["fallback_queries"], overflow=st["overflow_queries"])
         log(f"{method}: {dt:.1f} s ({nq / dt:.0f} cols/s), filtered pass {fms:.0f} ms "
-            f"({res[method]['filter_frac_of_peak'] * 100:.0f}% of peak), fallback "
-            f"{st['fallback_queries']}")
-        del nn
-    ns = 256
-    Xs = X[:, :ns]
-    t = time.perf_counter()
-    sd.sparseKNN(Xs, dX, k=k, method="euclidean", verbose=False, context=ctx)
-    dt = time.perf_counter() - t
-    res["euclidean_sample"] = dict(query_cols=ns, seconds=dt, cols_per_s=ns / dt,
-                                   path="exact FP64 merge walk")
-    log(f"euclidean (exact path): {ns} queries in {dt:.1f} s")
+            f"({res[method]['filter_frac_of_peak'] * 100:.0f}% of peak), escalated "
+            f"{st['escalated_queries']}, fallback {st['fallback_queries']}")
+    secs = sum(res[m]["seconds"] for m in res)
     out = {
         "metric": "exact-kNN query cols/s (k=20), large sparse self-search",
-        "value": 2 * nq / (res["binary"]["seconds"] + res["cosine"]["seconds"]),
+        "value": 3 * nq / secs,
         "unit": "query cols/s", "n_gpus": 1, "higher_is_better": True,
-        "dtype": "fp4 0/1 patterns + bf16 slab, fp64 re-rank", "data": "synthetic (seeded NB/log1p)",
-        "config": {"workload": "X 30000 x 1000000 CSC d=0.03 NB/log1p, self-search k=20: binary "
-                               "and cosine over all columns, euclidean on 256 query columns"},
+        "dtype": "fp4 0/1 patterns (binary), bf16 (cosine), split bf16 hi/lo (euclidean) slab, "
+                 "fp64 re-rank", "data": "synthetic (seeded NB/log1p)",
+        "config": {"workload": "X 30000 x 1000000 CSC d=0.03 NB/log1p, k=20: binary, cosine "
+                               "and euclidean over all 1000000 query columns"},
         "methods": res,
     }
     print(json.dumps(out), flush=True)

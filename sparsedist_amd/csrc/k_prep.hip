@@ -116,7 +116,7 @@ __global__ __launch_bounds__(256) void k_densify(const int64_t* __restrict__ p,
     const int64_t o = r - k0;
     if (KIND == kDenseF4Pattern) {
       atomicOr(reinterpret_cast<uint32_t*>(lds) + (o >> 3), 0x2u << ((o & 7) * 4));
-    } else if (KIND == kDenseBf16Unit || KIND == kDenseBf16Raw) {
+    } else if (KIND == kDenseBf16Unit) {
       reinterpret_cast<uint16_t*>(lds)[o] = f32_to_bf16_rne((float)vals[e]);
     } else if (KIND == kDenseF32Pattern) {
       reinterpret_cast<float*>(lds)[o] = 1.0f;
@@ -127,6 +127,57 @@ __global__ __launch_bounds__(256) void k_densify(const int64_t* __restrict__ p,
   __syncthreads();
   uint4* dst = reinterpret_cast<uint4*>(out + (ELEM ? (j * kpad + k0) * ELEM : (j * kpad + k0) / 2));
   for (int64_t t = threadIdx.x; t < bytes / 16; t += blockDim.x) dst[t] = l4[t];
+}
+
+// Split layout: one workgroup per (column, K chunk) as above; LDS holds the
+// chunk's hi values followed by its lo values, and the chunk is written to
+// the row's three segments [hi | hi | lo] (each kpad elements).  v - hi is
+// exact in FP64 (hi is v rounded to 8 significant bits, so the difference
+// has at most 45 significant bits), then rounded like hi.
+__global__ __launch_bounds__(256) void k_densify_split(const int64_t* __restrict__ p,
+                                                       const int32_t* __restrict__ ri,
+                                                       const double* __restrict__ vals,
+                                                       int64_t ncol, int64_t kpad, int64_t chunk,
+                                                       uint8_t* __restrict__ out) {
+  extern __shared__ __align__(16) uint8_t lds[];
+  const int64_t nchunks = (kpad + chunk - 1) / chunk;
+  const int64_t j = blockIdx.x / nchunks;
+  const int64_t c = blockIdx.x % nchunks;
+  if (j >= ncol) return;
+  const int64_t k0 = c * chunk;
+  const int64_t len = (kpad - k0 < chunk) ? (kpad - k0) : chunk;  // multiple of 256
+  uint16_t* hi = reinterpret_cast<uint16_t*>(lds);
+  uint16_t* lo = hi + len;
+  uint4* l4 = reinterpret_cast<uint4*>(lds);
+  for (int64_t t = threadIdx.x; t < len * 4 / 16; t += blockDim.x) l4[t] = make_uint4(0, 0, 0, 0);
+  __syncthreads();
+  for (int64_t e = p[j] + threadIdx.x, ee = p[j + 1]; e < ee; e += blockDim.x) {
+    const int64_t r = ri[e];
+    if (r < k0 || r >= k0 + len) continue;
+    const double v = vals[e];
+    const uint16_t h = f32_to_bf16_rne((float)v);
+    const double hv = (double)__uint_as_float((uint32_t)h << 16);
+    hi[r - k0] = h;
+    lo[r - k0] = f32_to_bf16_rne((float)(v - hv));
+  }
+  __syncthreads();
+  const int64_t seg = kpad * 2;  // bytes
+  uint8_t* row = out + j * 3 * seg + k0 * 2;
+  const int64_t n16 = len * 2 / 16;
+  const uint4* h4 = reinterpret_cast<const uint4*>(hi);
+  const uint4* o4 = reinterpret_cast<const uint4*>(lo);
+  for (int64_t t = threadIdx.x; t < n16; t += blockDim.x) {
+    const uint4 hv = h4[t];
+    reinterpret_cast<uint4*>(row)[t] = hv;
+    reinterpret_cast<uint4*>(row + seg)[t] = hv;
+    reinterpret_cast<uint4*>(row + 2 * seg)[t] = o4[t];
+  }
+}
+
+// FP32 copies of the FP64 squared norms.
+__global__ void k_to_f32(const double* __restrict__ in, int64_t n, float* __restrict__ out) {
+  const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (j < n) out[j] = (float)in[j];
 }
 
 }  // namespace
@@ -167,8 +218,11 @@ int prep_sq(sd_context* ctx, sd_matrix* m) {
   if (m->sq_ready) return SD_OK;
   SD_TRY(m->sqn.alloc(sizeof(double) * (size_t)(m->ncol + 1)));
   const int64_t nb = (m->ncol + 255) / 256;
+  SD_TRY(m->sqf.alloc(sizeof(float) * (size_t)(m->ncol + 1)));
   hipLaunchKernelGGL(k_sqnorm, dim3((unsigned)nb), dim3(256), 0, ctx->stream, m->p.as<int64_t>(),
                      m->x.as<double>(), m->ncol, m->sqn.as<double>());
+  hipLaunchKernelGGL(k_to_f32, dim3((unsigned)nb), dim3(256), 0, ctx->stream, m->sqn.as<double>(),
+                     m->ncol, m->sqf.as<float>());
   SD_HIP(hipGetLastError());
   SD_HIP(hipStreamSynchronize(ctx->stream));
   m->sq_ready = true;
@@ -193,10 +247,13 @@ int prep_dense(sd_context* ctx, sd_matrix* m, int kind, int64_t kpad) {
   // rows per chunk: 64 KB of LDS, a multiple of 128 rows (whole 16-byte
   // stores, and whole 32-bit words of eight FP4 rows)
   const int64_t chunk_bytes = 64 * 1024;
-  const int64_t chunk = kind == kDenseF4Pattern ? chunk_bytes * 2 : chunk_bytes / (row_bytes / kpad);
+  const int64_t chunk = kind == kDenseF4Pattern    ? chunk_bytes * 2
+                        : kind == kDenseBf16Split ? chunk_bytes / 4  // hi and lo in LDS
+                                                  : chunk_bytes / (row_bytes / kpad);
   const int64_t nchunks = (kpad + chunk - 1) / chunk;
   const int64_t grid = m->ncol * nchunks;
-  const size_t lds = (size_t)dense_row_bytes(kind, kpad < chunk ? kpad : chunk);
+  const size_t lds = kind == kDenseBf16Split ? (size_t)(kpad < chunk ? kpad : chunk) * 4
+                                             : (size_t)dense_row_bytes(kind, kpad < chunk ? kpad : chunk);
   if (grid * 256 >= ((int64_t)1 << 32))
     return set_error(SD_EUNSUPPORTED, "densify grid too large (%lld workgroups)", (long long)grid);
   if (grid > 0) {
@@ -211,10 +268,9 @@ int prep_dense(sd_context* ctx, sd_matrix* m, int kind, int64_t kpad) {
                            ctx->stream, m->p.as<int64_t>(), m->i.as<int32_t>(), vals, m->ncol, kpad,
                            chunk, dst);
         break;
-      case kDenseBf16Raw:
-        hipLaunchKernelGGL((k_densify<kDenseBf16Raw, 2>), dim3((unsigned)grid), dim3(256), lds,
-                           ctx->stream, m->p.as<int64_t>(), m->i.as<int32_t>(), vals, m->ncol, kpad,
-                           chunk, dst);
+      case kDenseBf16Split:
+        hipLaunchKernelGGL(k_densify_split, dim3((unsigned)grid), dim3(256), lds, ctx->stream,
+                           m->p.as<int64_t>(), m->i.as<int32_t>(), vals, m->ncol, kpad, chunk, dst);
         break;
       case kDenseF32Unit:
         hipLaunchKernelGGL((k_densify<kDenseF32Unit, 4>), dim3((unsigned)grid), dim3(256), lds,

@@ -17,6 +17,8 @@
 // one lane owns one query and keeps its L best candidates sorted in registers.
 #include <hip/hip_bf16.h>
 
+#include <cfloat>
+
 #include "pair_math.h"
 #include "sd_internal.h"
 #include "topk_block.h"
@@ -95,6 +97,29 @@ __device__ inline bool better(float a, uint32_t ia, float b, uint32_t ib) {
 __device__ inline float worst_key(float*) { return -INFINITY; }
 __device__ inline double key_to_double(float k) { return (double)k; }
 
+// Euclidean key: minus a lower bound of the squared distance
+// D = |q|^2 + |r|^2 - 2 q.r, from FP32 squared norms qa, ra and the slab's
+// dot x.  With e = e0 + e1 nnz(q) the dot obeys |x - q.r| <= e S, where
+// S = sum |q_i r_i| <= |q||r| <= (qa + ra) / 2; for non-negative data S is
+// q.r itself and q.r <= x / (1 - e) (the host folds 1 / (1 - e) into e0, e1).
+// kEucRound covers the FP32 rounding of qa, ra and of this formula (at most
+// six roundings of quantities below 2 (qa + ra), 2^-24 each); kEucAbs covers
+// flushed subnormal products.  Larger keys are better, like every key here.
+constexpr float kEucRound = 4.76837158203125e-07f;  // 2^-21
+constexpr float kEucAbs = 1e-30f;
+__device__ inline float euc_key(float qa, float ra, float x, uint32_t qnnz, float e0, float e1,
+                                int nonneg) {
+  const float n2 = qa + ra;
+  const float s = n2 - 2.f * x;
+  const float e = e0 + e1 * (float)qnnz;
+  const float err = nonneg ? 2.f * e * fabsf(x) + kEucRound * n2 + kEucAbs
+                           : (e + kEucRound) * n2 + kEucAbs;
+  return err - s;
+}
+__device__ inline int qseg_offset(int soff, int qseg) {
+  return (qseg == 0 || soff < qseg) ? soff : (soff < 2 * qseg ? soff + qseg : soff - qseg);
+}
+
 template <class K>
 struct Entry {
   K key;
@@ -126,6 +151,11 @@ struct FusedKernelArgs {
   int32_t* cnt;         // per query position: candidates appended
   double* fkey;         // [nqry][cap]
   int32_t* fidx;
+  // split layouts: query rows are read with the segments at byte offsets
+  // [qseg, 2 qseg) and [2 qseg, 3 qseg) swapped (0: read as stored)
+  int qseg;
+  float euc_e0, euc_e1;  // euclidean: dot error bound (e0 + e1 nnz(q)) sum |q_i r_i|
+  int euc_nonneg;        // both matrices >= 0: that sum is the dot product
 };
 
 namespace {
@@ -174,7 +204,8 @@ __device__ inline void wait_vm() {
   __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | 0x0F70);
 }
 
-template <class Op, class K, int L, int WM>
+// QS: query rows are read with the split layout's segments swapped (qseg).
+template <class Op, class K, int L, int WM, bool QS>
 __global__ __launch_bounds__(WM * 256, 1) void k_fused(FusedKernelArgs P) {
   constexpr int WAVES = WM * 4;
   constexpr int NT = WAVES * 64;
@@ -296,6 +327,7 @@ __global__ __launch_bounds__(WM * 256, 1) void k_fused(FusedKernelArgs P) {
     }
     auto issue = [&](uint8_t* st, int64_t c) {
       const int soff = (int)(c * kRowBytes);
+      const int soff_b = QS ? qseg_offset(soff, P.qseg) : soff;
 #pragma unroll
       for (int u = 0; u < A_INST; ++u)
         __builtin_amdgcn_raw_ptr_buffer_load_lds(
@@ -303,7 +335,8 @@ __global__ __launch_bounds__(WM * 256, 1) void k_fused(FusedKernelArgs P) {
 #pragma unroll
       for (int u = 0; u < B_INST; ++u)
         __builtin_amdgcn_raw_ptr_buffer_load_lds(
-            rb, (lds_void*)(st + A_BYTES + (wave * B_INST + u) * 1024), 16, (int)vb[u], soff, 0, 0);
+            rb, (lds_void*)(st + A_BYTES + (wave * B_INST + u) * 1024), 16, (int)vb[u], soff_b, 0,
+            0);
     };
     const int swz = (n >> 1) & 7;
     // Operand fragments of one MFMA step (16 K bytes per lane): lanes with h
@@ -446,11 +479,13 @@ __global__ __launch_bounds__(WM * 256, 1) void k_fused(FusedKernelArgs P) {
           } else {
             float sv;
             __builtin_memcpy(&sv, &bits, 4);
-            if (P.method == SD_EUCLIDEAN)
-              key = -(qaux + s_raux[rloc] - 2.f * sv);
-            else
+            if (P.method == SD_EUCLIDEAN) {
+              key = euc_key(qaux, s_raux[rloc], sv, nj, P.euc_e0, P.euc_e1, P.euc_nonneg);
+              if (P.self_mode && rg == my_q) key = FLT_MAX;  // distance 0
+            } else {
               key = sv;
-            if (P.self_mode && rg == my_q) key = 2.0f;  // restored self similarity
+              if (P.self_mode && rg == my_q) key = 2.0f;  // restored self similarity
+            }
           }
           if (better(key, (uint32_t)rg, list[L - 1].key, list[L - 1].idx)) {
             Entry<K> carry{key, (uint32_t)rg};
@@ -508,7 +543,7 @@ struct WideLayout {
   __device__ static int swz(int row) { return (row >> kRowShift) & (kSlots - 1); }
 };
 
-template <class Op, class K, int RB>
+template <class Op, class K, int RB, bool QS>
 __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
   using Lay = WideLayout<RB>;
   constexpr int NT = 512;
@@ -577,6 +612,7 @@ __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
   typedef __attribute__((address_space(3))) void lds_void;
   auto issue = [&](uint8_t* st, int64_t c) {
     const int soff = (int)(c * RB);
+    const int soff_b = QS ? qseg_offset(soff, P.qseg) : soff;
 #pragma unroll
     for (int u = 0; u < A_INST; ++u)
       __builtin_amdgcn_raw_ptr_buffer_load_lds(
@@ -584,7 +620,7 @@ __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
 #pragma unroll
     for (int u = 0; u < B_INST; ++u)
       __builtin_amdgcn_raw_ptr_buffer_load_lds(
-          rb, (lds_void*)(st + A_BYTES + (wave * B_INST + u) * 1024), 16, (int)vb[u], soff, 0, 0);
+          rb, (lds_void*)(st + A_BYTES + (wave * B_INST + u) * 1024), 16, (int)vb[u], soff_b, 0, 0);
   };
 
   typename Op::Acc acc[4][2];
@@ -709,8 +745,13 @@ __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
       const uint32_t d = (uint32_t)ni + qn[j] - cc;
       kd = (d == 0) ? 1.0 : key_to_double(KeyJ{cc, d});
     } else {
-      float key = (P.method == SD_EUCLIDEAN) ? -(qa[j] + s_raux[rloc] - 2.f * x) : x;
-      if (self) key = 2.0f;  // restored self similarity
+      float key;
+      if (P.method == SD_EUCLIDEAN) {
+        key = self ? FLT_MAX  // distance 0
+                   : euc_key(qa[j], s_raux[rloc], x, qn[j], P.euc_e0, P.euc_e1, P.euc_nonneg);
+      } else {
+        key = self ? 2.0f : x;  // restored self similarity
+      }
       kd = (double)key;
     }
     return kd >= thr_d[j];
@@ -1045,18 +1086,114 @@ __global__ __launch_bounds__(256) void k_rerank(RerankArgs R) {
   }
 }
 
+// ---- euclidean re-rank ----------------------------------------------------------
+// One wave per query, lane t < L takes candidate t: its exact squared
+// distance num (the reference's merge walk, euc_num) and distance sqrt(num).
+// Lanes are sorted by (distance, index).  Certificate: every reference
+// outside the list has key <= the list's last key, so its true squared
+// distance D is at least LB = -key[L-1] (euc_key's lower bound), and the
+// reference's FP64 value of D is at least D (1 - rho) with
+// rho = (nnz(q) + nrow + 4) 2^-52 (a sum of at most nnz(q) + nrow
+// non-negative terms, each a rounded square of a rounded difference).  The
+// list holds the exact top k when the k-th squared distance num_k satisfies
+// num_k (1 + 2^-49) < LB (1 - rho): then every outside reference's value
+// exceeds num_k by more than a rounding of sqrt, so it ranks strictly after.
+constexpr int kRerankEucQCap = 1024;  // query entries cached in LDS per wave
+
+__global__ __launch_bounds__(256) void k_rerank_euc(RerankArgs R) {
+  __shared__ int32_t s_qr[4][kRerankEucQCap];
+  __shared__ double s_qv[4][kRerankEucQCap];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t j = blockIdx.x * 4 + wave;
+  if (j >= R.nqry) return;  // whole waves leave; no barrier follows
+  const int L = R.L;
+  const int64_t jq = R.qlist ? (int64_t)R.qlist[j] : j;
+  const int64_t q0 = R.ex.qry.p[jq], qn = R.ex.qry.p[jq + 1] - q0;
+  const int32_t* qr = R.ex.qry.i + q0;
+  const double* qv = R.ex.qry.x + q0;
+  if (qn <= kRerankEucQCap) {
+    for (int64_t e = lane; e < qn; e += 64) {
+      s_qr[wave][e] = qr[e];
+      s_qv[wave][e] = qv[e];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    qr = s_qr[wave];
+    qv = s_qv[wave];
+  }
+  const int32_t ci = (lane < L) ? R.cand_idx[j * L + lane] : -1;
+  const bool deg_q = R.qdeg && R.qdeg[jq];
+  const bool valid = ci >= 0 && !deg_q;
+  double num = 0.0, v = 0.0;
+  if (valid) {
+    const int64_t a0 = R.ex.ref.p[ci], na = R.ex.ref.p[ci + 1] - a0;
+    num = euc_num(R.ex.ref.i + a0, R.ex.ref.x + a0, na, qr, qv, qn);
+    v = sqrt(num);
+  }
+  uint64_t key = valid ? order_key(v, 0) : ~0ULL;
+  uint32_t id = valid ? (uint32_t)ci : 0xFFFFFFFFu;
+  double nv = num, vv = v;
+  for (int size = 2; size <= 64; size <<= 1) {
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      const int partner = lane ^ stride;
+      const uint64_t pk = wave_shfl_u64(key, partner);
+      const uint32_t pi = (uint32_t)__shfl((int)id, partner);
+      const double pn = __shfl(nv, partner);
+      const double pv = __shfl(vv, partner);
+      const bool is_lo = (lane & stride) == 0;
+      const bool asc = (lane & size) == 0;
+      const bool mine_less = key < pk || (key == pk && id < pi);
+      const bool take_min = (is_lo == asc);
+      const bool keep_mine = take_min ? mine_less : !mine_less;
+      if (!keep_mine) {
+        key = pk;
+        id = pi;
+        nv = pn;
+        vv = pv;
+      }
+    }
+  }
+  const int nvalid = __popcll(__ballot(valid));
+  bool certified = true;
+  if (nvalid >= L && R.k <= nvalid) {
+    const double lb = -R.cand_key[j * L + (L - 1)];
+    const double numk = __shfl(nv, R.k - 1);
+    const double rho = ((double)qn + (double)R.ex.ref.nrow + 4.0) * 0x1p-52;
+    certified = lb > 0.0 && numk * (1.0 + 0x1p-49) < lb * (1.0 - rho);
+  }
+  if (deg_q) certified = true;
+  if (!certified) {
+    if (lane == 0) {
+      const int at = atomicAdd(R.fail_count, 1);
+      R.fail_list[at] = (int32_t)jq;
+    }
+    return;
+  }
+  if (lane < R.k) {
+    const int64_t e = (int64_t)lane * R.out_ld + jq;
+    if (lane < nvalid && !deg_q) {
+      R.out_idx[e] = (int32_t)id + 1;
+      R.out_val[e] = vv;
+    } else {
+      R.out_idx[e] = kNaInt;
+      R.out_val[e] = __longlong_as_double((long long)kNaRealBits);
+    }
+  }
+}
+
 }  // namespace
 
 // ----------------------------------------------------------------------------
 // host launchers
 // ----------------------------------------------------------------------------
 
-template <class Op, class K, int L, int WM>
+template <class Op, class K, int L, int WM, bool QS>
 static int launch_fused_t(sd_context* ctx, FusedKernelArgs& P) {
   const int64_t nblk = (int64_t)P.qtiles * P.splits;
   if (nblk * WM * 256 >= ((int64_t)1 << 32))
     return set_error(SD_EUNSUPPORTED, "slab grid too large (%lld workgroups)", (long long)nblk);
-  hipLaunchKernelGGL((k_fused<Op, K, L, WM>), dim3((unsigned)nblk), dim3(WM * 256), 0,
+  hipLaunchKernelGGL((k_fused<Op, K, L, WM, QS>), dim3((unsigned)nblk), dim3(WM * 256), 0,
                      ctx->stream, P);
   SD_HIP(hipGetLastError());
   return SD_OK;
@@ -1065,7 +1202,7 @@ static int launch_fused_t(sd_context* ctx, FusedKernelArgs& P) {
 // K bytes per row per chunk of the filtered pass (64 or 128)
 constexpr int kFilterRowBytes = 128;
 
-template <class Op, class K, int RB>
+template <class Op, class K, int RB, bool QS = false>
 static int launch_filter_t(sd_context* ctx, FusedKernelArgs& P) {
   if (P.kpad_bytes <= 0 || P.kpad_bytes % (WideLayout<RB>::kStages * RB) != 0)
     return set_error(SD_EINVAL, "internal: padded row bytes %lld are not whole chunk groups",
@@ -1079,22 +1216,22 @@ static int launch_filter_t(sd_context* ctx, FusedKernelArgs& P) {
   const int64_t gx = std::min<int64_t>((need + 7) / 8 * 8, (int64_t)1 << 20);
   const int64_t gy = (need + gx - 1) / gx;
   if (gy > 65535) return set_error(SD_EUNSUPPORTED, "slab grid too large (%lld tiles)", (long long)need);
-  hipLaunchKernelGGL((k_filter_wide<Op, K, RB>), dim3((unsigned)gx, (unsigned)gy), dim3(512), 0,
+  hipLaunchKernelGGL((k_filter_wide<Op, K, RB, QS>), dim3((unsigned)gx, (unsigned)gy), dim3(512), 0,
                      ctx->stream, P);
   SD_HIP(hipGetLastError());
   return SD_OK;
 }
 
-template <class Op, class K, int WM>
+template <class Op, class K, int WM, bool QS = false>
 static int launch_fused_L(sd_context* ctx, FusedKernelArgs& P, int L) {
   switch (L) {
-    case 8: return launch_fused_t<Op, K, 8, WM>(ctx, P);
-    case 16: return launch_fused_t<Op, K, 16, WM>(ctx, P);
-    case 24: return launch_fused_t<Op, K, 24, WM>(ctx, P);
-    case 32: return launch_fused_t<Op, K, 32, WM>(ctx, P);
-    case 40: return launch_fused_t<Op, K, 40, WM>(ctx, P);
-    case 48: return launch_fused_t<Op, K, 48, WM>(ctx, P);
-    case 64: return launch_fused_t<Op, K, 64, WM>(ctx, P);
+    case 8: return launch_fused_t<Op, K, 8, WM, QS>(ctx, P);
+    case 16: return launch_fused_t<Op, K, 16, WM, QS>(ctx, P);
+    case 24: return launch_fused_t<Op, K, 24, WM, QS>(ctx, P);
+    case 32: return launch_fused_t<Op, K, 32, WM, QS>(ctx, P);
+    case 40: return launch_fused_t<Op, K, 40, WM, QS>(ctx, P);
+    case 48: return launch_fused_t<Op, K, 48, WM, QS>(ctx, P);
+    case 64: return launch_fused_t<Op, K, 64, WM, QS>(ctx, P);
     default: return set_error(SD_EINVAL, "internal: unsupported list length %d", L);
   }
 }
@@ -1158,6 +1295,10 @@ int fused_slab_topl(sd_context* ctx, const FusedArgs& a, double* cand_key, int32
   P.qry_p = a.qry_p;
   P.ref_aux = a.ref_aux;
   P.qry_aux = a.qry_aux;
+  P.qseg = (int)dense_qseg_bytes(a.kind, a.kpad);
+  P.euc_e0 = a.euc_e0;
+  P.euc_e1 = a.euc_e1;
+  P.euc_nonneg = a.euc_nonneg;
   P.ref_deg = a.ref_deg;
   P.self_mode = a.self_mode;
   P.include_self = a.include_self;
@@ -1179,6 +1320,7 @@ int fused_slab_topl(sd_context* ctx, const FusedArgs& a, double* cand_key, int32
     return launch_fused_L<OpF4, KeyJ, WM>(ctx, P, a.L);
   }
   if (a.prec == SD_PREC_FP32) return launch_fused_L<OpF32, float, WM>(ctx, P, a.L);
+  if (P.qseg) return launch_fused_L<OpBf16, float, WM, true>(ctx, P, a.L);
   return launch_fused_L<OpBf16, float, WM>(ctx, P, a.L);
 }
 
@@ -1207,6 +1349,10 @@ int fused_filter_pass(sd_context* ctx, const FusedArgs& a, const double* thr, in
   P.qry_p = a.qry_p;
   P.ref_aux = a.ref_aux;
   P.qry_aux = a.qry_aux;
+  P.qseg = (int)dense_qseg_bytes(a.kind, a.kpad);
+  P.euc_e0 = a.euc_e0;
+  P.euc_e1 = a.euc_e1;
+  P.euc_nonneg = a.euc_nonneg;
   P.ref_deg = a.ref_deg;
   P.self_mode = a.self_mode;
   P.include_self = a.include_self;
@@ -1226,6 +1372,7 @@ int fused_filter_pass(sd_context* ctx, const FusedArgs& a, const double* thr, in
     return launch_filter_t<OpF4, KeyJ, kFilterRowBytes>(ctx, P);
   }
   if (a.prec == SD_PREC_FP32) return launch_filter_t<OpF32, float, kFilterRowBytes>(ctx, P);
+  if (P.qseg) return launch_filter_t<OpBf16, float, kFilterRowBytes, true>(ctx, P);
   return launch_filter_t<OpBf16, float, kFilterRowBytes>(ctx, P);
 }
 
@@ -1287,6 +1434,16 @@ int rerank_cosine(sd_context* ctx, const RerankArgs& R) {
   KernelTimer timer(ctx, &ctx->stats.rerank_ms, &ctx->stats.rerank_launches);
   hipLaunchKernelGGL(k_rerank, dim3((unsigned)((R.nqry + 3) / 4)), dim3(256),
                      R.map_words > 0 ? 4 * kRerankWaveBytes : 0, ctx->stream, R);
+  SD_HIP(hipGetLastError());
+  return SD_OK;
+}
+
+int rerank_euclid(sd_context* ctx, const RerankArgs& R) {
+  if (R.nqry <= 0) return SD_OK;
+  if (R.L > 64) return set_error(SD_EINVAL, "internal: re-rank list of %d > 64", R.L);
+  KernelTimer timer(ctx, &ctx->stats.rerank_ms, &ctx->stats.rerank_launches);
+  hipLaunchKernelGGL(k_rerank_euc, dim3((unsigned)((R.nqry + 3) / 4)), dim3(256), 0, ctx->stream,
+                     R);
   SD_HIP(hipGetLastError());
   return SD_OK;
 }

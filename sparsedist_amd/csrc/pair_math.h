@@ -58,10 +58,12 @@ __device__ inline double cos_value(const int32_t* __restrict__ ra, const double*
   return s;
 }
 
-// Euclidean (reference fastEuclidean2, src/sparseDist.cpp:854-872).
-__device__ inline double euc_value(const int32_t* __restrict__ ra, const double* __restrict__ xa,
-                                   int64_t na, const int32_t* __restrict__ rb,
-                                   const double* __restrict__ xb, int64_t nb) {
+// Euclidean (reference fastEuclidean2, src/sparseDist.cpp:854-872): the sum
+// of squared differences over the union of stored rows in ascending row
+// order; the distance is its square root.
+__device__ inline double euc_num(const int32_t* __restrict__ ra, const double* __restrict__ xa,
+                                 int64_t na, const int32_t* __restrict__ rb,
+                                 const double* __restrict__ xb, int64_t nb) {
   int64_t a = 0, b = 0;
   double num = 0.0;
   while (a < na && b < nb) {
@@ -81,7 +83,12 @@ __device__ inline double euc_value(const int32_t* __restrict__ ra, const double*
   }
   for (; a < na; ++a) num += xa[a] * xa[a];
   for (; b < nb; ++b) num += xb[b] * xb[b];
-  return sqrt(num);
+  return num;
+}
+__device__ inline double euc_value(const int32_t* __restrict__ ra, const double* __restrict__ xa,
+                                   int64_t na, const int32_t* __restrict__ rb,
+                                   const double* __restrict__ xb, int64_t nb) {
+  return sqrt(euc_num(ra, xa, na, rb, xb, nb));
 }
 
 // Manhattan (reference fastManhattan2, src/sparseDist.cpp:750-773).

@@ -297,6 +297,52 @@ int fused_candidate_lists(sd_context* ctx, FusedArgs fa, const double** out_key,
   return SD_OK;
 }
 
+// Constants of the euclidean keys' error bound (euc_key in k_slab.hip): the
+// slab's dot product x of a query q with nnz(q) = m stored values against a
+// reference r obeys |x - q.r| <= (e0 + e1 m) sum |q_i r_i|.
+//  * split BF16 (hi = bf16(v), lo = bf16(v - hi), each within ub = 2^-8 +
+//    2^-23 relative, the second term of ub for the FP32 step): the three
+//    products hi.hi + hi.lo + lo.hi miss lo.lo and the rounding of lo, at most
+//    3.1 ub^2 |q_i r_i| per row; the FP32 accumulation adds at most 3 m terms
+//    whose magnitudes sum to at most 1.02 sum |q_i r_i|, each addition within
+//    2^-23 relative (twice the round-to-nearest unit, so the bound holds for
+//    any rounding of the accumulator), inflated by 1 / (1 - n 2^-23) for the
+//    n = 3 nrow additions of the longest column.
+//  * FP32: each input within 2^-24 and each product rounded once (2^-23
+//    allowed): 4 2^-23 per row; m accumulated terms.
+// For non-negative data the key needs q.r <= x / (1 - e), so both constants
+// are divided by 1 - e at the longest column.  The path is taken only for
+// finite values in [1e-15, 1e15] (no FP32 overflow of squares or norms, no
+// subnormal BF16 parts) and e <= 0.25.
+struct EucBound {
+  float e0 = 0.f, e1 = 0.f;
+  int nonneg = 0;
+};
+bool euc_bound(const sd_matrix* ref, const sd_matrix* X, bool f32, EucBound* out) {
+  for (const sd_matrix* m : {ref, X}) {
+    if (m->any_nonfinite) return false;
+    if (m->nnz > 0 && (m->abs_min < 1e-15 || m->abs_max > 1e15)) return false;
+    if ((double)m->nrow * m->abs_max * m->abs_max > 1e36) return false;
+  }
+  const double nrow = (double)std::max<int64_t>(X->nrow, 1);
+  const double u23 = std::ldexp(1.0, -23);
+  const double ub = std::ldexp(1.0, -8) + u23;
+  const double terms = f32 ? 1.0 : 3.0;               // accumulated terms per row
+  const double n = terms * nrow;
+  if (n * u23 >= 0.2) return false;
+  const double g = 1.0 / (1.0 - n * u23);
+  double e0 = f32 ? 4.0 * u23 : 3.1 * ub * ub;
+  double e1 = terms * 1.02 * u23 * g;
+  const double emax = e0 + e1 * nrow;
+  if (emax > 0.25) return false;
+  const double fold = 1.0 / (1.0 - emax);
+  // round the constants up to FP32
+  out->e0 = std::nextafter((float)(e0 * fold), INFINITY);
+  out->e1 = std::nextafter((float)(e1 * fold), INFINITY);
+  out->nonneg = (!ref->any_negative && !X->any_negative) ? 1 : 0;
+  return true;
+}
+
 int run_knn(sd_context* ctx, sd_matrix* X, sd_matrix* Y, const sd_knn_params* p, const KnnPlan& plan,
             int32_t* h_idx, double* h_val) {
   sd_matrix* ref = plan.selfmode ? X : Y;
@@ -327,17 +373,23 @@ int run_knn(sd_context* ctx, sd_matrix* X, sd_matrix* Y, const sd_knn_params* p,
     if (ref->tuned_margin > 0) margin = ref->tuned_margin;
   }
   // densified layout: binary -> 0/1 patterns (FP4 nibbles, or FP32 on the
-  // plain path); cosine -> unit-normalised values (BF16, or FP32)
+  // plain path); cosine -> unit-normalised values (BF16, or FP32);
+  // euclidean -> raw values (split BF16 hi/lo, or FP32)
   const bool use_f32 = (prec == SD_PREC_FP32);
-  const int kind = method == SD_BINARY ? (use_f32 ? kDenseF32Pattern : kDenseF4Pattern)
-                                       : (use_f32 ? kDenseF32Unit : kDenseBf16Unit);
+  const int kind = method == SD_BINARY   ? (use_f32 ? kDenseF32Pattern : kDenseF4Pattern)
+                   : method == SD_COSINE ? (use_f32 ? kDenseF32Unit : kDenseBf16Unit)
+                                         : (use_f32 ? kDenseF32Raw : kDenseBf16Split);
   const int64_t kpad = round_up(std::max<int64_t>(ref->nrow, 1), dense_row_align(kind));
   const bool fits = fused_fits(dense_row_bytes(kind, kpad));
   bool fused_ok_binary = (method == SD_BINARY) && prec != SD_PREC_EXACT &&
                          fused_list_len(k) > 0 && fits;
   bool fused_ok_cosine = (method == SD_COSINE) && prec != SD_PREC_EXACT &&
                          fused_list_len(k + margin) > 0 && nref >= 1 && fits;
-  if (fused_ok_binary || fused_ok_cosine) {
+  EucBound eb;
+  bool fused_ok_euclid = (method == SD_EUCLIDEAN) && prec != SD_PREC_EXACT &&
+                         fused_list_len(k + margin) > 0 && nref >= 1 && fits &&
+                         euc_bound(ref, X, use_f32, &eb);
+  if (fused_ok_binary || fused_ok_cosine || fused_ok_euclid) {
     // SD_PREC_AUTO: the cost model may prefer the merge walk (tiny batches
     // against very sparse references); both paths give identical results
     if (prec == SD_PREC_AUTO &&
@@ -346,21 +398,27 @@ int run_knn(sd_context* ctx, sd_matrix* X, sd_matrix* Y, const sd_knn_params* p,
                           (double)ref->nnz / (double)std::max<int64_t>(nref, 1))) {
       fused_ok_binary = false;
       fused_ok_cosine = false;
+      fused_ok_euclid = false;
     }
   }
+  const bool fused_rerank = fused_ok_cosine || fused_ok_euclid;
   ExactArgs ea = exact_args(ref, X, method, p->dist);
   ea.ref_deg = ref_deg;
   ea.self_mode = plan.selfmode;
   ea.include_self = p->include_self;
 
-  if (fused_ok_binary || fused_ok_cosine) {
+  if (fused_ok_binary || fused_rerank) {
     ++ctx->stats.fused_calls;
     {
       std::lock_guard<std::mutex> lk(ref->prep_mu);
       SD_TRY(prep_dense(ctx, ref, kind, kpad));
+      if (fused_ok_euclid) SD_TRY(prep_sq(ctx, ref));
       SD_HIP(hipStreamSynchronize(ctx->stream));
     }
-    if (!plan.selfmode) SD_TRY(prep_dense(ctx, X, kind, kpad));
+    if (!plan.selfmode) {
+      SD_TRY(prep_dense(ctx, X, kind, kpad));
+      if (fused_ok_euclid) SD_TRY(prep_sq(ctx, X));
+    }
     SD_TRY(check_cancel(ctx));
 
     FusedArgs fa;
@@ -377,6 +435,13 @@ int run_knn(sd_context* ctx, sd_matrix* X, sd_matrix* Y, const sd_knn_params* p,
     fa.ref_deg = ref_deg;
     fa.self_mode = plan.selfmode;
     fa.include_self = p->include_self;
+    if (fused_ok_euclid) {
+      fa.ref_aux = ref->sqf.as<float>();
+      fa.qry_aux = X->sqf.as<float>();
+      fa.euc_e0 = eb.e0;
+      fa.euc_e1 = eb.e1;
+      fa.euc_nonneg = eb.nonneg;
+    }
     fa.L = fused_list_len(method == SD_BINARY ? k : k + margin);
     const int L = fa.L;
     const double* lkey = nullptr;
@@ -387,9 +452,12 @@ int run_knn(sd_context* ctx, sd_matrix* X, sd_matrix* Y, const sd_knn_params* p,
                                 p->drop_degenerate ? X->deg.as<uint8_t>() : nullptr,
                                 out_idx.as<int32_t>(), out_val.as<double>()));
     } else {
-      // cosine: FP64 re-rank with certificate; queries it cannot clear get a
-      // second fused pass over just those columns with the widest list, and
-      // only what is still uncertified goes to the exact FP64 slab.
+      // cosine / euclidean: FP64 re-rank with certificate; queries it cannot
+      // clear get a second fused pass over just those columns with the widest
+      // list, and only what is still uncertified goes to the exact FP64 slab.
+      auto rerank = [&](const RerankArgs& RA) {
+        return fused_ok_euclid ? rerank_euclid(ctx, RA) : rerank_cosine(ctx, RA);
+      };
       DevBuf& fl = ctx->scratch[7];
       DevBuf& fl2 = ctx->scratch[8];
       SD_TRY(fl.ensure(sizeof(int32_t) * (size_t)(nq + 16)));
@@ -414,7 +482,7 @@ int run_knn(sd_context* ctx, sd_matrix* X, sd_matrix* Y, const sd_knn_params* p,
       R.qlist = nullptr;
       R.out_ld = nq;
       R.map_words = rerank_map_words(ref->nrow);
-      SD_TRY(rerank_cosine(ctx, R));
+      SD_TRY(rerank(R));
       int32_t nfail = 0;
       SD_HIP(hipMemcpyAsync(&nfail, fail_count, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
       SD_HIP(hipStreamSynchronize(ctx->stream));
@@ -466,7 +534,7 @@ int run_knn(sd_context* ctx, sd_matrix* X, sd_matrix* Y, const sd_knn_params* p,
         R2.qlist = fl.as<int32_t>();
         R2.fail_list = fl2.as<int32_t>();
         R2.fail_count = fail_count2;
-        SD_TRY(rerank_cosine(ctx, R2));
+        SD_TRY(rerank(R2));
         ctx->stats.escalated_queries += nfail;
         SD_HIP(hipMemcpyAsync(&nfail, fail_count2, sizeof(int32_t), hipMemcpyDeviceToHost,
                               ctx->stream));
@@ -625,7 +693,12 @@ int sd_matrix_upload_csc(sd_context* ctx, int64_t nrow, int64_t ncol, const int6
         prev = r;
         const double v = values[e];
         if (v == 0.0) continue;
-        if (!std::isfinite(v)) m->any_nonfinite = true;
+        if (!std::isfinite(v)) {
+          m->any_nonfinite = true;
+        } else {
+          m->abs_min = std::min(m->abs_min, std::fabs(v));
+          m->abs_max = std::max(m->abs_max, std::fabs(v));
+        }
         if (v < 0.0) m->any_negative = true;
         ri.push_back(r);
         vx.push_back(v);
@@ -679,6 +752,7 @@ int sd_matrix_drop_cache(sd_matrix* m) {
   m->css.release();
   m->centre_ready = false;
   m->sqn.release();
+  m->sqf.release();
   m->sq_ready = false;
   for (int t = 0; t < kDenseKinds; ++t) {
     m->dense[t].release();

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <atomic>
+#include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <mutex>
@@ -113,6 +114,7 @@ struct sd_matrix {
   std::vector<int64_t> hp;         // host copy of colptr (planning)
   bool any_nonfinite = false;      // values contain NA/NaN/Inf
   bool any_negative = false;
+  double abs_min = INFINITY, abs_max = 0.0;  // over finite stored values
   std::vector<uint8_t> hdeg;       // host copy of degenerate flags
   sd::DevBuf deg;                  // uint8 per column: all stored values == 0
   // cosine: unit-normalised values (nnz) and per-column max |v|
@@ -121,9 +123,10 @@ struct sd_matrix {
   // pearson / covariance: per-column mean and centred sum of squares
   bool centre_ready = false;
   sd::DevBuf mean, css;
-  // euclidean: per-column sum of squares (FP64, merge order irrelevant)
+  // euclidean: per-column sum of squares (FP64, merge order irrelevant) and
+  // the same rounded to FP32 for the slab epilogues
   bool sq_ready = false;
-  sd::DevBuf sqn;
+  sd::DevBuf sqn, sqf;
   // cosine re-rank margin tuned on this matrix as a reference (0 = default)
   int tuned_margin = 0;
   // densified layouts, one per kind: [ncol][kpad] column-contiguous
@@ -144,8 +147,11 @@ struct sd_graph {
 
 namespace sd {
 
-// densified layout kinds
-enum DenseKind { kDenseF4Pattern = 0, kDenseBf16Unit = 1, kDenseBf16Raw = 2, kDenseF32Unit = 3,
+// densified layout kinds.  kDenseBf16Split stores each value v as
+// hi = bf16(v) and lo = bf16(v - hi) in three row segments [hi | hi | lo]; a
+// query read with its last two segments swapped, [hi | lo | hi], gives the
+// dot product hi.hi + hi.lo + lo.hi in one MFMA pass over 3 kpad.
+enum DenseKind { kDenseF4Pattern = 0, kDenseBf16Unit = 1, kDenseBf16Split = 2, kDenseF32Unit = 3,
                  kDenseF32Raw = 4, kDenseF32Pattern = 5, kDenseKinds = 6 };
 
 // ---- prep (k_prep.hip) ------------------------------------------------------
@@ -193,6 +199,11 @@ struct FusedArgs {
   const int64_t* qry_p = nullptr;
   const float* ref_aux = nullptr;   // per-column float aux (euclid: ||x||^2)
   const float* qry_aux = nullptr;
+  // euclidean keys: the dot product's error bound is (euc_e0 + euc_e1 nnz(q))
+  // times sum |q_i r_i| (see euc_key in k_slab.hip); euc_nonneg: every value
+  // of both matrices is >= 0, so that sum is the dot product itself
+  float euc_e0 = 0.f, euc_e1 = 0.f;
+  int euc_nonneg = 0;
   const uint8_t* ref_deg = nullptr;
   int self_mode = 0, include_self = 0;
   int L = 0;                        // list length kept per query
@@ -229,8 +240,14 @@ int rerank_map_words(int64_t nrow);
 inline int64_t dense_row_align(int kind) { return kind == kDenseF4Pattern ? 512 : 256; }
 inline int64_t dense_row_bytes(int kind, int64_t kpad) {
   return kind == kDenseF4Pattern ? kpad / 2
-         : (kind == kDenseBf16Unit || kind == kDenseBf16Raw) ? kpad * 2
-                                                             : kpad * 4;
+         : kind == kDenseBf16Unit ? kpad * 2
+         : kind == kDenseBf16Split ? kpad * 6
+                                   : kpad * 4;
+}
+// Bytes of one segment whose query copy is read swapped with the next one
+// (the split layout's hi | lo), 0 for layouts read as they are.
+inline int64_t dense_qseg_bytes(int kind, int64_t kpad) {
+  return kind == kDenseBf16Split ? kpad * 2 : 0;
 }
 // Cost-model choice between the densified path and the FP64 merge walk for
 // a search of nq query columns against nref reference columns.
@@ -257,6 +274,10 @@ int merge_lists(sd_context* ctx, const double* key, const int32_t* idx, int nlis
 int finish_exact_lists(sd_context* ctx, const double* key, const int32_t* idx, int64_t nqry, int L,
                        int k, int dist, const uint8_t* qdeg, int32_t* out_idx, double* out_val);
 int rerank_cosine(sd_context* ctx, const RerankArgs& R);
+// Euclidean: exact FP64 distances of the candidates in the reference's
+// arithmetic; the certificate compares the k-th squared distance with the
+// lower bound that the list's last key stands for.
+int rerank_euclid(sd_context* ctx, const RerankArgs& R);
 int topk_rows(sd_context* ctx, const double* D, int64_t nrow, int64_t ncol, int k, int decreasing,
               int32_t* best_idx, double* best_val, int64_t col_offset, const int32_t* self_col);
 

@@ -19,14 +19,13 @@ namespace sd {
 bool tune_prefer_exact(const sd_context* ctx, int kind, int64_t nq, int64_t nref, int64_t kpad,
                        double nnz_q, double nnz_ref) {
   const TuneRates& t = ctx->tune;
-  const double rate = kind == kDenseF4Pattern   ? t.fp4_ops
-                      : kind == kDenseBf16Unit ? t.bf16_ops
-                                               : t.fp32_ops;
-  const double fixed = kind == kDenseF4Pattern   ? t.fp4_fixed_s
-                       : kind == kDenseBf16Unit ? t.bf16_fixed_s
-                                                : t.fp32_fixed_s;
+  const bool bf16 = kind == kDenseBf16Unit || kind == kDenseBf16Split;
+  const double rate = kind == kDenseF4Pattern ? t.fp4_ops : bf16 ? t.bf16_ops : t.fp32_ops;
+  const double fixed = kind == kDenseF4Pattern ? t.fp4_fixed_s : bf16 ? t.bf16_fixed_s : t.fp32_fixed_s;
+  // the split layout runs three products per row
+  const double kwork = (double)kpad * (kind == kDenseBf16Split ? 3.0 : 1.0);
   const double qcols = (double)((nq + 255) / 256 * 256);
-  const double fused = fixed + qcols * (double)nref * 2.0 * (double)kpad / rate;
+  const double fused = fixed + qcols * (double)nref * 2.0 * kwork / rate;
   const double exact =
       t.exact_fixed_s + (double)nq * (double)nref * (nnz_q + nnz_ref) / t.merge_steps;
   return exact < fused;

@@ -1,0 +1,128 @@
+"""The euclidean slab keys' error bound, checked against an emulation of the
+arithmetic on the CPU.
+
+The slab computes x = sum hi_q hi_r + hi_q lo_r + lo_q hi_r (hi = bf16(v),
+lo = bf16(v - hi), products exact in FP32, FP32 accumulation) and keys each
+pair with minus a lower bound of the squared distance (euc_key in
+sparsedist_amd/csrc/k_slab.hip, constants from euc_bound in
+sparsedist_amd/csrc/sd_api.cpp).  Here the same rounding steps are emulated
+with numpy (bf16 by bit rounding of FP32, the accumulation as a sequential
+FP32 cumulative sum in a random order) and the exact dot products come from
+math.fsum, so the check does not reuse the formula it tests:
+
+* |x - q.r| <= (e0 + e1 m) sum |q_i r_i| for nonnegative and signed data;
+* the key's lower bound never exceeds the exact squared distance;
+* a slab that dropped the lo.hi term breaks the first bound (the check has
+  teeth for the mistake it guards against).
+"""
+import math
+
+import numpy as np
+import pytest
+
+U23 = 2.0 ** -23
+UB = 2.0 ** -8 + U23
+K_EUC_ROUND = np.float32(2.0 ** -21)
+K_EUC_ABS = np.float32(1e-30)
+
+
+def bf16(x32):
+    """Round FP32 to BF16 (round to nearest even), returned as FP32."""
+    b = np.asarray(x32, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    b = (b + 0x7FFF + ((b >> 16) & 1)) & 0xFFFF0000
+    return b.astype(np.uint32).view(np.float32)
+
+
+def split(v):
+    hi = bf16(v.astype(np.float32))
+    lo = bf16((v - hi.astype(np.float64)).astype(np.float32))
+    return hi, lo
+
+
+def constants(nrow, terms=3.0):
+    n = terms * nrow
+    g = 1.0 / (1.0 - n * U23)
+    e0 = 3.1 * UB * UB
+    e1 = terms * 1.02 * U23 * g
+    return e0, e1
+
+
+def slab_dot(q, r, rng, drop_lo_hi=False):
+    qh, ql = split(q)
+    rh, rl = split(r)
+    parts = [qh.astype(np.float64) * rh, qh.astype(np.float64) * rl]
+    if not drop_lo_hi:
+        parts.append(ql.astype(np.float64) * rh)
+    t = np.concatenate(parts).astype(np.float32)  # products of BF16 pairs are exact
+    t = t[t != 0]
+    rng.shuffle(t)
+    return float(np.cumsum(t, dtype=np.float32)[-1]) if t.size else 0.0
+
+
+def pair(rng, nrow, nnz, overlap, signed):
+    rows_q = rng.choice(nrow, nnz, replace=False)
+    keep = rows_q[: int(overlap * nnz)]
+    rest = np.setdiff1d(np.arange(nrow), rows_q)
+    rows_r = np.concatenate([keep, rng.choice(rest, nnz - keep.size, replace=False)])
+    q = np.zeros(nrow)
+    r = np.zeros(nrow)
+    # log1p-like magnitudes, with values just past BF16 rounding ties mixed in
+    vq = np.log1p(rng.gamma(0.5, 4.0, nnz) * 50) + 0.01
+    vr = np.log1p(rng.gamma(0.5, 4.0, nnz) * 50) + 0.01
+    tie = rng.random(nnz) < 0.3
+    vq[tie] = (1.0 + 2.0 ** -8 + 2.0 ** -20) * 2.0 ** rng.integers(-3, 3, tie.sum())
+    if signed:
+        vq *= rng.choice([-1.0, 1.0], nnz)
+        vr *= rng.choice([-1.0, 1.0], nnz)
+    q[rows_q] = vq
+    r[rows_r] = vr
+    return q, r
+
+
+def key_lower_bound(qa, ra, x, m, e0, e1, nonneg):
+    """euc_key's arithmetic in FP32, returning the lower bound (minus the key)."""
+    f = np.float32
+    qa, ra, x = f(qa), f(ra), f(x)
+    n2 = f(qa + ra)
+    s = f(n2 - f(f(2.0) * x))
+    e = f(f(e0) + f(f(e1) * f(m)))
+    if nonneg:
+        err = f(f(f(f(2.0) * e) * f(abs(x))) + f(K_EUC_ROUND * n2))
+    else:
+        err = f(f(f(e + K_EUC_ROUND) * n2))
+    err = f(err + K_EUC_ABS)
+    return -float(f(err - s))
+
+
+@pytest.mark.parametrize("signed", [False, True])
+def test_dot_bound_and_lower_bound(signed):
+    rng = np.random.default_rng(7 + signed)
+    nrow = 4000
+    e0, e1 = constants(nrow)
+    emax = e0 + e1 * nrow
+    fold = 1.0 / (1.0 - emax)
+    for trial in range(60):
+        nnz = int(rng.integers(50, 1500))
+        q, r = pair(rng, nrow, nnz, rng.random(), signed)
+        x = slab_dot(q, r, rng)
+        exact = math.fsum(q * r)
+        s_abs = math.fsum(np.abs(q * r))
+        assert abs(x - exact) <= (e0 + e1 * nnz) * s_abs + 1e-300, trial
+        d2 = math.fsum((q - r) ** 2)
+        lb = key_lower_bound(math.fsum(q * q), math.fsum(r * r), x, nnz, e0 * fold, e1 * fold,
+                             not signed)
+        assert lb <= d2, (trial, lb, d2)
+
+
+def test_missing_cross_term_is_caught():
+    rng = np.random.default_rng(11)
+    nrow = 4000
+    e0, e1 = constants(nrow)
+    broken = 0
+    for trial in range(20):
+        q, r = pair(rng, nrow, 800, 0.9, False)
+        x = slab_dot(q, r, rng, drop_lo_hi=True)
+        exact = math.fsum(q * r)
+        s_abs = math.fsum(np.abs(q * r))
+        broken += abs(x - exact) > (e0 + e1 * 800) * s_abs
+    assert broken >= 15

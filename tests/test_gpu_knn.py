@@ -337,7 +337,7 @@ def test_certificate_escalation_and_fallback(oracle_lib, gpu_ctx):
     assert st["fallback_queries"] > 0
 
 
-@pytest.mark.parametrize("method", ["binary", "cosine"])
+@pytest.mark.parametrize("method", ["binary", "cosine", "euclidean"])
 def test_two_pass_filter_path(oracle_lib, gpu_ctx, method):
     """Enough reference columns (>= 128 tiles of 128) for the threshold
     pre-pass + filtered full pass; queries are a subset so the oracle stays
@@ -355,3 +355,60 @@ def test_two_pass_filter_path(oracle_lib, gpu_ctx, method):
     o = oracle_lib.knn(X, Y, k=20, method=method, nthreads=NT)
     same((g.idx, g.dist), o, method)
     assert st["overflow_queries"] >= 1
+
+
+@pytest.mark.parametrize("prec", ["bf16", "fp32"])
+def test_euclidean_count_matrix_certified(oracle_lib, gpu_ctx, prec):
+    """Euclidean on the count-matrix rows and density: split-BF16 (or FP32)
+    slab with per-pair lower-bound keys, FP64 re-rank.  Nearest distances on
+    such data sit within a fraction of a percent of each other, so this also
+    checks that the bound is tight enough to certify nearly every query."""
+    X = synth.nb_log1p(20000, 1200, 0.06, 12)
+    gpu_ctx.stats(reset=True)
+    g = sd.sparseKNN(X, k=20, method="euclidean", precision=prec, verbose=False)
+    st = gpu_ctx.stats(reset=True)
+    o = oracle_lib.knn(X, k=20, method="euclidean", nthreads=NT)
+    same((g.idx, g.dist), o, f"euclidean {prec}")
+    assert st["fused_calls"] == 1
+    # at most 5% need the wider second list, at most 1% the exact fallback
+    assert st["escalated_queries"] <= 60 and st["fallback_queries"] <= 12, st
+
+
+def test_euclidean_signed_and_near_duplicates(oracle_lib, gpu_ctx):
+    """Signed values take the general bound (sum |q_i r_i| <= |q||r|); columns
+    equal up to 1e-7 relative noise cannot be ordered by any slab and go to the
+    exact fallback.  Both must match the oracle."""
+    X = rand_sparse(300, 500, 0.1, 51, signed=True, empty_cols=(3,))
+    Y = rand_sparse(300, 900, 0.1, 52, signed=True, empty_cols=(8,))
+    for include_self in (False, True):
+        g = sd.sparseKNN(X, k=7, method="euclidean", include_self=include_self,
+                         precision="bf16", verbose=False)
+        o = oracle_lib.knn(X, k=7, method="euclidean", include_self=include_self, nthreads=NT)
+        same((g.idx, g.dist), o, f"signed self include_self={include_self}")
+    g = sd.sparseKNN(X, Y, k=7, method="euclidean", precision="bf16", verbose=False)
+    o = oracle_lib.knn(X, Y, k=7, method="euclidean", nthreads=NT)
+    same((g.idx, g.dist), o, "signed cross")
+    rng = np.random.default_rng(53)
+    base = synth.abs_normal(400, 1, 0.2, 54).toarray()
+    cols = [base * (1 + 1e-7 * rng.standard_normal(base.shape)) for _ in range(120)]
+    cols.append(synth.abs_normal(400, 100, 0.2, 55).toarray())
+    Xd = sp.csc_matrix(np.hstack(cols))
+    gpu_ctx.stats(reset=True)
+    g = sd.sparseKNN(Xd, k=10, method="euclidean", precision="bf16", verbose=False)
+    st = gpu_ctx.stats(reset=True)
+    o = oracle_lib.knn(Xd, k=10, method="euclidean", nthreads=NT)
+    same((g.idx, g.dist), o, "near duplicates")
+    assert st["fallback_queries"] > 0
+
+
+def test_euclidean_out_of_range_values_take_exact_path(oracle_lib, gpu_ctx):
+    """Values outside [1e-15, 1e15] (FP32 squares would overflow or BF16 lo
+    parts become subnormal) are searched on the FP64 merge walk."""
+    X = rand_sparse(60, 80, 0.3, 56)
+    X.data[::7] *= 1e17
+    gpu_ctx.stats(reset=True)
+    g = sd.sparseKNN(X, k=5, method="euclidean", precision="bf16", verbose=False)
+    st = gpu_ctx.stats(reset=True)
+    o = oracle_lib.knn(X, k=5, method="euclidean", nthreads=NT)
+    same((g.idx, g.dist), o, "large values")
+    assert st["fused_calls"] == 0 and st["exact_calls"] == 1
