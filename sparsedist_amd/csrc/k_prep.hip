@@ -174,6 +174,17 @@ __global__ __launch_bounds__(256) void k_densify_split(const int64_t* __restrict
   }
 }
 
+// FP32 pearson aux: sqrt(nrow) mean and 1 / sqrt(centred sum of squares)
+// (0 when the sum is 0: a constant column, whose correlations are all 0).
+__global__ void k_centre_f32(const double* __restrict__ mean, const double* __restrict__ css,
+                             int64_t ncol, int64_t nrow, float* __restrict__ cm,
+                             float* __restrict__ ci) {
+  const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (j >= ncol) return;
+  cm[j] = (float)(sqrt((double)nrow) * mean[j]);
+  ci[j] = css[j] > 0.0 ? (float)(1.0 / sqrt(css[j])) : 0.f;
+}
+
 // FP32 copies of the FP64 squared norms.
 __global__ void k_to_f32(const double* __restrict__ in, int64_t n, float* __restrict__ out) {
   const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -205,9 +216,14 @@ int prep_centre(sd_context* ctx, sd_matrix* m) {
   if (m->centre_ready) return SD_OK;
   SD_TRY(m->mean.alloc(sizeof(double) * (size_t)(m->ncol + 1)));
   SD_TRY(m->css.alloc(sizeof(double) * (size_t)(m->ncol + 1)));
+  SD_TRY(m->cmf.alloc(sizeof(float) * (size_t)(m->ncol + 1)));
+  SD_TRY(m->cif.alloc(sizeof(float) * (size_t)(m->ncol + 1)));
   const int64_t nb = (m->ncol + 255) / 256;
   hipLaunchKernelGGL(k_centre, dim3((unsigned)nb), dim3(256), 0, ctx->stream, m->p.as<int64_t>(),
                      m->x.as<double>(), m->ncol, m->nrow, m->mean.as<double>(), m->css.as<double>());
+  hipLaunchKernelGGL(k_centre_f32, dim3((unsigned)nb), dim3(256), 0, ctx->stream,
+                     m->mean.as<double>(), m->css.as<double>(), m->ncol, m->nrow,
+                     m->cmf.as<float>(), m->cif.as<float>());
   SD_HIP(hipGetLastError());
   SD_HIP(hipStreamSynchronize(ctx->stream));
   m->centre_ready = true;

@@ -116,6 +116,25 @@ __device__ inline float euc_key(float qa, float ra, float x, uint32_t qnnz, floa
                            : (e + kEucRound) * n2 + kEucAbs;
   return err - s;
 }
+// Pearson key: an upper bound of the correlation
+// r = (q.y - n mq my) / (sq sy) from FP32 aux a = sqrt(n) mean and
+// c = 1 / sqrt(centred sum of squares) per column and the slab's raw dot x.
+// The dot obeys |x - q.y| <= e S with S = sum |q_i y_i|: S = q.y <= x / (1 - e)
+// for non-negative data (folded into e by the host), else
+// S <= |q||y| = sq sy sqrt(1 + (aq cq)^2) sqrt(1 + (ay cy)^2).  kPearRound
+// covers the FP32 rounding of the aux values and of this formula (at most
+// ten roundings, 2^-24 each, of quantities below (|x| + |t|) c_q c_y or S).
+constexpr float kPearRound = 9.5367431640625e-07f;  // 2^-20
+__device__ inline float pear_key(float qa, float qc, float ra, float rc, float x, uint32_t qnnz,
+                                 float e0, float e1, int nonneg) {
+  const float t = qa * ra;  // n mean_q mean_y
+  const float sc = qc * rc;
+  const float r = (x - t) * sc;
+  const float e = e0 + e1 * (float)qnnz;
+  const float s = nonneg ? fabsf(x) * sc
+                         : sqrtf(1.f + (qa * qc) * (qa * qc)) * sqrtf(1.f + (ra * rc) * (ra * rc));
+  return r + ((e + kPearRound) * s + kPearRound * (fabsf(x) + fabsf(t)) * sc + kEucAbs);
+}
 __device__ inline int qseg_offset(int soff, int qseg) {
   return (qseg == 0 || soff < qseg) ? soff : (soff < 2 * qseg ? soff + qseg : soff - qseg);
 }
@@ -137,6 +156,8 @@ struct FusedKernelArgs {
   const int64_t* qry_p;
   const float* ref_aux;
   const float* qry_aux;
+  const float* ref_aux2;
+  const float* qry_aux2;
   const uint8_t* ref_deg;
   int self_mode, include_self, method;
   int splits, qtiles, tiles_per_split;
@@ -154,8 +175,8 @@ struct FusedKernelArgs {
   // split layouts: query rows are read with the segments at byte offsets
   // [qseg, 2 qseg) and [2 qseg, 3 qseg) swapped (0: read as stored)
   int qseg;
-  float euc_e0, euc_e1;  // euclidean: dot error bound (e0 + e1 nnz(q)) sum |q_i r_i|
-  int euc_nonneg;        // both matrices >= 0: that sum is the dot product
+  float dot_e0, dot_e1;  // euclidean: dot error bound (e0 + e1 nnz(q)) sum |q_i r_i|
+  int dot_nonneg;        // both matrices >= 0: that sum is the dot product
 };
 
 namespace {
@@ -204,8 +225,14 @@ __device__ inline void wait_vm() {
   __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | 0x0F70);
 }
 
-// QS: query rows are read with the split layout's segments swapped (qseg).
-template <class Op, class K, int L, int WM, bool QS>
+// Key kinds of the FP32-accumulator kernels, fixed at compile time so each
+// instantiation carries only its own epilogue: the score itself (cosine),
+// minus a squared-distance lower bound (euclidean), a correlation upper
+// bound (pearson).  The last two read query rows with the split layout's
+// segments swapped (qseg; 0 on the FP32 layout, which is read as stored).
+enum KeyKind { kKeyScore = 0, kKeyEuclid = 1, kKeyPearson = 2 };
+
+template <class Op, class K, int L, int WM, int MK>
 __global__ __launch_bounds__(WM * 256, 1) void k_fused(FusedKernelArgs P) {
   constexpr int WAVES = WM * 4;
   constexpr int NT = WAVES * 64;
@@ -231,6 +258,7 @@ __global__ __launch_bounds__(WM * 256, 1) void k_fused(FusedKernelArgs P) {
   uint32_t* xch = reinterpret_cast<uint32_t*>(st0);
   __shared__ int32_t s_rnnz[TM];
   __shared__ float s_raux[TM];
+  __shared__ float s_raux2[TM];
   __shared__ uint8_t s_rdeg[TM];
 
   const int tid = threadIdx.x;
@@ -245,10 +273,11 @@ __global__ __launch_bounds__(WM * 256, 1) void k_fused(FusedKernelArgs P) {
   const bool q_ok = my_pos < P.nqry;
   const int64_t my_q = q_ok ? (P.qlist ? (int64_t)P.qlist[my_pos] : my_pos) : -1;
   uint32_t nj = 0;
-  float qaux = 0.f;
+  float qaux = 0.f, qaux2 = 0.f;
   if (q_ok) {
     nj = (uint32_t)(P.qry_p[my_q + 1] - P.qry_p[my_q]);
     if (P.qry_aux) qaux = P.qry_aux[my_q];
+    if (P.qry_aux2) qaux2 = P.qry_aux2[my_q];
   }
 
   Entry<K> list[L];
@@ -298,10 +327,12 @@ __global__ __launch_bounds__(WM * 256, 1) void k_fused(FusedKernelArgs P) {
       if (r < P.nref) {
         s_rnnz[t] = (int32_t)(P.ref_p[r + 1] - P.ref_p[r]);
         s_raux[t] = P.ref_aux ? P.ref_aux[r] : 0.f;
+        s_raux2[t] = P.ref_aux2 ? P.ref_aux2[r] : 0.f;
         s_rdeg[t] = P.ref_deg ? P.ref_deg[r] : 0;
       } else {
         s_rnnz[t] = 0;
         s_raux[t] = 0.f;
+        s_raux2[t] = 0.f;
         s_rdeg[t] = 1;
       }
     }
@@ -327,7 +358,7 @@ __global__ __launch_bounds__(WM * 256, 1) void k_fused(FusedKernelArgs P) {
     }
     auto issue = [&](uint8_t* st, int64_t c) {
       const int soff = (int)(c * kRowBytes);
-      const int soff_b = QS ? qseg_offset(soff, P.qseg) : soff;
+      const int soff_b = MK != kKeyScore ? qseg_offset(soff, P.qseg) : soff;
 #pragma unroll
       for (int u = 0; u < A_INST; ++u)
         __builtin_amdgcn_raw_ptr_buffer_load_lds(
@@ -479,9 +510,13 @@ __global__ __launch_bounds__(WM * 256, 1) void k_fused(FusedKernelArgs P) {
           } else {
             float sv;
             __builtin_memcpy(&sv, &bits, 4);
-            if (P.method == SD_EUCLIDEAN) {
-              key = euc_key(qaux, s_raux[rloc], sv, nj, P.euc_e0, P.euc_e1, P.euc_nonneg);
+            if constexpr (MK == kKeyEuclid) {
+              key = euc_key(qaux, s_raux[rloc], sv, nj, P.dot_e0, P.dot_e1, P.dot_nonneg);
               if (P.self_mode && rg == my_q) key = FLT_MAX;  // distance 0
+            } else if constexpr (MK == kKeyPearson) {
+              key = pear_key(qaux, qaux2, s_raux[rloc], s_raux2[rloc], sv, nj, P.dot_e0, P.dot_e1,
+                             P.dot_nonneg);
+              if (P.self_mode && rg == my_q) key = FLT_MAX;  // restored self correlation
             } else {
               key = sv;
               if (P.self_mode && rg == my_q) key = 2.0f;  // restored self similarity
@@ -543,7 +578,7 @@ struct WideLayout {
   __device__ static int swz(int row) { return (row >> kRowShift) & (kSlots - 1); }
 };
 
-template <class Op, class K, int RB, bool QS>
+template <class Op, class K, int RB, int MK>
 __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
   using Lay = WideLayout<RB>;
   constexpr int NT = 512;
@@ -561,6 +596,7 @@ __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
   __shared__ __align__(16) uint8_t st3[NS > 2 ? STAGE_BYTES : 16];
   __shared__ __align__(16) int32_t s_rnnz[TM];
   __shared__ float s_raux[TM];
+  __shared__ float s_raux2[TM];
   __shared__ uint8_t s_rdeg[TM];
 
   const int tid = threadIdx.x;
@@ -580,10 +616,12 @@ __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
     if (r < P.nref) {
       s_rnnz[t] = (int32_t)(P.ref_p[r + 1] - P.ref_p[r]);
       s_raux[t] = P.ref_aux ? P.ref_aux[r] : 0.f;
+      s_raux2[t] = P.ref_aux2 ? P.ref_aux2[r] : 0.f;
       s_rdeg[t] = P.ref_deg ? P.ref_deg[r] : 0;
     } else {
       s_rnnz[t] = 0;
       s_raux[t] = 0.f;
+      s_raux2[t] = 0.f;
       s_rdeg[t] = 1;
     }
   }
@@ -612,7 +650,7 @@ __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
   typedef __attribute__((address_space(3))) void lds_void;
   auto issue = [&](uint8_t* st, int64_t c) {
     const int soff = (int)(c * RB);
-    const int soff_b = QS ? qseg_offset(soff, P.qseg) : soff;
+    const int soff_b = MK != kKeyScore ? qseg_offset(soff, P.qseg) : soff;
 #pragma unroll
     for (int u = 0; u < A_INST; ++u)
       __builtin_amdgcn_raw_ptr_buffer_load_lds(
@@ -719,7 +757,7 @@ __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
   int64_t qp[2], qc[2];
   bool qv[2];
   uint32_t qn[2];
-  float qa[2], thr_f[2];
+  float qa[2], qa2[2], thr_f[2];
   double thr_d[2];
 #pragma unroll
   for (int j = 0; j < 2; ++j) {
@@ -728,6 +766,7 @@ __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
     qc[j] = qv[j] ? (P.qlist ? (int64_t)P.qlist[qp[j]] : qp[j]) : -1;
     qn[j] = qv[j] ? (uint32_t)(P.qry_p[qc[j] + 1] - P.qry_p[qc[j]]) : 0u;
     qa[j] = (qv[j] && P.qry_aux) ? P.qry_aux[qc[j]] : 0.f;
+    qa2[j] = (qv[j] && P.qry_aux2) ? P.qry_aux2[qc[j]] : 0.f;
     thr_d[j] = qv[j] ? P.thr[qp[j]] : INFINITY;
     // float screen a little below the threshold; the exact test follows
     const float t = (float)thr_d[j];
@@ -746,9 +785,13 @@ __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
       kd = (d == 0) ? 1.0 : key_to_double(KeyJ{cc, d});
     } else {
       float key;
-      if (P.method == SD_EUCLIDEAN) {
+      if constexpr (MK == kKeyEuclid) {
         key = self ? FLT_MAX  // distance 0
-                   : euc_key(qa[j], s_raux[rloc], x, qn[j], P.euc_e0, P.euc_e1, P.euc_nonneg);
+                   : euc_key(qa[j], s_raux[rloc], x, qn[j], P.dot_e0, P.dot_e1, P.dot_nonneg);
+      } else if constexpr (MK == kKeyPearson) {
+        key = self ? FLT_MAX  // restored self correlation
+                   : pear_key(qa[j], qa2[j], s_raux[rloc], s_raux2[rloc], x, qn[j], P.dot_e0,
+                              P.dot_e1, P.dot_nonneg);
       } else {
         key = self ? 2.0f : x;  // restored self similarity
       }
@@ -763,7 +806,7 @@ __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
       // c / (ni + nj - c) >= thr  <=>  c >= thr (ni + nj - c)
       const float d = (float)((uint32_t)ni + qn[j]) - x;
       return x >= thr_f[j] * d || d <= 0.f;
-    } else if (P.method == SD_EUCLIDEAN) {
+    } else if constexpr (MK != kKeyScore) {
       return true;
     } else {
       return x >= thr_f[j];
@@ -1086,21 +1129,26 @@ __global__ __launch_bounds__(256) void k_rerank(RerankArgs R) {
   }
 }
 
-// ---- euclidean re-rank ----------------------------------------------------------
-// One wave per query, lane t < L takes candidate t: its exact squared
-// distance num (the reference's merge walk, euc_num) and distance sqrt(num).
-// Lanes are sorted by (distance, index).  Certificate: every reference
-// outside the list has key <= the list's last key, so its true squared
-// distance D is at least LB = -key[L-1] (euc_key's lower bound), and the
-// reference's FP64 value of D is at least D (1 - rho) with
-// rho = (nnz(q) + nrow + 4) 2^-52 (a sum of at most nnz(q) + nrow
-// non-negative terms, each a rounded square of a rounded difference).  The
-// list holds the exact top k when the k-th squared distance num_k satisfies
-// num_k (1 + 2^-49) < LB (1 - rho): then every outside reference's value
-// exceeds num_k by more than a rounding of sqrt, so it ranks strictly after.
+// ---- euclidean / pearson re-rank ---------------------------------------------
+// One wave per query, lane t < L takes candidate t and computes its exact
+// FP64 value in the reference's arithmetic; lanes are sorted by (value in the
+// method's direction, index).  Every reference outside the list has a key at
+// most the list's last key, which bounds its true value:
+//  * euclidean: the key is minus a lower bound LB of the squared distance D.
+//    The reference's FP64 sum of squares is at least D (1 - rho) with
+//    rho = (nnz(q) + nrow + 4) 2^-52 (at most nnz(q) + nrow non-negative
+//    terms, each a rounded square of a rounded difference).  Certified when
+//    the k-th squared distance num_k satisfies num_k (1 + 2^-49) < LB (1 - rho):
+//    every outside value then exceeds num_k by more than a rounding of sqrt.
+//  * pearson: the key is an upper bound UB of the correlation r.  The FP64
+//    correlation differs from r by at most delta = (nnz(q) + 2 nrow + 32) 2^-50
+//    (merge-walk and sum-of-squares rounding, relative to sq sy).  Certified
+//    when the k-th correlation exceeds UB + delta + 1e-12 (the last term keeps
+//    1 - r distinct after rounding in the distance form).
 constexpr int kRerankEucQCap = 1024;  // query entries cached in LDS per wave
 
-__global__ __launch_bounds__(256) void k_rerank_euc(RerankArgs R) {
+template <int M>
+__global__ __launch_bounds__(256) void k_rerank_exact(RerankArgs R) {
   __shared__ int32_t s_qr[4][kRerankEucQCap];
   __shared__ double s_qv[4][kRerankEucQCap];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -1125,21 +1173,30 @@ __global__ __launch_bounds__(256) void k_rerank_euc(RerankArgs R) {
   const int32_t ci = (lane < L) ? R.cand_idx[j * L + lane] : -1;
   const bool deg_q = R.qdeg && R.qdeg[jq];
   const bool valid = ci >= 0 && !deg_q;
-  double num = 0.0, v = 0.0;
+  double c = 0.0, v = 0.0;  // euclidean: c = sum of squares; pearson: c = r
   if (valid) {
-    const int64_t a0 = R.ex.ref.p[ci], na = R.ex.ref.p[ci + 1] - a0;
-    num = euc_num(R.ex.ref.i + a0, R.ex.ref.x + a0, na, qr, qv, qn);
-    v = sqrt(num);
+    if (M == SD_EUCLIDEAN) {
+      const int64_t a0 = R.ex.ref.p[ci], na = R.ex.ref.p[ci + 1] - a0;
+      c = euc_num(R.ex.ref.i + a0, R.ex.ref.x + a0, na, qr, qv, qn);
+      v = sqrt(c);
+    } else {
+      if (R.ex.self_mode && R.ex.include_self && ci == jq)
+        c = 1.0;  // restored self value (R/sparseKNN.R:270-279)
+      else
+        c = pear_value(R.ex, ci, qr, qv, qn, jq);
+      v = R.ex.dist ? (1.0 - c) : c;
+      if (v == 0.0) v = 0.0;
+    }
   }
-  uint64_t key = valid ? order_key(v, 0) : ~0ULL;
+  uint64_t key = valid ? order_key(v, R.decreasing) : ~0ULL;
   uint32_t id = valid ? (uint32_t)ci : 0xFFFFFFFFu;
-  double nv = num, vv = v;
+  double cv = c, vv = v;
   for (int size = 2; size <= 64; size <<= 1) {
     for (int stride = size >> 1; stride > 0; stride >>= 1) {
       const int partner = lane ^ stride;
       const uint64_t pk = wave_shfl_u64(key, partner);
       const uint32_t pi = (uint32_t)__shfl((int)id, partner);
-      const double pn = __shfl(nv, partner);
+      const double pc = __shfl(cv, partner);
       const double pv = __shfl(vv, partner);
       const bool is_lo = (lane & stride) == 0;
       const bool asc = (lane & size) == 0;
@@ -1149,7 +1206,7 @@ __global__ __launch_bounds__(256) void k_rerank_euc(RerankArgs R) {
       if (!keep_mine) {
         key = pk;
         id = pi;
-        nv = pn;
+        cv = pc;
         vv = pv;
       }
     }
@@ -1157,10 +1214,17 @@ __global__ __launch_bounds__(256) void k_rerank_euc(RerankArgs R) {
   const int nvalid = __popcll(__ballot(valid));
   bool certified = true;
   if (nvalid >= L && R.k <= nvalid) {
-    const double lb = -R.cand_key[j * L + (L - 1)];
-    const double numk = __shfl(nv, R.k - 1);
-    const double rho = ((double)qn + (double)R.ex.ref.nrow + 4.0) * 0x1p-52;
-    certified = lb > 0.0 && numk * (1.0 + 0x1p-49) < lb * (1.0 - rho);
+    const double lastkey = R.cand_key[j * L + (L - 1)];
+    const double ck = __shfl(cv, R.k - 1);
+    const double nrow = (double)R.ex.ref.nrow;
+    if (M == SD_EUCLIDEAN) {
+      const double lb = -lastkey;
+      const double rho = ((double)qn + nrow + 4.0) * 0x1p-52;
+      certified = lb > 0.0 && ck * (1.0 + 0x1p-49) < lb * (1.0 - rho);
+    } else {
+      const double delta = ((double)qn + 2.0 * nrow + 32.0) * 0x1p-50;
+      certified = ck > lastkey + delta + 1e-12;
+    }
   }
   if (deg_q) certified = true;
   if (!certified) {
@@ -1188,12 +1252,12 @@ __global__ __launch_bounds__(256) void k_rerank_euc(RerankArgs R) {
 // host launchers
 // ----------------------------------------------------------------------------
 
-template <class Op, class K, int L, int WM, bool QS>
+template <class Op, class K, int L, int WM, int MK>
 static int launch_fused_t(sd_context* ctx, FusedKernelArgs& P) {
   const int64_t nblk = (int64_t)P.qtiles * P.splits;
   if (nblk * WM * 256 >= ((int64_t)1 << 32))
     return set_error(SD_EUNSUPPORTED, "slab grid too large (%lld workgroups)", (long long)nblk);
-  hipLaunchKernelGGL((k_fused<Op, K, L, WM, QS>), dim3((unsigned)nblk), dim3(WM * 256), 0,
+  hipLaunchKernelGGL((k_fused<Op, K, L, WM, MK>), dim3((unsigned)nblk), dim3(WM * 256), 0,
                      ctx->stream, P);
   SD_HIP(hipGetLastError());
   return SD_OK;
@@ -1202,7 +1266,7 @@ static int launch_fused_t(sd_context* ctx, FusedKernelArgs& P) {
 // K bytes per row per chunk of the filtered pass (64 or 128)
 constexpr int kFilterRowBytes = 128;
 
-template <class Op, class K, int RB, bool QS = false>
+template <class Op, class K, int RB, int MK = kKeyScore>
 static int launch_filter_t(sd_context* ctx, FusedKernelArgs& P) {
   if (P.kpad_bytes <= 0 || P.kpad_bytes % (WideLayout<RB>::kStages * RB) != 0)
     return set_error(SD_EINVAL, "internal: padded row bytes %lld are not whole chunk groups",
@@ -1216,22 +1280,22 @@ static int launch_filter_t(sd_context* ctx, FusedKernelArgs& P) {
   const int64_t gx = std::min<int64_t>((need + 7) / 8 * 8, (int64_t)1 << 20);
   const int64_t gy = (need + gx - 1) / gx;
   if (gy > 65535) return set_error(SD_EUNSUPPORTED, "slab grid too large (%lld tiles)", (long long)need);
-  hipLaunchKernelGGL((k_filter_wide<Op, K, RB, QS>), dim3((unsigned)gx, (unsigned)gy), dim3(512), 0,
+  hipLaunchKernelGGL((k_filter_wide<Op, K, RB, MK>), dim3((unsigned)gx, (unsigned)gy), dim3(512), 0,
                      ctx->stream, P);
   SD_HIP(hipGetLastError());
   return SD_OK;
 }
 
-template <class Op, class K, int WM, bool QS = false>
+template <class Op, class K, int WM, int MK = kKeyScore>
 static int launch_fused_L(sd_context* ctx, FusedKernelArgs& P, int L) {
   switch (L) {
-    case 8: return launch_fused_t<Op, K, 8, WM, QS>(ctx, P);
-    case 16: return launch_fused_t<Op, K, 16, WM, QS>(ctx, P);
-    case 24: return launch_fused_t<Op, K, 24, WM, QS>(ctx, P);
-    case 32: return launch_fused_t<Op, K, 32, WM, QS>(ctx, P);
-    case 40: return launch_fused_t<Op, K, 40, WM, QS>(ctx, P);
-    case 48: return launch_fused_t<Op, K, 48, WM, QS>(ctx, P);
-    case 64: return launch_fused_t<Op, K, 64, WM, QS>(ctx, P);
+    case 8: return launch_fused_t<Op, K, 8, WM, MK>(ctx, P);
+    case 16: return launch_fused_t<Op, K, 16, WM, MK>(ctx, P);
+    case 24: return launch_fused_t<Op, K, 24, WM, MK>(ctx, P);
+    case 32: return launch_fused_t<Op, K, 32, WM, MK>(ctx, P);
+    case 40: return launch_fused_t<Op, K, 40, WM, MK>(ctx, P);
+    case 48: return launch_fused_t<Op, K, 48, WM, MK>(ctx, P);
+    case 64: return launch_fused_t<Op, K, 64, WM, MK>(ctx, P);
     default: return set_error(SD_EINVAL, "internal: unsupported list length %d", L);
   }
 }
@@ -1295,10 +1359,12 @@ int fused_slab_topl(sd_context* ctx, const FusedArgs& a, double* cand_key, int32
   P.qry_p = a.qry_p;
   P.ref_aux = a.ref_aux;
   P.qry_aux = a.qry_aux;
+  P.ref_aux2 = a.ref_aux2;
+  P.qry_aux2 = a.qry_aux2;
   P.qseg = (int)dense_qseg_bytes(a.kind, a.kpad);
-  P.euc_e0 = a.euc_e0;
-  P.euc_e1 = a.euc_e1;
-  P.euc_nonneg = a.euc_nonneg;
+  P.dot_e0 = a.dot_e0;
+  P.dot_e1 = a.dot_e1;
+  P.dot_nonneg = a.dot_nonneg;
   P.ref_deg = a.ref_deg;
   P.self_mode = a.self_mode;
   P.include_self = a.include_self;
@@ -1319,8 +1385,13 @@ int fused_slab_topl(sd_context* ctx, const FusedArgs& a, double* cand_key, int32
     if (a.kind == kDenseF32Pattern) return launch_fused_L<OpF32, KeyJ, WM>(ctx, P, a.L);
     return launch_fused_L<OpF4, KeyJ, WM>(ctx, P, a.L);
   }
-  if (a.prec == SD_PREC_FP32) return launch_fused_L<OpF32, float, WM>(ctx, P, a.L);
-  if (P.qseg) return launch_fused_L<OpBf16, float, WM, true>(ctx, P, a.L);
+  if (a.prec == SD_PREC_FP32) {
+    if (a.method == SD_EUCLIDEAN) return launch_fused_L<OpF32, float, WM, kKeyEuclid>(ctx, P, a.L);
+    if (a.method == SD_PEARSON) return launch_fused_L<OpF32, float, WM, kKeyPearson>(ctx, P, a.L);
+    return launch_fused_L<OpF32, float, WM>(ctx, P, a.L);
+  }
+  if (a.method == SD_EUCLIDEAN) return launch_fused_L<OpBf16, float, WM, kKeyEuclid>(ctx, P, a.L);
+  if (a.method == SD_PEARSON) return launch_fused_L<OpBf16, float, WM, kKeyPearson>(ctx, P, a.L);
   return launch_fused_L<OpBf16, float, WM>(ctx, P, a.L);
 }
 
@@ -1349,10 +1420,12 @@ int fused_filter_pass(sd_context* ctx, const FusedArgs& a, const double* thr, in
   P.qry_p = a.qry_p;
   P.ref_aux = a.ref_aux;
   P.qry_aux = a.qry_aux;
+  P.ref_aux2 = a.ref_aux2;
+  P.qry_aux2 = a.qry_aux2;
   P.qseg = (int)dense_qseg_bytes(a.kind, a.kpad);
-  P.euc_e0 = a.euc_e0;
-  P.euc_e1 = a.euc_e1;
-  P.euc_nonneg = a.euc_nonneg;
+  P.dot_e0 = a.dot_e0;
+  P.dot_e1 = a.dot_e1;
+  P.dot_nonneg = a.dot_nonneg;
   P.ref_deg = a.ref_deg;
   P.self_mode = a.self_mode;
   P.include_self = a.include_self;
@@ -1371,9 +1444,15 @@ int fused_filter_pass(sd_context* ctx, const FusedArgs& a, const double* thr, in
     if (a.kind == kDenseF32Pattern) return launch_filter_t<OpF32, KeyJ, kFilterRowBytes>(ctx, P);
     return launch_filter_t<OpF4, KeyJ, kFilterRowBytes>(ctx, P);
   }
-  if (a.prec == SD_PREC_FP32) return launch_filter_t<OpF32, float, kFilterRowBytes>(ctx, P);
-  if (P.qseg) return launch_filter_t<OpBf16, float, kFilterRowBytes, true>(ctx, P);
-  return launch_filter_t<OpBf16, float, kFilterRowBytes>(ctx, P);
+  constexpr int RB = kFilterRowBytes;
+  if (a.prec == SD_PREC_FP32) {
+    if (a.method == SD_EUCLIDEAN) return launch_filter_t<OpF32, float, RB, kKeyEuclid>(ctx, P);
+    if (a.method == SD_PEARSON) return launch_filter_t<OpF32, float, RB, kKeyPearson>(ctx, P);
+    return launch_filter_t<OpF32, float, RB>(ctx, P);
+  }
+  if (a.method == SD_EUCLIDEAN) return launch_filter_t<OpBf16, float, RB, kKeyEuclid>(ctx, P);
+  if (a.method == SD_PEARSON) return launch_filter_t<OpBf16, float, RB, kKeyPearson>(ctx, P);
+  return launch_filter_t<OpBf16, float, RB>(ctx, P);
 }
 
 int select_lists(sd_context* ctx, const double* fkey, const int32_t* fidx, const int32_t* cnt,
@@ -1438,14 +1517,23 @@ int rerank_cosine(sd_context* ctx, const RerankArgs& R) {
   return SD_OK;
 }
 
-int rerank_euclid(sd_context* ctx, const RerankArgs& R) {
+template <int M>
+static int rerank_exact(sd_context* ctx, const RerankArgs& R) {
   if (R.nqry <= 0) return SD_OK;
   if (R.L > 64) return set_error(SD_EINVAL, "internal: re-rank list of %d > 64", R.L);
   KernelTimer timer(ctx, &ctx->stats.rerank_ms, &ctx->stats.rerank_launches);
-  hipLaunchKernelGGL(k_rerank_euc, dim3((unsigned)((R.nqry + 3) / 4)), dim3(256), 0, ctx->stream,
-                     R);
+  hipLaunchKernelGGL(k_rerank_exact<M>, dim3((unsigned)((R.nqry + 3) / 4)), dim3(256), 0,
+                     ctx->stream, R);
   SD_HIP(hipGetLastError());
   return SD_OK;
+}
+
+int rerank_euclid(sd_context* ctx, const RerankArgs& R) {
+  return rerank_exact<SD_EUCLIDEAN>(ctx, R);
+}
+
+int rerank_pearson(sd_context* ctx, const RerankArgs& R) {
+  return rerank_exact<SD_PEARSON>(ctx, R);
 }
 
 }  // namespace sd

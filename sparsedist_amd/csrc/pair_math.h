@@ -146,6 +146,13 @@ __device__ inline double centred_dot(const int32_t* __restrict__ ra, const doubl
   return s;
 }
 
+// Pearson correlation of reference column i and query column j (similarity
+// form): centred cross product over the square roots of the centred sums of
+// squares; non-finite -> 0, clamped to [-1, 1] (src/sparseDist.cpp:426-429).
+struct ExactArgs;
+__device__ inline double pear_value(const ExactArgs& A, int64_t i, const int32_t* qr,
+                                    const double* qv, int64_t qn, int64_t j);
+
 // Value of one (reference column i, query column j) pair in the reference's
 // arithmetic, before any kNN masking.  `qr/qv` may point into LDS.
 __device__ inline double exact_pair(const ExactArgs& A, int64_t i, const int32_t* qr, const double* qv,
@@ -166,12 +173,7 @@ __device__ inline double exact_pair(const ExactArgs& A, int64_t i, const int32_t
     case SD_MANHATTAN:
       return man_value(ra, A.ref.x + a0, na, qr, qv, qn);
     case SD_PEARSON: {
-      const double s = centred_dot(ra, A.ref.x + a0, na, A.ref_mean[i], qr, qv, qn,
-                                   A.qry_mean[j], A.ref.nrow);
-      double c = s / (sqrt(A.ref_css[i]) * sqrt(A.qry_css[j]));
-      if (!isfinite(c)) c = 0.0;
-      if (c > 1.0) c = 1.0;
-      if (c < -1.0) c = -1.0;
+      const double c = pear_value(A, i, qr, qv, qn, j);
       const double v = A.dist ? (1.0 - c) : c;
       return (v == 0.0) ? 0.0 : v;
     }
@@ -183,6 +185,18 @@ __device__ inline double exact_pair(const ExactArgs& A, int64_t i, const int32_t
       return (v == 0.0) ? 0.0 : v;
     }
   }
+}
+
+__device__ inline double pear_value(const ExactArgs& A, int64_t i, const int32_t* qr,
+                                    const double* qv, int64_t qn, int64_t j) {
+  const int64_t a0 = A.ref.p[i], na = A.ref.p[i + 1] - a0;
+  const double s = centred_dot(A.ref.i + a0, A.ref.x + a0, na, A.ref_mean[i], qr, qv, qn,
+                               A.qry_mean[j], A.ref.nrow);
+  double c = s / (sqrt(A.ref_css[i]) * sqrt(A.qry_css[j]));
+  if (!isfinite(c)) c = 0.0;
+  if (c > 1.0) c = 1.0;
+  if (c < -1.0) c = -1.0;
+  return c;
 }
 
 }  // namespace sd

@@ -297,7 +297,8 @@ int fused_candidate_lists(sd_context* ctx, FusedArgs fa, const double** out_key,
   return SD_OK;
 }
 
-// Constants of the euclidean keys' error bound (euc_key in k_slab.hip): the
+// Constants of the euclidean and pearson keys' error bound (euc_key and
+// pear_key in k_slab.hip): the
 // slab's dot product x of a query q with nnz(q) = m stored values against a
 // reference r obeys |x - q.r| <= (e0 + e1 m) sum |q_i r_i|.
 //  * split BF16 (hi = bf16(v), lo = bf16(v - hi), each within ub = 2^-8 +
@@ -314,11 +315,11 @@ int fused_candidate_lists(sd_context* ctx, FusedArgs fa, const double** out_key,
 // are divided by 1 - e at the longest column.  The path is taken only for
 // finite values in [1e-15, 1e15] (no FP32 overflow of squares or norms, no
 // subnormal BF16 parts) and e <= 0.25.
-struct EucBound {
+struct DotBound {
   float e0 = 0.f, e1 = 0.f;
   int nonneg = 0;
 };
-bool euc_bound(const sd_matrix* ref, const sd_matrix* X, bool f32, EucBound* out) {
+bool dot_bound(const sd_matrix* ref, const sd_matrix* X, bool f32, DotBound* out) {
   for (const sd_matrix* m : {ref, X}) {
     if (m->any_nonfinite) return false;
     if (m->nnz > 0 && (m->abs_min < 1e-15 || m->abs_max > 1e15)) return false;
@@ -374,7 +375,7 @@ int run_knn(sd_context* ctx, sd_matrix* X, sd_matrix* Y, const sd_knn_params* p,
   }
   // densified layout: binary -> 0/1 patterns (FP4 nibbles, or FP32 on the
   // plain path); cosine -> unit-normalised values (BF16, or FP32);
-  // euclidean -> raw values (split BF16 hi/lo, or FP32)
+  // euclidean and pearson -> raw values (split BF16 hi/lo, or FP32)
   const bool use_f32 = (prec == SD_PREC_FP32);
   const int kind = method == SD_BINARY   ? (use_f32 ? kDenseF32Pattern : kDenseF4Pattern)
                    : method == SD_COSINE ? (use_f32 ? kDenseF32Unit : kDenseBf16Unit)
@@ -385,11 +386,13 @@ int run_knn(sd_context* ctx, sd_matrix* X, sd_matrix* Y, const sd_knn_params* p,
                          fused_list_len(k) > 0 && fits;
   bool fused_ok_cosine = (method == SD_COSINE) && prec != SD_PREC_EXACT &&
                          fused_list_len(k + margin) > 0 && nref >= 1 && fits;
-  EucBound eb;
-  bool fused_ok_euclid = (method == SD_EUCLIDEAN) && prec != SD_PREC_EXACT &&
-                         fused_list_len(k + margin) > 0 && nref >= 1 && fits &&
-                         euc_bound(ref, X, use_f32, &eb);
-  if (fused_ok_binary || fused_ok_cosine || fused_ok_euclid) {
+  DotBound eb;
+  const bool raw_dot = method == SD_EUCLIDEAN || method == SD_PEARSON;
+  const bool fused_ok_raw = raw_dot && prec != SD_PREC_EXACT && fused_list_len(k + margin) > 0 &&
+                            nref >= 1 && fits && dot_bound(ref, X, use_f32, &eb);
+  bool fused_ok_euclid = fused_ok_raw && method == SD_EUCLIDEAN;
+  bool fused_ok_pearson = fused_ok_raw && method == SD_PEARSON;
+  if (fused_ok_binary || fused_ok_cosine || fused_ok_euclid || fused_ok_pearson) {
     // SD_PREC_AUTO: the cost model may prefer the merge walk (tiny batches
     // against very sparse references); both paths give identical results
     if (prec == SD_PREC_AUTO &&
@@ -399,9 +402,10 @@ int run_knn(sd_context* ctx, sd_matrix* X, sd_matrix* Y, const sd_knn_params* p,
       fused_ok_binary = false;
       fused_ok_cosine = false;
       fused_ok_euclid = false;
+      fused_ok_pearson = false;
     }
   }
-  const bool fused_rerank = fused_ok_cosine || fused_ok_euclid;
+  const bool fused_rerank = fused_ok_cosine || fused_ok_euclid || fused_ok_pearson;
   ExactArgs ea = exact_args(ref, X, method, p->dist);
   ea.ref_deg = ref_deg;
   ea.self_mode = plan.selfmode;
@@ -438,9 +442,16 @@ int run_knn(sd_context* ctx, sd_matrix* X, sd_matrix* Y, const sd_knn_params* p,
     if (fused_ok_euclid) {
       fa.ref_aux = ref->sqf.as<float>();
       fa.qry_aux = X->sqf.as<float>();
-      fa.euc_e0 = eb.e0;
-      fa.euc_e1 = eb.e1;
-      fa.euc_nonneg = eb.nonneg;
+    } else if (fused_ok_pearson) {
+      fa.ref_aux = ref->cmf.as<float>();
+      fa.qry_aux = X->cmf.as<float>();
+      fa.ref_aux2 = ref->cif.as<float>();
+      fa.qry_aux2 = X->cif.as<float>();
+    }
+    if (fused_ok_euclid || fused_ok_pearson) {
+      fa.dot_e0 = eb.e0;
+      fa.dot_e1 = eb.e1;
+      fa.dot_nonneg = eb.nonneg;
     }
     fa.L = fused_list_len(method == SD_BINARY ? k : k + margin);
     const int L = fa.L;
@@ -452,11 +463,14 @@ int run_knn(sd_context* ctx, sd_matrix* X, sd_matrix* Y, const sd_knn_params* p,
                                 p->drop_degenerate ? X->deg.as<uint8_t>() : nullptr,
                                 out_idx.as<int32_t>(), out_val.as<double>()));
     } else {
-      // cosine / euclidean: FP64 re-rank with certificate; queries it cannot
-      // clear get a second fused pass over just those columns with the widest
-      // list, and only what is still uncertified goes to the exact FP64 slab.
+      // cosine / euclidean / pearson: FP64 re-rank with certificate; queries
+      // it cannot clear get a second fused pass over just those columns with
+      // the widest list, and only what is still uncertified goes to the exact
+      // FP64 slab.
       auto rerank = [&](const RerankArgs& RA) {
-        return fused_ok_euclid ? rerank_euclid(ctx, RA) : rerank_cosine(ctx, RA);
+        return fused_ok_euclid    ? rerank_euclid(ctx, RA)
+               : fused_ok_pearson ? rerank_pearson(ctx, RA)
+                                  : rerank_cosine(ctx, RA);
       };
       DevBuf& fl = ctx->scratch[7];
       DevBuf& fl2 = ctx->scratch[8];
@@ -750,6 +764,8 @@ int sd_matrix_drop_cache(sd_matrix* m) {
   m->unit_ready = false;
   m->mean.release();
   m->css.release();
+  m->cmf.release();
+  m->cif.release();
   m->centre_ready = false;
   m->sqn.release();
   m->sqf.release();

@@ -120,9 +120,10 @@ struct sd_matrix {
   // cosine: unit-normalised values (nnz) and per-column max |v|
   bool unit_ready = false;
   sd::DevBuf unit, scale;
-  // pearson / covariance: per-column mean and centred sum of squares
+  // pearson / covariance: per-column mean and centred sum of squares, and
+  // their FP32 forms for the slab epilogue (sqrt(nrow) mean, 1 / sqrt(css))
   bool centre_ready = false;
-  sd::DevBuf mean, css;
+  sd::DevBuf mean, css, cmf, cif;
   // euclidean: per-column sum of squares (FP64, merge order irrelevant) and
   // the same rounded to FP32 for the slab epilogues
   bool sq_ready = false;
@@ -197,13 +198,18 @@ struct FusedArgs {
   const int32_t* qlist = nullptr;   // optional: queries are these columns (positions 0..nqry)
   const int64_t* ref_p = nullptr;   // column pointers (nnz per column)
   const int64_t* qry_p = nullptr;
-  const float* ref_aux = nullptr;   // per-column float aux (euclid: ||x||^2)
+  // per-column FP32 aux: euclidean ||x||^2; pearson sqrt(nrow) mean (aux)
+  // and 1 / sqrt(centred sum of squares) (aux2, 0 for a constant column)
+  const float* ref_aux = nullptr;
   const float* qry_aux = nullptr;
-  // euclidean keys: the dot product's error bound is (euc_e0 + euc_e1 nnz(q))
-  // times sum |q_i r_i| (see euc_key in k_slab.hip); euc_nonneg: every value
-  // of both matrices is >= 0, so that sum is the dot product itself
-  float euc_e0 = 0.f, euc_e1 = 0.f;
-  int euc_nonneg = 0;
+  const float* ref_aux2 = nullptr;
+  const float* qry_aux2 = nullptr;
+  // euclidean / pearson keys: the raw dot product's error bound is
+  // (dot_e0 + dot_e1 nnz(q)) times sum |q_i r_i| (euc_key, pear_key in
+  // k_slab.hip); dot_nonneg: every value of both matrices is >= 0, so that
+  // sum is the dot product itself
+  float dot_e0 = 0.f, dot_e1 = 0.f;
+  int dot_nonneg = 0;
   const uint8_t* ref_deg = nullptr;
   int self_mode = 0, include_self = 0;
   int L = 0;                        // list length kept per query
@@ -278,6 +284,10 @@ int rerank_cosine(sd_context* ctx, const RerankArgs& R);
 // arithmetic; the certificate compares the k-th squared distance with the
 // lower bound that the list's last key stands for.
 int rerank_euclid(sd_context* ctx, const RerankArgs& R);
+// Pearson: exact FP64 correlations (this build's centred definition); the
+// certificate compares the k-th correlation with the upper bound that the
+// list's last key stands for.
+int rerank_pearson(sd_context* ctx, const RerankArgs& R);
 int topk_rows(sd_context* ctx, const double* D, int64_t nrow, int64_t ncol, int k, int decreasing,
               int32_t* best_idx, double* best_val, int64_t col_offset, const int32_t* self_col);
 

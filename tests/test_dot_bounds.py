@@ -1,17 +1,18 @@
-"""The euclidean slab keys' error bound, checked against an emulation of the
-arithmetic on the CPU.
+"""The euclidean and pearson slab keys' error bounds, checked against an
+emulation of the arithmetic on the CPU.
 
 The slab computes x = sum hi_q hi_r + hi_q lo_r + lo_q hi_r (hi = bf16(v),
 lo = bf16(v - hi), products exact in FP32, FP32 accumulation) and keys each
 pair with minus a lower bound of the squared distance (euc_key in
-sparsedist_amd/csrc/k_slab.hip, constants from euc_bound in
-sparsedist_amd/csrc/sd_api.cpp).  Here the same rounding steps are emulated
+sparsedist_amd/csrc/k_slab.hip) or an upper bound of the correlation
+(pear_key), with constants from dot_bound in sparsedist_amd/csrc/sd_api.cpp.  Here the same rounding steps are emulated
 with numpy (bf16 by bit rounding of FP32, the accumulation as a sequential
 FP32 cumulative sum in a random order) and the exact dot products come from
 math.fsum, so the check does not reuse the formula it tests:
 
 * |x - q.r| <= (e0 + e1 m) sum |q_i r_i| for nonnegative and signed data;
-* the key's lower bound never exceeds the exact squared distance;
+* the euclidean key's lower bound never exceeds the exact squared distance,
+  and the pearson key's upper bound is never below the exact correlation;
 * a slab that dropped the lo.hi term breaks the first bound (the check has
   teeth for the mistake it guards against).
 """
@@ -126,3 +127,53 @@ def test_missing_cross_term_is_caught():
         s_abs = math.fsum(np.abs(q * r))
         broken += abs(x - exact) > (e0 + e1 * 800) * s_abs
     assert broken >= 15
+
+
+K_PEAR_ROUND = np.float32(2.0 ** -20)
+
+
+def pear_upper_bound(qa, qc, ra, rc, x, m, e0, e1, nonneg):
+    """pear_key's arithmetic in FP32."""
+    f = np.float32
+    qa, qc, ra, rc, x = f(qa), f(qc), f(ra), f(rc), f(x)
+    t = f(qa * ra)
+    sc = f(qc * rc)
+    r = f(f(x - t) * sc)
+    e = f(f(e0) + f(f(e1) * f(m)))
+    if nonneg:
+        s = f(f(abs(x)) * sc)
+    else:
+        s = f(f(np.sqrt(f(f(1.0) + f(f(qa * qc) * f(qa * qc))))) *
+              f(np.sqrt(f(f(1.0) + f(f(ra * rc) * f(ra * rc))))))
+    err = f(f(f(e + K_PEAR_ROUND) * s) + f(f(K_PEAR_ROUND * f(f(abs(x)) + f(abs(t)))) * sc))
+    err = f(err + K_EUC_ABS)
+    return float(f(r + err))
+
+
+def centred(v, n):
+    mean = math.fsum(v) / n
+    css = math.fsum((v - mean) ** 2)
+    return mean, css
+
+
+@pytest.mark.parametrize("signed", [False, True])
+def test_pearson_upper_bound(signed):
+    rng = np.random.default_rng(21 + signed)
+    nrow = 3000
+    e0, e1 = constants(nrow)
+    emax = e0 + e1 * nrow
+    fold = 1.0 / (1.0 - emax)
+    for trial in range(60):
+        nnz = int(rng.integers(30, 1200))
+        q, r = pair(rng, nrow, nnz, rng.random(), signed)
+        if trial % 15 == 0:
+            r[r != 0] = 2.5  # stored values all equal (not constant over the rows)
+        x = slab_dot(q, r, rng)
+        mq, cq = centred(q, nrow)
+        mr, cr = centred(r, nrow)
+        exact = math.fsum((q - mq) * (r - mr)) / (math.sqrt(cq) * math.sqrt(cr))
+        ub = pear_upper_bound(math.sqrt(nrow) * mq, 1.0 / math.sqrt(cq), math.sqrt(nrow) * mr,
+                              1.0 / math.sqrt(cr), x, nnz, e0 * fold, e1 * fold, not signed)
+        assert exact <= ub, (trial, exact, ub)
+        # the bound is useful: within a few hundred ppm of the value
+        assert ub - exact < 2e-3, (trial, ub - exact)

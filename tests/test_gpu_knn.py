@@ -337,7 +337,7 @@ def test_certificate_escalation_and_fallback(oracle_lib, gpu_ctx):
     assert st["fallback_queries"] > 0
 
 
-@pytest.mark.parametrize("method", ["binary", "cosine", "euclidean"])
+@pytest.mark.parametrize("method", ["binary", "cosine", "euclidean", "pearson"])
 def test_two_pass_filter_path(oracle_lib, gpu_ctx, method):
     """Enough reference columns (>= 128 tiles of 128) for the threshold
     pre-pass + filtered full pass; queries are a subset so the oracle stays
@@ -412,3 +412,43 @@ def test_euclidean_out_of_range_values_take_exact_path(oracle_lib, gpu_ctx):
     o = oracle_lib.knn(X, k=5, method="euclidean", nthreads=NT)
     same((g.idx, g.dist), o, "large values")
     assert st["fused_calls"] == 0 and st["exact_calls"] == 1
+
+
+@pytest.mark.parametrize("prec", ["bf16", "fp32"])
+@pytest.mark.parametrize("dist", [True, False])
+def test_pearson_count_matrix_certified(oracle_lib, gpu_ctx, prec, dist):
+    """Pearson on the count-matrix rows and density: raw-value slab (split
+    BF16 or FP32), per-pair upper-bound keys of the correlation, FP64
+    re-rank in this build's centred definition."""
+    X = synth.nb_log1p(20000, 1200, 0.06, 12)
+    gpu_ctx.stats(reset=True)
+    g = sd.sparseKNN(X, k=20, method="pearson", dist=dist, precision=prec, verbose=False)
+    st = gpu_ctx.stats(reset=True)
+    o = oracle_lib.knn(X, k=20, method="pearson", dist=dist, nthreads=NT)
+    same((g.idx, g.dist), o, f"pearson {prec} dist={dist}")
+    assert st["fused_calls"] == 1
+    assert st["escalated_queries"] <= 60 and st["fallback_queries"] <= 12, st
+
+
+def test_pearson_signed_constant_and_self(oracle_lib, gpu_ctx):
+    """Signed values (general bound), constant columns (correlation 0 by
+    convention), self restore and cross search on the fused pearson path."""
+    X = rand_sparse(200, 400, 0.15, 61, signed=True, empty_cols=(5,))
+    Xd = X.toarray()
+    Xd[:, 7] = 2.5  # a constant column: every stored row holds the same value
+    Xd[:, 9] = 0.0
+    Xd[3, 9] = 1.0  # a single stored value
+    X = sp.csc_matrix(Xd)
+    Y = rand_sparse(200, 700, 0.15, 62, signed=True)
+    for dist in (True, False):
+        for include_self in (False, True):
+            for drop in (True, False):
+                lab = f"dist={dist} self={include_self} drop={drop}"
+                g = sd.sparseKNN(X, k=8, method="pearson", dist=dist, include_self=include_self,
+                                 drop_degenerate=drop, precision="bf16", verbose=False)
+                o = oracle_lib.knn(X, k=8, method="pearson", dist=dist,
+                                   include_self=include_self, drop_degenerate=drop, nthreads=NT)
+                same((g.idx, g.dist), o, lab)
+        g = sd.sparseKNN(X, Y, k=8, method="pearson", dist=dist, precision="bf16", verbose=False)
+        o = oracle_lib.knn(X, Y, k=8, method="pearson", dist=dist, nthreads=NT)
+        same((g.idx, g.dist), o, f"cross dist={dist}")
