@@ -455,12 +455,13 @@ def large_main(args, sd, synth, workers):
         ctx.set_timing(False)
         fms = st["filter_ms"] / max(st["filter_launches"], 1)
         peak = MI355X_PEAK["fp4" if method == "binary" else "bf16"]
-        # dense-equivalent operations of one filtered pass (the split layout
-        # runs three products per row)
-        rows = X.shape[0] * (3 if method == "euclidean" else 1)
+        # dense-equivalent operations of one filtered pass (the split BF16
+        # layout runs three products per row)
+        rows = X.shape[0] * max(st["slab_products"], 1)
         qpass = nq / max(st["filter_launches"], 1)
         flop = 2.0 * rows * nq * qpass
         res[method] = dict(seconds=dt, cols_per_s=nq / dt, filter_ms=fms,
+                           slab_products_per_row=st["slab_products"],
                            filter_frac_of_peak=flop / (fms / 1e3) / peak,
                            certified=st["certified_queries"], escalated=st["escalated_queries"],
                            fallback=st["fallback_queries"], overflow=st["overflow_queries"])
@@ -472,7 +473,7 @@ def large_main(args, sd, synth, workers):
         "metric": "exact-kNN query cols/s (k=20), large sparse self-search",
         "value": 3 * nq / secs,
         "unit": "query cols/s", "n_gpus": 1, "higher_is_better": True,
-        "dtype": "fp4 0/1 patterns (binary), bf16 (cosine), split bf16 hi/lo (euclidean) slab, "
+        "dtype": "fp4 0/1 patterns (binary), bf16 (cosine, euclidean on non-negative data) slab, "
                  "fp64 re-rank", "data": "synthetic (seeded NB/log1p)",
         "config": {"workload": "X 30000 x 1000000 CSC d=0.03 NB/log1p, k=20: binary, cosine "
                                "and euclidean over all 1000000 query columns"},

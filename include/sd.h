@@ -109,7 +109,9 @@ typedef struct {
   int64_t filter_launches;
   int64_t fused_calls;        /* searches run on the densified MFMA path */
   int64_t exact_calls;        /* searches run on the FP64 merge walk */
-  int64_t last_margin;        /* cosine: re-rank margin m of the last fused search */
+  int64_t last_margin;        /* re-rank margin m of the last fused search */
+  int64_t slab_products;      /* MFMA products per row of the last fused search's layout:
+                                 3 for split BF16 (hi.hi + hi.lo + lo.hi), else 1 */
 } sd_stats;
 int sd_context_stats(sd_context* ctx, sd_stats* out, int reset);
 /* Enable/disable event timing of kernels (off by default). */

@@ -116,7 +116,7 @@ __global__ __launch_bounds__(256) void k_densify(const int64_t* __restrict__ p,
     const int64_t o = r - k0;
     if (KIND == kDenseF4Pattern) {
       atomicOr(reinterpret_cast<uint32_t*>(lds) + (o >> 3), 0x2u << ((o & 7) * 4));
-    } else if (KIND == kDenseBf16Unit) {
+    } else if (KIND == kDenseBf16Unit || KIND == kDenseBf16Raw) {
       reinterpret_cast<uint16_t*>(lds)[o] = f32_to_bf16_rne((float)vals[e]);
     } else if (KIND == kDenseF32Pattern) {
       reinterpret_cast<float*>(lds)[o] = 1.0f;
@@ -281,6 +281,11 @@ int prep_dense(sd_context* ctx, sd_matrix* m, int kind, int64_t kpad) {
         break;
       case kDenseBf16Unit:
         hipLaunchKernelGGL((k_densify<kDenseBf16Unit, 2>), dim3((unsigned)grid), dim3(256), lds,
+                           ctx->stream, m->p.as<int64_t>(), m->i.as<int32_t>(), vals, m->ncol, kpad,
+                           chunk, dst);
+        break;
+      case kDenseBf16Raw:
+        hipLaunchKernelGGL((k_densify<kDenseBf16Raw, 2>), dim3((unsigned)grid), dim3(256), lds,
                            ctx->stream, m->p.as<int64_t>(), m->i.as<int32_t>(), vals, m->ncol, kpad,
                            chunk, dst);
         break;

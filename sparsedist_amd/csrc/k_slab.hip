@@ -135,8 +135,15 @@ __device__ inline float pear_key(float qa, float qc, float ra, float rc, float x
                          : sqrtf(1.f + (qa * qc) * (qa * qc)) * sqrtf(1.f + (ra * rc) * (ra * rc));
   return r + ((e + kPearRound) * s + kPearRound * (fabsf(x) + fabsf(t)) * sc + kEucAbs);
 }
+// Split layout's query offset: segment 1 <-> 2.  Written as arithmetic, not
+// selects, so that the K loop stays one basic block (a branch there makes the
+// wait-count pass merge states at the join and wait for every load), and
+// read back as a wave-uniform scalar: the buffer load's scalar offset must
+// not come from a vector register (the compiler would loop over lanes).
 __device__ inline int qseg_offset(int soff, int qseg) {
-  return (qseg == 0 || soff < qseg) ? soff : (soff < 2 * qseg ? soff + qseg : soff - qseg);
+  const int in1 = (int)(soff >= qseg) & (int)(soff < 2 * qseg);
+  const int in2 = (int)(soff >= 2 * qseg);
+  return __builtin_amdgcn_readfirstlane(soff + qseg * (in1 - in2));
 }
 
 template <class K>
@@ -228,11 +235,11 @@ __device__ inline void wait_vm() {
 // Key kinds of the FP32-accumulator kernels, fixed at compile time so each
 // instantiation carries only its own epilogue: the score itself (cosine),
 // minus a squared-distance lower bound (euclidean), a correlation upper
-// bound (pearson).  The last two read query rows with the split layout's
-// segments swapped (qseg; 0 on the FP32 layout, which is read as stored).
+// bound (pearson).  QS: query rows are read with the split layout's
+// segments swapped (qseg).
 enum KeyKind { kKeyScore = 0, kKeyEuclid = 1, kKeyPearson = 2 };
 
-template <class Op, class K, int L, int WM, int MK>
+template <class Op, class K, int L, int WM, int MK, bool QS>
 __global__ __launch_bounds__(WM * 256, 1) void k_fused(FusedKernelArgs P) {
   constexpr int WAVES = WM * 4;
   constexpr int NT = WAVES * 64;
@@ -358,7 +365,7 @@ __global__ __launch_bounds__(WM * 256, 1) void k_fused(FusedKernelArgs P) {
     }
     auto issue = [&](uint8_t* st, int64_t c) {
       const int soff = (int)(c * kRowBytes);
-      const int soff_b = MK != kKeyScore ? qseg_offset(soff, P.qseg) : soff;
+      const int soff_b = QS ? qseg_offset(soff, P.qseg) : soff;
 #pragma unroll
       for (int u = 0; u < A_INST; ++u)
         __builtin_amdgcn_raw_ptr_buffer_load_lds(
@@ -578,7 +585,7 @@ struct WideLayout {
   __device__ static int swz(int row) { return (row >> kRowShift) & (kSlots - 1); }
 };
 
-template <class Op, class K, int RB, int MK>
+template <class Op, class K, int RB, int MK, bool QS>
 __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
   using Lay = WideLayout<RB>;
   constexpr int NT = 512;
@@ -650,7 +657,7 @@ __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
   typedef __attribute__((address_space(3))) void lds_void;
   auto issue = [&](uint8_t* st, int64_t c) {
     const int soff = (int)(c * RB);
-    const int soff_b = MK != kKeyScore ? qseg_offset(soff, P.qseg) : soff;
+    const int soff_b = QS ? qseg_offset(soff, P.qseg) : soff;
 #pragma unroll
     for (int u = 0; u < A_INST; ++u)
       __builtin_amdgcn_raw_ptr_buffer_load_lds(
@@ -1252,12 +1259,12 @@ __global__ __launch_bounds__(256) void k_rerank_exact(RerankArgs R) {
 // host launchers
 // ----------------------------------------------------------------------------
 
-template <class Op, class K, int L, int WM, int MK>
+template <class Op, class K, int L, int WM, int MK, bool QS>
 static int launch_fused_t(sd_context* ctx, FusedKernelArgs& P) {
   const int64_t nblk = (int64_t)P.qtiles * P.splits;
   if (nblk * WM * 256 >= ((int64_t)1 << 32))
     return set_error(SD_EUNSUPPORTED, "slab grid too large (%lld workgroups)", (long long)nblk);
-  hipLaunchKernelGGL((k_fused<Op, K, L, WM, MK>), dim3((unsigned)nblk), dim3(WM * 256), 0,
+  hipLaunchKernelGGL((k_fused<Op, K, L, WM, MK, QS>), dim3((unsigned)nblk), dim3(WM * 256), 0,
                      ctx->stream, P);
   SD_HIP(hipGetLastError());
   return SD_OK;
@@ -1266,7 +1273,7 @@ static int launch_fused_t(sd_context* ctx, FusedKernelArgs& P) {
 // K bytes per row per chunk of the filtered pass (64 or 128)
 constexpr int kFilterRowBytes = 128;
 
-template <class Op, class K, int RB, int MK = kKeyScore>
+template <class Op, class K, int RB, int MK = kKeyScore, bool QS = false>
 static int launch_filter_t(sd_context* ctx, FusedKernelArgs& P) {
   if (P.kpad_bytes <= 0 || P.kpad_bytes % (WideLayout<RB>::kStages * RB) != 0)
     return set_error(SD_EINVAL, "internal: padded row bytes %lld are not whole chunk groups",
@@ -1280,22 +1287,22 @@ static int launch_filter_t(sd_context* ctx, FusedKernelArgs& P) {
   const int64_t gx = std::min<int64_t>((need + 7) / 8 * 8, (int64_t)1 << 20);
   const int64_t gy = (need + gx - 1) / gx;
   if (gy > 65535) return set_error(SD_EUNSUPPORTED, "slab grid too large (%lld tiles)", (long long)need);
-  hipLaunchKernelGGL((k_filter_wide<Op, K, RB, MK>), dim3((unsigned)gx, (unsigned)gy), dim3(512), 0,
+  hipLaunchKernelGGL((k_filter_wide<Op, K, RB, MK, QS>), dim3((unsigned)gx, (unsigned)gy), dim3(512), 0,
                      ctx->stream, P);
   SD_HIP(hipGetLastError());
   return SD_OK;
 }
 
-template <class Op, class K, int WM, int MK = kKeyScore>
+template <class Op, class K, int WM, int MK = kKeyScore, bool QS = false>
 static int launch_fused_L(sd_context* ctx, FusedKernelArgs& P, int L) {
   switch (L) {
-    case 8: return launch_fused_t<Op, K, 8, WM, MK>(ctx, P);
-    case 16: return launch_fused_t<Op, K, 16, WM, MK>(ctx, P);
-    case 24: return launch_fused_t<Op, K, 24, WM, MK>(ctx, P);
-    case 32: return launch_fused_t<Op, K, 32, WM, MK>(ctx, P);
-    case 40: return launch_fused_t<Op, K, 40, WM, MK>(ctx, P);
-    case 48: return launch_fused_t<Op, K, 48, WM, MK>(ctx, P);
-    case 64: return launch_fused_t<Op, K, 64, WM, MK>(ctx, P);
+    case 8: return launch_fused_t<Op, K, 8, WM, MK, QS>(ctx, P);
+    case 16: return launch_fused_t<Op, K, 16, WM, MK, QS>(ctx, P);
+    case 24: return launch_fused_t<Op, K, 24, WM, MK, QS>(ctx, P);
+    case 32: return launch_fused_t<Op, K, 32, WM, MK, QS>(ctx, P);
+    case 40: return launch_fused_t<Op, K, 40, WM, MK, QS>(ctx, P);
+    case 48: return launch_fused_t<Op, K, 48, WM, MK, QS>(ctx, P);
+    case 64: return launch_fused_t<Op, K, 64, WM, MK, QS>(ctx, P);
     default: return set_error(SD_EINVAL, "internal: unsupported list length %d", L);
   }
 }
@@ -1390,8 +1397,12 @@ int fused_slab_topl(sd_context* ctx, const FusedArgs& a, double* cand_key, int32
     if (a.method == SD_PEARSON) return launch_fused_L<OpF32, float, WM, kKeyPearson>(ctx, P, a.L);
     return launch_fused_L<OpF32, float, WM>(ctx, P, a.L);
   }
-  if (a.method == SD_EUCLIDEAN) return launch_fused_L<OpBf16, float, WM, kKeyEuclid>(ctx, P, a.L);
-  if (a.method == SD_PEARSON) return launch_fused_L<OpBf16, float, WM, kKeyPearson>(ctx, P, a.L);
+  if (a.method == SD_EUCLIDEAN)
+    return P.qseg ? launch_fused_L<OpBf16, float, WM, kKeyEuclid, true>(ctx, P, a.L)
+                  : launch_fused_L<OpBf16, float, WM, kKeyEuclid>(ctx, P, a.L);
+  if (a.method == SD_PEARSON)
+    return P.qseg ? launch_fused_L<OpBf16, float, WM, kKeyPearson, true>(ctx, P, a.L)
+                  : launch_fused_L<OpBf16, float, WM, kKeyPearson>(ctx, P, a.L);
   return launch_fused_L<OpBf16, float, WM>(ctx, P, a.L);
 }
 
@@ -1450,8 +1461,12 @@ int fused_filter_pass(sd_context* ctx, const FusedArgs& a, const double* thr, in
     if (a.method == SD_PEARSON) return launch_filter_t<OpF32, float, RB, kKeyPearson>(ctx, P);
     return launch_filter_t<OpF32, float, RB>(ctx, P);
   }
-  if (a.method == SD_EUCLIDEAN) return launch_filter_t<OpBf16, float, RB, kKeyEuclid>(ctx, P);
-  if (a.method == SD_PEARSON) return launch_filter_t<OpBf16, float, RB, kKeyPearson>(ctx, P);
+  if (a.method == SD_EUCLIDEAN)
+    return P.qseg ? launch_filter_t<OpBf16, float, RB, kKeyEuclid, true>(ctx, P)
+                  : launch_filter_t<OpBf16, float, RB, kKeyEuclid>(ctx, P);
+  if (a.method == SD_PEARSON)
+    return P.qseg ? launch_filter_t<OpBf16, float, RB, kKeyPearson, true>(ctx, P)
+                  : launch_filter_t<OpBf16, float, RB, kKeyPearson>(ctx, P);
   return launch_filter_t<OpBf16, float, RB>(ctx, P);
 }
 

@@ -298,9 +298,9 @@ int fused_candidate_lists(sd_context* ctx, FusedArgs fa, const double** out_key,
 }
 
 // Constants of the euclidean and pearson keys' error bound (euc_key and
-// pear_key in k_slab.hip): the
-// slab's dot product x of a query q with nnz(q) = m stored values against a
-// reference r obeys |x - q.r| <= (e0 + e1 m) sum |q_i r_i|.
+// pear_key in k_slab.hip): the slab's dot product x of a query q with
+// nnz(q) = m stored values against a reference r obeys
+// |x - q.r| <= (e0 + e1 m) sum |q_i r_i|.
 //  * split BF16 (hi = bf16(v), lo = bf16(v - hi), each within ub = 2^-8 +
 //    2^-23 relative, the second term of ub for the FP32 step): the three
 //    products hi.hi + hi.lo + lo.hi miss lo.lo and the rounding of lo, at most
@@ -309,8 +309,9 @@ int fused_candidate_lists(sd_context* ctx, FusedArgs fa, const double** out_key,
 //    2^-23 relative (twice the round-to-nearest unit, so the bound holds for
 //    any rounding of the accumulator), inflated by 1 / (1 - n 2^-23) for the
 //    n = 3 nrow additions of the longest column.
+//  * plain BF16: each product within (1 + ub)^2 - 1 = 2 ub + ub^2; m terms.
 //  * FP32: each input within 2^-24 and each product rounded once (2^-23
-//    allowed): 4 2^-23 per row; m accumulated terms.
+//    allowed): 4 2^-23 per row; m terms.
 // For non-negative data the key needs q.r <= x / (1 - e), so both constants
 // are divided by 1 - e at the longest column.  The path is taken only for
 // finite values in [1e-15, 1e15] (no FP32 overflow of squares or norms, no
@@ -319,7 +320,7 @@ struct DotBound {
   float e0 = 0.f, e1 = 0.f;
   int nonneg = 0;
 };
-bool dot_bound(const sd_matrix* ref, const sd_matrix* X, bool f32, DotBound* out) {
+bool dot_bound(const sd_matrix* ref, const sd_matrix* X, int kind, DotBound* out) {
   for (const sd_matrix* m : {ref, X}) {
     if (m->any_nonfinite) return false;
     if (m->nnz > 0 && (m->abs_min < 1e-15 || m->abs_max > 1e15)) return false;
@@ -328,12 +329,22 @@ bool dot_bound(const sd_matrix* ref, const sd_matrix* X, bool f32, DotBound* out
   const double nrow = (double)std::max<int64_t>(X->nrow, 1);
   const double u23 = std::ldexp(1.0, -23);
   const double ub = std::ldexp(1.0, -8) + u23;
-  const double terms = f32 ? 1.0 : 3.0;               // accumulated terms per row
+  const double terms = kind == kDenseBf16Split ? 3.0 : 1.0;  // accumulated terms per row
   const double n = terms * nrow;
   if (n * u23 >= 0.2) return false;
   const double g = 1.0 / (1.0 - n * u23);
-  double e0 = f32 ? 4.0 * u23 : 3.1 * ub * ub;
-  double e1 = terms * 1.02 * u23 * g;
+  double e0, mag;  // per-row representation error; magnitude of the summed terms
+  if (kind == kDenseBf16Split) {
+    e0 = 3.1 * ub * ub;
+    mag = 1.02;
+  } else if (kind == kDenseBf16Raw) {
+    e0 = 2.0 * ub + ub * ub;
+    mag = (1.0 + ub) * (1.0 + ub);
+  } else {
+    e0 = 4.0 * u23;
+    mag = 1.0 + 4.0 * u23;
+  }
+  const double e1 = terms * mag * u23 * g;
   const double emax = e0 + e1 * nrow;
   if (emax > 0.25) return false;
   const double fold = 1.0 / (1.0 - emax);
@@ -375,11 +386,20 @@ int run_knn(sd_context* ctx, sd_matrix* X, sd_matrix* Y, const sd_knn_params* p,
   }
   // densified layout: binary -> 0/1 patterns (FP4 nibbles, or FP32 on the
   // plain path); cosine -> unit-normalised values (BF16, or FP32);
-  // euclidean and pearson -> raw values (split BF16 hi/lo, or FP32)
+  // euclidean and pearson -> raw values (BF16 for non-negative data until a
+  // search on this reference falls back too often, else split BF16 hi/lo;
+  // or FP32)
   const bool use_f32 = (prec == SD_PREC_FP32);
+  bool raw_plain = !ref->any_negative && !X->any_negative;
+  if (raw_plain) {
+    std::lock_guard<std::mutex> lk(ref->prep_mu);
+    raw_plain = !ref->raw_split;
+  }
   const int kind = method == SD_BINARY   ? (use_f32 ? kDenseF32Pattern : kDenseF4Pattern)
                    : method == SD_COSINE ? (use_f32 ? kDenseF32Unit : kDenseBf16Unit)
-                                         : (use_f32 ? kDenseF32Raw : kDenseBf16Split);
+                   : use_f32             ? kDenseF32Raw
+                   : raw_plain           ? kDenseBf16Raw
+                                         : kDenseBf16Split;
   const int64_t kpad = round_up(std::max<int64_t>(ref->nrow, 1), dense_row_align(kind));
   const bool fits = fused_fits(dense_row_bytes(kind, kpad));
   bool fused_ok_binary = (method == SD_BINARY) && prec != SD_PREC_EXACT &&
@@ -389,7 +409,7 @@ int run_knn(sd_context* ctx, sd_matrix* X, sd_matrix* Y, const sd_knn_params* p,
   DotBound eb;
   const bool raw_dot = method == SD_EUCLIDEAN || method == SD_PEARSON;
   const bool fused_ok_raw = raw_dot && prec != SD_PREC_EXACT && fused_list_len(k + margin) > 0 &&
-                            nref >= 1 && fits && dot_bound(ref, X, use_f32, &eb);
+                            nref >= 1 && fits && dot_bound(ref, X, kind, &eb);
   bool fused_ok_euclid = fused_ok_raw && method == SD_EUCLIDEAN;
   bool fused_ok_pearson = fused_ok_raw && method == SD_PEARSON;
   if (fused_ok_binary || fused_ok_cosine || fused_ok_euclid || fused_ok_pearson) {
@@ -413,6 +433,7 @@ int run_knn(sd_context* ctx, sd_matrix* X, sd_matrix* Y, const sd_knn_params* p,
 
   if (fused_ok_binary || fused_rerank) {
     ++ctx->stats.fused_calls;
+    ctx->stats.slab_products = kind == kDenseBf16Split ? 3 : 1;
     {
       std::lock_guard<std::mutex> lk(ref->prep_mu);
       SD_TRY(prep_dense(ctx, ref, kind, kpad));
@@ -557,6 +578,10 @@ int run_knn(sd_context* ctx, sd_matrix* X, sd_matrix* Y, const sd_knn_params* p,
       }
       ctx->stats.fallback_queries += nfail;
       ctx->stats.certified_queries += nq - nfail;
+      if (kind == kDenseBf16Raw && (int64_t)nfail * 100 > nq) {
+        std::lock_guard<std::mutex> lk(ref->prep_mu);
+        ref->raw_split = true;  // later searches on this reference use hi/lo
+      }
       if (nfail > 0)
         SD_TRY(exact_knn_columns(ctx, ea, fail_ids, nfail, nq, k, plan.dec, 0,
                                  out_idx.as<int32_t>(), out_val.as<double>()));

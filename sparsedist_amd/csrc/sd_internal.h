@@ -130,9 +130,13 @@ struct sd_matrix {
   sd::DevBuf sqn, sqf;
   // cosine re-rank margin tuned on this matrix as a reference (0 = default)
   int tuned_margin = 0;
+  // euclidean / pearson on non-negative data start on plain BF16 values and
+  // move to the split layout once a search on this reference had to fall
+  // back to the exact path for more than 1% of its queries
+  bool raw_split = false;
   // densified layouts, one per kind: [ncol][kpad] column-contiguous
-  int64_t dense_kpad[6] = {0, 0, 0, 0, 0, 0};
-  sd::DevBuf dense[6];
+  int64_t dense_kpad[7] = {0, 0, 0, 0, 0, 0, 0};
+  sd::DevBuf dense[7];
   sd::Csc view() const {
     sd::Csc c;
     c.nrow = nrow; c.ncol = ncol;
@@ -152,8 +156,11 @@ namespace sd {
 // hi = bf16(v) and lo = bf16(v - hi) in three row segments [hi | hi | lo]; a
 // query read with its last two segments swapped, [hi | lo | hi], gives the
 // dot product hi.hi + hi.lo + lo.hi in one MFMA pass over 3 kpad.
+// kDenseBf16Raw holds bf16(v) only: for non-negative data the keys' error
+// bound is relative to each pair's own dot product, which plain BF16 meets.
 enum DenseKind { kDenseF4Pattern = 0, kDenseBf16Unit = 1, kDenseBf16Split = 2, kDenseF32Unit = 3,
-                 kDenseF32Raw = 4, kDenseF32Pattern = 5, kDenseKinds = 6 };
+                 kDenseF32Raw = 4, kDenseF32Pattern = 5, kDenseBf16Raw = 6, kDenseKinds = 7 };
+static_assert(sizeof(sd_matrix::dense) / sizeof(DevBuf) == kDenseKinds, "one layout slot per kind");
 
 // ---- prep (k_prep.hip) ------------------------------------------------------
 int prep_flags(sd_context* ctx, sd_matrix* m);                 // deg, nonfinite, negative
@@ -245,10 +252,10 @@ int rerank_map_words(int64_t nrow);
 // 256 rows.  The FP4 pattern layout packs two rows per byte.
 inline int64_t dense_row_align(int kind) { return kind == kDenseF4Pattern ? 512 : 256; }
 inline int64_t dense_row_bytes(int kind, int64_t kpad) {
-  return kind == kDenseF4Pattern ? kpad / 2
-         : kind == kDenseBf16Unit ? kpad * 2
-         : kind == kDenseBf16Split ? kpad * 6
-                                   : kpad * 4;
+  return kind == kDenseF4Pattern                             ? kpad / 2
+         : (kind == kDenseBf16Unit || kind == kDenseBf16Raw) ? kpad * 2
+         : kind == kDenseBf16Split                           ? kpad * 6
+                                                             : kpad * 4;
 }
 // Bytes of one segment whose query copy is read swapped with the next one
 // (the split layout's hi | lo), 0 for layouts read as they are.

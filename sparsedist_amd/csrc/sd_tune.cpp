@@ -19,7 +19,7 @@ namespace sd {
 bool tune_prefer_exact(const sd_context* ctx, int kind, int64_t nq, int64_t nref, int64_t kpad,
                        double nnz_q, double nnz_ref) {
   const TuneRates& t = ctx->tune;
-  const bool bf16 = kind == kDenseBf16Unit || kind == kDenseBf16Split;
+  const bool bf16 = kind == kDenseBf16Unit || kind == kDenseBf16Split || kind == kDenseBf16Raw;
   const double rate = kind == kDenseF4Pattern ? t.fp4_ops : bf16 ? t.bf16_ops : t.fp32_ops;
   const double fixed = kind == kDenseF4Pattern ? t.fp4_fixed_s : bf16 ? t.bf16_fixed_s : t.fp32_fixed_s;
   // the split layout runs three products per row

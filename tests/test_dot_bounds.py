@@ -14,7 +14,9 @@ math.fsum, so the check does not reuse the formula it tests:
 * the euclidean key's lower bound never exceeds the exact squared distance,
   and the pearson key's upper bound is never below the exact correlation;
 * a slab that dropped the lo.hi term breaks the first bound (the check has
-  teeth for the mistake it guards against).
+  teeth for the mistake it guards against);
+* the plain BF16 layout (hi only, used for non-negative data) meets its own,
+  wider constants.
 """
 import math
 
@@ -177,3 +179,39 @@ def test_pearson_upper_bound(signed):
         assert exact <= ub, (trial, exact, ub)
         # the bound is useful: within a few hundred ppm of the value
         assert ub - exact < 2e-3, (trial, ub - exact)
+
+
+def plain_constants(nrow):
+    g = 1.0 / (1.0 - nrow * U23)
+    return 2.0 * UB + UB * UB, (1.0 + UB) ** 2 * U23 * g
+
+
+def plain_dot(q, r, rng):
+    t = (bf16(q.astype(np.float32)).astype(np.float64) * bf16(r.astype(np.float32))).astype(
+        np.float32)
+    t = t[t != 0]
+    rng.shuffle(t)
+    return float(np.cumsum(t, dtype=np.float32)[-1]) if t.size else 0.0
+
+
+def test_plain_bf16_bound_nonnegative():
+    rng = np.random.default_rng(31)
+    nrow = 4000
+    e0, e1 = plain_constants(nrow)
+    fold = 1.0 / (1.0 - (e0 + e1 * nrow))
+    worst = 0.0
+    for trial in range(60):
+        nnz = int(rng.integers(50, 1500))
+        q, r = pair(rng, nrow, nnz, rng.random(), False)
+        x = plain_dot(q, r, rng)
+        exact = math.fsum(q * r)
+        bound = (e0 + e1 * nnz) * exact
+        assert abs(x - exact) <= bound, trial
+        worst = max(worst, abs(x - exact) / bound)
+        d2 = math.fsum((q - r) ** 2)
+        lb = key_lower_bound(math.fsum(q * q), math.fsum(r * r), x, nnz, e0 * fold, e1 * fold,
+                             True)
+        assert lb <= d2, (trial, lb, d2)
+    # the representation term is attained to within a factor of a few (values
+    # just past rounding ties are in every pair), so the constant is not loose
+    assert worst > 0.05
