@@ -432,6 +432,11 @@ def large_main(args, sd, synth, workers):
     dX = sd.DeviceMatrix(X, ctx)
     log(f"uploaded in {time.time() - t0:.1f}s")
     k, nq = 20, X.shape[1]
+    # full-size parity on a seeded sample of query columns: the exact FP64
+    # merge walk over all 10^6 reference columns must give the same idx and
+    # bit-identical dist as the fused search
+    sample = np.sort(np.random.default_rng(77).choice(nq, 512, replace=False))
+    Xs = X[:, sample]
     res = {}
     for method in ("binary", "cosine", "euclidean"):
         dX.drop_cache()
@@ -440,15 +445,21 @@ def large_main(args, sd, synth, workers):
         t = time.perf_counter()
         if method == "euclidean":
             blk = 125_000
+            got_i = np.empty((sample.size, k), np.int32)
+            got_d = np.empty((sample.size, k))
             for b0 in range(0, nq, blk):
                 nn = sd.sparseKNN(X[:, b0:b0 + blk], dX, k=k, method=method, verbose=False,
                                   context=ctx)
+                inb = (sample >= b0) & (sample < b0 + blk)
+                got_i[inb] = nn.idx[sample[inb] - b0]
+                got_d[inb] = nn.dist[sample[inb] - b0]
                 del nn
                 log(f"euclidean block {b0 // blk + 1}/{(nq + blk - 1) // blk}: "
                     f"{time.perf_counter() - t:.1f} s")
         else:
             nn = sd.sparseKNN(dX, k=k, method=method, block_size=8192, verbose=False,
                               context=ctx)
+            got_i, got_d = nn.idx[sample].copy(), nn.dist[sample].copy()
             del nn
         dt = time.perf_counter() - t
         st = ctx.stats(reset=True)
@@ -468,6 +479,30 @@ def large_main(args, sd, synth, workers):
         log(f"{method}: {dt:.1f} s ({nq / dt:.0f} cols/s), filtered pass {fms:.0f} ms "
             f"({res[method]['filter_frac_of_peak'] * 100:.0f}% of peak), escalated "
             f"{st['escalated_queries']}, fallback {st['fallback_queries']}")
+        t = time.perf_counter()
+        if method == "euclidean":  # the fused blocks were cross searches
+            ex = sd.sparseKNN(Xs, dX, k=k, method=method, precision="exact", verbose=False,
+                              context=ctx)
+            want_i, want_d = ex.idx, ex.dist
+        else:  # self-search without self: drop each column's own entry from k + 1
+            ex = sd.sparseKNN(Xs, dX, k=k + 1, method=method, precision="exact",
+                              verbose=False, context=ctx)
+            want_i = np.empty((sample.size, k), np.int32)
+            want_d = np.empty((sample.size, k))
+            for r, c in enumerate(sample):
+                keep = ex.idx[r] != c + 1
+                if keep.all():
+                    keep[-1] = False
+                want_i[r] = ex.idx[r][keep][:k]
+                want_d[r] = ex.dist[r][keep][:k]
+        same_i = bool(np.array_equal(got_i, want_i))
+        same_d = bool(np.array_equal(got_d.view(np.int64), want_d.view(np.int64)))
+        res[method]["sample_parity"] = dict(queries=int(sample.size), idx_identical=same_i,
+                                            dist_bit_identical=same_d,
+                                            exact_seconds=time.perf_counter() - t)
+        log(f"{method}: exact FP64 check of {sample.size} sampled queries against all "
+            f"{nq} references: idx identical {same_i}, dist identical {same_d} "
+            f"({time.perf_counter() - t:.1f} s)")
     secs = sum(res[m]["seconds"] for m in res)
     out = {
         "metric": "exact-kNN query cols/s (k=20), large sparse self-search",
