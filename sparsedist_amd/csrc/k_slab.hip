@@ -102,18 +102,26 @@ __device__ inline double key_to_double(float k) { return (double)k; }
 // dot x.  With e = e0 + e1 nnz(q) the dot obeys |x - q.r| <= e S, where
 // S = sum |q_i r_i| <= |q||r| <= (qa + ra) / 2; for non-negative data S is
 // q.r itself and q.r <= x / (1 - e) (the host folds 1 / (1 - e) into e0, e1).
-// kEucRound covers the FP32 rounding of qa, ra and of this formula (at most
-// six roundings of quantities below 2 (qa + ra), 2^-24 each); kEucAbs covers
-// flushed subnormal products.  Larger keys are better, like every key here.
+// So D >= s - err with s = qa + ra - 2x and err = 2e |x| + kEucRound (qa + ra)
+// (non-negative data) or (e + kEucRound) (qa + ra) (signed).  kEucRound covers
+// the FP32 rounding of qa, ra and of this formula (at most six roundings of
+// quantities below 2 (qa + ra), 2^-24 each); kEucAbs covers flushed
+// subnormal products.  The per-query weights w1, w2 select the case without
+// a branch (a branch per value made the compiler spill); larger keys are
+// better, like every key here.
 constexpr float kEucRound = 4.76837158203125e-07f;  // 2^-21
 constexpr float kEucAbs = 1e-30f;
-__device__ inline float euc_key(float qa, float ra, float x, uint32_t qnnz, float e0, float e1,
-                                int nonneg) {
+struct KeyWeights {
+  float w1, w2, w3;
+};
+__device__ inline KeyWeights euc_weights(uint32_t qnnz, float e0, float e1, int nonneg) {
+  const float e = e0 + e1 * (float)qnnz;
+  return KeyWeights{nonneg ? 2.f * e : 0.f, nonneg ? kEucRound : e + kEucRound, 0.f};
+}
+__device__ inline float euc_key(const KeyWeights& w, float qa, float ra, float x) {
   const float n2 = qa + ra;
   const float s = n2 - 2.f * x;
-  const float e = e0 + e1 * (float)qnnz;
-  const float err = nonneg ? 2.f * e * fabsf(x) + kEucRound * n2 + kEucAbs
-                           : (e + kEucRound) * n2 + kEucAbs;
+  const float err = (w.w1 * fabsf(x) + w.w2 * n2) + kEucAbs;
   return err - s;
 }
 // Pearson key: an upper bound of the correlation
@@ -121,19 +129,27 @@ __device__ inline float euc_key(float qa, float ra, float x, uint32_t qnnz, floa
 // c = 1 / sqrt(centred sum of squares) per column and the slab's raw dot x.
 // The dot obeys |x - q.y| <= e S with S = sum |q_i y_i|: S = q.y <= x / (1 - e)
 // for non-negative data (folded into e by the host), else
-// S <= |q||y| = sq sy sqrt(1 + (aq cq)^2) sqrt(1 + (ay cy)^2).  kPearRound
-// covers the FP32 rounding of the aux values and of this formula (at most
-// ten roundings, 2^-24 each, of quantities below (|x| + |t|) c_q c_y or S).
+// S <= |q||y| = sq sy g_q g_y with g = sqrt(1 + (a c)^2).  The bound adds
+// (e + kPearRound) S c_q c_y + kPearRound (|x| + |t|) c_q c_y to the estimate;
+// kPearRound covers the FP32 rounding of the aux values and of this formula
+// (at most ten roundings, 2^-24 each, of quantities below (|x| + |t|) c_q c_y
+// or S c_q c_y).  Weights: w1 = e + kPearRound on |x| c_q c_y (non-negative
+// data) and w2 = (e + kPearRound) g_q on g_y (signed), the other one 0.
 constexpr float kPearRound = 9.5367431640625e-07f;  // 2^-20
-__device__ inline float pear_key(float qa, float qc, float ra, float rc, float x, uint32_t qnnz,
-                                 float e0, float e1, int nonneg) {
+__device__ inline KeyWeights pear_weights(float qa, float qc, uint32_t qnnz, float e0, float e1,
+                                          int nonneg) {
+  const float ew = (e0 + e1 * (float)qnnz) + kPearRound;
+  const float gq = sqrtf(1.f + (qa * qc) * (qa * qc));
+  return KeyWeights{nonneg ? ew : 0.f, nonneg ? 0.f : ew * gq, 0.f};
+}
+__device__ inline float pear_key(const KeyWeights& w, float qa, float qc, float ra, float rc,
+                                 float x) {
   const float t = qa * ra;  // n mean_q mean_y
   const float sc = qc * rc;
   const float r = (x - t) * sc;
-  const float e = e0 + e1 * (float)qnnz;
-  const float s = nonneg ? fabsf(x) * sc
-                         : sqrtf(1.f + (qa * qc) * (qa * qc)) * sqrtf(1.f + (ra * rc) * (ra * rc));
-  return r + ((e + kPearRound) * s + kPearRound * (fabsf(x) + fabsf(t)) * sc + kEucAbs);
+  const float gy = sqrtf(1.f + (ra * rc) * (ra * rc));
+  return r + (((w.w1 * (fabsf(x) * sc) + w.w2 * gy) + kPearRound * (fabsf(x) + fabsf(t)) * sc) +
+              kEucAbs);
 }
 // Split layout's query offset: segment 1 <-> 2.  Written as arithmetic, not
 // selects, so that the K loop stays one basic block (a branch there makes the
@@ -286,6 +302,9 @@ __global__ __launch_bounds__(WM * 256, 1) void k_fused(FusedKernelArgs P) {
     if (P.qry_aux) qaux = P.qry_aux[my_q];
     if (P.qry_aux2) qaux2 = P.qry_aux2[my_q];
   }
+  const KeyWeights kw = MK == kKeyPearson
+                            ? pear_weights(qaux, qaux2, nj, P.dot_e0, P.dot_e1, P.dot_nonneg)
+                            : euc_weights(nj, P.dot_e0, P.dot_e1, P.dot_nonneg);
 
   Entry<K> list[L];
 #pragma unroll
@@ -518,11 +537,10 @@ __global__ __launch_bounds__(WM * 256, 1) void k_fused(FusedKernelArgs P) {
             float sv;
             __builtin_memcpy(&sv, &bits, 4);
             if constexpr (MK == kKeyEuclid) {
-              key = euc_key(qaux, s_raux[rloc], sv, nj, P.dot_e0, P.dot_e1, P.dot_nonneg);
+              key = euc_key(kw, qaux, s_raux[rloc], sv);
               if (P.self_mode && rg == my_q) key = FLT_MAX;  // distance 0
             } else if constexpr (MK == kKeyPearson) {
-              key = pear_key(qaux, qaux2, s_raux[rloc], s_raux2[rloc], sv, nj, P.dot_e0, P.dot_e1,
-                             P.dot_nonneg);
+              key = pear_key(kw, qaux, qaux2, s_raux[rloc], s_raux2[rloc], sv);
               if (P.self_mode && rg == my_q) key = FLT_MAX;  // restored self correlation
             } else {
               key = sv;
@@ -765,6 +783,7 @@ __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
   bool qv[2];
   uint32_t qn[2];
   float qa[2], qa2[2], thr_f[2];
+  KeyWeights kw[2];
   double thr_d[2];
 #pragma unroll
   for (int j = 0; j < 2; ++j) {
@@ -774,6 +793,8 @@ __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
     qn[j] = qv[j] ? (uint32_t)(P.qry_p[qc[j] + 1] - P.qry_p[qc[j]]) : 0u;
     qa[j] = (qv[j] && P.qry_aux) ? P.qry_aux[qc[j]] : 0.f;
     qa2[j] = (qv[j] && P.qry_aux2) ? P.qry_aux2[qc[j]] : 0.f;
+    kw[j] = MK == kKeyPearson ? pear_weights(qa[j], qa2[j], qn[j], P.dot_e0, P.dot_e1, P.dot_nonneg)
+                              : euc_weights(qn[j], P.dot_e0, P.dot_e1, P.dot_nonneg);
     thr_d[j] = qv[j] ? P.thr[qp[j]] : INFINITY;
     // float screen a little below the threshold; the exact test follows
     const float t = (float)thr_d[j];
@@ -794,11 +815,10 @@ __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
       float key;
       if constexpr (MK == kKeyEuclid) {
         key = self ? FLT_MAX  // distance 0
-                   : euc_key(qa[j], s_raux[rloc], x, qn[j], P.dot_e0, P.dot_e1, P.dot_nonneg);
+                   : euc_key(kw[j], qa[j], s_raux[rloc], x);
       } else if constexpr (MK == kKeyPearson) {
         key = self ? FLT_MAX  // restored self correlation
-                   : pear_key(qa[j], qa2[j], s_raux[rloc], s_raux2[rloc], x, qn[j], P.dot_e0,
-                              P.dot_e1, P.dot_nonneg);
+                   : pear_key(kw[j], qa[j], qa2[j], s_raux[rloc], s_raux2[rloc], x);
       } else {
         key = self ? 2.0f : x;  // restored self similarity
       }
@@ -806,22 +826,27 @@ __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
     }
     return kd >= thr_d[j];
   };
-  // cheap screen (may keep too much, never too little)
-  auto screen = [&](const float x, int j, int ni, int64_t rg) -> bool {
-    if (P.self_mode && rg == qc[j]) return true;
+  // cheap screen (may keep too much, never too little); the self value is
+  // marked separately below
+  auto screen = [&](const float x, int j, int ni, float ra, float rc) -> bool {
     if constexpr (sizeof(K) == 8) {
       // c / (ni + nj - c) >= thr  <=>  c >= thr (ni + nj - c)
       const float d = (float)((uint32_t)ni + qn[j]) - x;
       return x >= thr_f[j] * d || d <= 0.f;
-    } else if constexpr (MK != kKeyScore) {
-      return true;
+    } else if constexpr (MK == kKeyEuclid) {
+      return euc_key(kw[j], qa[j], ra, x) >= thr_f[j];
+    } else if constexpr (MK == kKeyPearson) {
+      return pear_key(kw[j], qa[j], qa2[j], ra, rc, x) >= thr_f[j];
     } else {
       return x >= thr_f[j];
     }
   };
-  // pass 1: screen and test every value; remember the passing ones
-  uint64_t pm[2] = {0ull, 0ull};  // bit 16b + r
-  int cnt[2] = {0, 0};
+  // Pass 1, unrolled over the 128 values: plain compares only, one bit per
+  // value (16 b + r of pm[j]).  The exact test and the append run in loops
+  // over the set bits (pass 2), so the epilogue's code stays a few thousand
+  // instructions: unrolled over all 128 values it outgrew the instruction
+  // cache and was fetched again for every tile.
+  uint64_t pm[2] = {0ull, 0ull};
 #pragma unroll
   for (int b = 0; b < 4; ++b)
 #pragma unroll
@@ -829,47 +854,83 @@ __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
       // the four references 8 q4 + 4h + 0..3 of block b are consecutive
       const int rl = wm * 128 + b * 32 + 8 * q4 + 4 * h;
       int4 nn = make_int4(0, 0, 0, 0);
+      float4 ra4 = make_float4(0.f, 0.f, 0.f, 0.f), rc4 = ra4;
       if constexpr (sizeof(K) == 8) nn = *reinterpret_cast<const int4*>(&s_rnnz[rl]);
+      if constexpr (MK != kKeyScore) ra4 = *reinterpret_cast<const float4*>(&s_raux[rl]);
+      if constexpr (MK == kKeyPearson) rc4 = *reinterpret_cast<const float4*>(&s_raux2[rl]);
       const int nis[4] = {nn.x, nn.y, nn.z, nn.w};
+      const float ras[4] = {ra4.x, ra4.y, ra4.z, ra4.w};
+      const float rcs[4] = {rc4.x, rc4.y, rc4.z, rc4.w};
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         const int r = 4 * q4 + e;
-        const int64_t rg = rg0 + b * 32 + 8 * q4 + e;
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
-          const float x = acc[b][j][r];
-          if (screen(x, j, nis[e], rg)) {
-            double kd;
-            if (exact(x, j, rl + e, nis[e], rg, kd)) {
-              pm[j] |= 1ull << (16 * b + r);
-              ++cnt[j];
-            }
-          }
-        }
+        for (int j = 0; j < 2; ++j)
+          if (screen(acc[b][j][r], j, nis[e], ras[e], rcs[e])) pm[j] |= 1ull << (16 * b + r);
+      }
+      // one group at a time: hoisting every group's LDS reads would hold
+      // 16 x 4 aux values in registers next to the accumulators
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  // the query's own column, kept by the self rule whatever its slab value
+  if (P.self_mode && P.include_self) {
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int64_t u = qc[j] - r_base - wm * 128;  // -1 - ... for invalid queries
+      if (u >= 0 && u < 128 && (((u & 31) >> 2) & 1) == h) {
+        const int w = (int)(u & 31);
+        pm[j] |= 1ull << (16 * (int)(u >> 5) + (((w >> 3) << 2) | (w & 3)));
       }
     }
+  }
+  // pass 2: exact test of the marked values, then one atomic per query to
+  // reserve buffer slots, then the writes
+  auto value_of = [&](int b, int j, int r) -> float {
+    float x = 0.f;
+#pragma unroll
+    for (int t = 0; t < 16; ++t) x = (r == t) ? (float)acc[b][j][t] : x;
+    return x;
+  };
+  int cnt[2] = {0, 0};
+#pragma unroll
+  for (int b = 0; b < 4; ++b)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      uint32_t m = (uint32_t)(pm[j] >> (16 * b)) & 0xFFFFu, keep = 0u;
+      while (m) {
+        const int r = __builtin_ctz(m);
+        m &= m - 1u;
+        const int rloc = wm * 128 + b * 32 + 8 * (r >> 2) + 4 * h + (r & 3);
+        const int64_t rg = rg0 + b * 32 + 8 * (r >> 2) + (r & 3);
+        double kd;
+        if (exact(value_of(b, j, r), j, rloc, s_rnnz[rloc], rg, kd)) keep |= 1u << r;
+      }
+      pm[j] = (pm[j] & ~(0xFFFFull << (16 * b))) | ((uint64_t)keep << (16 * b));
+      cnt[j] += __popc(keep);
+    }
   if (cnt[0] + cnt[1] == 0) return;
-  // pass 2: reserve with one atomic per query, write the passing values
   int at[2];
 #pragma unroll
   for (int j = 0; j < 2; ++j) at[j] = cnt[j] ? atomicAdd(&P.cnt[qp[j]], cnt[j]) : 0;
 #pragma unroll
   for (int b = 0; b < 4; ++b)
 #pragma unroll
-    for (int r = 0; r < 16; ++r)
-#pragma unroll
-      for (int j = 0; j < 2; ++j) {
-        if (!((pm[j] >> (16 * b + r)) & 1ull)) continue;
+    for (int j = 0; j < 2; ++j) {
+      uint32_t m = (uint32_t)(pm[j] >> (16 * b)) & 0xFFFFu;
+      while (m) {
+        const int r = __builtin_ctz(m);
+        m &= m - 1u;
         const int rloc = wm * 128 + b * 32 + 8 * (r >> 2) + 4 * h + (r & 3);
         const int64_t rg = rg0 + b * 32 + 8 * (r >> 2) + (r & 3);
         double kd;
-        exact(acc[b][j][r], j, rloc, s_rnnz[rloc], rg, kd);
+        exact(value_of(b, j, r), j, rloc, s_rnnz[rloc], rg, kd);
         if (at[j] < P.cap) {
           P.fkey[qp[j] * P.cap + at[j]] = kd;
           P.fidx[qp[j] * P.cap + at[j]] = (int32_t)rg;
         }
         ++at[j];
       }
+    }
 }
 
 // ---- merge per-split lists: top-L by (key desc, idx asc) --------------------

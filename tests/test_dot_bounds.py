@@ -83,17 +83,16 @@ def pair(rng, nrow, nnz, overlap, signed):
 
 
 def key_lower_bound(qa, ra, x, m, e0, e1, nonneg):
-    """euc_key's arithmetic in FP32, returning the lower bound (minus the key)."""
+    """euc_weights + euc_key's arithmetic in FP32, returning the lower bound
+    (minus the key)."""
     f = np.float32
     qa, ra, x = f(qa), f(ra), f(x)
+    e = f(f(e0) + f(f(e1) * f(m)))
+    w1 = f(f(2.0) * e) if nonneg else f(0.0)
+    w2 = K_EUC_ROUND if nonneg else f(e + K_EUC_ROUND)
     n2 = f(qa + ra)
     s = f(n2 - f(f(2.0) * x))
-    e = f(f(e0) + f(f(e1) * f(m)))
-    if nonneg:
-        err = f(f(f(f(2.0) * e) * f(abs(x))) + f(K_EUC_ROUND * n2))
-    else:
-        err = f(f(f(e + K_EUC_ROUND) * n2))
-    err = f(err + K_EUC_ABS)
+    err = f(f(f(w1 * f(abs(x))) + f(w2 * n2)) + K_EUC_ABS)
     return -float(f(err - s))
 
 
@@ -135,19 +134,19 @@ K_PEAR_ROUND = np.float32(2.0 ** -20)
 
 
 def pear_upper_bound(qa, qc, ra, rc, x, m, e0, e1, nonneg):
-    """pear_key's arithmetic in FP32."""
+    """pear_weights + pear_key's arithmetic in FP32."""
     f = np.float32
     qa, qc, ra, rc, x = f(qa), f(qc), f(ra), f(rc), f(x)
+    ew = f(f(f(e0) + f(f(e1) * f(m))) + K_PEAR_ROUND)
+    gq = f(np.sqrt(f(f(1.0) + f(f(qa * qc) * f(qa * qc)))))
+    w1 = ew if nonneg else f(0.0)
+    w2 = f(0.0) if nonneg else f(ew * gq)
     t = f(qa * ra)
     sc = f(qc * rc)
     r = f(f(x - t) * sc)
-    e = f(f(e0) + f(f(e1) * f(m)))
-    if nonneg:
-        s = f(f(abs(x)) * sc)
-    else:
-        s = f(f(np.sqrt(f(f(1.0) + f(f(qa * qc) * f(qa * qc))))) *
-              f(np.sqrt(f(f(1.0) + f(f(ra * rc) * f(ra * rc))))))
-    err = f(f(f(e + K_PEAR_ROUND) * s) + f(f(K_PEAR_ROUND * f(f(abs(x)) + f(abs(t)))) * sc))
+    gy = f(np.sqrt(f(f(1.0) + f(f(ra * rc) * f(ra * rc)))))
+    err = f(f(w1 * f(f(abs(x)) * sc)) + f(w2 * gy))
+    err = f(err + f(f(K_PEAR_ROUND * f(f(abs(x)) + f(abs(t)))) * sc))
     err = f(err + K_EUC_ABS)
     return float(f(r + err))
 
