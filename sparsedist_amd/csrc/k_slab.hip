@@ -986,11 +986,14 @@ __global__ __launch_bounds__(256) void k_select(const double* __restrict__ fkey,
                                                 int64_t nqry, int L, double* __restrict__ out_key,
                                                 int32_t* __restrict__ out_idx,
                                                 int32_t* __restrict__ over_list,
-                                                int32_t* __restrict__ over_count) {
+                                                int32_t* __restrict__ over_count,
+                                                int min_count) {
   __shared__ BlockTopKStorage<512> st;
   const int64_t q = blockIdx.x;
   const int n = cnt[q];
-  if (n > cap) {
+  // overflowed, or fewer candidates than the threshold guarantees complete
+  // lists for: the query is searched again by a full list pass
+  if (n > cap || n < min_count) {
     if (threadIdx.x == 0) over_list[atomicAdd(over_count, 1)] = (int32_t)q;
     for (int t = threadIdx.x; t < L; t += blockDim.x) out_idx[q * L + t] = -1;
     return;
@@ -1116,12 +1119,27 @@ __global__ __launch_bounds__(256) void k_rerank(RerankArgs R) {
           const int32_t* ra = R.ex.ref.i + a0;
           const double* ua = R.ex.ref_unit + a0;
           double acc = 0.0;
-          for (int64_t e = 0; e < na; ++e) {
-            const int32_t r = ra[e];
+          // entries in ascending row order, so the FP64 sum keeps the
+          // reference's order; the row indices stream in 16-byte loads (the
+          // index array is 256-byte aligned, so entry a0 + e is 16-byte
+          // aligned when (a0 + e) % 4 == 0) and a unit value is read only for
+          // a row the query also stores
+          auto step = [&](int32_t r, int64_t e) {
             const uint32_t word = bm[r >> 5];
             const uint32_t bit = 1u << (r & 31);
             if (word & bit) acc += ua[e] * qv[pre[r >> 5] + __popc(word & (bit - 1u))];
+          };
+          int64_t e = 0;
+          const int64_t head = min(na, (int64_t)((4 - (a0 & 3)) & 3));
+          for (; e < head; ++e) step(ra[e], e);
+          for (; e + 4 <= na; e += 4) {
+            const int4 r4 = *reinterpret_cast<const int4*>(ra + e);
+            step(r4.x, e);
+            step(r4.y, e + 1);
+            step(r4.z, e + 2);
+            step(r4.w, e + 3);
           }
+          for (; e < na; ++e) step(ra[e], e);
           c = acc;
           if (!(c == c)) c = 0.0;
           if (c > 1.0) c = 1.0;
@@ -1533,10 +1551,10 @@ int fused_filter_pass(sd_context* ctx, const FusedArgs& a, const double* thr, in
 
 int select_lists(sd_context* ctx, const double* fkey, const int32_t* fidx, const int32_t* cnt,
                  int cap, int64_t nqry, int L, double* out_key, int32_t* out_idx,
-                 int32_t* over_list, int32_t* over_count) {
+                 int32_t* over_list, int32_t* over_count, int min_count) {
   if (nqry <= 0) return SD_OK;
   hipLaunchKernelGGL(k_select, dim3((unsigned)nqry), dim3(256), 0, ctx->stream, fkey, fidx, cnt, cap,
-                     nqry, L, out_key, out_idx, over_list, over_count);
+                     nqry, L, out_key, out_idx, over_list, over_count, min_count);
   SD_HIP(hipGetLastError());
   return SD_OK;
 }

@@ -223,38 +223,48 @@ int fused_candidate_lists(sd_context* ctx, FusedArgs fa, const double** out_key,
     int64_t splits = (2 * (int64_t)ctx->cu_count + qtiles - 1) / qtiles;
     a.splits = (int)std::min<int64_t>(round_up(std::max<int64_t>(splits, 1), 8), 4096);
     const int nlists = fused_effective_splits(a);
-    SD_TRY(kbuf.ensure(sizeof(double) * (size_t)nlists * (size_t)a.nqry * (size_t)L + 64));
-    SD_TRY(ibuf.ensure(sizeof(int32_t) * (size_t)nlists * (size_t)a.nqry * (size_t)L + 64));
+    const int La = a.L;
+    SD_TRY(kbuf.ensure(sizeof(double) * (size_t)nlists * (size_t)a.nqry * (size_t)La + 64));
+    SD_TRY(ibuf.ensure(sizeof(int32_t) * (size_t)nlists * (size_t)a.nqry * (size_t)La + 64));
     SD_TRY(fused_slab_topl(ctx, a, kbuf.as<double>(), ibuf.as<int32_t>()));
     *rk = kbuf.as<double>();
     *ri = ibuf.as<int32_t>();
     if (nlists > 1) {
-      SD_TRY(mkbuf.ensure(sizeof(double) * (size_t)a.nqry * (size_t)L + 64));
-      SD_TRY(mibuf.ensure(sizeof(int32_t) * (size_t)a.nqry * (size_t)L + 64));
-      SD_TRY(merge_lists(ctx, *rk, *ri, nlists, a.nqry, L, mkbuf.as<double>(), mibuf.as<int32_t>()));
+      SD_TRY(mkbuf.ensure(sizeof(double) * (size_t)a.nqry * (size_t)La + 64));
+      SD_TRY(mibuf.ensure(sizeof(int32_t) * (size_t)a.nqry * (size_t)La + 64));
+      SD_TRY(merge_lists(ctx, *rk, *ri, nlists, a.nqry, La, mkbuf.as<double>(),
+                         mibuf.as<int32_t>()));
       *rk = mkbuf.as<double>();
       *ri = mibuf.as<int32_t>();
     }
     return SD_OK;
   };
-  const int64_t stride = 32;  // pre-pass samples 1/32 of the reference tiles
-  if (rtiles < 4 * stride) {
+  // The pre-pass samples every stride-th reference tile and takes each
+  // query's Lp-th key there as its threshold.  About stride x Lp references
+  // of the whole set score at or above it (the Lp-th of a 1/stride sample),
+  // so a sparser sample with a proportionally smaller rank keeps the
+  // candidate count and costs less: 1/64 at rank ~L/2 for large references,
+  // 1/32 at rank L below 256 tiles (a sample of at least 4 tiles).
+  if (rtiles < 4 * 32) {
     fa.tile_stride = 1;
     return list_pass(fa, ctx->scratch[3], ctx->scratch[4], ctx->scratch[5], ctx->scratch[6],
                      out_key, out_idx);
   }
+  const int64_t stride = rtiles >= 4 * 64 ? 64 : 32;
+  const int Lp = stride == 64 ? fused_list_len((L + 1) / 2) : L;
   // 1. thresholds from every stride-th reference tile
   const double* pk = nullptr;
   const int32_t* pi = nullptr;
   FusedArgs pre = fa;
   pre.tile_stride = (int)stride;
+  pre.L = Lp;
   SD_TRY(list_pass(pre, ctx->scratch[3], ctx->scratch[4], ctx->scratch[5], ctx->scratch[6], &pk,
                    &pi));
   DevBuf& thr = ctx->scratch[13];
   SD_TRY(thr.ensure(sizeof(double) * (size_t)nq + 64));
-  SD_TRY(list_thresholds(ctx, pk, pi, nq, L, thr.as<double>()));
-  // 2. filtered full pass
-  const int cap = (int)std::min<int64_t>(4096, 2 * L * stride + 256);
+  SD_TRY(list_thresholds(ctx, pk, pi, nq, Lp, thr.as<double>()));
+  // 2. filtered full pass: room for twice the expected candidates
+  const int cap = (int)std::min<int64_t>(4096, 2 * Lp * stride + 256);
   DevBuf& cnt = ctx->scratch[14];
   DevBuf& fk = ctx->scratch[9];
   DevBuf& fi = ctx->scratch[10];
@@ -273,8 +283,12 @@ int fused_candidate_lists(sd_context* ctx, FusedArgs fa, const double** out_key,
   SD_TRY(ov.ensure(sizeof(int32_t) * (size_t)(nq + 16)));
   int32_t* over_count = ov.as<int32_t>() + nq;
   SD_HIP(hipMemsetAsync(over_count, 0, sizeof(int32_t), ctx->stream));
+  // A threshold from rank Lp < L of the sample can exceed the true L-th key;
+  // at least L candidates at or above it prove it does not (then L references
+  // score >= thr, so the L-th key is >= thr and the lists are complete).
   SD_TRY(select_lists(ctx, fk.as<double>(), fi.as<int32_t>(), cnt.as<int32_t>(), cap, nq, L,
-                      mk.as<double>(), mi.as<int32_t>(), ov.as<int32_t>(), over_count));
+                      mk.as<double>(), mi.as<int32_t>(), ov.as<int32_t>(), over_count,
+                      Lp < L ? L : 0));
   int32_t nover = 0;
   SD_HIP(hipMemcpyAsync(&nover, over_count, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
   SD_HIP(hipStreamSynchronize(ctx->stream));

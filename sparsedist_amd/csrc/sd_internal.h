@@ -274,9 +274,12 @@ bool fused_fits(int64_t row_bytes);
 // >= thr[query] are appended to fkey/fidx ([nqry][cap]), counted in cnt.
 int fused_filter_pass(sd_context* ctx, const FusedArgs& a, const double* thr, int cap,
                       int32_t* cnt, double* fkey, int32_t* fidx);
+// Per-query top-L of the appended candidates.  Queries with more than cap
+// candidates (buffer overflow) or fewer than min_count (the threshold cannot
+// vouch for complete lists) are listed in over_list for a full pass.
 int select_lists(sd_context* ctx, const double* fkey, const int32_t* fidx, const int32_t* cnt,
                  int cap, int64_t nqry, int L, double* out_key, int32_t* out_idx,
-                 int32_t* over_list, int32_t* over_count);
+                 int32_t* over_list, int32_t* over_count, int min_count);
 int list_thresholds(sd_context* ctx, const double* key, const int32_t* idx, int64_t nqry, int L,
                     double* thr);
 int scatter_lists(sd_context* ctx, const double* key, const int32_t* idx, const int32_t* rows,

@@ -457,3 +457,27 @@ def test_pearson_signed_constant_and_self(oracle_lib, gpu_ctx):
         g = sd.sparseKNN(X, Y, k=8, method="pearson", dist=dist, precision="bf16", verbose=False)
         o = oracle_lib.knn(X, Y, k=8, method="pearson", dist=dist, nthreads=NT)
         same((g.idx, g.dist), o, f"cross dist={dist}")
+
+
+@pytest.mark.parametrize("method,copies", [("cosine", 29), ("binary", 16)])
+def test_sparse_sample_threshold_redo(oracle_lib, gpu_ctx, method, copies):
+    """Large references (>= 256 tiles of 128) take thresholds from every 64th
+    tile at about half the list rank (L = 40 -> 24 for cosine, 24 -> 16 for
+    binary at k = 20).  Such a threshold can exceed the true L-th key: here
+    copies of the first query fill the first sampled tile beyond that rank,
+    so the sample's key at that rank is the top score and fewer than L
+    candidates pass; the query is searched again in full and every result
+    still matches the oracle."""
+    Y = synth.nb_log1p(600, 40000, 0.05, 43)
+    Yd = Y.tolil()
+    base = Y[:, 100]
+    for c in range(copies):
+        Yd[:, c] = base
+    Y = Yd.tocsc()
+    X = sp.hstack([base, Y[:, 5000:5060]]).tocsc()
+    gpu_ctx.stats(reset=True)
+    g = sd.sparseKNN(X, Y, k=20, method=method, precision="bf16", verbose=False)
+    st = gpu_ctx.stats(reset=True)
+    o = oracle_lib.knn(X, Y, k=20, method=method, nthreads=NT)
+    same((g.idx, g.dist), o, method)
+    assert st["overflow_queries"] >= 1
