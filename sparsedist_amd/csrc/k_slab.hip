@@ -1053,7 +1053,13 @@ __device__ inline uint64_t wave_shfl_u64(uint64_t v, int src) {
 // the common rows are added in ascending row order, exactly the order of the
 // reference's merge walk (src/sparseDist.cpp:294-298).  When the bitmap does
 // not fit (very tall matrices) the lane runs the merge walk itself.
-constexpr int kRerankWaveBytes = 16384;  // LDS per wave
+// LDS per wave: the query's row bitmap (one bit per row) and the exclusive
+// prefix of its per-word popcounts, which turn a row into the position of
+// the query's value; sized to the row count so that many waves share a CU.
+constexpr int kRerankWaveMax = 16384;
+__host__ __device__ inline int rerank_wave_bytes(int nwords) {
+  return ((((nwords + 3) & ~3) * 4 + ((nwords + 7) & ~7) * 2) + 15) & ~15;
+}
 
 __global__ __launch_bounds__(256) void k_rerank(RerankArgs R) {
   extern __shared__ __align__(16) uint8_t rsm[];
@@ -1064,21 +1070,16 @@ __global__ __launch_bounds__(256) void k_rerank(RerankArgs R) {
   const int64_t jq = active ? (R.qlist ? (int64_t)R.qlist[j] : j) : 0;  // query column
   const int64_t q0 = R.ex.qry.p[jq], qn = active ? R.ex.qry.p[jq + 1] - q0 : 0;
   const int nwords = R.map_words;
-  uint32_t* bm = reinterpret_cast<uint32_t*>(rsm + wave * kRerankWaveBytes);
+  uint32_t* bm = reinterpret_cast<uint32_t*>(rsm + wave * rerank_wave_bytes(nwords));
   uint16_t* pre = reinterpret_cast<uint16_t*>(bm + ((nwords + 3) & ~3));
-  double* qv_l = reinterpret_cast<double*>(pre + ((nwords + 7) & ~7));
-  const int64_t qcap = nwords > 0 ? (kRerankWaveBytes - ((nwords + 3) & ~3) * 4 -
-                                     ((nwords + 7) & ~7) * 2) / 8
-                                  : 0;
   const bool use_map = nwords > 0 && active && qn < 65536;
-  const double* qv = (qn <= qcap) ? qv_l : R.ex.qry_unit + q0;
+  const double* qv = R.ex.qry_unit + q0;  // read through the caches
   for (int w = lane; w < nwords; w += 64) bm[w] = 0u;
   __syncthreads();
   if (use_map) {
     for (int64_t e = lane; e < qn; e += 64) {
       const int32_t r = R.ex.qry.i[q0 + e];
       atomicOr(&bm[r >> 5], 1u << (r & 31));
-      if (qn <= qcap) qv_l[e] = R.ex.qry_unit[q0 + e];
     }
   }
   __syncthreads();
@@ -1598,15 +1599,15 @@ int finish_exact_lists(sd_context* ctx, const double* key, const int32_t* idx, i
 
 int rerank_map_words(int64_t nrow) {
   const int64_t nwords = (nrow + 31) / 32;
-  const int64_t bytes = ((nwords + 3) & ~3LL) * 4 + ((nwords + 7) & ~7LL) * 2;
-  return (bytes + 64 * 8 <= kRerankWaveBytes) ? (int)nwords : 0;
+  return (nwords < (1 << 20) && rerank_wave_bytes((int)nwords) <= kRerankWaveMax) ? (int)nwords
+                                                                                : 0;
 }
 
 int rerank_cosine(sd_context* ctx, const RerankArgs& R) {
   if (R.nqry <= 0) return SD_OK;
   KernelTimer timer(ctx, &ctx->stats.rerank_ms, &ctx->stats.rerank_launches);
   hipLaunchKernelGGL(k_rerank, dim3((unsigned)((R.nqry + 3) / 4)), dim3(256),
-                     R.map_words > 0 ? 4 * kRerankWaveBytes : 0, ctx->stream, R);
+                     R.map_words > 0 ? 4 * rerank_wave_bytes(R.map_words) : 0, ctx->stream, R);
   SD_HIP(hipGetLastError());
   return SD_OK;
 }
