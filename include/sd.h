@@ -143,6 +143,7 @@ typedef struct {
   double exact_fixed_ms;  /* per-search overhead of the merge walk */
   int32_t calibrated;     /* 1 once sd_autotune has measured them */
   int32_t reserved;
+  double step_latency_us; /* merge walk: time per sequential step of one pair */
 } sd_tune_info;
 int sd_autotune(sd_context* ctx, int32_t measure, sd_tune_info* out);
 

@@ -259,5 +259,6 @@ Rcpp::List autotune(bool measure) {
       Rcpp::Named("merge_steps") = t.merge_steps, Rcpp::Named("bf16_fixed_ms") = t.bf16_fixed_ms,
       Rcpp::Named("fp4_fixed_ms") = t.fp4_fixed_ms,
       Rcpp::Named("exact_fixed_ms") = t.exact_fixed_ms,
+      Rcpp::Named("step_latency_us") = t.step_latency_us,
       Rcpp::Named("calibrated") = (bool)t.calibrated);
 }

@@ -60,7 +60,8 @@ class TuneInfo(C.Structure):
     _fields_ = [("bf16_ops", C.c_double), ("fp4_ops", C.c_double), ("merge_steps", C.c_double),
                 ("bf16_fixed_ms", C.c_double), ("fp4_fixed_ms", C.c_double),
                 ("exact_fixed_ms", C.c_double),
-                ("calibrated", C.c_int32), ("reserved", C.c_int32)]
+                ("calibrated", C.c_int32), ("reserved", C.c_int32),
+                ("step_latency_us", C.c_double)]
 
 
 class ServerParams(C.Structure):

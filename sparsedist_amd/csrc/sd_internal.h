@@ -77,10 +77,11 @@ namespace sd {
 // Rates of the path chooser's cost model (sd_autotune measures them; the
 // defaults were measured on an MI355X with the count-matrix workload).
 struct TuneRates {
-  double bf16_ops = 1.2e15;    // filtered pass, dense-equivalent ops/s
-  double fp4_ops = 3.5e15;
+  double bf16_ops = 1.4e15;    // filtered pass, dense-equivalent ops/s (measured on MI355X)
+  double fp4_ops = 5.2e15;
   double fp32_ops = 1.2e14;
   double merge_steps = 8.0e10; // exact slab merge-walk steps/s
+  double step_latency_s = 0.4e-6;  // one pair's walk is sequential: time per step
   double bf16_fixed_s = 1.5e-3;  // per-search overheads
   double fp4_fixed_s = 0.5e-3;
   double fp32_fixed_s = 1.5e-3;

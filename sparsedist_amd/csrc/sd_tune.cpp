@@ -4,7 +4,12 @@
 // costs, on the densified MFMA path, a fixed overhead plus whole query tiles
 // of 256 columns times 2 r nref dense-equivalent operations at the filtered
 // pass's rate; on the FP64 merge walk it costs a smaller overhead plus
-// nq nref (nnz_q + nnz_ref) merge steps.  Both paths are exact, so the choice
+// nq nref (nnz_q + nnz_ref) merge steps at the walk's throughput, but no less
+// than one pair's nnz_q + nnz_ref sequential steps (a lone query's walks run
+// side by side, each at the latency of a dependent load per step).  The
+// densified path of the re-ranked methods adds that floor for its FP64
+// re-rank: one candidate's column per lane (cosine walks the reference
+// column against a bitmap of the query; euclidean and pearson walk both).  Both paths are exact, so the choice
 // changes only the time.  sd_autotune replaces the default rates by ones
 // measured on synthetic inputs of known size.
 #include <algorithm>
@@ -25,9 +30,13 @@ bool tune_prefer_exact(const sd_context* ctx, int kind, int64_t nq, int64_t nref
   // the split layout runs three products per row
   const double kwork = (double)kpad * (kind == kDenseBf16Split ? 3.0 : 1.0);
   const double qcols = (double)((nq + 255) / 256 * 256);
-  const double fused = fixed + qcols * (double)nref * 2.0 * kwork / rate;
+  const bool pattern = kind == kDenseF4Pattern || kind == kDenseF32Pattern;
+  const bool unit = kind == kDenseBf16Unit || kind == kDenseF32Unit;
+  const double rerank = pattern ? 0.0 : (unit ? nnz_ref : nnz_q + nnz_ref) * t.step_latency_s;
+  const double fused = fixed + qcols * (double)nref * 2.0 * kwork / rate + rerank;
   const double exact =
-      t.exact_fixed_s + (double)nq * (double)nref * (nnz_q + nnz_ref) / t.merge_steps;
+      t.exact_fixed_s + std::max((double)nq * (double)nref * (nnz_q + nnz_ref) / t.merge_steps,
+                                 (nnz_q + nnz_ref) * t.step_latency_s);
   return exact < fused;
 }
 
@@ -116,9 +125,11 @@ int calibrate(sd_context* ctx) {
   const SynthCsc Q2h = synth_csc(kRows, 8192, kPer, 3);
   const SynthCsc E1h = synth_csc(kRows, 8, kPer, 4);
   const SynthCsc E2h = synth_csc(kRows, 40, kPer, 5);
-  sd_matrix *Y = nullptr, *Q1 = nullptr, *Q2 = nullptr, *E1 = nullptr, *E2 = nullptr;
+  const SynthCsc E0h = synth_csc(kRows, 1, 4 * kPer, 6);
+  sd_matrix *Y = nullptr, *Q1 = nullptr, *Q2 = nullptr, *E1 = nullptr, *E2 = nullptr,
+            *E0 = nullptr;
   auto release = [&]() {
-    for (sd_matrix* m : {Y, Q1, Q2, E1, E2})
+    for (sd_matrix* m : {Y, Q1, Q2, E1, E2, E0})
       if (m) sd_matrix_free(m);
   };
   int rc = SD_OK;
@@ -129,6 +140,7 @@ int calibrate(sd_context* ctx) {
     if ((rc = upload(ctx, kRows, Q2h, &Q2)) != SD_OK) break;
     if ((rc = upload(ctx, kRows, E1h, &E1)) != SD_OK) break;
     if ((rc = upload(ctx, kRows, E2h, &E2)) != SD_OK) break;
+    if ((rc = upload(ctx, kRows, E0h, &E0)) != SD_OK) break;
     // densified path: two query batch sizes per kind; slope = rate, intercept = overhead
     const int methods[2] = {SD_COSINE, SD_BINARY};
     const int kinds[2] = {kDenseBf16Unit, kDenseF4Pattern};
@@ -158,6 +170,11 @@ int calibrate(sd_context* ctx) {
       t.merge_steps = steps;
       t.exact_fixed_s = std::max(e1 - s1 / steps, 5e-5);
     }
+    // a lone query with 1024 stored rows: each pair walks 1280 steps in turn
+    double e0 = 0;
+    if ((rc = best_of(3, ctx, E0, Y, SD_COSINE, SD_PREC_EXACT, &e0)) != SD_OK) break;
+    const double lat = (e0 - t.exact_fixed_s) / (4.0 * kPer + kPer);
+    if (plausible(lat, 1e-9, 1e-4)) t.step_latency_s = lat;
     t.calibrated = 1;
   } while (false);
   release();
@@ -187,6 +204,6 @@ extern "C" int sd_autotune(sd_context* ctx, int32_t measure, sd_tune_info* out) 
   const sd::TuneRates& t = ctx->tune;
   *out = sd_tune_info{t.bf16_ops,          t.fp4_ops,     t.merge_steps,
                       t.bf16_fixed_s * 1e3, t.fp4_fixed_s * 1e3, t.exact_fixed_s * 1e3,
-                      t.calibrated,         0};
+                      t.calibrated,         0,             t.step_latency_s * 1e6};
   return SD_OK;
 }
