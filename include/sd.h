@@ -297,6 +297,7 @@ typedef struct {
   int64_t pending, active;  /* waiting for admission / admitted and unfinished */
   int64_t reserved_bytes;   /* current reservations of admitted requests */
   int64_t query_cols;       /* query columns of completed requests */
+  int64_t packed_items;     /* work items that carried columns of several requests */
 } sd_server_stats;
 
 /* Y must stay alive until sd_server_destroy; ctx gives the device. */

@@ -84,7 +84,8 @@ class RequestInfo(C.Structure):
 class ServerStats(C.Structure):
     _fields_ = [("submitted", C.c_int64), ("admitted", C.c_int64), ("completed", C.c_int64),
                 ("refused", C.c_int64), ("pending", C.c_int64), ("active", C.c_int64),
-                ("reserved_bytes", C.c_int64), ("query_cols", C.c_int64)]
+                ("reserved_bytes", C.c_int64), ("query_cols", C.c_int64),
+                ("packed_items", C.c_int64)]
 
 
 _lib = None

@@ -152,35 +152,85 @@ Request* pick_locked(sd_server* s) {
   return best;
 }
 
-// One work item: query columns [c0, c1) of r against Y on the slot's context.
-int run_item(sd_server* s, sd_context* ctx, Request* r, int64_t c0, int64_t c1) {
-  const int64_t nc = c1 - c0;
-  const int64_t base = r->colptr[c0];
+// A piece of a work item: query columns [c0, c1) of request r.
+struct Seg {
+  Request* r;
+  int64_t c0, c1;
+};
+
+// Requests whose columns may share one search: every parameter that can
+// change a result is equal (the block size is a memory hint only).
+bool same_search(const sd_knn_params& a, const sd_knn_params& b) {
+  return a.k == b.k && a.method == b.method && a.dist == b.dist &&
+         a.drop_degenerate == b.drop_degenerate && a.precision == b.precision &&
+         a.margin == b.margin;
+}
+
+// One search of the pieces' columns, concatenated in order, against Y on the
+// slot's context; each piece's results go to its own request.  A query's
+// result does not depend on the other columns of the search, so packing
+// changes no answer.
+int run_segs(sd_server* s, sd_context* ctx, const std::vector<Seg>& segs) {
+  const Request* r0 = segs[0].r;
+  int64_t nc = 0, nnz = 0;
+  for (const Seg& g : segs) {
+    nc += g.c1 - g.c0;
+    nnz += g.r->colptr[g.c1] - g.r->colptr[g.c0];
+  }
   std::vector<int64_t> p((size_t)nc + 1);
-  for (int64_t j = 0; j <= nc; ++j) p[j] = r->colptr[c0 + j] - base;
+  std::vector<int32_t> ri;
+  std::vector<double> vx;
+  const int32_t* rows = nullptr;
+  const double* vals = nullptr;
+  if (segs.size() == 1) {  // one request's columns: its arrays are contiguous
+    const int64_t base = r0->colptr[segs[0].c0];
+    for (int64_t j = 0; j <= nc; ++j) p[j] = r0->colptr[segs[0].c0 + j] - base;
+    rows = r0->rowind + base;
+    vals = r0->values + base;
+  } else {
+    ri.resize((size_t)nnz);
+    vx.resize((size_t)nnz);
+    int64_t j = 0, e = 0;
+    for (const Seg& g : segs) {
+      const int64_t b0 = g.r->colptr[g.c0], b1 = g.r->colptr[g.c1];
+      std::copy(g.r->rowind + b0, g.r->rowind + b1, ri.begin() + e);
+      std::copy(g.r->values + b0, g.r->values + b1, vx.begin() + e);
+      for (int64_t c = g.c0; c < g.c1; ++c) p[(size_t)j++] = e + g.r->colptr[c] - b0;
+      e += b1 - b0;
+    }
+    p[(size_t)nc] = e;
+    rows = ri.data();
+    vals = vx.data();
+  }
   sd_matrix* X = nullptr;
-  SD_TRY(sd_matrix_upload_csc(ctx, r->nrow, nc, p.data(), r->rowind + base, r->values + base,
-                              &X));
-  const int32_t k = r->knn.k;
+  SD_TRY(sd_matrix_upload_csc(ctx, r0->nrow, nc, p.data(), rows, vals, &X));
+  const int32_t k = r0->knn.k;
   std::vector<int32_t> idx((size_t)nc * k);
   std::vector<double> dist((size_t)nc * k);
   int32_t keff = 0, warn = 0;
-  sd_knn_params prm = r->knn;
+  sd_knn_params prm = r0->knn;
   prm.include_self = 0;
   int rc = sd_knn(ctx, X, s->Y, &prm, idx.data(), dist.data(), &keff, &warn);
   sd_matrix_free(X);
   if (rc != SD_OK) return rc;
-  // columns [c0, c1) belong to this item alone, so no lock is needed here;
-  // k_eff is a property of (Y, k) and equal for every item.
-  for (int32_t t = 0; t < keff; ++t) {
-    std::copy(idx.begin() + (size_t)t * nc, idx.begin() + (size_t)(t + 1) * nc,
-              r->idx.begin() + (size_t)t * r->ncol + c0);
-    std::copy(dist.begin() + (size_t)t * nc, dist.begin() + (size_t)(t + 1) * nc,
-              r->dist.begin() + (size_t)t * r->ncol + c0);
+  // columns [c0, c1) of each request belong to this item alone, so no lock
+  // is needed here; k_eff is a property of (Y, k) and equal for every item.
+  int64_t off = 0;
+  for (const Seg& g : segs) {
+    const int64_t n = g.c1 - g.c0;
+    for (int32_t t = 0; t < keff; ++t) {
+      std::copy(idx.begin() + (size_t)t * nc + off, idx.begin() + (size_t)t * nc + off + n,
+                g.r->idx.begin() + (size_t)t * g.r->ncol + g.c0);
+      std::copy(dist.begin() + (size_t)t * nc + off, dist.begin() + (size_t)t * nc + off + n,
+                g.r->dist.begin() + (size_t)t * g.r->ncol + g.c0);
+    }
+    off += n;
   }
   std::lock_guard<std::mutex> lk(s->mu);
-  r->k_eff = keff;
-  r->warn = warn;
+  for (const Seg& g : segs) {
+    g.r->k_eff = keff;
+    g.r->warn = warn;
+  }
   return SD_OK;
 }
 
@@ -208,48 +258,96 @@ int run_snn(sd_server* s, sd_context* ctx, Request* r) {
   return SD_OK;
 }
 
+// The compatible admitted request, other than those in segs, with the fewest
+// unscheduled columns (ties: earliest).
+Request* pick_compatible_locked(sd_server* s, const std::vector<Seg>& segs) {
+  Request* best = nullptr;
+  for (Request* r : s->active) {
+    if (r->next_col >= r->ncol || r->status != SD_OK) continue;
+    if (!same_search(r->knn, segs[0].r->knn)) continue;
+    bool taken = false;
+    for (const Seg& g : segs) taken = taken || g.r == r;
+    if (taken) continue;
+    const int64_t left = r->ncol - r->next_col;
+    if (!best || left < best->ncol - best->next_col ||
+        (left == best->ncol - best->next_col && r->ticket < best->ticket))
+      best = r;
+  }
+  return best;
+}
+
 void worker(sd_server* s, int slot) {
   sd_context* ctx = s->ctx[slot];
   (void)hipSetDevice(s->device);
+  constexpr int64_t kTile = 256;  // query columns of one slab tile
   for (;;) {
-    Request* r = nullptr;
-    int64_t c0 = 0, c1 = 0;
+    std::vector<Seg> segs;
     {
       std::unique_lock<std::mutex> lk(s->mu);
       s->cv_work.wait(lk, [&] { return s->stop || pick_locked(s) != nullptr; });
       if (s->stop) return;
-      r = pick_locked(s);
-      c0 = r->next_col;
-      c1 = std::min(r->ncol, c0 + s->chunk);
+      Request* r = pick_locked(s);
+      const int64_t c0 = r->next_col, c1 = std::min(r->ncol, c0 + s->chunk);
       r->next_col = c1;
       ++r->inflight;
-    }
-    int rc = guarded_call([&] { return run_item(s, ctx, r, c0, c1); });
-    const std::string msg = rc == SD_OK ? std::string() : std::string(sd_last_error());
-    bool complete = false, need_snn = false;
-    {
-      std::lock_guard<std::mutex> lk(s->mu);
-      --r->inflight;
-      if (rc == SD_OK) {
-        r->done_cols += c1 - c0;
-      } else if (r->status == SD_OK) {
-        r->status = rc;
-        r->error = msg;
-        r->next_col = r->ncol;  // schedule nothing more of it
+      segs.push_back(Seg{r, c0, c1});
+      // Continuous batching: a slab computes whole tiles of 256 query
+      // columns, so columns of other waiting requests with the same search
+      // fill the last tile at no extra slab work.
+      int64_t total = c1 - c0;
+      const int64_t target = std::min(s->chunk, (total + kTile - 1) / kTile * kTile);
+      while (total < target) {
+        Request* o = pick_compatible_locked(s, segs);
+        if (!o) break;
+        const int64_t take = std::min(o->ncol - o->next_col, target - total);
+        segs.push_back(Seg{o, o->next_col, o->next_col + take});
+        o->next_col += take;
+        ++o->inflight;
+        total += take;
       }
-      // the item that returns last, with nothing left to schedule, completes it
-      complete = r->inflight == 0 && r->next_col == r->ncol;
-      need_snn = complete && r->status == SD_OK && r->snn;
-      if (complete && !need_snn) finish_locked(s, r);
+      if (segs.size() > 1) ++s->st.packed_items;
     }
-    if (need_snn) {
-      rc = guarded_call([&] { return run_snn(s, ctx, r); });
-      std::lock_guard<std::mutex> lk(s->mu);
-      if (rc != SD_OK) {
-        r->status = rc;
-        r->error = sd_last_error();
+    // a failed packed search is run again piece by piece, so that an error
+    // (say, non-finite input of one request) is charged to its own request
+    std::vector<int> rcs(segs.size(), SD_OK);
+    std::vector<std::string> msgs(segs.size());
+    int rc = guarded_call([&] { return run_segs(s, ctx, segs); });
+    if (rc != SD_OK && segs.size() > 1) {
+      for (size_t i = 0; i < segs.size(); ++i) {
+        rcs[i] = guarded_call([&] { return run_segs(s, ctx, {segs[i]}); });
+        if (rcs[i] != SD_OK) msgs[i] = sd_last_error();
       }
-      finish_locked(s, r);
+    } else if (rc != SD_OK) {
+      rcs[0] = rc;
+      msgs[0] = sd_last_error();
+    }
+    for (size_t i = 0; i < segs.size(); ++i) {
+      Request* r = segs[i].r;
+      bool complete = false, need_snn = false;
+      {
+        std::lock_guard<std::mutex> lk(s->mu);
+        --r->inflight;
+        if (rcs[i] == SD_OK) {
+          r->done_cols += segs[i].c1 - segs[i].c0;
+        } else if (r->status == SD_OK) {
+          r->status = rcs[i];
+          r->error = msgs[i];
+          r->next_col = r->ncol;  // schedule nothing more of it
+        }
+        // the item that returns last, with nothing left to schedule, completes it
+        complete = r->inflight == 0 && r->next_col == r->ncol;
+        need_snn = complete && r->status == SD_OK && r->snn;
+        if (complete && !need_snn) finish_locked(s, r);
+      }
+      if (need_snn) {
+        const int src = guarded_call([&] { return run_snn(s, ctx, r); });
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (src != SD_OK) {
+          r->status = src;
+          r->error = sd_last_error();
+        }
+        finish_locked(s, r);
+      }
     }
   }
 }

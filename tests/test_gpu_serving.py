@@ -143,3 +143,30 @@ def test_request_errors_are_reported(gpu_ctx):
         # the server keeps serving after a failed request
         ok = srv.result(srv.submit(synth.nb_log1p(200, 10, 0.05, 81), k=5, method="cosine"))
         assert ok.idx.shape == (10, 5)
+
+
+def test_small_requests_share_a_tile(gpu_ctx):
+    """Small requests with the same search are packed into one work item (a
+    slab computes whole tiles of 256 query columns): while a large request
+    holds the only slot, twenty small ones queue up, and the slot then serves
+    them together; each result is still the one-shot search's, and a request
+    with other parameters is never mixed in."""
+    Y = synth.nb_log1p(300, 8000, 0.05, 91)
+    big = synth.nb_log1p(300, 6000, 0.05, 92)
+    small = [synth.nb_log1p(300, 3 + (i % 5), 0.05, 200 + i) for i in range(20)]
+    other = synth.nb_log1p(300, 4, 0.05, 300)
+    with sd.Server(Y, slots=1, chunk_cols=2048) as srv:
+        tb = srv.submit(big, k=10, method="cosine")
+        ts = [srv.submit(X, k=10, method="cosine") for X in small]
+        to = srv.submit(other, k=15, method="cosine")
+        res = [srv.result(t) for t in ts]
+        ro = srv.result(to)
+        srv.result(tb)
+        st = srv.stats()
+    assert st["packed_items"] >= 1
+    for X, r in zip(small, res):
+        alone = sd.sparseKNN(X, Y, k=10, method="cosine", verbose=False)
+        assert np.array_equal(r.idx, alone.idx)
+        assert np.array_equal(bits(r.dist), bits(alone.dist))
+    alone = sd.sparseKNN(other, Y, k=15, method="cosine", verbose=False)
+    assert np.array_equal(ro.idx, alone.idx)
