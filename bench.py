@@ -117,6 +117,10 @@ def main():
     ap.add_argument("--sweep", action="store_true",
                     help="path-chooser sweep: densified vs merge walk vs auto over density "
                          "and batch size (20000-row references of 20000 columns)")
+    ap.add_argument("--exact", action="store_true",
+                    help="FP64 merge-walk path (the one the cost model picks for sparse "
+                         "references): 2048 queries against a 20000 x 20000 reference at "
+                         "density 0.002, binary and cosine")
     ap.add_argument("--large", action="store_true",
                     help="large sparse self-search only (30000 x 1000000, density 0.03)")
     ap.add_argument("--cross", action="store_true",
@@ -151,6 +155,8 @@ def main():
         return large_main(args, sd, synth, workers)
     if args.sweep:
         return sweep_main(args, sd, synth, workers)
+    if args.exact:
+        return exact_main(args, sd, synth, workers)
     t0 = time.time()
     X = synth.nb_log1p(args.nrow, args.ncol, args.density, seed=20)
     log(f"generated X {X.shape} nnz={X.nnz} in {time.time() - t0:.1f}s")
@@ -515,6 +521,37 @@ def large_main(args, sd, synth, workers):
         "methods": res,
     }
     print(json.dumps(out), flush=True)
+
+
+def exact_main(args, sd, synth, workers):
+    """The FP64 merge walk (k_exact_slab + k_topk_dense) on the inputs where the
+    cost model prefers it: every pair walks the reference column against the
+    query staged in LDS.  Algorithmic bytes per query: the reference CSC once
+    (12 B per stored value), the slab column written and read back by the
+    top-k (16 B per reference)."""
+    Y = synth.nb_log1p_parallel(20_000, 20_000, 0.002, 81, workers)
+    X = synth.nb_log1p_parallel(20_000, 2048, 0.002, 91, workers)
+    ctx = sd.default_context()
+    dY = sd.DeviceMatrix(Y, ctx)
+    dX = sd.DeviceMatrix(X, ctx)
+    res = {}
+    for method in ("binary", "cosine"):
+        sd.sparseKNN(dX, dY, k=10, method=method, precision="exact", verbose=False, context=ctx)
+        best = 1e30
+        for _ in range(3):
+            t = time.perf_counter()
+            sd.sparseKNN(dX, dY, k=10, method=method, precision="exact", verbose=False,
+                         context=ctx)
+            best = min(best, time.perf_counter() - t)
+        alg = X.shape[1] * (12.0 * Y.nnz + 16.0 * Y.shape[1])
+        res[method] = dict(ms=best * 1e3, cols_per_s=X.shape[1] / best,
+                           algorithmic_bytes=alg, algorithmic_gb_per_s=alg / best / 1e9)
+        log(f"{method}: {best * 1e3:.2f} ms, {alg / best / 1e9:.0f} GB/s algorithmic")
+    print(json.dumps({"metric": "FP64 merge-walk kNN (k=10) query cols/s", "unit": "query cols/s",
+                      "value": 2 * X.shape[1] / sum(r["ms"] / 1e3 for r in res.values()),
+                      "config": {"workload": "2048 queries vs Y 20000 x 20000, d=0.002, "
+                                             "NB/log1p, precision=exact"},
+                      "methods": res}), flush=True)
 
 
 def sweep_main(args, sd, synth, workers):
