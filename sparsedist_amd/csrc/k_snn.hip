@@ -26,12 +26,13 @@ __global__ void k_validate(const int32_t* __restrict__ idx, int64_t total, int n
 // One thread per node: gather, insertion-sort, de-duplicate.  self_space:
 // neighbour ids name the nodes themselves (a node's own id is dropped, and
 // added back when include_self); otherwise they name another column space.
-__global__ void k_build_sets(const int32_t* __restrict__ idx, int n, int k, int include_self,
-                             int self_space, int32_t* __restrict__ sets,
+// Sets are rows of ks >= k + 1 entries, ks a multiple of 4 (16-byte rows).
+__global__ void k_build_sets(const int32_t* __restrict__ idx, int n, int k, int ks,
+                             int include_self, int self_space, int32_t* __restrict__ sets,
                              int32_t* __restrict__ len) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  int32_t* S = sets + (int64_t)i * (k + 1);
+  int32_t* S = sets + (int64_t)i * ks;
   int m = 0;
   if (include_self && self_space) S[m++] = i;
   for (int t = 0; t < k; ++t) {
@@ -69,7 +70,13 @@ __device__ inline int inter_size(const int32_t* a, int na, const int32_t* b, int
 }
 
 // w(i, t) over the kNN edge set (reference snnJaccard, src/snn.cpp:198-250).
-__global__ void k_knn_weights(const int32_t* __restrict__ idx, int n, int k,
+// KS > 0: both sets (rows of KS entries) are read with 16-byte loads into
+// registers and intersected by comparing every pair of entries (sets are
+// sorted and duplicate-free, so each entry matches at most once): a row
+// read per set instead of one dependent load per merge step.  KS == 0: the
+// merge walk over rows of ks entries.
+template <int KS>
+__global__ void k_knn_weights(const int32_t* __restrict__ idx, int n, int k, int ks,
                               const int32_t* __restrict__ sets, const int32_t* __restrict__ len,
                               double* __restrict__ w) {
   const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -79,10 +86,27 @@ __global__ void k_knn_weights(const int32_t* __restrict__ idx, int n, int k,
   double out = __longlong_as_double((long long)kNaRealBits);
   if (v != kNaInt && len[i] > 0) {
     const int j = v - 1;
-    if (len[j] > 0) {
-      const int u = inter_size(sets + (int64_t)i * (k + 1), len[i], sets + (int64_t)j * (k + 1),
-                               len[j]);
-      out = (double)u / (double)(len[i] + len[j] - u);
+    const int li = len[i], lj = len[j];
+    if (lj > 0) {
+      int u = 0;
+      if constexpr (KS > 0) {
+        int32_t a[KS], b[KS];
+        const int4* ri = reinterpret_cast<const int4*>(sets + (int64_t)i * KS);
+        const int4* rj = reinterpret_cast<const int4*>(sets + (int64_t)j * KS);
+#pragma unroll
+        for (int q = 0; q < KS / 4; ++q) {
+          const int4 x = ri[q], y = rj[q];
+          a[4 * q] = x.x; a[4 * q + 1] = x.y; a[4 * q + 2] = x.z; a[4 * q + 3] = x.w;
+          b[4 * q] = y.x; b[4 * q + 1] = y.y; b[4 * q + 2] = y.z; b[4 * q + 3] = y.w;
+        }
+#pragma unroll
+        for (int t = 0; t < KS; ++t)
+#pragma unroll
+          for (int s = 0; s < KS; ++s) u += (t < li && s < lj && a[t] == b[s]) ? 1 : 0;
+      } else {
+        u = inter_size(sets + (int64_t)i * ks, li, sets + (int64_t)j * ks, lj);
+      }
+      out = (double)u / (double)(li + lj - u);
     }
   }
   w[e] = out;
@@ -148,20 +172,20 @@ __global__ void k_colptr(const uint64_t* __restrict__ keys, int64_t m, int64_t n
 
 // ---- shared edge set ----------------------------------------------------------
 __global__ void k_rev_count(const int32_t* __restrict__ sets, const int32_t* __restrict__ len,
-                            int n, int k, int64_t* __restrict__ cnt) {
+                            int n, int ks, int64_t* __restrict__ cnt) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   for (int a = 0; a < len[i]; ++a)
-    atomicAdd((unsigned long long*)&cnt[sets[(int64_t)i * (k + 1) + a]], 1ULL);
+    atomicAdd((unsigned long long*)&cnt[sets[(int64_t)i * ks + a]], 1ULL);
 }
 
 __global__ void k_rev_fill(const int32_t* __restrict__ sets, const int32_t* __restrict__ len,
-                           int n, int k, const int64_t* __restrict__ rp,
+                           int n, int ks, const int64_t* __restrict__ rp,
                            int64_t* __restrict__ fill, int32_t* __restrict__ rv) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   for (int a = 0; a < len[i]; ++a) {
-    const int32_t m = sets[(int64_t)i * (k + 1) + a];
+    const int32_t m = sets[(int64_t)i * ks + a];
     const unsigned long long at = atomicAdd((unsigned long long*)&fill[m], 1ULL);
     rv[rp[m] + (int64_t)at] = i;
   }
@@ -312,11 +336,12 @@ int snn_build(sd_context* ctx, const int32_t* d_idx, int n, int k, int include_s
 
   KernelTimer timer(ctx, &ctx->stats.snn_ms, &ctx->stats.snn_launches);
   DevBuf sets, len;
-  SD_TRY(sets.alloc(sizeof(int32_t) * (size_t)n * (size_t)(k + 1)));
+  const int ks = (k + 1 + 3) & ~3;  // set row stride: 16-byte rows
+  SD_TRY(sets.alloc(sizeof(int32_t) * (size_t)n * (size_t)ks));
   SD_TRY(len.alloc(sizeof(int32_t) * (size_t)n));
   const unsigned gn = (unsigned)((n + 255) / 256);
-  hipLaunchKernelGGL(k_build_sets, dim3(gn), dim3(256), 0, ctx->stream, d_idx, n, k, include_self,
-                     (int)self_space, sets.as<int32_t>(), len.as<int32_t>());
+  hipLaunchKernelGGL(k_build_sets, dim3(gn), dim3(256), 0, ctx->stream, d_idx, n, k, ks,
+                     include_self, (int)self_space, sets.as<int32_t>(), len.as<int32_t>());
   SD_HIP(hipGetLastError());
 
   const int end_bit = 32 + bits_for(n);
@@ -328,8 +353,25 @@ int snn_build(sd_context* ctx, const int32_t* d_idx, int n, int k, int include_s
       dw = w.as<double>();
     }
     const unsigned ge = (unsigned)((total + 255) / 256);
-    hipLaunchKernelGGL(k_knn_weights, dim3(ge), dim3(256), 0, ctx->stream, d_idx, n, k,
-                       sets.as<int32_t>(), len.as<int32_t>(), dw);
+    switch (ks) {
+#define SD_WEIGHTS(KS)                                                                   \
+  case KS:                                                                               \
+    hipLaunchKernelGGL(k_knn_weights<KS>, dim3(ge), dim3(256), 0, ctx->stream, d_idx, n, k, \
+                       ks, sets.as<int32_t>(), len.as<int32_t>(), dw);                   \
+    break;
+      SD_WEIGHTS(4)
+      SD_WEIGHTS(8)
+      SD_WEIGHTS(12)
+      SD_WEIGHTS(16)
+      SD_WEIGHTS(20)
+      SD_WEIGHTS(24)
+      SD_WEIGHTS(28)
+      SD_WEIGHTS(32)
+#undef SD_WEIGHTS
+      default:
+        hipLaunchKernelGGL(k_knn_weights<0>, dim3(ge), dim3(256), 0, ctx->stream, d_idx, n, k,
+                           ks, sets.as<int32_t>(), len.as<int32_t>(), dw);
+    }
     SD_HIP(hipGetLastError());
     if (!g) {
       SD_HIP(hipStreamSynchronize(ctx->stream));
@@ -365,7 +407,7 @@ int snn_build(sd_context* ctx, const int32_t* d_idx, int n, int k, int include_s
   SD_HIP(hipMemsetAsync(rc.ptr, 0, sizeof(int64_t) * (size_t)(nids + 1), ctx->stream));
   SD_HIP(hipMemsetAsync(fill.ptr, 0, sizeof(int64_t) * (size_t)(nids + 1), ctx->stream));
   hipLaunchKernelGGL(k_rev_count, dim3(gn), dim3(256), 0, ctx->stream, sets.as<int32_t>(),
-                     len.as<int32_t>(), n, k, rc.as<int64_t>());
+                     len.as<int32_t>(), n, ks, rc.as<int64_t>());
   size_t bytes = 0;
   SD_HIP(rocprim::exclusive_scan(nullptr, bytes, rc.as<int64_t>(), rp.as<int64_t>(), (int64_t)0,
                                  (size_t)(nids + 1), rocprim::plus<int64_t>(), ctx->stream));
@@ -379,7 +421,7 @@ int snn_build(sd_context* ctx, const int32_t* d_idx, int n, int k, int include_s
   SD_HIP(hipStreamSynchronize(ctx->stream));
   SD_TRY(rv.alloc(sizeof(int32_t) * (size_t)nrev + 16));
   hipLaunchKernelGGL(k_rev_fill, dim3(gn), dim3(256), 0, ctx->stream, sets.as<int32_t>(),
-                     len.as<int32_t>(), n, k, rp.as<int64_t>(), fill.as<int64_t>(),
+                     len.as<int32_t>(), n, ks, rp.as<int64_t>(), fill.as<int64_t>(),
                      rv.as<int32_t>());
   SD_TRY(pc.alloc(sizeof(int64_t) * (size_t)(nids + 1)));
   SD_TRY(poff.alloc(sizeof(int64_t) * (size_t)(nids + 1)));
