@@ -1133,6 +1133,20 @@ __global__ __launch_bounds__(256) void k_rerank(RerankArgs R) {
           int64_t e = 0;
           const int64_t head = min(na, (int64_t)((4 - (a0 & 3)) & 3));
           for (; e < head; ++e) step(ra[e], e);
+          // eight 16-byte loads in flight per lane before the walk uses them
+          for (; e + 32 <= na; e += 32) {
+            const int4* r32 = reinterpret_cast<const int4*>(ra + e);
+            int4 x[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) x[u] = r32[u];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+              step(x[u].x, e + 4 * u);
+              step(x[u].y, e + 4 * u + 1);
+              step(x[u].z, e + 4 * u + 2);
+              step(x[u].w, e + 4 * u + 3);
+            }
+          }
           for (; e + 4 <= na; e += 4) {
             const int4 r4 = *reinterpret_cast<const int4*>(ra + e);
             step(r4.x, e);
