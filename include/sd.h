@@ -112,6 +112,9 @@ typedef struct {
   int64_t last_margin;        /* re-rank margin m of the last fused search */
   int64_t slab_products;      /* MFMA products per row of the last fused search's layout:
                                  3 for split BF16 (hi.hi + hi.lo + lo.hi), else 1 */
+  int64_t filter_pairs;       /* (query, reference) pairs scored by filtered passes: a
+                                 self-search over all columns at once with a symmetric
+                                 method (binary, cosine) scores each unordered pair once */
 } sd_stats;
 int sd_context_stats(sd_context* ctx, sd_stats* out, int reset);
 /* Enable/disable event timing of kernels (off by default). */

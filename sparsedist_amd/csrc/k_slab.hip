@@ -230,6 +230,32 @@ __device__ inline bool map_tile(int64_t b, int64_t nblk, int qtiles, int rtiles,
   return qt < qtiles;
 }
 
+// Symmetric self-search: the query and reference tiles are the same columns,
+// so only tiles with rt >= qt are visited, in the same (group of 4 query
+// tiles, reference tile, query tile in group) order; group g covers reference
+// tiles 4g .. rtiles - 1, 4 (rtiles - 4g) slots, of which the few with
+// rt < qt near the diagonal stay empty.  Slots before group g:
+// cum(g) = 4 g rtiles - 8 g (g - 1).
+__device__ inline int64_t sym_slots_before(int64_t g, int64_t rtiles) {
+  return 4 * g * rtiles - 8 * g * (g - 1);
+}
+__device__ inline bool map_tile_sym(int64_t b, int64_t nblk, int qtiles, int rtiles, int& qt,
+                                    int64_t& rt) {
+  const int64_t lin = (b & 7) * (nblk >> 3) + (b >> 3);
+  const int64_t groups = (qtiles + 3) / 4;
+  if (lin >= sym_slots_before(groups, rtiles)) return false;
+  int64_t lo = 0, hi = groups;  // cum(lo) <= lin < cum(hi)
+  while (hi - lo > 1) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (sym_slots_before(mid, rtiles) <= lin) lo = mid;
+    else hi = mid;
+  }
+  const int64_t rem = lin - sym_slots_before(lo, rtiles);
+  rt = 4 * lo + rem / 4;
+  qt = (int)(4 * lo + rem % 4);
+  return qt < qtiles && rt >= qt;
+}
+
 // Workgroup barrier for LDS only: wait for this wave's LDS operations, then
 // s_barrier.  Unlike __syncthreads() it does not wait for outstanding global
 // loads, which is what lets direct-to-LDS loads stay in flight across it.
@@ -603,7 +629,11 @@ struct WideLayout {
   __device__ static int swz(int row) { return (row >> kRowShift) & (kSlots - 1); }
 };
 
-template <class Op, class K, int RB, int MK, bool QS>
+// SYM: symmetric self-search (queries are the reference columns, one launch
+// over all of them, a symmetric key): only tiles with rt >= qt run, and a
+// tile with rt > qt also screens every value for the reference column as the
+// query and the query column as its candidate.
+template <class Op, class K, int RB, int MK, bool QS, bool SYM>
 __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
   using Lay = WideLayout<RB>;
   constexpr int NT = 512;
@@ -623,6 +653,7 @@ __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
   __shared__ float s_raux[TM];
   __shared__ float s_raux2[TM];
   __shared__ uint8_t s_rdeg[TM];
+  __shared__ __align__(16) float s_rthr[SYM ? TM : 4];  // mirror screen thresholds
 
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
@@ -633,9 +664,14 @@ __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
   // the grid is two-dimensional when one dimension cannot hold every tile
   // (a dispatch counts work-items in 32 bits); dispatch order is x-major
   const int64_t blk = (int64_t)blockIdx.y * gridDim.x + blockIdx.x;
-  if (!map_tile(blk, (int64_t)gridDim.x * gridDim.y, P.qtiles, P.rtiles, qt, rt)) return;
+  if constexpr (SYM) {
+    if (!map_tile_sym(blk, (int64_t)gridDim.x * gridDim.y, P.qtiles, P.rtiles, qt, rt)) return;
+  } else {
+    if (!map_tile(blk, (int64_t)gridDim.x * gridDim.y, P.qtiles, P.rtiles, qt, rt)) return;
+  }
   const int64_t q_base = (int64_t)qt * TN;
   const int64_t r_base = rt * TM;
+  const bool mirror = SYM && rt > qt;  // workgroup-uniform
   for (int t = tid; t < TM; t += NT) {
     const int64_t r = r_base + t;
     if (r < P.nref) {
@@ -643,11 +679,18 @@ __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
       s_raux[t] = P.ref_aux ? P.ref_aux[r] : 0.f;
       s_raux2[t] = P.ref_aux2 ? P.ref_aux2[r] : 0.f;
       s_rdeg[t] = P.ref_deg ? P.ref_deg[r] : 0;
+      if constexpr (SYM) {
+        // reference r as a query (query position r): a float screen a little
+        // below its threshold, as for the tile's own queries
+        const float f = (float)P.thr[r];
+        s_rthr[t] = f - fabsf(f) * 9.5367431640625e-07f - 1e-30f;
+      }
     } else {
       s_rnnz[t] = 0;
       s_raux[t] = 0.f;
       s_raux2[t] = 0.f;
       s_rdeg[t] = 1;
+      if constexpr (SYM) s_rthr[t] = INFINITY;
     }
   }
 
@@ -841,6 +884,84 @@ __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
       return x >= thr_f[j];
     }
   };
+  auto value_of = [&](int b, int j, int r) -> float {
+    float x = 0.f;
+#pragma unroll
+    for (int t = 0; t < 16; ++t) x = (r == t) ? (float)acc[b][j][t] : x;
+    return x;
+  };
+  // Mirror screen (symmetric tiles off the diagonal): reference column rg as
+  // the query, against its own threshold, with this lane's query column as
+  // its candidate.  No self pair lies off the diagonal.
+  if constexpr (SYM) {
+    if (mirror) {
+      uint64_t pmr[2] = {0ull, 0ull};
+#pragma unroll
+      for (int b = 0; b < 4; ++b)
+#pragma unroll
+        for (int q4 = 0; q4 < 4; ++q4) {
+          const int rl = wm * 128 + b * 32 + 8 * q4 + 4 * h;
+          const float4 t4 = *reinterpret_cast<const float4*>(&s_rthr[rl]);
+          int4 nn = make_int4(0, 0, 0, 0);
+          if constexpr (sizeof(K) == 8) nn = *reinterpret_cast<const int4*>(&s_rnnz[rl]);
+          const float ts[4] = {t4.x, t4.y, t4.z, t4.w};
+          const int nis[4] = {nn.x, nn.y, nn.z, nn.w};
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const int r = 4 * q4 + e;
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+              const float x = acc[b][j][r];
+              bool pass;
+              if constexpr (sizeof(K) == 8) {
+                const float d = (float)((uint32_t)nis[e] + qn[j]) - x;
+                pass = x >= ts[e] * d || d <= 0.f;
+              } else {
+                pass = x >= ts[e];
+              }
+              if (pass) pmr[j] |= 1ull << (16 * b + r);
+            }
+          }
+          __builtin_amdgcn_sched_barrier(0);
+        }
+      // candidates are valid, non-degenerate query columns only
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+        if (!qv[j] || (P.ref_deg && P.ref_deg[qc[j]])) pmr[j] = 0ull;
+      // appends: exact test against the reference column's own threshold,
+      // then one slot per surviving pair in that column's buffer (survivors
+      // are rare, so an atomic each costs little)
+#pragma unroll
+      for (int b = 0; b < 4; ++b)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          uint32_t m = (uint32_t)(pmr[j] >> (16 * b)) & 0xFFFFu;
+          while (m) {
+            const int r = __builtin_ctz(m);
+            m &= m - 1u;
+            const int rloc = wm * 128 + b * 32 + 8 * (r >> 2) + 4 * h + (r & 3);
+            const int64_t rg = rg0 + b * 32 + 8 * (r >> 2) + (r & 3);
+            if (rg >= P.nref) continue;
+            const float x = value_of(b, j, r);
+            double kd;
+            if constexpr (sizeof(K) == 8) {
+              const uint32_t cc = (uint32_t)x;
+              const uint32_t d = (uint32_t)s_rnnz[rloc] + qn[j] - cc;
+              kd = (d == 0) ? 1.0 : key_to_double(KeyJ{cc, d});
+            } else {
+              kd = (double)x;
+            }
+            if (kd >= P.thr[rg]) {
+              const int at = atomicAdd(&P.cnt[rg], 1);
+              if (at < P.cap) {
+                P.fkey[rg * P.cap + at] = kd;
+                P.fidx[rg * P.cap + at] = (int32_t)qc[j];
+              }
+            }
+          }
+        }
+    }
+  }
   // Pass 1, unrolled over the 128 values: plain compares only, one bit per
   // value (16 b + r of pm[j]).  The exact test and the append run in loops
   // over the set bits (pass 2), so the epilogue's code stays a few thousand
@@ -885,12 +1006,6 @@ __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
   }
   // pass 2: exact test of the marked values, then one atomic per query to
   // reserve buffer slots, then the writes
-  auto value_of = [&](int b, int j, int r) -> float {
-    float x = 0.f;
-#pragma unroll
-    for (int t = 0; t < 16; ++t) x = (r == t) ? (float)acc[b][j][t] : x;
-    return x;
-  };
   int cnt[2] = {0, 0};
 #pragma unroll
   for (int b = 0; b < 4; ++b)
@@ -908,29 +1023,30 @@ __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
       pm[j] = (pm[j] & ~(0xFFFFull << (16 * b))) | ((uint64_t)keep << (16 * b));
       cnt[j] += __popc(keep);
     }
-  if (cnt[0] + cnt[1] == 0) return;
-  int at[2];
+  if (cnt[0] + cnt[1] != 0) {
+    int at[2];
 #pragma unroll
-  for (int j = 0; j < 2; ++j) at[j] = cnt[j] ? atomicAdd(&P.cnt[qp[j]], cnt[j]) : 0;
+    for (int j = 0; j < 2; ++j) at[j] = cnt[j] ? atomicAdd(&P.cnt[qp[j]], cnt[j]) : 0;
 #pragma unroll
-  for (int b = 0; b < 4; ++b)
+    for (int b = 0; b < 4; ++b)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      uint32_t m = (uint32_t)(pm[j] >> (16 * b)) & 0xFFFFu;
-      while (m) {
-        const int r = __builtin_ctz(m);
-        m &= m - 1u;
-        const int rloc = wm * 128 + b * 32 + 8 * (r >> 2) + 4 * h + (r & 3);
-        const int64_t rg = rg0 + b * 32 + 8 * (r >> 2) + (r & 3);
-        double kd;
-        exact(value_of(b, j, r), j, rloc, s_rnnz[rloc], rg, kd);
-        if (at[j] < P.cap) {
-          P.fkey[qp[j] * P.cap + at[j]] = kd;
-          P.fidx[qp[j] * P.cap + at[j]] = (int32_t)rg;
+      for (int j = 0; j < 2; ++j) {
+        uint32_t m = (uint32_t)(pm[j] >> (16 * b)) & 0xFFFFu;
+        while (m) {
+          const int r = __builtin_ctz(m);
+          m &= m - 1u;
+          const int rloc = wm * 128 + b * 32 + 8 * (r >> 2) + 4 * h + (r & 3);
+          const int64_t rg = rg0 + b * 32 + 8 * (r >> 2) + (r & 3);
+          double kd;
+          exact(value_of(b, j, r), j, rloc, s_rnnz[rloc], rg, kd);
+          if (at[j] < P.cap) {
+            P.fkey[qp[j] * P.cap + at[j]] = kd;
+            P.fidx[qp[j] * P.cap + at[j]] = (int32_t)rg;
+          }
+          ++at[j];
         }
-        ++at[j];
       }
-    }
+  }
 }
 
 // ---- merge per-split lists: top-L by (key desc, idx asc) --------------------
@@ -1367,7 +1483,7 @@ static int launch_fused_t(sd_context* ctx, FusedKernelArgs& P) {
 // K bytes per row per chunk of the filtered pass (64 or 128)
 constexpr int kFilterRowBytes = 128;
 
-template <class Op, class K, int RB, int MK = kKeyScore, bool QS = false>
+template <class Op, class K, int RB, int MK = kKeyScore, bool QS = false, bool SYM = false>
 static int launch_filter_t(sd_context* ctx, FusedKernelArgs& P) {
   if (P.kpad_bytes <= 0 || P.kpad_bytes % (WideLayout<RB>::kStages * RB) != 0)
     return set_error(SD_EINVAL, "internal: padded row bytes %lld are not whole chunk groups",
@@ -1375,13 +1491,18 @@ static int launch_filter_t(sd_context* ctx, FusedKernelArgs& P) {
   P.qtiles = (int)((P.nqry + kWideTile - 1) / kWideTile);
   P.rtiles = (int)((P.nref + kWideTile - 1) / kWideTile);
   P.qgroups = (P.qtiles + 3) / 4;
-  const int64_t need = (int64_t)P.qgroups * 4 * P.rtiles;
+  if (SYM && (P.nqry != P.nref || P.qtiles != P.rtiles))
+    return set_error(SD_EINVAL, "internal: symmetric pass needs the query set to be the references");
+  // slots: groups of 4 query tiles x their reference tiles (from the group's
+  // first tile on, for the symmetric pass)
+  const int64_t G = P.qgroups, R = P.rtiles;
+  const int64_t need = SYM ? 4 * G * R - 8 * G * (G - 1) : G * 4 * R;
   // x: a multiple of 8 (whole rounds of the XCD round-robin) small enough that
   // x * 512 work-items fit a dispatch's 32-bit grid size; y: the rest
   const int64_t gx = std::min<int64_t>((need + 7) / 8 * 8, (int64_t)1 << 20);
   const int64_t gy = (need + gx - 1) / gx;
   if (gy > 65535) return set_error(SD_EUNSUPPORTED, "slab grid too large (%lld tiles)", (long long)need);
-  hipLaunchKernelGGL((k_filter_wide<Op, K, RB, MK, QS>), dim3((unsigned)gx, (unsigned)gy), dim3(512), 0,
+  hipLaunchKernelGGL((k_filter_wide<Op, K, RB, MK, QS, SYM>), dim3((unsigned)gx, (unsigned)gy), dim3(512), 0,
                      ctx->stream, P);
   SD_HIP(hipGetLastError());
   return SD_OK;
@@ -1545,15 +1666,24 @@ int fused_filter_pass(sd_context* ctx, const FusedArgs& a, const double* thr, in
   P.fidx = fidx;
   SD_HIP(hipMemsetAsync(cnt, 0, sizeof(int32_t) * (size_t)a.nqry, ctx->stream));
   KernelTimer timer(ctx, &ctx->stats.filter_ms, &ctx->stats.filter_launches);
-  if (a.method == SD_BINARY) {
-    if (a.kind == kDenseF32Pattern) return launch_filter_t<OpF32, KeyJ, kFilterRowBytes>(ctx, P);
-    return launch_filter_t<OpF4, KeyJ, kFilterRowBytes>(ctx, P);
-  }
+  // A self-search over all columns at once with a symmetric key (Jaccard,
+  // cosine) scores each unordered pair once.
+  const bool sym = a.self_mode && !a.qlist && a.nqry == a.nref && a.qry_dense == a.ref_dense &&
+                   a.qry_p == a.ref_p && (a.method == SD_BINARY || a.method == SD_COSINE);
+  ctx->stats.filter_pairs += sym ? a.nqry * (a.nqry + 1) / 2 : a.nqry * a.nref;
   constexpr int RB = kFilterRowBytes;
+  if (a.method == SD_BINARY) {
+    if (a.kind == kDenseF32Pattern)
+      return sym ? launch_filter_t<OpF32, KeyJ, RB, kKeyScore, false, true>(ctx, P)
+                 : launch_filter_t<OpF32, KeyJ, RB>(ctx, P);
+    return sym ? launch_filter_t<OpF4, KeyJ, RB, kKeyScore, false, true>(ctx, P)
+               : launch_filter_t<OpF4, KeyJ, RB>(ctx, P);
+  }
   if (a.prec == SD_PREC_FP32) {
     if (a.method == SD_EUCLIDEAN) return launch_filter_t<OpF32, float, RB, kKeyEuclid>(ctx, P);
     if (a.method == SD_PEARSON) return launch_filter_t<OpF32, float, RB, kKeyPearson>(ctx, P);
-    return launch_filter_t<OpF32, float, RB>(ctx, P);
+    return sym ? launch_filter_t<OpF32, float, RB, kKeyScore, false, true>(ctx, P)
+               : launch_filter_t<OpF32, float, RB>(ctx, P);
   }
   if (a.method == SD_EUCLIDEAN)
     return P.qseg ? launch_filter_t<OpBf16, float, RB, kKeyEuclid, true>(ctx, P)
@@ -1561,7 +1691,8 @@ int fused_filter_pass(sd_context* ctx, const FusedArgs& a, const double* thr, in
   if (a.method == SD_PEARSON)
     return P.qseg ? launch_filter_t<OpBf16, float, RB, kKeyPearson, true>(ctx, P)
                   : launch_filter_t<OpBf16, float, RB, kKeyPearson>(ctx, P);
-  return launch_filter_t<OpBf16, float, RB>(ctx, P);
+  return sym ? launch_filter_t<OpBf16, float, RB, kKeyScore, false, true>(ctx, P)
+             : launch_filter_t<OpBf16, float, RB>(ctx, P);
 }
 
 int select_lists(sd_context* ctx, const double* fkey, const int32_t* fidx, const int32_t* cnt,

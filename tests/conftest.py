@@ -29,3 +29,20 @@ def oracle_lib():
 def gpu_ctx():
     import sparsedist_amd as sd
     return sd.default_context()
+
+
+def self_rows_from_cross(oi, od, qcols, k, include_self):
+    """Self-search rows (distances, ascending) of the columns qcols of Y, from
+    a cross search of Y[:, qcols] against Y with k + 1 neighbours: the
+    query's own column leaves the list, and with include_self it comes back
+    at its restored value 0 in (value, index) order (R/sparseKNN.R:270-279)."""
+    gi = np.empty((len(qcols), k), np.int32)
+    gd = np.empty((len(qcols), k), np.float64)
+    for t, q in enumerate(qcols):
+        pairs = [(d, i) for i, d in zip(oi[t].tolist(), od[t].tolist()) if i != q + 1]
+        if include_self:
+            pairs.append((0.0, q + 1))
+            pairs.sort()
+        gi[t] = [i for _, i in pairs[:k]]
+        gd[t] = [d for d, _ in pairs[:k]]
+    return gi, gd

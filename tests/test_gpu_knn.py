@@ -11,7 +11,7 @@ import pytest
 import scipy.sparse as sp
 
 import sparsedist_amd as sd
-from conftest import bits
+from conftest import bits, self_rows_from_cross
 from sparsedist_amd import synth
 
 pytestmark = pytest.mark.gpu
@@ -355,6 +355,38 @@ def test_two_pass_filter_path(oracle_lib, gpu_ctx, method):
     o = oracle_lib.knn(X, Y, k=20, method=method, nthreads=NT)
     same((g.idx, g.dist), o, method)
     assert st["overflow_queries"] >= 1
+
+
+@pytest.mark.parametrize("method", ["binary", "cosine"])
+@pytest.mark.parametrize("include_self", [False, True])
+def test_symmetric_self_filter(oracle_lib, gpu_ctx, method, include_self):
+    """A self-search over 20000 columns at once scores each unordered pair once
+    (tiles with reference tile >= query tile, each value screened for both
+    columns as the query).  A heavily tied block overflows candidate buffers
+    from both directions; the last tile is partial.  Sampled rows, including
+    those, must match the oracle."""
+    n, k = 20000, 20
+    Y = synth.nb_log1p(600, n, 0.05, 43)
+    Yd = Y.tolil()
+    for c in range(3000, 7700):
+        Yd[:, c] = Y[:, 0]
+    Y = Yd.tocsc()
+    gpu_ctx.stats(reset=True)
+    gpu_ctx.set_timing(True)  # launches are counted while timing is on
+    try:
+        g = sd.sparseKNN(Y, k=k, method=method, include_self=include_self, precision="bf16",
+                         verbose=False)
+    finally:
+        gpu_ctx.set_timing(False)
+    st = gpu_ctx.stats(reset=True)
+    assert st["filter_launches"] == 1 and st["filter_pairs"] == n * (n + 1) // 2, st
+    assert st["overflow_queries"] >= 1
+    rng = np.random.default_rng(7)
+    qcols = sorted(set(rng.choice(n, 200, replace=False).tolist()) |
+                   {0, 1, 255, 256, 2999, 3000, 5000, 7699, 7700, 19967, 19968, 19999})
+    o = oracle_lib.knn(Y[:, qcols], Y, k=k + 1, method=method, nthreads=NT)
+    want = self_rows_from_cross(o[0], o[1], qcols, k, include_self)
+    same((g.idx[qcols], g.dist[qcols]), want, f"{method} include_self={include_self}")
 
 
 @pytest.mark.parametrize("prec", ["bf16", "fp32"])

@@ -8,7 +8,7 @@ import pytest
 import scipy.sparse as sp
 
 import brute
-from conftest import bits
+from conftest import bits, self_rows_from_cross
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 FIX = json.load(open(os.path.join(HERE, "golden", "fixtures.json")))
@@ -144,6 +144,26 @@ def test_knn_include_self_restores_exact_values(oracle_lib):
     idx, dst, _ = oracle_lib.knn(X, k=3, method="cosine", include_self=True)
     assert np.array_equal(idx[:, 0], np.arange(1, 13))
     assert np.array_equal(bits(dst[:, 0]), bits(np.zeros(12)))
+
+
+@pytest.mark.parametrize("method", ["binary", "cosine"])
+@pytest.mark.parametrize("include_self", [False, True])
+def test_self_rows_from_cross_search(oracle_lib, method, include_self):
+    """The GPU tests check sampled rows of large self-searches against a cross
+    search with k + 1 neighbours, minus the query's own column (plus its
+    restored value with include_self).  That construction must reproduce the
+    oracle's self mode, including tied duplicate columns."""
+    X = rand_sparse(60, 150, 0.15, 31).tolil()
+    for c in range(40, 70):
+        X[:, c] = X[:, 3]  # 31 identical columns: ties wider than k + 1
+    X = X.tocsc()
+    k = 8
+    o = oracle_lib.knn(X, k=k, method=method, include_self=include_self)
+    qcols = [0, 3, 5, 40, 41, 69, 70, 149]
+    c = oracle_lib.knn(X[:, qcols], X, k=k + 1, method=method)
+    gi, gd = self_rows_from_cross(c[0], c[1], qcols, k, include_self)
+    assert np.array_equal(gi, o[0][qcols])
+    assert np.array_equal(bits(gd), bits(o[1][qcols]))
 
 
 def test_cross_equals_two_knn_calls(oracle_lib):
