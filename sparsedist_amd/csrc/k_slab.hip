@@ -654,6 +654,8 @@ __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
   __shared__ float s_raux2[TM];
   __shared__ uint8_t s_rdeg[TM];
   __shared__ __align__(16) float s_rthr[SYM ? TM : 4];  // mirror screen thresholds
+  __shared__ int32_t s_mcnt[SYM ? TM : 1];                // mirror hits per reference
+  __shared__ int32_t s_npool;                              // mirror hits past a column's slots
 
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
@@ -672,6 +674,7 @@ __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
   const int64_t q_base = (int64_t)qt * TN;
   const int64_t r_base = rt * TM;
   const bool mirror = SYM && rt > qt;  // workgroup-uniform
+  if (SYM && tid == 0) s_npool = 0;
   for (int t = tid; t < TM; t += NT) {
     const int64_t r = r_base + t;
     if (r < P.nref) {
@@ -684,13 +687,17 @@ __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
         // below its threshold, as for the tile's own queries
         const float f = (float)P.thr[r];
         s_rthr[t] = f - fabsf(f) * 9.5367431640625e-07f - 1e-30f;
+        s_mcnt[t] = 0;
       }
     } else {
       s_rnnz[t] = 0;
       s_raux[t] = 0.f;
       s_raux2[t] = 0.f;
       s_rdeg[t] = 1;
-      if constexpr (SYM) s_rthr[t] = INFINITY;
+      if constexpr (SYM) {
+        s_rthr[t] = INFINITY;
+        s_mcnt[t] = 0;
+      }
     }
   }
 
@@ -890,78 +897,6 @@ __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
     for (int t = 0; t < 16; ++t) x = (r == t) ? (float)acc[b][j][t] : x;
     return x;
   };
-  // Mirror screen (symmetric tiles off the diagonal): reference column rg as
-  // the query, against its own threshold, with this lane's query column as
-  // its candidate.  No self pair lies off the diagonal.
-  if constexpr (SYM) {
-    if (mirror) {
-      uint64_t pmr[2] = {0ull, 0ull};
-#pragma unroll
-      for (int b = 0; b < 4; ++b)
-#pragma unroll
-        for (int q4 = 0; q4 < 4; ++q4) {
-          const int rl = wm * 128 + b * 32 + 8 * q4 + 4 * h;
-          const float4 t4 = *reinterpret_cast<const float4*>(&s_rthr[rl]);
-          int4 nn = make_int4(0, 0, 0, 0);
-          if constexpr (sizeof(K) == 8) nn = *reinterpret_cast<const int4*>(&s_rnnz[rl]);
-          const float ts[4] = {t4.x, t4.y, t4.z, t4.w};
-          const int nis[4] = {nn.x, nn.y, nn.z, nn.w};
-#pragma unroll
-          for (int e = 0; e < 4; ++e) {
-            const int r = 4 * q4 + e;
-#pragma unroll
-            for (int j = 0; j < 2; ++j) {
-              const float x = acc[b][j][r];
-              bool pass;
-              if constexpr (sizeof(K) == 8) {
-                const float d = (float)((uint32_t)nis[e] + qn[j]) - x;
-                pass = x >= ts[e] * d || d <= 0.f;
-              } else {
-                pass = x >= ts[e];
-              }
-              if (pass) pmr[j] |= 1ull << (16 * b + r);
-            }
-          }
-          __builtin_amdgcn_sched_barrier(0);
-        }
-      // candidates are valid, non-degenerate query columns only
-#pragma unroll
-      for (int j = 0; j < 2; ++j)
-        if (!qv[j] || (P.ref_deg && P.ref_deg[qc[j]])) pmr[j] = 0ull;
-      // appends: exact test against the reference column's own threshold,
-      // then one slot per surviving pair in that column's buffer (survivors
-      // are rare, so an atomic each costs little)
-#pragma unroll
-      for (int b = 0; b < 4; ++b)
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-          uint32_t m = (uint32_t)(pmr[j] >> (16 * b)) & 0xFFFFu;
-          while (m) {
-            const int r = __builtin_ctz(m);
-            m &= m - 1u;
-            const int rloc = wm * 128 + b * 32 + 8 * (r >> 2) + 4 * h + (r & 3);
-            const int64_t rg = rg0 + b * 32 + 8 * (r >> 2) + (r & 3);
-            if (rg >= P.nref) continue;
-            const float x = value_of(b, j, r);
-            double kd;
-            if constexpr (sizeof(K) == 8) {
-              const uint32_t cc = (uint32_t)x;
-              const uint32_t d = (uint32_t)s_rnnz[rloc] + qn[j] - cc;
-              kd = (d == 0) ? 1.0 : key_to_double(KeyJ{cc, d});
-            } else {
-              kd = (double)x;
-            }
-            if (kd >= P.thr[rg]) {
-              const int at = atomicAdd(&P.cnt[rg], 1);
-              if (at < P.cap) {
-                P.fkey[rg * P.cap + at] = kd;
-                P.fidx[rg * P.cap + at] = (int32_t)qc[j];
-              }
-            }
-          }
-        }
-    }
-  }
   // Pass 1, unrolled over the 128 values: plain compares only, one bit per
   // value (16 b + r of pm[j]).  The exact test and the append run in loops
   // over the set bits (pass 2), so the epilogue's code stays a few thousand
@@ -1046,6 +981,139 @@ __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
           ++at[j];
         }
       }
+  }
+  // Mirror screen (symmetric tiles off the diagonal): reference column rg as
+  // the query, against its own threshold, with this lane's query column as
+  // its candidate.  No self pair lies off the diagonal.
+  if constexpr (SYM) {
+    if (mirror) {
+      uint64_t pmr[2] = {0ull, 0ull};
+#pragma unroll
+      for (int b = 0; b < 4; ++b)
+#pragma unroll
+        for (int q4 = 0; q4 < 4; ++q4) {
+          const int rl = wm * 128 + b * 32 + 8 * q4 + 4 * h;
+          const float4 t4 = *reinterpret_cast<const float4*>(&s_rthr[rl]);
+          int4 nn = make_int4(0, 0, 0, 0);
+          if constexpr (sizeof(K) == 8) nn = *reinterpret_cast<const int4*>(&s_rnnz[rl]);
+          const float ts[4] = {t4.x, t4.y, t4.z, t4.w};
+          const int nis[4] = {nn.x, nn.y, nn.z, nn.w};
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const int r = 4 * q4 + e;
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+              const float x = acc[b][j][r];
+              bool pass;
+              if constexpr (sizeof(K) == 8) {
+                const float d = (float)((uint32_t)nis[e] + qn[j]) - x;
+                pass = x >= ts[e] * d || d <= 0.f;
+              } else {
+                pass = x >= ts[e];
+              }
+              if (pass) pmr[j] |= 1ull << (16 * b + r);
+            }
+          }
+          // the group's compares complete here: otherwise they sink below
+          // every group's LDS reads, whose results then spill
+          asm volatile("" : "+v"(pmr[0]), "+v"(pmr[1]));
+          __builtin_amdgcn_sched_barrier(0);
+        }
+      // candidates are valid, non-degenerate query columns only
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+        if (!qv[j] || (P.ref_deg && P.ref_deg[qc[j]])) pmr[j] = 0ull;
+      // Gather the marked pairs per reference column in LDS (the stage
+      // buffers are free once every wave's refills have landed): kMirrorSlots
+      // per column in stage 0, further ones in a pool in stage 1.  Then one
+      // thread per column computes the keys, runs the exact test against the
+      // column's threshold and reserves its buffer slots with one atomic, and
+      // the pool is tested entry by entry.  The lanes compute no key here, so
+      // the accumulators stay in registers.  A pool that fills up pushes the
+      // column's count past the buffer size, which sends that column to the
+      // full per-query pass.
+      constexpr int kMirrorSlots = 16;
+      constexpr int kPool = STAGE_BYTES / 16;
+      static_assert(TM * kMirrorSlots * 12 <= STAGE_BYTES, "mirror lists fit one stage");
+      float* m_x = reinterpret_cast<float*>(st0);
+      int32_t* m_idx = reinterpret_cast<int32_t*>(st0 + TM * kMirrorSlots * 4);
+      uint32_t* m_nnz = reinterpret_cast<uint32_t*>(st0 + TM * kMirrorSlots * 8);
+      int4* pool = reinterpret_cast<int4*>(st1);  // {value bits, candidate, its nnz, column}
+      // exact key of slab value x of tile column rloc for a candidate with cn
+      // stored entries
+      auto mirror_key = [&](float x, int rloc, uint32_t cn) -> double {
+        if constexpr (sizeof(K) == 8) {
+          const uint32_t cc = (uint32_t)x;
+          const uint32_t d = (uint32_t)s_rnnz[rloc] + cn - cc;
+          return (d == 0) ? 1.0 : key_to_double(KeyJ{cc, d});
+        } else {
+          return (double)x;
+        }
+      };
+      lds_barrier();  // no wave reads the stages any more; refills have landed (vmcnt 0)
+#pragma unroll
+      for (int b = 0; b < 4; ++b)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          uint32_t m = (uint32_t)(pmr[j] >> (16 * b)) & 0xFFFFu;
+          while (m) {
+            const int r = __builtin_ctz(m);
+            m &= m - 1u;
+            const int rloc = wm * 128 + b * 32 + 8 * (r >> 2) + 4 * h + (r & 3);
+            const float x = value_of(b, j, r);
+            const int slot = atomicAdd(&s_mcnt[rloc], 1);
+            if (slot < kMirrorSlots) {
+              m_x[rloc * kMirrorSlots + slot] = x;
+              m_idx[rloc * kMirrorSlots + slot] = (int32_t)qc[j];
+              m_nnz[rloc * kMirrorSlots + slot] = qn[j];
+            } else {
+              const int ps = atomicAdd(&s_npool, 1);
+              if (ps < kPool)
+                pool[ps] = make_int4(__float_as_int(x), (int)qc[j], (int)qn[j], rloc);
+              else if (r_base + rloc < P.nref)
+                atomicAdd(&P.cnt[r_base + rloc], P.cap + 1);
+            }
+          }
+        }
+      lds_barrier();
+      if (tid < TM) {
+        const int64_t rg = r_base + tid;
+        const int nm = min(s_mcnt[tid], kMirrorSlots);
+        if (nm > 0 && rg < P.nref) {
+          const double t = P.thr[rg];
+          int keep = 0;
+          for (int e = 0; e < nm; ++e)
+            keep += mirror_key(m_x[tid * kMirrorSlots + e], tid, m_nnz[tid * kMirrorSlots + e]) >= t;
+          if (keep > 0) {
+            int at = atomicAdd(&P.cnt[rg], keep);
+            for (int e = 0; e < nm; ++e) {
+              const double kd =
+                  mirror_key(m_x[tid * kMirrorSlots + e], tid, m_nnz[tid * kMirrorSlots + e]);
+              if (kd < t) continue;
+              if (at < P.cap) {
+                P.fkey[rg * P.cap + at] = kd;
+                P.fidx[rg * P.cap + at] = m_idx[tid * kMirrorSlots + e];
+              }
+              ++at;
+            }
+          }
+        }
+      }
+      const int np = min(s_npool, kPool);
+      for (int e = tid; e < np; e += NT) {
+        const int4 v = pool[e];
+        const int64_t rg = r_base + v.w;
+        if (rg >= P.nref) continue;
+        const double kd = mirror_key(__int_as_float(v.x), v.w, (uint32_t)v.z);
+        if (kd >= P.thr[rg]) {
+          const int at = atomicAdd(&P.cnt[rg], 1);
+          if (at < P.cap) {
+            P.fkey[rg * P.cap + at] = kd;
+            P.fidx[rg * P.cap + at] = v.y;
+          }
+        }
+      }
+    }
   }
 }
 
