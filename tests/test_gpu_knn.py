@@ -382,7 +382,7 @@ def test_symmetric_self_filter(oracle_lib, gpu_ctx, method, include_self):
     assert st["filter_launches"] == 1 and st["filter_pairs"] == n * (n + 1) // 2, st
     assert st["overflow_queries"] >= 1
     rng = np.random.default_rng(7)
-    qcols = sorted(set(rng.choice(n, 200, replace=False).tolist()) |
+    qcols = sorted(set(rng.choice(n, 600, replace=False).tolist()) |
                    {0, 1, 255, 256, 2999, 3000, 5000, 7699, 7700, 19967, 19968, 19999})
     o = oracle_lib.knn(Y[:, qcols], Y, k=k + 1, method=method, nthreads=NT)
     want = self_rows_from_cross(o[0], o[1], qcols, k, include_self)
