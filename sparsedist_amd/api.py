@@ -403,6 +403,9 @@ def _idx_matrix(idx):
         raise ValueError("'idx' must have at least one row and one column.")
     if idx.dtype.kind not in "iuf":
         raise ValueError("'idx' must be a numeric or integer matrix.")
+    if idx.dtype == np.int32 and idx.flags["F_CONTIGUOUS"] and idx.min() >= 1 \
+            and idx.max() <= n:
+        return idx  # all entries are row indices (no NA): nothing to convert
     if idx.dtype.kind == "f":
         a = idx.astype(np.float64)
         na = np.isnan(a)

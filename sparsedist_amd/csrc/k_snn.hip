@@ -249,8 +249,8 @@ int sort_pairs(sd_context* ctx, DevBuf& kin, DevBuf& vin, DevBuf& kout, DevBuf& 
   SD_HIP(rocprim::radix_sort_pairs(nullptr, bytes, kin.as<uint64_t>(), kout.as<uint64_t>(),
                                    vin.as<double>(), vout.as<double>(), (size_t)m, 0, end_bit,
                                    ctx->stream));
-  DevBuf tmp;
-  SD_TRY(tmp.alloc(bytes + 16));
+  DevBuf& tmp = ctx->snn_scratch[10];
+  SD_TRY(tmp.ensure(bytes + 16));
   SD_HIP(rocprim::radix_sort_pairs(tmp.ptr, bytes, kin.as<uint64_t>(), kout.as<uint64_t>(),
                                    vin.as<double>(), vout.as<double>(), (size_t)m, 0, end_bit,
                                    ctx->stream));
@@ -276,17 +276,18 @@ int assemble_csc(sd_context* ctx, DevBuf& keys, DevBuf& vals, int64_t m, int64_t
     SD_HIP(hipStreamSynchronize(ctx->stream));
     return SD_OK;
   }
-  DevBuf flag, pos;
-  SD_TRY(flag.alloc(sizeof(int64_t) * (size_t)m));
-  SD_TRY(pos.alloc(sizeof(int64_t) * (size_t)(m + 1)));
+  DevBuf& flag = ctx->snn_scratch[11];
+  DevBuf& pos = ctx->snn_scratch[12];
+  SD_TRY(flag.ensure(sizeof(int64_t) * (size_t)m));
+  SD_TRY(pos.ensure(sizeof(int64_t) * (size_t)(m + 1)));
   const unsigned gb = (unsigned)((m + 255) / 256);
   hipLaunchKernelGGL(k_run_heads, dim3(gb), dim3(256), 0, ctx->stream, keys.as<uint64_t>(), m,
                      flag.as<int64_t>());
   size_t bytes = 0;
   SD_HIP(rocprim::exclusive_scan(nullptr, bytes, flag.as<int64_t>(), pos.as<int64_t>(), (int64_t)0,
                                  (size_t)m, rocprim::plus<int64_t>(), ctx->stream));
-  DevBuf tmp;
-  SD_TRY(tmp.alloc(bytes + 16));
+  DevBuf& tmp = ctx->snn_scratch[13];
+  SD_TRY(tmp.ensure(bytes + 16));
   SD_HIP(rocprim::exclusive_scan(tmp.ptr, bytes, flag.as<int64_t>(), pos.as<int64_t>(), (int64_t)0,
                                  (size_t)m, rocprim::plus<int64_t>(), ctx->stream));
   int64_t last_pos = 0, last_flag = 0;
@@ -299,8 +300,8 @@ int assemble_csc(sd_context* ctx, DevBuf& keys, DevBuf& vals, int64_t m, int64_t
   g->nnz = nu;
   SD_TRY(g->rowind.alloc(sizeof(int32_t) * (size_t)nu + 16));
   SD_TRY(g->val.alloc(sizeof(double) * (size_t)nu + 16));
-  DevBuf ukeys;
-  SD_TRY(ukeys.alloc(sizeof(uint64_t) * (size_t)nu + 16));
+  DevBuf& ukeys = ctx->snn_scratch[14];
+  SD_TRY(ukeys.ensure(sizeof(uint64_t) * (size_t)nu + 16));
   hipLaunchKernelGGL(k_scatter_heads, dim3(gb), dim3(256), 0, ctx->stream, keys.as<uint64_t>(),
                      vals.as<double>(), pos.as<int64_t>(), m, g->rowind.as<int32_t>(),
                      g->val.as<double>(), ukeys.as<uint64_t>());
@@ -321,8 +322,8 @@ int snn_build(sd_context* ctx, const int32_t* d_idx, int n, int k, int include_s
   if (!self_space && edges == 0)
     return set_error(SD_EINVAL, "internal: kNN-edge weights need neighbour ids in the node space");
   const int64_t total = (int64_t)n * k;
-  DevBuf bad;
-  SD_TRY(bad.alloc(sizeof(int)));
+  DevBuf& bad = ctx->snn_scratch[1];
+  SD_TRY(bad.ensure(sizeof(int)));
   SD_HIP(hipMemsetAsync(bad.ptr, 0, sizeof(int), ctx->stream));
   hipLaunchKernelGGL(k_validate, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, ctx->stream,
                      d_idx, total, nids, bad.as<int>());
@@ -335,10 +336,11 @@ int snn_build(sd_context* ctx, const int32_t* d_idx, int n, int k, int include_s
                                     : "neighbour ids must be NA or 1-based column indices of the reference.");
 
   KernelTimer timer(ctx, &ctx->stats.snn_ms, &ctx->stats.snn_launches);
-  DevBuf sets, len;
+  DevBuf& sets = ctx->snn_scratch[2];
+  DevBuf& len = ctx->snn_scratch[3];
   const int ks = (k + 1 + 3) & ~3;  // set row stride: 16-byte rows
-  SD_TRY(sets.alloc(sizeof(int32_t) * (size_t)n * (size_t)ks));
-  SD_TRY(len.alloc(sizeof(int32_t) * (size_t)n));
+  SD_TRY(sets.ensure(sizeof(int32_t) * (size_t)n * (size_t)ks));
+  SD_TRY(len.ensure(sizeof(int32_t) * (size_t)n));
   const unsigned gn = (unsigned)((n + 255) / 256);
   hipLaunchKernelGGL(k_build_sets, dim3(gn), dim3(256), 0, ctx->stream, d_idx, n, k, ks,
                      include_self, (int)self_space, sets.as<int32_t>(), len.as<int32_t>());
@@ -346,10 +348,10 @@ int snn_build(sd_context* ctx, const int32_t* d_idx, int n, int k, int include_s
 
   const int end_bit = 32 + bits_for(n);
   if (edges == 0) {
-    DevBuf w;
+    DevBuf& w = ctx->snn_scratch[4];
     double* dw = d_w_out;
     if (!dw) {
-      SD_TRY(w.alloc(sizeof(double) * (size_t)total));
+      SD_TRY(w.ensure(sizeof(double) * (size_t)total));
       dw = w.as<double>();
     }
     const unsigned ge = (unsigned)((total + 255) / 256);
@@ -378,10 +380,14 @@ int snn_build(sd_context* ctx, const int32_t* d_idx, int n, int k, int include_s
       return SD_OK;
     }
     const int64_t cap = total * (symmetrise ? 2 : 1);
-    DevBuf keys, vals, keys2, vals2, cnt;
-    SD_TRY(keys.alloc(sizeof(uint64_t) * (size_t)cap + 16));
-    SD_TRY(vals.alloc(sizeof(double) * (size_t)cap + 16));
-    SD_TRY(cnt.alloc(sizeof(unsigned long long)));
+    DevBuf& keys = ctx->snn_scratch[5];
+    DevBuf& vals = ctx->snn_scratch[6];
+    DevBuf& cnt = ctx->snn_scratch[7];
+    DevBuf& keys2 = ctx->snn_scratch[8];
+    DevBuf& vals2 = ctx->snn_scratch[9];
+    SD_TRY(keys.ensure(sizeof(uint64_t) * (size_t)cap + 16));
+    SD_TRY(vals.ensure(sizeof(double) * (size_t)cap + 16));
+    SD_TRY(cnt.ensure(sizeof(unsigned long long)));
     SD_HIP(hipMemsetAsync(cnt.ptr, 0, sizeof(unsigned long long), ctx->stream));
     hipLaunchKernelGGL(k_knn_emit, dim3(ge), dim3(256), 0, ctx->stream, d_idx, dw, n, k, prune,
                        symmetrise, keys.as<uint64_t>(), vals.as<double>(),
@@ -390,20 +396,25 @@ int snn_build(sd_context* ctx, const int32_t* d_idx, int n, int k, int include_s
     SD_HIP(hipMemcpyAsync(&m, cnt.ptr, sizeof(m), hipMemcpyDeviceToHost, ctx->stream));
     SD_HIP(hipStreamSynchronize(ctx->stream));
     if (m > 0) {
-      SD_TRY(keys2.alloc(sizeof(uint64_t) * (size_t)m + 16));
-      SD_TRY(vals2.alloc(sizeof(double) * (size_t)m + 16));
+      SD_TRY(keys2.ensure(sizeof(uint64_t) * (size_t)m + 16));
+      SD_TRY(vals2.ensure(sizeof(double) * (size_t)m + 16));
       SD_TRY(sort_pairs(ctx, keys, vals, keys2, vals2, (int64_t)m, end_bit));
     }
     return assemble_csc(ctx, keys2, vals2, (int64_t)m, n, g);
   }
 
   // ---- edges = "shared" ------------------------------------------------------
-  DevBuf rc, rp, fill, rv, pc, poff;
+  DevBuf& rc = ctx->snn_scratch[15];
+  DevBuf& rp = ctx->snn_scratch[16];
+  DevBuf& fill = ctx->snn_scratch[17];
+  DevBuf& rv = ctx->snn_scratch[18];
+  DevBuf& pc = ctx->snn_scratch[19];
+  DevBuf& poff = ctx->snn_scratch[20];
   // inverted index over the neighbour ids: R(m) = nodes whose set holds m
   const unsigned gm = (unsigned)((nids + 255) / 256);
-  SD_TRY(rc.alloc(sizeof(int64_t) * (size_t)(nids + 1)));
-  SD_TRY(rp.alloc(sizeof(int64_t) * (size_t)(nids + 1)));
-  SD_TRY(fill.alloc(sizeof(int64_t) * (size_t)(nids + 1)));
+  SD_TRY(rc.ensure(sizeof(int64_t) * (size_t)(nids + 1)));
+  SD_TRY(rp.ensure(sizeof(int64_t) * (size_t)(nids + 1)));
+  SD_TRY(fill.ensure(sizeof(int64_t) * (size_t)(nids + 1)));
   SD_HIP(hipMemsetAsync(rc.ptr, 0, sizeof(int64_t) * (size_t)(nids + 1), ctx->stream));
   SD_HIP(hipMemsetAsync(fill.ptr, 0, sizeof(int64_t) * (size_t)(nids + 1), ctx->stream));
   hipLaunchKernelGGL(k_rev_count, dim3(gn), dim3(256), 0, ctx->stream, sets.as<int32_t>(),
@@ -411,27 +422,27 @@ int snn_build(sd_context* ctx, const int32_t* d_idx, int n, int k, int include_s
   size_t bytes = 0;
   SD_HIP(rocprim::exclusive_scan(nullptr, bytes, rc.as<int64_t>(), rp.as<int64_t>(), (int64_t)0,
                                  (size_t)(nids + 1), rocprim::plus<int64_t>(), ctx->stream));
-  DevBuf tmp;
-  SD_TRY(tmp.alloc(bytes + 16));
+  DevBuf& tmp = ctx->snn_scratch[13];
+  SD_TRY(tmp.ensure(bytes + 16));
   SD_HIP(rocprim::exclusive_scan(tmp.ptr, bytes, rc.as<int64_t>(), rp.as<int64_t>(), (int64_t)0,
                                  (size_t)(nids + 1), rocprim::plus<int64_t>(), ctx->stream));
   int64_t nrev = 0;
   SD_HIP(hipMemcpyAsync(&nrev, rp.as<int64_t>() + nids, sizeof(int64_t), hipMemcpyDeviceToHost,
                         ctx->stream));
   SD_HIP(hipStreamSynchronize(ctx->stream));
-  SD_TRY(rv.alloc(sizeof(int32_t) * (size_t)nrev + 16));
+  SD_TRY(rv.ensure(sizeof(int32_t) * (size_t)nrev + 16));
   hipLaunchKernelGGL(k_rev_fill, dim3(gn), dim3(256), 0, ctx->stream, sets.as<int32_t>(),
                      len.as<int32_t>(), n, ks, rp.as<int64_t>(), fill.as<int64_t>(),
                      rv.as<int32_t>());
-  SD_TRY(pc.alloc(sizeof(int64_t) * (size_t)(nids + 1)));
-  SD_TRY(poff.alloc(sizeof(int64_t) * (size_t)(nids + 1)));
+  SD_TRY(pc.ensure(sizeof(int64_t) * (size_t)(nids + 1)));
+  SD_TRY(poff.ensure(sizeof(int64_t) * (size_t)(nids + 1)));
   SD_HIP(hipMemsetAsync(pc.ptr, 0, sizeof(int64_t) * (size_t)(nids + 1), ctx->stream));
   hipLaunchKernelGGL(k_pair_count, dim3(gm), dim3(256), 0, ctx->stream, rp.as<int64_t>(), nids,
                      pc.as<int64_t>());
   bytes = 0;
   SD_HIP(rocprim::exclusive_scan(nullptr, bytes, pc.as<int64_t>(), poff.as<int64_t>(), (int64_t)0,
                                  (size_t)(nids + 1), rocprim::plus<int64_t>(), ctx->stream));
-  SD_TRY(tmp.alloc(bytes + 16));
+  SD_TRY(tmp.ensure(bytes + 16));
   SD_HIP(rocprim::exclusive_scan(tmp.ptr, bytes, pc.as<int64_t>(), poff.as<int64_t>(), (int64_t)0,
                                  (size_t)(nids + 1), rocprim::plus<int64_t>(), ctx->stream));
   int64_t npairs = 0;
@@ -442,23 +453,27 @@ int snn_build(sd_context* ctx, const int32_t* d_idx, int n, int k, int include_s
     DevBuf e1, e2;
     return assemble_csc(ctx, e1, e2, 0, n, g);
   }
-  DevBuf pk, pk2;
-  SD_TRY(pk.alloc(sizeof(uint64_t) * (size_t)npairs + 16));
-  SD_TRY(pk2.alloc(sizeof(uint64_t) * (size_t)npairs + 16));
+  DevBuf& pk = ctx->snn_scratch[21];
+  DevBuf& pk2 = ctx->snn_scratch[22];
+  SD_TRY(pk.ensure(sizeof(uint64_t) * (size_t)npairs + 16));
+  SD_TRY(pk2.ensure(sizeof(uint64_t) * (size_t)npairs + 16));
   hipLaunchKernelGGL(k_pair_emit, dim3((unsigned)nids), dim3(256), 0, ctx->stream, rp.as<int64_t>(),
                      rv.as<int32_t>(), nids, poff.as<int64_t>(), pk.as<uint64_t>());
   SD_HIP(hipGetLastError());
   bytes = 0;
   SD_HIP(rocprim::radix_sort_keys(nullptr, bytes, pk.as<uint64_t>(), pk2.as<uint64_t>(),
                                   (size_t)npairs, 0, end_bit, ctx->stream));
-  SD_TRY(tmp.alloc(bytes + 16));
+  SD_TRY(tmp.ensure(bytes + 16));
   SD_HIP(rocprim::radix_sort_keys(tmp.ptr, bytes, pk.as<uint64_t>(), pk2.as<uint64_t>(),
                                   (size_t)npairs, 0, end_bit, ctx->stream));
-  pk.release();
-  DevBuf keys, vals, keys2, vals2, cnt;
-  SD_TRY(keys.alloc(sizeof(uint64_t) * (size_t)npairs * 2 + 16));
-  SD_TRY(vals.alloc(sizeof(double) * (size_t)npairs * 2 + 16));
-  SD_TRY(cnt.alloc(sizeof(unsigned long long)));
+  DevBuf& keys = ctx->snn_scratch[5];
+  DevBuf& vals = ctx->snn_scratch[6];
+  DevBuf& cnt = ctx->snn_scratch[7];
+  DevBuf& keys2 = ctx->snn_scratch[8];
+  DevBuf& vals2 = ctx->snn_scratch[9];
+  SD_TRY(keys.ensure(sizeof(uint64_t) * (size_t)npairs * 2 + 16));
+  SD_TRY(vals.ensure(sizeof(double) * (size_t)npairs * 2 + 16));
+  SD_TRY(cnt.ensure(sizeof(unsigned long long)));
   SD_HIP(hipMemsetAsync(cnt.ptr, 0, sizeof(unsigned long long), ctx->stream));
   hipLaunchKernelGGL(k_shared_emit, dim3((unsigned)((npairs + 255) / 256)), dim3(256), 0,
                      ctx->stream, pk2.as<uint64_t>(), npairs, len.as<int32_t>(), prune,
@@ -466,10 +481,9 @@ int snn_build(sd_context* ctx, const int32_t* d_idx, int n, int k, int include_s
   unsigned long long m = 0;
   SD_HIP(hipMemcpyAsync(&m, cnt.ptr, sizeof(m), hipMemcpyDeviceToHost, ctx->stream));
   SD_HIP(hipStreamSynchronize(ctx->stream));
-  pk2.release();
   if (m > 0) {
-    SD_TRY(keys2.alloc(sizeof(uint64_t) * (size_t)m + 16));
-    SD_TRY(vals2.alloc(sizeof(double) * (size_t)m + 16));
+    SD_TRY(keys2.ensure(sizeof(uint64_t) * (size_t)m + 16));
+    SD_TRY(vals2.ensure(sizeof(double) * (size_t)m + 16));
     SD_TRY(sort_pairs(ctx, keys, vals, keys2, vals2, (int64_t)m, end_bit));
   }
   return assemble_csc(ctx, keys2, vals2, (int64_t)m, n, g);

@@ -1005,11 +1005,11 @@ int sd_snn(sd_context* ctx, const int32_t* idx, int32_t n, int32_t k, int32_t in
     if (edges != 0 && edges != 1) return set_error(SD_EINVAL, "edges must be 0 (knn) or 1 (shared)");
     SD_HIP(hipSetDevice(ctx->device));
     ctx->cancel.store(0);
-    DevBuf di;
+    DevBuf& di = ctx->snn_scratch[0];
     const int32_t* d_idx = idx;
     if (!idx_on_device) {
       const size_t total = (size_t)n * (size_t)k;
-      SD_TRY(di.alloc(sizeof(int32_t) * total));
+      SD_TRY(di.ensure(sizeof(int32_t) * total));
       SD_HIP(hipMemcpyAsync(di.ptr, idx, sizeof(int32_t) * total, hipMemcpyHostToDevice,
                             ctx->stream));
       d_idx = di.as<int32_t>();

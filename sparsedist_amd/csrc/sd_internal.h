@@ -101,6 +101,8 @@ struct sd_context {
   int cu_count = 256;
   // scratch reused across calls (grow-only)
   sd::DevBuf scratch[24];
+  // the SNN builder's temporaries, one slot each (grow-only)
+  sd::DevBuf snn_scratch[24];
   std::vector<uint8_t> host_scratch;
 };
 
