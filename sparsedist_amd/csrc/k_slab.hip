@@ -663,6 +663,10 @@ __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
   __shared__ __align__(16) float s_rthr[SYM ? TM : 4];  // mirror screen thresholds
   __shared__ int32_t s_mcnt[SYM ? TM : 1];                // mirror hits per reference
   __shared__ int32_t s_npool;                              // mirror hits past a column's slots
+  // euclidean / pearson: the reference column's key weights as a query
+  constexpr bool MW = SYM && MK != kKeyScore;
+  __shared__ __align__(16) float s_rw1[MW ? TM : 4];
+  __shared__ __align__(16) float s_rw2[MW ? TM : 4];
 
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
@@ -695,6 +699,15 @@ __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
         const float f = (float)P.thr[r];
         s_rthr[t] = f - fabsf(f) * 9.5367431640625e-07f - 1e-30f;
         s_mcnt[t] = 0;
+        if constexpr (MW) {
+          const KeyWeights w =
+              MK == kKeyPearson
+                  ? pear_weights(s_raux[t], s_raux2[t], (uint32_t)s_rnnz[t], P.dot_e0, P.dot_e1,
+                                 P.dot_nonneg)
+                  : euc_weights((uint32_t)s_rnnz[t], P.dot_e0, P.dot_e1, P.dot_nonneg);
+          s_rw1[t] = w.w1;
+          s_rw2[t] = w.w2;
+        }
       }
     } else {
       s_rnnz[t] = 0;
@@ -704,6 +717,10 @@ __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
       if constexpr (SYM) {
         s_rthr[t] = INFINITY;
         s_mcnt[t] = 0;
+      }
+      if constexpr (MW) {
+        s_rw1[t] = 0.f;
+        s_rw2[t] = 0.f;
       }
     }
   }
@@ -1002,9 +1019,20 @@ __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
           const int rl = wm * 128 + b * 32 + 8 * q4 + 4 * h;
           const float4 t4 = *reinterpret_cast<const float4*>(&s_rthr[rl]);
           int4 nn = make_int4(0, 0, 0, 0);
+          float4 w14 = make_float4(0.f, 0.f, 0.f, 0.f), w24 = w14, ra4 = w14, rc4 = w14;
           if constexpr (sizeof(K) == 8) nn = *reinterpret_cast<const int4*>(&s_rnnz[rl]);
+          if constexpr (MW) {
+            w14 = *reinterpret_cast<const float4*>(&s_rw1[rl]);
+            w24 = *reinterpret_cast<const float4*>(&s_rw2[rl]);
+            ra4 = *reinterpret_cast<const float4*>(&s_raux[rl]);
+          }
+          if constexpr (MW && MK == kKeyPearson) rc4 = *reinterpret_cast<const float4*>(&s_raux2[rl]);
           const float ts[4] = {t4.x, t4.y, t4.z, t4.w};
           const int nis[4] = {nn.x, nn.y, nn.z, nn.w};
+          const float w1s[4] = {w14.x, w14.y, w14.z, w14.w};
+          const float w2s[4] = {w24.x, w24.y, w24.z, w24.w};
+          const float ras[4] = {ra4.x, ra4.y, ra4.z, ra4.w};
+          const float rcs[4] = {rc4.x, rc4.y, rc4.z, rc4.w};
 #pragma unroll
           for (int e = 0; e < 4; ++e) {
             const int r = 4 * q4 + e;
@@ -1015,6 +1043,11 @@ __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
               if constexpr (sizeof(K) == 8) {
                 const float d = (float)((uint32_t)nis[e] + qn[j]) - x;
                 pass = x >= ts[e] * d || d <= 0.f;
+              } else if constexpr (MK == kKeyEuclid) {
+                pass = euc_key(KeyWeights{w1s[e], w2s[e], 0.f}, ras[e], qa[j], x) >= ts[e];
+              } else if constexpr (MK == kKeyPearson) {
+                pass = pear_key(KeyWeights{w1s[e], w2s[e], 0.f}, ras[e], rcs[e], qa[j], qa2[j],
+                                x) >= ts[e];
               } else {
                 pass = x >= ts[e];
               }
@@ -1046,8 +1079,21 @@ __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
       int32_t* m_idx = reinterpret_cast<int32_t*>(st0 + TM * kMirrorSlots * 4);
       uint32_t* m_nnz = reinterpret_cast<uint32_t*>(st0 + TM * kMirrorSlots * 8);
       int4* pool = reinterpret_cast<int4*>(st1);  // {value bits, candidate, its nnz, column}
-      // exact key of slab value x of tile column rloc for a candidate with cn
-      // stored entries
+      // key of slab value x for reference column rloc as the query and this
+      // lane's query column j as its candidate (euclidean, pearson; Jaccard
+      // and cosine store the value itself)
+      auto mirror_fkey = [&](float x, int rloc, int j) -> float {
+        if constexpr (MK == kKeyEuclid) {
+          return euc_key(KeyWeights{s_rw1[rloc], s_rw2[rloc], 0.f}, s_raux[rloc], qa[j], x);
+        } else if constexpr (MK == kKeyPearson) {
+          return pear_key(KeyWeights{s_rw1[rloc], s_rw2[rloc], 0.f}, s_raux[rloc], s_raux2[rloc],
+                          qa[j], qa2[j], x);
+        } else {
+          return x;
+        }
+      };
+      // exact key of stored value x (the slab value for Jaccard, with cn the
+      // candidate's stored entries; the float key otherwise) of tile column rloc
       auto mirror_key = [&](float x, int rloc, uint32_t cn) -> double {
         if constexpr (sizeof(K) == 8) {
           const uint32_t cc = (uint32_t)x;
@@ -1067,7 +1113,7 @@ __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
             const int r = __builtin_ctz(m);
             m &= m - 1u;
             const int rloc = wm * 128 + b * 32 + 8 * (r >> 2) + 4 * h + (r & 3);
-            const float x = value_of(b, j, r);
+            const float x = mirror_fkey(value_of(b, j, r), rloc, j);
             const int slot = atomicAdd(&s_mcnt[rloc], 1);
             if (slot < kMirrorSlots) {
               m_x[rloc * kMirrorSlots + slot] = x;
@@ -1743,10 +1789,11 @@ int fused_filter_pass(sd_context* ctx, const FusedArgs& a, const double* thr, in
   P.fidx = fidx;
   SD_HIP(hipMemsetAsync(cnt, 0, sizeof(int32_t) * (size_t)a.nqry, ctx->stream));
   KernelTimer timer(ctx, &ctx->stats.filter_ms, &ctx->stats.filter_launches);
-  // A self-search over all columns at once with a symmetric key (Jaccard,
-  // cosine) scores each unordered pair once.
+  // A self-search over all columns at once scores each unordered pair once:
+  // the slab value is symmetric (the split layout's three products too), and
+  // so is each key given the query column's weights.
   const bool sym = a.self_mode && !a.qlist && a.nqry == a.nref && a.qry_dense == a.ref_dense &&
-                   a.qry_p == a.ref_p && (a.method == SD_BINARY || a.method == SD_COSINE);
+                   a.qry_p == a.ref_p && a.qry_aux == a.ref_aux && a.qry_aux2 == a.ref_aux2;
   ctx->stats.filter_pairs += sym ? a.nqry * (a.nqry + 1) / 2 : a.nqry * a.nref;
   constexpr int RB = kFilterRowBytes;
   if (a.method == SD_BINARY) {
@@ -1757,17 +1804,29 @@ int fused_filter_pass(sd_context* ctx, const FusedArgs& a, const double* thr, in
                : launch_filter_t<OpF4, KeyJ, RB>(ctx, P);
   }
   if (a.prec == SD_PREC_FP32) {
-    if (a.method == SD_EUCLIDEAN) return launch_filter_t<OpF32, float, RB, kKeyEuclid>(ctx, P);
-    if (a.method == SD_PEARSON) return launch_filter_t<OpF32, float, RB, kKeyPearson>(ctx, P);
+    if (a.method == SD_EUCLIDEAN)
+      return sym ? launch_filter_t<OpF32, float, RB, kKeyEuclid, false, true>(ctx, P)
+                 : launch_filter_t<OpF32, float, RB, kKeyEuclid>(ctx, P);
+    if (a.method == SD_PEARSON)
+      return sym ? launch_filter_t<OpF32, float, RB, kKeyPearson, false, true>(ctx, P)
+                 : launch_filter_t<OpF32, float, RB, kKeyPearson>(ctx, P);
     return sym ? launch_filter_t<OpF32, float, RB, kKeyScore, false, true>(ctx, P)
                : launch_filter_t<OpF32, float, RB>(ctx, P);
   }
-  if (a.method == SD_EUCLIDEAN)
+  if (a.method == SD_EUCLIDEAN) {
+    if (sym)
+      return P.qseg ? launch_filter_t<OpBf16, float, RB, kKeyEuclid, true, true>(ctx, P)
+                    : launch_filter_t<OpBf16, float, RB, kKeyEuclid, false, true>(ctx, P);
     return P.qseg ? launch_filter_t<OpBf16, float, RB, kKeyEuclid, true>(ctx, P)
                   : launch_filter_t<OpBf16, float, RB, kKeyEuclid>(ctx, P);
-  if (a.method == SD_PEARSON)
+  }
+  if (a.method == SD_PEARSON) {
+    if (sym)
+      return P.qseg ? launch_filter_t<OpBf16, float, RB, kKeyPearson, true, true>(ctx, P)
+                    : launch_filter_t<OpBf16, float, RB, kKeyPearson, false, true>(ctx, P);
     return P.qseg ? launch_filter_t<OpBf16, float, RB, kKeyPearson, true>(ctx, P)
                   : launch_filter_t<OpBf16, float, RB, kKeyPearson>(ctx, P);
+  }
   return sym ? launch_filter_t<OpBf16, float, RB, kKeyScore, false, true>(ctx, P)
              : launch_filter_t<OpBf16, float, RB>(ctx, P);
 }

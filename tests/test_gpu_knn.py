@@ -357,16 +357,22 @@ def test_two_pass_filter_path(oracle_lib, gpu_ctx, method):
     assert st["overflow_queries"] >= 1
 
 
-@pytest.mark.parametrize("method", ["binary", "cosine"])
+@pytest.mark.parametrize("method,data", [("binary", "counts"), ("cosine", "counts"),
+                                         ("euclidean", "counts"), ("euclidean", "signed"),
+                                         ("pearson", "counts"), ("pearson", "signed")])
 @pytest.mark.parametrize("include_self", [False, True])
-def test_symmetric_self_filter(oracle_lib, gpu_ctx, method, include_self):
+def test_symmetric_self_filter(oracle_lib, gpu_ctx, method, data, include_self):
     """A self-search over 20000 columns at once scores each unordered pair once
     (tiles with reference tile >= query tile, each value screened for both
-    columns as the query).  A heavily tied block overflows candidate buffers
-    from both directions; the last tile is partial.  Sampled rows, including
-    those, must match the oracle."""
+    columns as the query, euclidean and pearson with the reference column's
+    own key weights; signed data takes the split layout).  A heavily tied
+    block overflows candidate buffers from both directions; the last tile is
+    partial.  Sampled rows, including those, must match the oracle."""
     n, k = 20000, 20
-    Y = synth.nb_log1p(600, n, 0.05, 43)
+    if data == "counts":
+        Y = synth.nb_log1p(600, n, 0.05, 43)
+    else:
+        Y = rand_sparse(600, n, 0.05, 44, signed=True)
     Yd = Y.tolil()
     for c in range(3000, 7700):
         Yd[:, c] = Y[:, 0]
