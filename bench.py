@@ -436,10 +436,8 @@ def cross_main(args, sd, synth):
 
 def large_main(args, sd, synth, workers):
     """Self-search kNN (k=20) on X = 30000 x 1000000 CSC, density 0.03, NB/log1p:
-    binary and cosine over every column; euclidean over every column in
-    blocks of 125,000 query columns against the resident matrix (a cross
-    search per block, so each query also finds itself at distance 0; the
-    blocks keep a progress line per block and bound the query-side memory)."""
+    binary, cosine and euclidean, each over every column in one call (the
+    filtered pass scores each unordered pair once)."""
     t0 = time.time()
     X = synth.nb_log1p_parallel(30_000, 1_000_000, 0.03, 70, workers)
     log(f"generated X {X.shape} nnz={X.nnz} in {time.time() - t0:.1f}s")
@@ -459,24 +457,9 @@ def large_main(args, sd, synth, workers):
         ctx.stats(reset=True)
         ctx.set_timing(True)
         t = time.perf_counter()
-        if method == "euclidean":
-            blk = 125_000
-            got_i = np.empty((sample.size, k), np.int32)
-            got_d = np.empty((sample.size, k))
-            for b0 in range(0, nq, blk):
-                nn = sd.sparseKNN(X[:, b0:b0 + blk], dX, k=k, method=method, verbose=False,
-                                  context=ctx)
-                inb = (sample >= b0) & (sample < b0 + blk)
-                got_i[inb] = nn.idx[sample[inb] - b0]
-                got_d[inb] = nn.dist[sample[inb] - b0]
-                del nn
-                log(f"euclidean block {b0 // blk + 1}/{(nq + blk - 1) // blk}: "
-                    f"{time.perf_counter() - t:.1f} s")
-        else:
-            nn = sd.sparseKNN(dX, k=k, method=method, block_size=8192, verbose=False,
-                              context=ctx)
-            got_i, got_d = nn.idx[sample].copy(), nn.dist[sample].copy()
-            del nn
+        nn = sd.sparseKNN(dX, k=k, method=method, block_size=8192, verbose=False, context=ctx)
+        got_i, got_d = nn.idx[sample].copy(), nn.dist[sample].copy()
+        del nn
         dt = time.perf_counter() - t
         st = ctx.stats(reset=True)
         ctx.set_timing(False)
@@ -485,8 +468,8 @@ def large_main(args, sd, synth, workers):
         # dense-equivalent operations of one filtered pass (the split BF16
         # layout runs three products per row)
         rows = X.shape[0] * max(st["slab_products"], 1)
-        qpass = nq / max(st["filter_launches"], 1)
-        flop = 2.0 * rows * nq * qpass
+        pairs = st["filter_pairs"] / max(st["filter_launches"], 1)  # scored per launch
+        flop = 2.0 * rows * pairs
         res[method] = dict(seconds=dt, cols_per_s=nq / dt, filter_ms=fms,
                            slab_products_per_row=st["slab_products"],
                            filter_frac_of_peak=flop / (fms / 1e3) / peak,
@@ -496,21 +479,17 @@ def large_main(args, sd, synth, workers):
             f"({res[method]['filter_frac_of_peak'] * 100:.0f}% of peak), escalated "
             f"{st['escalated_queries']}, fallback {st['fallback_queries']}")
         t = time.perf_counter()
-        if method == "euclidean":  # the fused blocks were cross searches
-            ex = sd.sparseKNN(Xs, dX, k=k, method=method, precision="exact", verbose=False,
-                              context=ctx)
-            want_i, want_d = ex.idx, ex.dist
-        else:  # self-search without self: drop each column's own entry from k + 1
-            ex = sd.sparseKNN(Xs, dX, k=k + 1, method=method, precision="exact",
-                              verbose=False, context=ctx)
-            want_i = np.empty((sample.size, k), np.int32)
-            want_d = np.empty((sample.size, k))
-            for r, c in enumerate(sample):
-                keep = ex.idx[r] != c + 1
-                if keep.all():
-                    keep[-1] = False
-                want_i[r] = ex.idx[r][keep][:k]
-                want_d[r] = ex.dist[r][keep][:k]
+        # self-search without self: drop each column's own entry from k + 1
+        ex = sd.sparseKNN(Xs, dX, k=k + 1, method=method, precision="exact", verbose=False,
+                          context=ctx)
+        want_i = np.empty((sample.size, k), np.int32)
+        want_d = np.empty((sample.size, k))
+        for r, c in enumerate(sample):
+            keep = ex.idx[r] != c + 1
+            if keep.all():
+                keep[-1] = False
+            want_i[r] = ex.idx[r][keep][:k]
+            want_d[r] = ex.dist[r][keep][:k]
         same_i = bool(np.array_equal(got_i, want_i))
         same_d = bool(np.array_equal(got_d.view(np.int64), want_d.view(np.int64)))
         res[method]["sample_parity"] = dict(queries=int(sample.size), idx_identical=same_i,
