@@ -243,15 +243,18 @@ int fused_candidate_lists(sd_context* ctx, FusedArgs fa, const double** out_key,
   // query's Lp-th key there as its threshold.  About stride x Lp references
   // of the whole set score at or above it (the Lp-th of a 1/stride sample),
   // so a sparser sample with a proportionally smaller rank keeps the
-  // candidate count and costs less: 1/64 at rank ~L/2 for large references,
-  // 1/32 at rank L below 256 tiles (a sample of at least 4 tiles).
+  // candidate count and costs less: 1/128 at rank ~L/4 from 512 tiles on,
+  // 1/64 at rank ~L/2 from 256, 1/32 at rank L below (a sample of at least
+  // 4 tiles).
   if (rtiles < 4 * 32) {
     fa.tile_stride = 1;
     return list_pass(fa, ctx->scratch[3], ctx->scratch[4], ctx->scratch[5], ctx->scratch[6],
                      out_key, out_idx);
   }
-  const int64_t stride = rtiles >= 4 * 64 ? 64 : 32;
-  const int Lp = stride == 64 ? fused_list_len((L + 1) / 2) : L;
+  const int64_t stride = rtiles >= 4 * 128 ? 128 : rtiles >= 4 * 64 ? 64 : 32;
+  const int Lp = stride == 128 ? fused_list_len((L + 3) / 4)
+                 : stride == 64 ? fused_list_len((L + 1) / 2)
+                                : L;
   // 1. thresholds from every stride-th reference tile
   const double* pk = nullptr;
   const int32_t* pi = nullptr;

@@ -498,15 +498,17 @@ def test_pearson_signed_constant_and_self(oracle_lib, gpu_ctx):
 
 
 @pytest.mark.parametrize("method,copies", [("cosine", 29), ("binary", 16)])
-def test_sparse_sample_threshold_redo(oracle_lib, gpu_ctx, method, copies):
-    """Large references (>= 256 tiles of 128) take thresholds from every 64th
-    tile at about half the list rank (L = 40 -> 24 for cosine, 24 -> 16 for
-    binary at k = 20).  Such a threshold can exceed the true L-th key: here
-    copies of the first query fill the first sampled tile beyond that rank,
-    so the sample's key at that rank is the top score and fewer than L
-    candidates pass; the query is searched again in full and every result
-    still matches the oracle."""
-    Y = synth.nb_log1p(600, 40000, 0.05, 43)
+@pytest.mark.parametrize("nref", [40000, 70000])
+def test_sparse_sample_threshold_redo(oracle_lib, gpu_ctx, method, copies, nref):
+    """Large references take thresholds from a sparse sample of tiles at a
+    reduced list rank: every 64th tile at about half the rank from 256 tiles
+    of 128 (L = 40 -> 24 for cosine, 24 -> 16 for binary at k = 20), every
+    128th at about a quarter from 512 tiles (40 -> 16, 24 -> 8).  Such a
+    threshold can exceed the true L-th key: here copies of the first query
+    fill the first sampled tile beyond that rank, so the sample's key at that
+    rank is the top score and fewer than L candidates pass; the query is
+    searched again in full and every result still matches the oracle."""
+    Y = synth.nb_log1p(600, nref, 0.05, 43)
     Yd = Y.tolil()
     base = Y[:, 100]
     for c in range(copies):
