@@ -113,8 +113,10 @@ typedef struct {
   int64_t slab_products;      /* MFMA products per row of the last fused search's layout:
                                  3 for split BF16 (hi.hi + hi.lo + lo.hi), else 1 */
   int64_t filter_pairs;       /* (query, reference) pairs scored by filtered passes: a
-                                 self-search over all columns at once with a symmetric
-                                 method (binary, cosine) scores each unordered pair once */
+                                 self-search over all columns at once scores each
+                                 unordered pair once */
+  int64_t buffer_escalations; /* searches whose wider second lists came from the filtered
+                                 pass's candidate buffers instead of a second slab pass */
 } sd_stats;
 int sd_context_stats(sd_context* ctx, sd_stats* out, int reset);
 /* Enable/disable event timing of kernels (off by default). */

@@ -1224,15 +1224,21 @@ __global__ __launch_bounds__(256) void k_select(const double* __restrict__ fkey,
                                                 int32_t* __restrict__ out_idx,
                                                 int32_t* __restrict__ over_list,
                                                 int32_t* __restrict__ over_count,
-                                                int min_count) {
+                                                int min_count,
+                                                const int32_t* __restrict__ qlist) {
   __shared__ BlockTopKStorage<512> st;
-  const int64_t q = blockIdx.x;
+  // list position b; its query's buffer q (listed mode: qlist[b])
+  const int64_t b = blockIdx.x;
+  const int64_t q = qlist ? (int64_t)qlist[b] : b;
   const int n = cnt[q];
   // overflowed, or fewer candidates than the threshold guarantees complete
   // lists for: the query is searched again by a full list pass
   if (n > cap || n < min_count) {
-    if (threadIdx.x == 0) over_list[atomicAdd(over_count, 1)] = (int32_t)q;
-    for (int t = threadIdx.x; t < L; t += blockDim.x) out_idx[q * L + t] = -1;
+    if (threadIdx.x == 0) {
+      const int at = atomicAdd(over_count, 1);
+      if (over_list) over_list[at] = (int32_t)q;
+    }
+    for (int t = threadIdx.x; t < L; t += blockDim.x) out_idx[b * L + t] = -1;
     return;
   }
   BlockTopK<512> tk(st, L, 1);
@@ -1244,8 +1250,8 @@ __global__ __launch_bounds__(256) void k_select(const double* __restrict__ fkey,
   __syncthreads();
   const int c = st.cnt;
   for (int t = threadIdx.x; t < L; t += blockDim.x) {
-    out_key[q * L + t] = (t < c) ? st.val[t] : 0.0;
-    out_idx[q * L + t] = (t < c) ? (int32_t)st.idx[t] : -1;
+    out_key[b * L + t] = (t < c) ? st.val[t] : 0.0;
+    out_idx[b * L + t] = (t < c) ? (int32_t)st.idx[t] : -1;
   }
 }
 
@@ -1833,10 +1839,10 @@ int fused_filter_pass(sd_context* ctx, const FusedArgs& a, const double* thr, in
 
 int select_lists(sd_context* ctx, const double* fkey, const int32_t* fidx, const int32_t* cnt,
                  int cap, int64_t nqry, int L, double* out_key, int32_t* out_idx,
-                 int32_t* over_list, int32_t* over_count, int min_count) {
+                 int32_t* over_list, int32_t* over_count, int min_count, const int32_t* qlist) {
   if (nqry <= 0) return SD_OK;
   hipLaunchKernelGGL(k_select, dim3((unsigned)nqry), dim3(256), 0, ctx->stream, fkey, fidx, cnt, cap,
-                     nqry, L, out_key, out_idx, over_list, over_count, min_count);
+                     nqry, L, out_key, out_idx, over_list, over_count, min_count, qlist);
   SD_HIP(hipGetLastError());
   return SD_OK;
 }

@@ -212,8 +212,19 @@ int plan_knn(const sd_matrix* X, const sd_matrix* Y, const sd_knn_params* p, Knn
 // those is the result.  Queries whose buffer overflows (many ties at the
 // threshold) get a full list-mode pass instead.  Small searches use the
 // list mode directly.
+// The filtered pass's candidate buffers, kept after the lists are selected:
+// every reference at or above its query's threshold, so the top-L' of a
+// buffer that holds at least L' entries and did not overflow is the query's
+// top-L' for any L' (fkey is null when the search took list mode only).
+struct CandBuffers {
+  const double* fkey = nullptr;
+  const int32_t* fidx = nullptr;
+  const int32_t* cnt = nullptr;
+  int cap = 0;
+};
+
 int fused_candidate_lists(sd_context* ctx, FusedArgs fa, const double** out_key,
-                          const int32_t** out_idx) {
+                          const int32_t** out_idx, CandBuffers* cb) {
   const int64_t nq = fa.nqry;
   const int L = fa.L;
   const int64_t rtiles = (fa.nref + 127) / 128;
@@ -311,6 +322,10 @@ int fused_candidate_lists(sd_context* ctx, FusedArgs fa, const double** out_key,
   }
   *out_key = mk.as<double>();
   *out_idx = mi.as<int32_t>();
+  cb->fkey = fk.as<double>();
+  cb->fidx = fi.as<int32_t>();
+  cb->cnt = cnt.as<int32_t>();
+  cb->cap = cap;
   return SD_OK;
 }
 
@@ -495,7 +510,8 @@ int run_knn(sd_context* ctx, sd_matrix* X, sd_matrix* Y, const sd_knn_params* p,
     const int L = fa.L;
     const double* lkey = nullptr;
     const int32_t* lidx = nullptr;
-    SD_TRY(fused_candidate_lists(ctx, fa, &lkey, &lidx));
+    CandBuffers cb;
+    SD_TRY(fused_candidate_lists(ctx, fa, &lkey, &lidx, &cb));
     if (method == SD_BINARY) {
       SD_TRY(finish_exact_lists(ctx, lkey, lidx, nq, L, k, p->dist,
                                 p->drop_degenerate ? X->deg.as<uint8_t>() : nullptr,
@@ -552,28 +568,54 @@ int run_knn(sd_context* ctx, sd_matrix* X, sd_matrix* Y, const sd_knn_params* p,
       const int Lmax = fused_list_len(64);
       if (nfail > 0 && L < Lmax) {
         SD_TRY(check_cancel(ctx));
-        FusedArgs fb = fa;
-        fb.qlist = fl.as<int32_t>();
-        fb.nqry = nfail;
-        fb.L = Lmax;
-        const int64_t qt2 = (nfail + 255) / 256;
-        fb.splits = (int)std::min<int64_t>(round_up((2 * (int64_t)ctx->cu_count + qt2 - 1) / qt2, 8), 4096);
-        const int nl2 = fused_effective_splits(fb);
-        DevBuf& ck2 = ctx->scratch[9];
-        DevBuf& ci2 = ctx->scratch[10];
         DevBuf& mk2 = ctx->scratch[11];
         DevBuf& mi2 = ctx->scratch[12];
-        SD_TRY(ck2.ensure(sizeof(double) * (size_t)nl2 * (size_t)nfail * Lmax + 64));
-        SD_TRY(ci2.ensure(sizeof(int32_t) * (size_t)nl2 * (size_t)nfail * Lmax + 64));
-        SD_TRY(fused_slab_topl(ctx, fb, ck2.as<double>(), ci2.as<int32_t>()));
-        const double* k2 = ck2.as<double>();
-        const int32_t* i2 = ci2.as<int32_t>();
-        if (nl2 > 1) {
+        const double* k2 = nullptr;
+        const int32_t* i2 = nullptr;
+        if (cb.fkey) {
+          // the wider lists straight from the filtered pass's buffers, when
+          // every failed query's buffer is complete for Lmax
           SD_TRY(mk2.ensure(sizeof(double) * (size_t)nfail * Lmax + 64));
           SD_TRY(mi2.ensure(sizeof(int32_t) * (size_t)nfail * Lmax + 64));
-          SD_TRY(merge_lists(ctx, k2, i2, nl2, nfail, Lmax, mk2.as<double>(), mi2.as<int32_t>()));
-          k2 = mk2.as<double>();
-          i2 = mi2.as<int32_t>();
+          DevBuf& bc = ctx->scratch[15];
+          SD_TRY(bc.ensure(64));
+          SD_HIP(hipMemsetAsync(bc.ptr, 0, sizeof(int32_t), ctx->stream));
+          SD_TRY(select_lists(ctx, cb.fkey, cb.fidx, cb.cnt, cb.cap, nfail, Lmax, mk2.as<double>(),
+                              mi2.as<int32_t>(), nullptr, bc.as<int32_t>(), Lmax,
+                              fl.as<int32_t>()));
+          int32_t nbad = 0;
+          SD_HIP(hipMemcpyAsync(&nbad, bc.ptr, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+          SD_HIP(hipStreamSynchronize(ctx->stream));
+          if (nbad == 0) {
+            k2 = mk2.as<double>();
+            i2 = mi2.as<int32_t>();
+          }
+        }
+        if (!k2) {
+          // second fused pass over just those columns with the widest list
+          FusedArgs fb = fa;
+          fb.qlist = fl.as<int32_t>();
+          fb.nqry = nfail;
+          fb.L = Lmax;
+          const int64_t qt2 = (nfail + 255) / 256;
+          fb.splits = (int)std::min<int64_t>(round_up((2 * (int64_t)ctx->cu_count + qt2 - 1) / qt2, 8), 4096);
+          const int nl2 = fused_effective_splits(fb);
+          DevBuf& ck2 = ctx->scratch[9];
+          DevBuf& ci2 = ctx->scratch[10];
+          SD_TRY(ck2.ensure(sizeof(double) * (size_t)nl2 * (size_t)nfail * Lmax + 64));
+          SD_TRY(ci2.ensure(sizeof(int32_t) * (size_t)nl2 * (size_t)nfail * Lmax + 64));
+          SD_TRY(fused_slab_topl(ctx, fb, ck2.as<double>(), ci2.as<int32_t>()));
+          k2 = ck2.as<double>();
+          i2 = ci2.as<int32_t>();
+          if (nl2 > 1) {
+            SD_TRY(mk2.ensure(sizeof(double) * (size_t)nfail * Lmax + 64));
+            SD_TRY(mi2.ensure(sizeof(int32_t) * (size_t)nfail * Lmax + 64));
+            SD_TRY(merge_lists(ctx, k2, i2, nl2, nfail, Lmax, mk2.as<double>(), mi2.as<int32_t>()));
+            k2 = mk2.as<double>();
+            i2 = mi2.as<int32_t>();
+          }
+        } else {
+          ++ctx->stats.buffer_escalations;
         }
         SD_TRY(fl2.ensure(sizeof(int32_t) * (size_t)(nfail + 16)));
         int32_t* fail_count2 = fl2.as<int32_t>() + nfail;

@@ -279,10 +279,13 @@ int fused_filter_pass(sd_context* ctx, const FusedArgs& a, const double* thr, in
                       int32_t* cnt, double* fkey, int32_t* fidx);
 // Per-query top-L of the appended candidates.  Queries with more than cap
 // candidates (buffer overflow) or fewer than min_count (the threshold cannot
-// vouch for complete lists) are listed in over_list for a full pass.
+// vouch for complete lists) are counted in over_count and, when over_list is
+// given, listed there for a full pass.  With qlist, list position b takes
+// query qlist[b]'s buffer (outputs by position).
 int select_lists(sd_context* ctx, const double* fkey, const int32_t* fidx, const int32_t* cnt,
                  int cap, int64_t nqry, int L, double* out_key, int32_t* out_idx,
-                 int32_t* over_list, int32_t* over_count, int min_count);
+                 int32_t* over_list, int32_t* over_count, int min_count,
+                 const int32_t* qlist = nullptr);
 int list_thresholds(sd_context* ctx, const double* key, const int32_t* idx, int64_t nqry, int L,
                     double* thr);
 int scatter_lists(sd_context* ctx, const double* key, const int32_t* idx, const int32_t* rows,

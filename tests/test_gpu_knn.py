@@ -497,6 +497,29 @@ def test_pearson_signed_constant_and_self(oracle_lib, gpu_ctx):
         same((g.idx, g.dist), o, f"cross dist={dist}")
 
 
+@pytest.mark.parametrize("method", ["cosine", "euclidean"])
+def test_escalation_lists_from_candidate_buffers(oracle_lib, gpu_ctx, method):
+    """A narrow re-rank margin (L = 24 at k = 20) makes the first certificate
+    fail often; the wider lists (L = 64) of the failed queries then come from
+    the filtered pass's candidate buffers (about 770 entries per query here,
+    every reference at or above the query's threshold) instead of a second
+    slab pass.  Sampled rows must match the oracle."""
+    n, k = 20000, 20
+    Y = synth.nb_log1p(2000, n, 0.05, 45)
+    gpu_ctx.stats(reset=True)
+    g = sd.sparseKNN(Y, k=k, method=method, precision="bf16", margin=1, verbose=False)
+    st = gpu_ctx.stats(reset=True)
+    if method == "cosine":
+        assert st["escalated_queries"] > 0, st
+    if st["escalated_queries"] > 0:
+        assert st["buffer_escalations"] == 1, st
+    rng = np.random.default_rng(9)
+    qcols = sorted(rng.choice(n, 400, replace=False).tolist())
+    o = oracle_lib.knn(Y[:, qcols], Y, k=k + 1, method=method, nthreads=NT)
+    want = self_rows_from_cross(o[0], o[1], qcols, k, False)
+    same((g.idx[qcols], g.dist[qcols]), want, method)
+
+
 @pytest.mark.parametrize("method,copies", [("cosine", 29), ("binary", 16)])
 @pytest.mark.parametrize("nref", [40000, 70000])
 def test_sparse_sample_threshold_redo(oracle_lib, gpu_ctx, method, copies, nref):
