@@ -277,8 +277,9 @@ int fused_candidate_lists(sd_context* ctx, FusedArgs fa, const double** out_key,
   DevBuf& thr = ctx->scratch[13];
   SD_TRY(thr.ensure(sizeof(double) * (size_t)nq + 64));
   SD_TRY(list_thresholds(ctx, pk, pi, nq, Lp, thr.as<double>()));
-  // 2. filtered full pass: room for twice the expected candidates
-  const int cap = (int)std::min<int64_t>(4096, 2 * Lp * stride + 256);
+  // 2. filtered full pass: room for three times the expected candidates
+  // (the count above the Lp-th of a sample varies by about 1/sqrt(Lp))
+  const int cap = (int)std::min<int64_t>(4096, 3 * Lp * stride + 256);
   DevBuf& cnt = ctx->scratch[14];
   DevBuf& fk = ctx->scratch[9];
   DevBuf& fi = ctx->scratch[10];

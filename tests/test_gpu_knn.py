@@ -365,9 +365,13 @@ def test_symmetric_self_filter(oracle_lib, gpu_ctx, method, data, include_self):
     """A self-search over 20000 columns at once scores each unordered pair once
     (tiles with reference tile >= query tile, each value screened for both
     columns as the query, euclidean and pearson with the reference column's
-    own key weights; signed data takes the split layout).  A heavily tied
-    block overflows candidate buffers from both directions; the last tile is
-    partial.  Sampled rows, including those, must match the oracle."""
+    own key weights).  Signed euclidean / pearson data takes the split layout,
+    whose mirrored values are not bit-identical to the threshold pre-pass's,
+    so it keeps the full grid: a rank-L threshold then left two pearson
+    queries with 17 and 18 candidates (caught by the whole-result check).  A
+    heavily tied block overflows candidate buffers from both directions; the
+    last tile is partial.  Sampled rows, including those, must match the
+    oracle, and every row must match a cross search of the same columns."""
     n, k = 20000, 20
     if data == "counts":
         Y = synth.nb_log1p(600, n, 0.05, 43)
@@ -385,7 +389,9 @@ def test_symmetric_self_filter(oracle_lib, gpu_ctx, method, data, include_self):
     finally:
         gpu_ctx.set_timing(False)
     st = gpu_ctx.stats(reset=True)
-    assert st["filter_launches"] == 1 and st["filter_pairs"] == n * (n + 1) // 2, st
+    split = data == "signed"  # split BF16 layout: full grid of tiles
+    pairs = n * n if split else n * (n + 1) // 2
+    assert st["filter_launches"] == 1 and st["filter_pairs"] == pairs, st
     assert st["overflow_queries"] >= 1
     rng = np.random.default_rng(7)
     qcols = sorted(set(rng.choice(n, 600, replace=False).tolist()) |
@@ -393,6 +399,12 @@ def test_symmetric_self_filter(oracle_lib, gpu_ctx, method, data, include_self):
     o = oracle_lib.knn(Y[:, qcols], Y, k=k + 1, method=method, nthreads=NT)
     want = self_rows_from_cross(o[0], o[1], qcols, k, include_self)
     same((g.idx[qcols], g.dist[qcols]), want, f"{method} include_self={include_self}")
+    # every row: a cross search against the same columns runs the full grid
+    # of tiles; with the self rules applied its rows equal the symmetric ones
+    c = sd.sparseKNN(Y, Y.copy(), k=k + 1, method=method, precision="bf16", verbose=False)
+    allq = list(range(n))
+    same((g.idx, g.dist), self_rows_from_cross(c.idx, c.dist, allq, k, include_self),
+         f"{method} include_self={include_self}: symmetric vs full grid")
 
 
 @pytest.mark.parametrize("prec", ["bf16", "fp32"])
