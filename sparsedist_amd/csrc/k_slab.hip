@@ -1795,18 +1795,13 @@ int fused_filter_pass(sd_context* ctx, const FusedArgs& a, const double* thr, in
   P.fidx = fidx;
   SD_HIP(hipMemsetAsync(cnt, 0, sizeof(int32_t) * (size_t)a.nqry, ctx->stream));
   KernelTimer timer(ctx, &ctx->stats.filter_ms, &ctx->stats.filter_launches);
-  // A self-search over all columns at once scores each unordered pair once.
-  // The mirrored value must equal, bit for bit, what the threshold pre-pass
-  // computed for the same pair with the roles swapped: a threshold at rank L
-  // of the sample is sound only if every sampled key at or above it is
-  // appended again.  Single-layout products are exact in FP32 (BF16, FP4
-  // counts) or commutative (FP32) and are summed in the same K order either
-  // way.  The split layout is not symmetric bit for bit: the query side reads
-  // its segments swapped, so the mirrored value sums hi.lo and lo.hi in the
-  // other order; split searches keep the full grid.
+  // A self-search over all columns at once scores each unordered pair once:
+  // the mirrored value is the same sum of products (on the split layout the
+  // query side reads its segments swapped, so hi.lo and lo.hi are added in
+  // the other order and the value may differ in the last bit, within the same
+  // error bound), and each key uses its own query column's weights.
   const bool sym = a.self_mode && !a.qlist && a.nqry == a.nref && a.qry_dense == a.ref_dense &&
-                   a.qry_p == a.ref_p && a.qry_aux == a.ref_aux && a.qry_aux2 == a.ref_aux2 &&
-                   P.qseg == 0;
+                   a.qry_p == a.ref_p && a.qry_aux == a.ref_aux && a.qry_aux2 == a.ref_aux2;
   ctx->stats.filter_pairs += sym ? a.nqry * (a.nqry + 1) / 2 : a.nqry * a.nref;
   constexpr int RB = kFilterRowBytes;
   if (a.method == SD_BINARY) {
@@ -1827,12 +1822,16 @@ int fused_filter_pass(sd_context* ctx, const FusedArgs& a, const double* thr, in
                : launch_filter_t<OpF32, float, RB>(ctx, P);
   }
   if (a.method == SD_EUCLIDEAN) {
-    if (sym) return launch_filter_t<OpBf16, float, RB, kKeyEuclid, false, true>(ctx, P);
+    if (sym)
+      return P.qseg ? launch_filter_t<OpBf16, float, RB, kKeyEuclid, true, true>(ctx, P)
+                    : launch_filter_t<OpBf16, float, RB, kKeyEuclid, false, true>(ctx, P);
     return P.qseg ? launch_filter_t<OpBf16, float, RB, kKeyEuclid, true>(ctx, P)
                   : launch_filter_t<OpBf16, float, RB, kKeyEuclid>(ctx, P);
   }
   if (a.method == SD_PEARSON) {
-    if (sym) return launch_filter_t<OpBf16, float, RB, kKeyPearson, false, true>(ctx, P);
+    if (sym)
+      return P.qseg ? launch_filter_t<OpBf16, float, RB, kKeyPearson, true, true>(ctx, P)
+                    : launch_filter_t<OpBf16, float, RB, kKeyPearson, false, true>(ctx, P);
     return P.qseg ? launch_filter_t<OpBf16, float, RB, kKeyPearson, true>(ctx, P)
                   : launch_filter_t<OpBf16, float, RB, kKeyPearson>(ctx, P);
   }

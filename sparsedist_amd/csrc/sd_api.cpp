@@ -298,12 +298,16 @@ int fused_candidate_lists(sd_context* ctx, FusedArgs fa, const double** out_key,
   SD_TRY(ov.ensure(sizeof(int32_t) * (size_t)(nq + 16)));
   int32_t* over_count = ov.as<int32_t>() + nq;
   SD_HIP(hipMemsetAsync(over_count, 0, sizeof(int32_t), ctx->stream));
-  // A threshold from rank Lp < L of the sample can exceed the true L-th key;
-  // at least L candidates at or above it prove it does not (then L references
-  // score >= thr, so the L-th key is >= thr and the lists are complete).
+  // A buffer holds exactly the references whose filtered-pass key is at or
+  // above the threshold.  With at least L of them, the L-th key is >= thr and
+  // every reference outside keys below it, so the buffer's top-L is the
+  // query's top-L.  That needs nothing from the pre-pass: a threshold from
+  // rank Lp < L can exceed the true L-th key, and the filtered pass may
+  // compute a key in another order than the pre-pass did (the symmetric
+  // pass's mirrored values on the split layout); either way fewer than L
+  // candidates send the query to a full pass instead.
   SD_TRY(select_lists(ctx, fk.as<double>(), fi.as<int32_t>(), cnt.as<int32_t>(), cap, nq, L,
-                      mk.as<double>(), mi.as<int32_t>(), ov.as<int32_t>(), over_count,
-                      Lp < L ? L : 0));
+                      mk.as<double>(), mi.as<int32_t>(), ov.as<int32_t>(), over_count, L));
   int32_t nover = 0;
   SD_HIP(hipMemcpyAsync(&nover, over_count, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
   SD_HIP(hipStreamSynchronize(ctx->stream));

@@ -366,12 +366,14 @@ def test_symmetric_self_filter(oracle_lib, gpu_ctx, method, data, include_self):
     (tiles with reference tile >= query tile, each value screened for both
     columns as the query, euclidean and pearson with the reference column's
     own key weights).  Signed euclidean / pearson data takes the split layout,
-    whose mirrored values are not bit-identical to the threshold pre-pass's,
-    so it keeps the full grid: a rank-L threshold then left two pearson
-    queries with 17 and 18 candidates (caught by the whole-result check).  A
-    heavily tied block overflows candidate buffers from both directions; the
-    last tile is partial.  Sampled rows, including those, must match the
-    oracle, and every row must match a cross search of the same columns."""
+    whose mirrored values may differ from the threshold pre-pass's in the last
+    bit: when that leaves a query fewer than L candidates (two pearson
+    queries here, before every short buffer was sent to the full pass, came
+    out with 17 and 18 neighbours; the whole-result check caught it) the query
+    is searched again in full.  A heavily tied block overflows candidate
+    buffers from both directions; the last tile is partial.  Sampled rows,
+    including those, must match the oracle, and every row must match a cross
+    search of the same columns."""
     n, k = 20000, 20
     if data == "counts":
         Y = synth.nb_log1p(600, n, 0.05, 43)
@@ -389,9 +391,7 @@ def test_symmetric_self_filter(oracle_lib, gpu_ctx, method, data, include_self):
     finally:
         gpu_ctx.set_timing(False)
     st = gpu_ctx.stats(reset=True)
-    split = data == "signed"  # split BF16 layout: full grid of tiles
-    pairs = n * n if split else n * (n + 1) // 2
-    assert st["filter_launches"] == 1 and st["filter_pairs"] == pairs, st
+    assert st["filter_launches"] == 1 and st["filter_pairs"] == n * (n + 1) // 2, st
     assert st["overflow_queries"] >= 1
     rng = np.random.default_rng(7)
     qcols = sorted(set(rng.choice(n, 600, replace=False).tolist()) |
