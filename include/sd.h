@@ -130,10 +130,13 @@ void* sd_context_stream(sd_context* ctx);
  * (whole query tiles of 256) at the filtered pass's rate; the FP64 merge
  * walk costs a smaller overhead plus n_ref (nnz_q + nnz_ref) merge steps per
  * query column.  Both paths give identical results, so the choice never
- * changes an answer.  For cosine the re-rank margin m adapts per reference
- * matrix: a search whose certificate sends more than 1% of its queries to the
- * wider second pass doubles m for the next search on that reference (up to
- * 48), and a search with none lowers it by 8 (down to 16).  The reference
+ * changes an answer.  The re-rank margin m (cosine, euclidean, pearson)
+ * adapts per reference matrix.  When the wider second lists come from the
+ * filtered pass's candidate buffers, a search whose certificate sends more
+ * than 15% of its queries to the second re-rank doubles m for the next search
+ * on that reference (up to 48), and one that sends fewer than 1% lowers it
+ * by 4 (down to 8).  When they need a second slab pass, more than 1% doubles
+ * m and a search with none lowers it by 8 (down to 16).  The reference
  * has no counterpart; its bench chooses between dense and sparse inputs by a
  * density sweep (bench/R/20-design.R:115-160).
  * sd_autotune measures the model's rates on this device (about a second of

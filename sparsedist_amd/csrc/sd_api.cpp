@@ -562,13 +562,24 @@ int run_knn(sd_context* ctx, sd_matrix* X, sd_matrix* Y, const sd_knn_params* p,
       const int32_t* fail_ids = fl.as<int32_t>();
       ctx->stats.last_margin = margin;
       if (prec == SD_PREC_AUTO && p->margin <= 0) {
-        // adapt this reference's margin: widen when the certificate sent more
-        // than 1% of the queries to the second pass, narrow after a clean search
+        // adapt this reference's margin.  After a filtered pass the wider
+        // lists come from the candidate buffers and a failed query costs only
+        // its second re-rank (64 lanes instead of L): widen when more than 15%
+        // fail, narrow by 4 (down to 8) when fewer than 1% do.  Otherwise a
+        // failed query costs a second slab pass: widen above 1%, narrow by 8
+        // (down to 16) only after a clean search.
         std::lock_guard<std::mutex> lk(ref->prep_mu);
-        if ((int64_t)nfail * 100 > nq)
-          ref->tuned_margin = std::min(2 * margin, 48);
-        else if (nfail == 0)
-          ref->tuned_margin = std::max(margin - 8, 16);
+        if (cb.fkey) {
+          if ((int64_t)nfail * 100 > 15 * nq)
+            ref->tuned_margin = std::min(2 * margin, 48);
+          else if ((int64_t)nfail * 100 < nq)
+            ref->tuned_margin = std::max(margin - 4, 8);
+        } else {
+          if ((int64_t)nfail * 100 > nq)
+            ref->tuned_margin = std::min(2 * margin, 48);
+          else if (nfail == 0)
+            ref->tuned_margin = std::max(margin - 8, 16);
+        }
       }
       const int Lmax = fused_list_len(64);
       if (nfail > 0 && L < Lmax) {

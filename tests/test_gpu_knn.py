@@ -532,6 +532,32 @@ def test_escalation_lists_from_candidate_buffers(oracle_lib, gpu_ctx, method):
     same((g.idx[qcols], g.dist[qcols]), want, method)
 
 
+def test_margin_adapts_per_reference(gpu_ctx):
+    """With precision = "auto" the re-rank margin adapts per reference: after a
+    filtered pass (candidate buffers serve the wider lists) more than 15% of
+    queries escalating doubles it (at most 48), fewer than 1% lowers it by 4
+    (at least 8).  Three searches on one resident reference follow that rule
+    from the default 16, and give identical results."""
+    Y = synth.nb_log1p(2000, 20000, 0.05, 46)
+    dY = sd.DeviceMatrix(Y, gpu_ctx)
+    margin, prev = 16, None
+    for it in range(3):
+        gpu_ctx.stats(reset=True)
+        g = sd.sparseKNN(dY, k=20, method="cosine", verbose=False, context=gpu_ctx)
+        st = gpu_ctx.stats(reset=True)
+        assert st["fused_calls"] == 1 and st["last_margin"] == margin, (it, st)
+        if prev is not None:
+            assert np.array_equal(g.idx, prev.idx)
+            assert np.array_equal(bits(g.dist), bits(prev.dist))
+        prev = g
+        # first-pass failures: escalated while L < 64, else sent to the exact path
+        esc = st["escalated_queries"] if 20 + margin <= 48 else st["fallback_queries"]
+        if esc * 100 > 15 * 20000:
+            margin = min(2 * margin, 48)
+        elif esc * 100 < 20000:
+            margin = max(margin - 4, 8)
+
+
 @pytest.mark.parametrize("method,copies", [("cosine", 29), ("binary", 16)])
 @pytest.mark.parametrize("nref", [40000, 70000])
 def test_sparse_sample_threshold_redo(oracle_lib, gpu_ctx, method, copies, nref):
