@@ -1011,6 +1011,17 @@ __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
   // its candidate.  No self pair lies off the diagonal.
   if constexpr (SYM) {
     if (mirror) {
+      // candidates are valid, non-degenerate query columns only.  The flag
+      // joins each compare below: clearing the marks afterwards under a
+      // divergent condition was dropped by the compiler (the flag load fed an
+      // empty branch), which let degenerate columns through.
+      bool cand_ok[2];
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const int64_t cq = qv[j] ? qc[j] : 0;  // in range for every lane
+        const int dg = P.ref_deg ? (int)P.ref_deg[cq] : 0;
+        cand_ok[j] = qv[j] & (dg == 0);
+      }
       uint64_t pmr[2] = {0ull, 0ull};
 #pragma unroll
       for (int b = 0; b < 4; ++b)
@@ -1051,7 +1062,7 @@ __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
               } else {
                 pass = x >= ts[e];
               }
-              if (pass) pmr[j] |= 1ull << (16 * b + r);
+              if (pass & cand_ok[j]) pmr[j] |= 1ull << (16 * b + r);
             }
           }
           // the group's compares complete here: otherwise they sink below
@@ -1059,10 +1070,6 @@ __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
           asm volatile("" : "+v"(pmr[0]), "+v"(pmr[1]));
           __builtin_amdgcn_sched_barrier(0);
         }
-      // candidates are valid, non-degenerate query columns only
-#pragma unroll
-      for (int j = 0; j < 2; ++j)
-        if (!qv[j] || (P.ref_deg && P.ref_deg[qc[j]])) pmr[j] = 0ull;
       // Gather the marked pairs per reference column in LDS (the stage
       // buffers are free once every wave's refills have landed): kMirrorSlots
       // per column in stage 0, further ones in a pool in stage 1.  Then one

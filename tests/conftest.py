@@ -38,7 +38,12 @@ def self_rows_from_cross(oi, od, qcols, k, include_self):
     at its restored value 0 in (value, index) order (R/sparseKNN.R:270-279)."""
     gi = np.empty((len(qcols), k), np.int32)
     gd = np.empty((len(qcols), k), np.float64)
+    na_int = np.iinfo(np.int32).min
     for t, q in enumerate(qcols):
+        if (oi[t] == na_int).all():  # degenerate query: an NA row either way
+            gi[t] = oi[t][:k]
+            gd[t] = od[t][:k]
+            continue
         pairs = [(d, i) for i, d in zip(oi[t].tolist(), od[t].tolist()) if i != q + 1]
         if include_self:
             pairs.append((0.0, q + 1))

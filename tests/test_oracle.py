@@ -156,10 +156,12 @@ def test_self_rows_from_cross_search(oracle_lib, method, include_self):
     X = rand_sparse(60, 150, 0.15, 31).tolil()
     for c in range(40, 70):
         X[:, c] = X[:, 3]  # 31 identical columns: ties wider than k + 1
+    X[:, 100] = 0  # an empty column: an NA row and never a candidate
     X = X.tocsc()
+    X.eliminate_zeros()
     k = 8
     o = oracle_lib.knn(X, k=k, method=method, include_self=include_self)
-    qcols = [0, 3, 5, 40, 41, 69, 70, 149]
+    qcols = [0, 3, 5, 40, 41, 69, 70, 100, 149]
     c = oracle_lib.knn(X[:, qcols], X, k=k + 1, method=method)
     gi, gd = self_rows_from_cross(c[0], c[1], qcols, k, include_self)
     assert np.array_equal(gi, o[0][qcols])
