@@ -517,6 +517,34 @@ def test_pearson_signed_constant_and_self(oracle_lib, gpu_ctx):
 
 
 @pytest.mark.parametrize("method", ["cosine", "euclidean"])
+def test_symmetric_self_filter_keeps_degenerate(oracle_lib, gpu_ctx, method):
+    """drop_degenerate = FALSE: empty columns are ordinary candidates and
+    queries of the symmetric pass (an empty column is at distance |q| in
+    euclidean; cosine takes the reference's 0 / 1 convention)."""
+    n, k = 20000, 20
+    Y = synth.nb_log1p(600, n, 0.05, 47).tolil()
+    for c in DEGENERATE_COLS:
+        Y[:, c] = 0
+    Y = Y.tocsc()
+    Y.eliminate_zeros()
+    gpu_ctx.stats(reset=True)
+    g = sd.sparseKNN(Y, k=k, method=method, drop_degenerate=False, precision="bf16",
+                     verbose=False)
+    st = gpu_ctx.stats(reset=True)
+    assert st["filter_pairs"] == n * (n + 1) // 2, st
+    rng = np.random.default_rng(11)
+    qcols = sorted(set(rng.choice(n, 300, replace=False).tolist()) | set(DEGENERATE_COLS))
+    o = oracle_lib.knn(Y[:, qcols], Y, k=k + 1, method=method, drop_degenerate=False,
+                       nthreads=NT)
+    same((g.idx[qcols], g.dist[qcols]), self_rows_from_cross(o[0], o[1], qcols, k, False),
+         method)
+    c = sd.sparseKNN(Y, Y.copy(), k=k + 1, method=method, drop_degenerate=False,
+                     precision="bf16", verbose=False)
+    same((g.idx, g.dist), self_rows_from_cross(c.idx, c.dist, list(range(n)), k, False),
+         f"{method}: symmetric vs full grid")
+
+
+@pytest.mark.parametrize("method", ["cosine", "euclidean"])
 def test_escalation_lists_from_candidate_buffers(oracle_lib, gpu_ctx, method):
     """A narrow re-rank margin (L = 24 at k = 20) makes the first certificate
     fail often; the wider lists (L = 64) of the failed queries then come from
