@@ -200,6 +200,20 @@ struct FusedKernelArgs {
   int qseg;
   float dot_e0, dot_e1;  // euclidean: dot error bound (e0 + e1 nnz(q)) sum |q_i r_i|
   int dot_nonneg;        // both matrices >= 0: that sum is the dot product
+  // filter mode, mirror (k_filter_wide with SYM): every value also screened
+  // for the reference column as a query of the mirrored search, whose
+  // per-query thresholds, counts and buffers (cap entries each) these are,
+  // with the query column as its candidate (mdeg: the candidates' degenerate
+  // flags).  tri = 1: a self-search, the mirrored search is this one and only
+  // tiles with rt >= qt run; tri = 0: a cross search, every tile runs and
+  // feeds the reverse search.
+  const double* mthr;
+  int mcap;
+  int32_t* mcnt;
+  double* mfkey;
+  int32_t* mfidx;
+  const uint8_t* mdeg;
+  int tri;
 };
 
 namespace {
@@ -636,10 +650,12 @@ struct WideLayout {
   __device__ static int swz(int row) { return (row >> kRowShift) & (kSlots - 1); }
 };
 
-// SYM: symmetric self-search (queries are the reference columns, one launch
-// over all of them, a symmetric key): only tiles with rt >= qt run, and a
-// tile with rt > qt also screens every value for the reference column as the
-// query and the query column as its candidate.
+// SYM: mirrored screening.  A self-search over all columns at once
+// (P.tri = 1) runs only tiles with rt >= qt, and a tile with rt > qt also
+// screens every value for the reference column as the query and the query
+// column as its candidate.  A cross search (P.tri = 0) runs every tile and
+// screens every value for the reverse search as well (the reference column
+// as its query, the query column as its candidate).
 template <class Op, class K, int RB, int MK, bool QS, bool SYM>
 __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
   using Lay = WideLayout<RB>;
@@ -678,13 +694,17 @@ __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
   // (a dispatch counts work-items in 32 bits); dispatch order is x-major
   const int64_t blk = (int64_t)blockIdx.y * gridDim.x + blockIdx.x;
   if constexpr (SYM) {
-    if (!map_tile_sym(blk, P.qtiles, P.rtiles, qt, rt)) return;
+    if (P.tri) {
+      if (!map_tile_sym(blk, P.qtiles, P.rtiles, qt, rt)) return;
+    } else {
+      if (!map_tile(blk, (int64_t)gridDim.x * gridDim.y, P.qtiles, P.rtiles, qt, rt)) return;
+    }
   } else {
     if (!map_tile(blk, (int64_t)gridDim.x * gridDim.y, P.qtiles, P.rtiles, qt, rt)) return;
   }
   const int64_t q_base = (int64_t)qt * TN;
   const int64_t r_base = rt * TM;
-  const bool mirror = SYM && rt > qt;  // workgroup-uniform
+  const bool mirror = SYM && (!P.tri || rt > qt);  // workgroup-uniform
   if (SYM && tid == 0) s_npool = 0;
   for (int t = tid; t < TM; t += NT) {
     const int64_t r = r_base + t;
@@ -696,7 +716,7 @@ __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
       if constexpr (SYM) {
         // reference r as a query (query position r): a float screen a little
         // below its threshold, as for the tile's own queries
-        const float f = (float)P.thr[r];
+        const float f = (float)P.mthr[r];
         s_rthr[t] = f - fabsf(f) * 9.5367431640625e-07f - 1e-30f;
         s_mcnt[t] = 0;
         if constexpr (MW) {
@@ -1019,7 +1039,7 @@ __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
 #pragma unroll
       for (int j = 0; j < 2; ++j) {
         const int64_t cq = qv[j] ? qc[j] : 0;  // in range for every lane
-        const int dg = P.ref_deg ? (int)P.ref_deg[cq] : 0;
+        const int dg = P.mdeg ? (int)P.mdeg[cq] : 0;
         cand_ok[j] = qv[j] & (dg == 0);
       }
       uint64_t pmr[2] = {0ull, 0ull};
@@ -1131,7 +1151,7 @@ __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
               if (ps < kPool)
                 pool[ps] = make_int4(__float_as_int(x), (int)qc[j], (int)qn[j], rloc);
               else if (r_base + rloc < P.nref)
-                atomicAdd(&P.cnt[r_base + rloc], P.cap + 1);
+                atomicAdd(&P.mcnt[r_base + rloc], P.mcap + 1);
             }
           }
         }
@@ -1140,19 +1160,19 @@ __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
         const int64_t rg = r_base + tid;
         const int nm = min(s_mcnt[tid], kMirrorSlots);
         if (nm > 0 && rg < P.nref) {
-          const double t = P.thr[rg];
+          const double t = P.mthr[rg];
           int keep = 0;
           for (int e = 0; e < nm; ++e)
             keep += mirror_key(m_x[tid * kMirrorSlots + e], tid, m_nnz[tid * kMirrorSlots + e]) >= t;
           if (keep > 0) {
-            int at = atomicAdd(&P.cnt[rg], keep);
+            int at = atomicAdd(&P.mcnt[rg], keep);
             for (int e = 0; e < nm; ++e) {
               const double kd =
                   mirror_key(m_x[tid * kMirrorSlots + e], tid, m_nnz[tid * kMirrorSlots + e]);
               if (kd < t) continue;
-              if (at < P.cap) {
-                P.fkey[rg * P.cap + at] = kd;
-                P.fidx[rg * P.cap + at] = m_idx[tid * kMirrorSlots + e];
+              if (at < P.mcap) {
+                P.mfkey[rg * P.mcap + at] = kd;
+                P.mfidx[rg * P.mcap + at] = m_idx[tid * kMirrorSlots + e];
               }
               ++at;
             }
@@ -1165,11 +1185,11 @@ __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
         const int64_t rg = r_base + v.w;
         if (rg >= P.nref) continue;
         const double kd = mirror_key(__int_as_float(v.x), v.w, (uint32_t)v.z);
-        if (kd >= P.thr[rg]) {
-          const int at = atomicAdd(&P.cnt[rg], 1);
-          if (at < P.cap) {
-            P.fkey[rg * P.cap + at] = kd;
-            P.fidx[rg * P.cap + at] = v.y;
+        if (kd >= P.mthr[rg]) {
+          const int at = atomicAdd(&P.mcnt[rg], 1);
+          if (at < P.mcap) {
+            P.mfkey[rg * P.mcap + at] = kd;
+            P.mfidx[rg * P.mcap + at] = v.y;
           }
         }
       }
@@ -1625,16 +1645,17 @@ static int launch_filter_t(sd_context* ctx, FusedKernelArgs& P) {
   P.qtiles = (int)((P.nqry + kWideTile - 1) / kWideTile);
   P.rtiles = (int)((P.nref + kWideTile - 1) / kWideTile);
   P.qgroups = (P.qtiles + 3) / 4;
-  if (SYM && (P.nqry != P.nref || P.qtiles != P.rtiles))
+  const bool tri = SYM && P.tri;
+  if (tri && (P.nqry != P.nref || P.qtiles != P.rtiles))
     return set_error(SD_EINVAL, "internal: symmetric pass needs the query set to be the references");
   // slots: groups of 4 query tiles x their reference tiles (from the group's
   // first tile on, for the symmetric pass)
   const int64_t G = P.qgroups, R = P.rtiles;
-  const int64_t need = SYM ? 4 * G * R - 8 * G * (G - 1) : G * 4 * R;
+  const int64_t need = tri ? 4 * G * R - 8 * G * (G - 1) : G * 4 * R;
   // x: a multiple of 8 (whole rounds of the XCD round-robin; 8 kSymChunk for
   // the symmetric order) small enough that x * 512 work-items fit a
   // dispatch's 32-bit grid size; y: the rest
-  const int64_t unit = SYM ? 8 * kSymChunk : 8;
+  const int64_t unit = tri ? 8 * kSymChunk : 8;
   const int64_t gx = std::min<int64_t>((need + unit - 1) / unit * unit, (int64_t)1 << 20);
   const int64_t gy = (need + gx - 1) / gx;
   if (gy > 65535) return set_error(SD_EUNSUPPORTED, "slab grid too large (%lld tiles)", (long long)need);
@@ -1801,15 +1822,39 @@ int fused_filter_pass(sd_context* ctx, const FusedArgs& a, const double* thr, in
   P.fkey = fkey;
   P.fidx = fidx;
   SD_HIP(hipMemsetAsync(cnt, 0, sizeof(int32_t) * (size_t)a.nqry, ctx->stream));
-  KernelTimer timer(ctx, &ctx->stats.filter_ms, &ctx->stats.filter_launches);
   // A self-search over all columns at once scores each unordered pair once:
   // the mirrored value is the same sum of products (on the split layout the
   // query side reads its segments swapped, so hi.lo and lo.hi are added in
   // the other order and the value may differ in the last bit, within the same
   // error bound), and each key uses its own query column's weights.
-  const bool sym = a.self_mode && !a.qlist && a.nqry == a.nref && a.qry_dense == a.ref_dense &&
+  const bool tri = a.self_mode && !a.qlist && a.nqry == a.nref && a.qry_dense == a.ref_dense &&
                    a.qry_p == a.ref_p && a.qry_aux == a.ref_aux && a.qry_aux2 == a.ref_aux2;
-  ctx->stats.filter_pairs += sym ? a.nqry * (a.nqry + 1) / 2 : a.nqry * a.nref;
+  // A cross search can feed the reverse search's buffers from the same tiles
+  // (cosine and Jaccard: keys without per-query weights).
+  const bool cross = !tri && a.mirror && !a.qlist &&
+                     (a.method == SD_COSINE || a.method == SD_BINARY);
+  const bool sym = tri || cross;
+  if (tri) {
+    P.tri = 1;
+    P.mthr = thr;
+    P.mcap = cap;
+    P.mcnt = cnt;
+    P.mfkey = fkey;
+    P.mfidx = fidx;
+    P.mdeg = a.ref_deg;
+  } else if (cross) {
+    P.tri = 0;
+    P.mthr = a.mirror->thr;
+    P.mcap = a.mirror->cap;
+    P.mcnt = a.mirror->cnt;
+    P.mfkey = a.mirror->fkey;
+    P.mfidx = a.mirror->fidx;
+    P.mdeg = a.mirror->deg;
+    SD_HIP(hipMemsetAsync(P.mcnt, 0, sizeof(int32_t) * (size_t)a.nref, ctx->stream));
+    a.mirror->filled = true;
+  }
+  KernelTimer timer(ctx, &ctx->stats.filter_ms, &ctx->stats.filter_launches);
+  ctx->stats.filter_pairs += tri ? a.nqry * (a.nqry + 1) / 2 : a.nqry * a.nref;
   constexpr int RB = kFilterRowBytes;
   if (a.method == SD_BINARY) {
     if (a.kind == kDenseF32Pattern)

@@ -257,7 +257,13 @@ int fused_candidate_lists(sd_context* ctx, FusedArgs fa, const double** out_key,
   // candidate count and costs less: 1/128 at rank ~L/4 from 512 tiles on,
   // 1/64 at rank ~L/2 from 256, 1/32 at rank L below (a sample of at least
   // 4 tiles).
+  CrossMirror& xm = ctx->xm;
   if (rtiles < 4 * 32) {
+    if (xm.phase == 1) {  // probe of a reverse search without a filtered pass
+      *out_key = nullptr;
+      *out_idx = nullptr;
+      return SD_OK;
+    }
     fa.tile_stride = 1;
     return list_pass(fa, ctx->scratch[3], ctx->scratch[4], ctx->scratch[5], ctx->scratch[6],
                      out_key, out_idx);
@@ -266,29 +272,75 @@ int fused_candidate_lists(sd_context* ctx, FusedArgs fa, const double** out_key,
   const int Lp = stride == 128 ? fused_list_len((L + 3) / 4)
                  : stride == 64 ? fused_list_len((L + 1) / 2)
                                 : L;
-  // 1. thresholds from every stride-th reference tile
-  const double* pk = nullptr;
-  const int32_t* pi = nullptr;
-  FusedArgs pre = fa;
-  pre.tile_stride = (int)stride;
-  pre.L = Lp;
-  SD_TRY(list_pass(pre, ctx->scratch[3], ctx->scratch[4], ctx->scratch[5], ctx->scratch[6], &pk,
-                   &pi));
-  DevBuf& thr = ctx->scratch[13];
-  SD_TRY(thr.ensure(sizeof(double) * (size_t)nq + 64));
-  SD_TRY(list_thresholds(ctx, pk, pi, nq, Lp, thr.as<double>()));
-  // 2. filtered full pass: room for three times the expected candidates
-  // (the count above the Lp-th of a sample varies by about 1/sqrt(Lp))
-  const int cap = (int)std::min<int64_t>(4096, 3 * Lp * stride + 256);
-  DevBuf& cnt = ctx->scratch[14];
-  DevBuf& fk = ctx->scratch[9];
-  DevBuf& fi = ctx->scratch[10];
-  SD_TRY(cnt.ensure(sizeof(int32_t) * (size_t)(nq + 16)));
-  SD_TRY(fk.ensure(sizeof(double) * (size_t)nq * (size_t)cap + 64));
-  SD_TRY(fi.ensure(sizeof(int32_t) * (size_t)nq * (size_t)cap + 64));
-  SD_TRY(check_cancel(ctx));
-  SD_TRY(fused_filter_pass(ctx, fa, thr.as<double>(), cap, cnt.as<int32_t>(), fk.as<double>(),
-                           fi.as<int32_t>()));
+  // room for three times the expected candidates (the count above the
+  // Lp-th of a sample varies by about 1/sqrt(Lp))
+  int cap = (int)std::min<int64_t>(4096, 3 * Lp * stride + 256);
+  // a search the cross mirror can serve: same layout, precision and method,
+  // all query columns, a key without per-query weights
+  auto mirrorable = [&](const FusedArgs& f) {
+    return xm.ready && !f.qlist && f.kind == xm.kind && f.kpad == xm.kpad && f.prec == xm.prec &&
+           f.method == xm.method;
+  };
+  DevBuf* cntb = &ctx->scratch[14];
+  DevBuf* fkb = &ctx->scratch[9];
+  DevBuf* fib = &ctx->scratch[10];
+  if (xm.phase == 3 && xm.target.filled && mirrorable(fa) && fa.L == xm.L && fa.nqry == xm.nqry &&
+      fa.nref == xm.nref) {
+    // the reverse search of a cross pair: its buffers were filled by the
+    // forward search's filtered pass
+    cntb = &xm.cnt;
+    fkb = &xm.fkey;
+    fib = &xm.fidx;
+    cap = xm.target.cap;
+  } else {
+    // 1. thresholds from every stride-th reference tile
+    const double* pk = nullptr;
+    const int32_t* pi = nullptr;
+    FusedArgs pre = fa;
+    pre.tile_stride = (int)stride;
+    pre.L = Lp;
+    SD_TRY(list_pass(pre, ctx->scratch[3], ctx->scratch[4], ctx->scratch[5], ctx->scratch[6], &pk,
+                     &pi));
+    DevBuf& thr = xm.phase == 1 ? xm.thr : ctx->scratch[13];
+    SD_TRY(thr.ensure(sizeof(double) * (size_t)nq + 64));
+    SD_TRY(list_thresholds(ctx, pk, pi, nq, Lp, thr.as<double>()));
+    if (xm.phase == 1) {
+      // probe of the reverse search: keep its thresholds and buffers for the
+      // forward search's filtered pass
+      if (!fa.qlist && (fa.method == SD_COSINE || fa.method == SD_BINARY)) {
+        SD_TRY(xm.cnt.ensure(sizeof(int32_t) * (size_t)(nq + 16)));
+        SD_TRY(xm.fkey.ensure(sizeof(double) * (size_t)nq * (size_t)cap + 64));
+        SD_TRY(xm.fidx.ensure(sizeof(int32_t) * (size_t)nq * (size_t)cap + 64));
+        xm.L = L;
+        xm.kind = fa.kind;
+        xm.prec = fa.prec;
+        xm.method = fa.method;
+        xm.kpad = fa.kpad;
+        xm.nqry = nq;
+        xm.nref = fa.nref;
+        xm.target = MirrorTarget{thr.as<double>(), cap, xm.cnt.as<int32_t>(), xm.fkey.as<double>(),
+                                 xm.fidx.as<int32_t>(), fa.ref_deg, false};
+        xm.ready = true;
+      }
+      *out_key = nullptr;
+      *out_idx = nullptr;
+      return SD_OK;
+    }
+    // 2. filtered full pass (a cross pair's forward search also feeds the
+    // reverse search's buffers)
+    SD_TRY(cntb->ensure(sizeof(int32_t) * (size_t)(nq + 16)));
+    SD_TRY(fkb->ensure(sizeof(double) * (size_t)nq * (size_t)cap + 64));
+    SD_TRY(fib->ensure(sizeof(int32_t) * (size_t)nq * (size_t)cap + 64));
+    SD_TRY(check_cancel(ctx));
+    FusedArgs ff = fa;
+    if (xm.phase == 2 && mirrorable(fa) && xm.nqry == fa.nref && xm.nref == fa.nqry)
+      ff.mirror = &xm.target;
+    SD_TRY(fused_filter_pass(ctx, ff, thr.as<double>(), cap, cntb->as<int32_t>(),
+                             fkb->as<double>(), fib->as<int32_t>()));
+  }
+  DevBuf& cnt = *cntb;
+  DevBuf& fk = *fkb;
+  DevBuf& fi = *fib;
   // 3. per-query top-L of the appended candidates
   DevBuf& mk = ctx->scratch[5];
   DevBuf& mi = ctx->scratch[6];
@@ -517,6 +569,7 @@ int run_knn(sd_context* ctx, sd_matrix* X, sd_matrix* Y, const sd_knn_params* p,
     const int32_t* lidx = nullptr;
     CandBuffers cb;
     SD_TRY(fused_candidate_lists(ctx, fa, &lkey, &lidx, &cb));
+    if (ctx->xm.phase == 1) return SD_OK;  // probe of a cross pair's reverse search
     if (method == SD_BINARY) {
       SD_TRY(finish_exact_lists(ctx, lkey, lidx, nq, L, k, p->dist,
                                 p->drop_degenerate ? X->deg.as<uint8_t>() : nullptr,
@@ -663,6 +716,7 @@ int run_knn(sd_context* ctx, sd_matrix* X, sd_matrix* Y, const sd_knn_params* p,
     }
   } else {
     // exact FP64 path over every query column
+    if (ctx->xm.phase == 1) return SD_OK;  // probe of a cross pair's reverse search
     ++ctx->stats.exact_calls;
     DevBuf& qc = ctx->scratch[7];
     SD_TRY(qc.ensure(sizeof(int32_t) * (size_t)(nq + 16)));
@@ -927,10 +981,36 @@ int sd_knn_cross(sd_context* ctx, const sd_matrix* X, const sd_matrix* Y,
     *warn = (pf.warn ? 1 : 0) | (pr.warn ? 2 : 0);
     *k_fwd = pf.k;
     *k_rev = pr.k;
-    SD_TRY(run_knn(ctx, const_cast<sd_matrix*>(X), const_cast<sd_matrix*>(Y), params, pf, fwd_idx,
-                   fwd_dist));
-    return run_knn(ctx, const_cast<sd_matrix*>(Y), const_cast<sd_matrix*>(X), params, pr, rev_idx,
-                   rev_dist);
+    sd_matrix* Xm = const_cast<sd_matrix*>(X);
+    sd_matrix* Ym = const_cast<sd_matrix*>(Y);
+    // One filtered pass for both directions when both are large enough for
+    // one (at least 128 reference tiles of 128 columns each way) and the key
+    // needs no per-query weights: probe the reverse search (its thresholds
+    // and buffers), run the forward search, whose filtered pass also feeds
+    // the reverse buffers, and resume the reverse search from them.  When
+    // the searches take other paths, each runs on its own as before.
+    CrossMirror& xm = ctx->xm;
+    struct Reset {
+      CrossMirror& m;
+      ~Reset() {
+        m.phase = 0;
+        m.ready = false;
+        m.target.filled = false;
+      }
+    } reset{xm};
+    xm.ready = false;
+    xm.target.filled = false;
+    if ((params->method == SD_COSINE || params->method == SD_BINARY) &&
+        params->precision != SD_PREC_EXACT && X->ncol >= 128 * 128 && Y->ncol >= 128 * 128) {
+      const int64_t fused = ctx->stats.fused_calls;
+      xm.phase = 1;
+      SD_TRY(run_knn(ctx, Ym, Xm, params, pr, rev_idx, rev_dist));  // writes no results
+      ctx->stats.fused_calls = fused;                               // counted when it resumes
+      xm.phase = 2;
+    }
+    SD_TRY(run_knn(ctx, Xm, Ym, params, pf, fwd_idx, fwd_dist));
+    if (xm.phase == 2) xm.phase = 3;
+    return run_knn(ctx, Ym, Xm, params, pr, rev_idx, rev_dist);
   });
 }
 

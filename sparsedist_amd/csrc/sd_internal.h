@@ -90,6 +90,36 @@ struct TuneRates {
 };
 }  // namespace sd
 
+namespace sd {
+
+// The candidate buffers of a second search, filled by the filtered pass of a
+// cross search in the other direction: S(y, x) is S(x, y), so each value of
+// the forward pass is also screened for the reverse search.
+struct MirrorTarget {
+  const double* thr = nullptr;   // per query of the reverse search
+  int cap = 0;                   // buffer entries per query
+  int32_t* cnt = nullptr;        // appended per query (zeroed by the pass)
+  double* fkey = nullptr;        // [nqry][cap]
+  int32_t* fidx = nullptr;
+  const uint8_t* deg = nullptr;  // degenerate flags of its candidates (may be null)
+  mutable bool filled = false;   // set by the pass that fed it
+};
+
+// A cross search run as one filtered pass (sd_knn_cross): the reverse
+// search is probed first (its thresholds and buffers, no outputs), the
+// forward search's filtered pass feeds those buffers, and the reverse search
+// then resumes from them.  Any mismatch leaves the searches independent.
+struct CrossMirror {
+  int phase = 0;       // 0 off, 1 probe the reverse search, 2 forward, 3 resume the reverse
+  bool ready = false;  // the probe set thresholds and buffers
+  DevBuf thr, cnt, fkey, fidx;
+  int L = 0, kind = -1, prec = 0, method = -1;
+  int64_t kpad = 0, nqry = 0, nref = 0;
+  MirrorTarget target;
+};
+
+}  // namespace sd
+
 struct sd_context {
   int device = 0;
   sd::TuneRates tune;
@@ -103,6 +133,7 @@ struct sd_context {
   sd::DevBuf scratch[24];
   // the SNN builder's temporaries, one slot each (grow-only)
   sd::DevBuf snn_scratch[24];
+  sd::CrossMirror xm;
   std::vector<uint8_t> host_scratch;
 };
 
@@ -225,6 +256,7 @@ struct FusedArgs {
   int L = 0;                        // list length kept per query
   int splits = 1;
   int tile_stride = 1;              // list mode: only every tile_stride-th reference tile
+  const MirrorTarget* mirror = nullptr;  // filtered pass: also feed the reverse search
 };
 // Candidate lists: per (split, query) L entries of (key double, idx int32),
 // key larger = better, sorted best first; unfilled slots idx = -1.

@@ -516,6 +516,47 @@ def test_pearson_signed_constant_and_self(oracle_lib, gpu_ctx):
         same((g.idx, g.dist), o, f"cross dist={dist}")
 
 
+@pytest.mark.parametrize("method", ["binary", "cosine"])
+def test_cross_search_one_filtered_pass(oracle_lib, gpu_ctx, method):
+    """sparseKNNCross with both sides large enough for a filtered pass: one
+    pass over the X x Y tiles feeds both directions (the reverse search's
+    buffers through the mirror, with its own thresholds).  A tied block in Y
+    overflows forward buffers and empty columns on both sides test the
+    degenerate rules.  Both directions must equal two independent sparseKNN
+    calls on every row, and sampled rows the oracle."""
+    nx, ny, k = 17000, 20000, 20
+    X = synth.nb_log1p(600, nx, 0.05, 48).tolil()
+    Y = synth.nb_log1p(600, ny, 0.05, 49).tolil()
+    for c in range(2000, 6500):
+        Y[:, c] = X[:, 5]
+    for c in (3, 16999):
+        X[:, c] = 0
+    for c in (10, 19999):
+        Y[:, c] = 0
+    X, Y = X.tocsc(), Y.tocsc()
+    X.eliminate_zeros()
+    Y.eliminate_zeros()
+    gpu_ctx.stats(reset=True)
+    gpu_ctx.set_timing(True)  # launches are counted while timing is on
+    try:
+        cr = sd.sparseKNNCross(X, Y, k=k, method=method, precision="bf16", verbose=False)
+    finally:
+        gpu_ctx.set_timing(False)
+    st = gpu_ctx.stats(reset=True)
+    assert st["filter_launches"] == 1 and st["filter_pairs"] == nx * ny, st
+    fw = sd.sparseKNN(X, Y, k=k, method=method, precision="bf16", verbose=False)
+    rv = sd.sparseKNN(Y, X, k=k, method=method, precision="bf16", verbose=False)
+    same((cr["forward"].idx, cr["forward"].dist), (fw.idx, fw.dist), "forward")
+    same((cr["reverse"].idx, cr["reverse"].dist), (rv.idx, rv.dist), "reverse")
+    rng = np.random.default_rng(12)
+    sf = sorted(set(rng.choice(nx, 200, replace=False).tolist()) | {3, 5, 16999})
+    sr = sorted(set(rng.choice(ny, 200, replace=False).tolist()) | {10, 2000, 6499, 19999})
+    of = oracle_lib.knn(X[:, sf], Y, k=k, method=method, nthreads=NT)
+    orv = oracle_lib.knn(Y[:, sr], X, k=k, method=method, nthreads=NT)
+    same((cr["forward"].idx[sf], cr["forward"].dist[sf]), of, "forward vs oracle")
+    same((cr["reverse"].idx[sr], cr["reverse"].dist[sr]), orv, "reverse vs oracle")
+
+
 @pytest.mark.parametrize("method", ["cosine", "euclidean"])
 def test_symmetric_self_filter_keeps_degenerate(oracle_lib, gpu_ctx, method):
     """drop_degenerate = FALSE: empty columns are ordinary candidates and
