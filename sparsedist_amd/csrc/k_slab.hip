@@ -677,6 +677,7 @@ __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
   __shared__ float s_raux2[TM];
   __shared__ uint8_t s_rdeg[TM];
   __shared__ __align__(16) float s_rthr[SYM ? TM : 4];  // mirror screen thresholds
+  __shared__ double s_rthrd[SYM ? TM : 1];                // mirror exact thresholds
   __shared__ int32_t s_mcnt[SYM ? TM : 1];                // mirror hits per reference
   __shared__ int32_t s_npool;                              // mirror hits past a column's slots
   // euclidean / pearson: the reference column's key weights as a query
@@ -716,8 +717,10 @@ __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
       if constexpr (SYM) {
         // reference r as a query (query position r): a float screen a little
         // below its threshold, as for the tile's own queries
-        const float f = (float)P.mthr[r];
+        const double td = P.mthr[r];
+        const float f = (float)td;
         s_rthr[t] = f - fabsf(f) * 9.5367431640625e-07f - 1e-30f;
+        s_rthrd[t] = td;
         s_mcnt[t] = 0;
         if constexpr (MW) {
           const KeyWeights w =
@@ -736,6 +739,7 @@ __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
       s_rdeg[t] = 1;
       if constexpr (SYM) {
         s_rthr[t] = INFINITY;
+        s_rthrd[t] = INFINITY;
         s_mcnt[t] = 0;
       }
       if constexpr (MW) {
@@ -1160,7 +1164,7 @@ __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
         const int64_t rg = r_base + tid;
         const int nm = min(s_mcnt[tid], kMirrorSlots);
         if (nm > 0 && rg < P.nref) {
-          const double t = P.mthr[rg];
+          const double t = s_rthrd[tid];
           int keep = 0;
           for (int e = 0; e < nm; ++e)
             keep += mirror_key(m_x[tid * kMirrorSlots + e], tid, m_nnz[tid * kMirrorSlots + e]) >= t;
@@ -1185,7 +1189,7 @@ __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
         const int64_t rg = r_base + v.w;
         if (rg >= P.nref) continue;
         const double kd = mirror_key(__int_as_float(v.x), v.w, (uint32_t)v.z);
-        if (kd >= P.mthr[rg]) {
+        if (kd >= s_rthrd[v.w]) {
           const int at = atomicAdd(&P.mcnt[rg], 1);
           if (at < P.mcap) {
             P.mfkey[rg * P.mcap + at] = kd;
