@@ -139,7 +139,7 @@ def main():
         print(json.dumps({"copy_bytes_read": a.numel() * 4, "copy_bytes_written": b.numel() * 4}))
         return
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
               "(the query-sharded multi-GPU design is described but not run).", file=sys.stderr)
         sys.exit(2)
 

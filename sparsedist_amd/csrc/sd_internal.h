@@ -332,7 +332,7 @@ int rerank_cosine(sd_context* ctx, const RerankArgs& R);
 // arithmetic; the certificate compares the k-th squared distance with the
 // lower bound that the list's last key stands for.
 int rerank_euclid(sd_context* ctx, const RerankArgs& R);
-// Pearson: exact FP64 correlations (this build's centred definition); the
+// Pearson: exact FP64 correlations (the library's centred definition); the
 // certificate compares the k-th correlation with the upper bound that the
 // list's last key stands for.
 int rerank_pearson(sd_context* ctx, const RerankArgs& R);

@@ -481,7 +481,7 @@ def test_euclidean_out_of_range_values_take_exact_path(oracle_lib, gpu_ctx):
 def test_pearson_count_matrix_certified(oracle_lib, gpu_ctx, prec, dist):
     """Pearson on the count-matrix rows and density: raw-value slab (split
     BF16 or FP32), per-pair upper-bound keys of the correlation, FP64
-    re-rank in this build's centred definition."""
+    re-rank in the library's centred definition."""
     X = synth.nb_log1p(20000, 1200, 0.06, 12)
     gpu_ctx.stats(reset=True)
     g = sd.sparseKNN(X, k=20, method="pearson", dist=dist, precision=prec, verbose=False)
