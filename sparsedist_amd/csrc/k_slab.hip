@@ -1394,41 +1394,49 @@ __global__ __launch_bounds__(256) void k_rerank(RerankArgs R) {
           const int32_t* ra = R.ex.ref.i + a0;
           const double* ua = R.ex.ref_unit + a0;
           double acc = 0.0;
-          // entries in ascending row order, so the FP64 sum keeps the
-          // reference's order; the row indices stream in 16-byte loads (the
-          // index array is 256-byte aligned, so entry a0 + e is 16-byte
-          // aligned when (a0 + e) % 4 == 0) and a unit value is read only for
-          // a row the query also stores
-          auto step = [&](int32_t r, int64_t e) {
-            const uint32_t word = bm[r >> 5];
-            const uint32_t bit = 1u << (r & 31);
-            if (word & bit) acc += ua[e] * qv[pre[r >> 5] + __popc(word & (bit - 1u))];
-          };
-          int64_t e = 0;
-          const int64_t head = min(na, (int64_t)((4 - (a0 & 3)) & 3));
-          for (; e < head; ++e) step(ra[e], e);
-          // eight 16-byte loads in flight per lane before the walk uses them
-          for (; e + 32 <= na; e += 32) {
-            const int4* r32 = reinterpret_cast<const int4*>(ra + e);
-            int4 x[8];
+          if (na == R.ex.ref.nrow && qn == na) {
+            // both columns store every row: the merge visits rows 0 .. n-1
+            // in order, so the FP64 sum runs over the value arrays directly
+            // (the same products in the same order; no index reads)
+#pragma unroll 8
+            for (int64_t e = 0; e < na; ++e) acc += ua[e] * qv[e];
+          } else {
+            // entries in ascending row order, so the FP64 sum keeps the
+            // reference's order; the row indices stream in 16-byte loads (the
+            // index array is 256-byte aligned, so entry a0 + e is 16-byte
+            // aligned when (a0 + e) % 4 == 0) and a unit value is read only for
+            // a row the query also stores
+            auto step = [&](int32_t r, int64_t e) {
+              const uint32_t word = bm[r >> 5];
+              const uint32_t bit = 1u << (r & 31);
+              if (word & bit) acc += ua[e] * qv[pre[r >> 5] + __popc(word & (bit - 1u))];
+            };
+            int64_t e = 0;
+            const int64_t head = min(na, (int64_t)((4 - (a0 & 3)) & 3));
+            for (; e < head; ++e) step(ra[e], e);
+            // eight 16-byte loads in flight per lane before the walk uses them
+            for (; e + 32 <= na; e += 32) {
+              const int4* r32 = reinterpret_cast<const int4*>(ra + e);
+              int4 x[8];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) x[u] = r32[u];
+              for (int u = 0; u < 8; ++u) x[u] = r32[u];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
-              step(x[u].x, e + 4 * u);
-              step(x[u].y, e + 4 * u + 1);
-              step(x[u].z, e + 4 * u + 2);
-              step(x[u].w, e + 4 * u + 3);
+              for (int u = 0; u < 8; ++u) {
+                step(x[u].x, e + 4 * u);
+                step(x[u].y, e + 4 * u + 1);
+                step(x[u].z, e + 4 * u + 2);
+                step(x[u].w, e + 4 * u + 3);
+              }
             }
+            for (; e + 4 <= na; e += 4) {
+              const int4 r4 = *reinterpret_cast<const int4*>(ra + e);
+              step(r4.x, e);
+              step(r4.y, e + 1);
+              step(r4.z, e + 2);
+              step(r4.w, e + 3);
+            }
+            for (; e < na; ++e) step(ra[e], e);
           }
-          for (; e + 4 <= na; e += 4) {
-            const int4 r4 = *reinterpret_cast<const int4*>(ra + e);
-            step(r4.x, e);
-            step(r4.y, e + 1);
-            step(r4.z, e + 2);
-            step(r4.w, e + 3);
-          }
-          for (; e < na; ++e) step(ra[e], e);
           c = acc;
           if (!(c == c)) c = 0.0;
           if (c > 1.0) c = 1.0;
