@@ -1841,10 +1841,13 @@ int fused_filter_pass(sd_context* ctx, const FusedArgs& a, const double* thr, in
   // error bound), and each key uses its own query column's weights.
   const bool tri = a.self_mode && !a.qlist && a.nqry == a.nref && a.qry_dense == a.ref_dense &&
                    a.qry_p == a.ref_p && a.qry_aux == a.ref_aux && a.qry_aux2 == a.ref_aux2;
-  // A cross search can feed the reverse search's buffers from the same tiles
-  // (cosine and Jaccard: keys without per-query weights).
+  // A cross search can feed the reverse search's buffers from the same tiles;
+  // euclidean and pearson keys take the reverse query's weights from the
+  // same dot error bound, which both directions must share.
   const bool cross = !tri && a.mirror && !a.qlist &&
-                     (a.method == SD_COSINE || a.method == SD_BINARY);
+                     (a.method == SD_COSINE || a.method == SD_BINARY ||
+                      (a.mirror->e0 == a.dot_e0 && a.mirror->e1 == a.dot_e1 &&
+                       a.mirror->nonneg == a.dot_nonneg));
   const bool sym = tri || cross;
   if (tri) {
     P.tri = 1;

@@ -307,7 +307,7 @@ int fused_candidate_lists(sd_context* ctx, FusedArgs fa, const double** out_key,
     if (xm.phase == 1) {
       // probe of the reverse search: keep its thresholds and buffers for the
       // forward search's filtered pass
-      if (!fa.qlist && (fa.method == SD_COSINE || fa.method == SD_BINARY)) {
+      if (!fa.qlist) {
         SD_TRY(xm.cnt.ensure(sizeof(int32_t) * (size_t)(nq + 16)));
         SD_TRY(xm.fkey.ensure(sizeof(double) * (size_t)nq * (size_t)cap + 64));
         SD_TRY(xm.fidx.ensure(sizeof(int32_t) * (size_t)nq * (size_t)cap + 64));
@@ -318,8 +318,10 @@ int fused_candidate_lists(sd_context* ctx, FusedArgs fa, const double** out_key,
         xm.kpad = fa.kpad;
         xm.nqry = nq;
         xm.nref = fa.nref;
-        xm.target = MirrorTarget{thr.as<double>(), cap, xm.cnt.as<int32_t>(), xm.fkey.as<double>(),
-                                 xm.fidx.as<int32_t>(), fa.ref_deg, false};
+        xm.target = MirrorTarget{thr.as<double>(), cap,       xm.cnt.as<int32_t>(),
+                                 xm.fkey.as<double>(), xm.fidx.as<int32_t>(), fa.ref_deg,
+                                 false,             fa.dot_e0, fa.dot_e1,
+                                 fa.dot_nonneg};
         xm.ready = true;
       }
       *out_key = nullptr;
@@ -984,11 +986,11 @@ int sd_knn_cross(sd_context* ctx, const sd_matrix* X, const sd_matrix* Y,
     sd_matrix* Xm = const_cast<sd_matrix*>(X);
     sd_matrix* Ym = const_cast<sd_matrix*>(Y);
     // One filtered pass for both directions when both are large enough for
-    // one (at least 128 reference tiles of 128 columns each way) and the key
-    // needs no per-query weights: probe the reverse search (its thresholds
-    // and buffers), run the forward search, whose filtered pass also feeds
-    // the reverse buffers, and resume the reverse search from them.  When
-    // the searches take other paths, each runs on its own as before.
+    // one (at least 128 reference tiles of 128 columns each way): probe the
+    // reverse search (its thresholds and buffers), run the forward search,
+    // whose filtered pass also feeds the reverse buffers, and resume the
+    // reverse search from them.  When the searches take other paths (or
+    // other layouts), each runs on its own as before.
     CrossMirror& xm = ctx->xm;
     struct Reset {
       CrossMirror& m;
@@ -1000,8 +1002,7 @@ int sd_knn_cross(sd_context* ctx, const sd_matrix* X, const sd_matrix* Y,
     } reset{xm};
     xm.ready = false;
     xm.target.filled = false;
-    if ((params->method == SD_COSINE || params->method == SD_BINARY) &&
-        params->precision != SD_PREC_EXACT && X->ncol >= 128 * 128 && Y->ncol >= 128 * 128) {
+    if (params->precision != SD_PREC_EXACT && X->ncol >= 128 * 128 && Y->ncol >= 128 * 128) {
       const int64_t fused = ctx->stats.fused_calls;
       xm.phase = 1;
       SD_TRY(run_knn(ctx, Ym, Xm, params, pr, rev_idx, rev_dist));  // writes no results

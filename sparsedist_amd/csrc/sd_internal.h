@@ -103,6 +103,10 @@ struct MirrorTarget {
   int32_t* fidx = nullptr;
   const uint8_t* deg = nullptr;  // degenerate flags of its candidates (may be null)
   mutable bool filled = false;   // set by the pass that fed it
+  // euclidean / pearson: the reverse search's dot error bound constants; the
+  // pass feeds it only when they equal its own (its keys use them)
+  float e0 = 0.f, e1 = 0.f;
+  int nonneg = 0;
 };
 
 // A cross search run as one filtered pass (sd_knn_cross): the reverse

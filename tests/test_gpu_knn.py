@@ -516,14 +516,15 @@ def test_pearson_signed_constant_and_self(oracle_lib, gpu_ctx):
         same((g.idx, g.dist), o, f"cross dist={dist}")
 
 
-@pytest.mark.parametrize("method", ["binary", "cosine"])
+@pytest.mark.parametrize("method", ["binary", "cosine", "euclidean", "pearson"])
 def test_cross_search_one_filtered_pass(oracle_lib, gpu_ctx, method):
     """sparseKNNCross with both sides large enough for a filtered pass: one
     pass over the X x Y tiles feeds both directions (the reverse search's
     buffers through the mirror, with its own thresholds).  A tied block in Y
     overflows forward buffers and empty columns on both sides test the
-    degenerate rules.  Both directions must equal two independent sparseKNN
-    calls on every row, and sampled rows the oracle."""
+    degenerate rules; euclidean and pearson keys use the reverse query's own
+    weights.  Both directions must equal two independent sparseKNN calls on
+    every row, and sampled rows the oracle."""
     nx, ny, k = 17000, 20000, 20
     X = synth.nb_log1p(600, nx, 0.05, 48).tolil()
     Y = synth.nb_log1p(600, ny, 0.05, 49).tolil()
