@@ -198,8 +198,8 @@ int sd_knn(sd_context* ctx, const sd_matrix* X, const sd_matrix* Y, const sd_knn
  * call (the reference's equivalence contract, R/sparseKNNCross.R:33-40).
  * Replaces the slab + topKBlock + topKRowsAccum (src/topKRows.cpp:104-248)
  * loop.  When both sides are large enough for a filtered pass (at least
- * 16384 columns each) and the method is cosine or binary, one pass over the
- * X x Y tiles serves both directions (the slab is the same, transposed).
+ * 16384 columns each) and both directions take the same layout, one pass
+ * over the X x Y tiles serves both (the slab is the same, transposed).
  * warn bit 0: forward k clamped; bit 1: reverse k clamped. */
 int sd_knn_cross(sd_context* ctx, const sd_matrix* X, const sd_matrix* Y,
                  const sd_knn_params* params, int32_t* fwd_idx, double* fwd_dist, int32_t* k_fwd,
