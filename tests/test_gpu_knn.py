@@ -516,8 +516,10 @@ def test_pearson_signed_constant_and_self(oracle_lib, gpu_ctx):
         same((g.idx, g.dist), o, f"cross dist={dist}")
 
 
-@pytest.mark.parametrize("method", ["binary", "cosine", "euclidean", "pearson"])
-def test_cross_search_one_filtered_pass(oracle_lib, gpu_ctx, method):
+@pytest.mark.parametrize("method,dist,drop", [("binary", True, True), ("cosine", True, True),
+                                              ("euclidean", True, True), ("pearson", True, True),
+                                              ("cosine", False, False)])
+def test_cross_search_one_filtered_pass(oracle_lib, gpu_ctx, method, dist, drop):
     """sparseKNNCross with both sides large enough for a filtered pass: one
     pass over the X x Y tiles feeds both directions (the reverse search's
     buffers through the mirror, with its own thresholds).  A tied block in Y
@@ -539,21 +541,22 @@ def test_cross_search_one_filtered_pass(oracle_lib, gpu_ctx, method):
     Y.eliminate_zeros()
     gpu_ctx.stats(reset=True)
     gpu_ctx.set_timing(True)  # launches are counted while timing is on
+    flags = dict(method=method, dist=dist, drop_degenerate=drop)
     try:
-        cr = sd.sparseKNNCross(X, Y, k=k, method=method, precision="bf16", verbose=False)
+        cr = sd.sparseKNNCross(X, Y, k=k, precision="bf16", verbose=False, **flags)
     finally:
         gpu_ctx.set_timing(False)
     st = gpu_ctx.stats(reset=True)
     assert st["filter_launches"] == 1 and st["filter_pairs"] == nx * ny, st
-    fw = sd.sparseKNN(X, Y, k=k, method=method, precision="bf16", verbose=False)
-    rv = sd.sparseKNN(Y, X, k=k, method=method, precision="bf16", verbose=False)
+    fw = sd.sparseKNN(X, Y, k=k, precision="bf16", verbose=False, **flags)
+    rv = sd.sparseKNN(Y, X, k=k, precision="bf16", verbose=False, **flags)
     same((cr["forward"].idx, cr["forward"].dist), (fw.idx, fw.dist), "forward")
     same((cr["reverse"].idx, cr["reverse"].dist), (rv.idx, rv.dist), "reverse")
     rng = np.random.default_rng(12)
     sf = sorted(set(rng.choice(nx, 200, replace=False).tolist()) | {3, 5, 16999})
     sr = sorted(set(rng.choice(ny, 200, replace=False).tolist()) | {10, 2000, 6499, 19999})
-    of = oracle_lib.knn(X[:, sf], Y, k=k, method=method, nthreads=NT)
-    orv = oracle_lib.knn(Y[:, sr], X, k=k, method=method, nthreads=NT)
+    of = oracle_lib.knn(X[:, sf], Y, k=k, nthreads=NT, **flags)
+    orv = oracle_lib.knn(Y[:, sr], X, k=k, nthreads=NT, **flags)
     same((cr["forward"].idx[sf], cr["forward"].dist[sf]), of, "forward vs oracle")
     same((cr["reverse"].idx[sr], cr["reverse"].dist[sr]), orv, "reverse vs oracle")
 
