@@ -1317,6 +1317,112 @@ __device__ inline uint64_t wave_shfl_u64(uint64_t v, int src) {
          ((uint64_t)(uint32_t)__shfl((int)hi, src) << 32);
 }
 
+// Wave-level top-L (L <= 64) of the candidate buffers, one wave per list
+// position, in BlockTopK's order: (order key, index) with the lower index
+// winning ties, non-finite values never admitted.  Lane i holds the i-th best
+// entry so far.  Candidates arrive 64 at a time; a chunk with none better
+// than the current L-th is skipped, otherwise it is sorted across the wave
+// (bitonic), merged with the list (the elementwise minimum against the
+// reversed chunk is a bitonic sequence holding the 64 best of both) and
+// cleaned back into order.  Over-full and short buffers are listed for a full
+// pass as in k_select.
+__global__ __launch_bounds__(256) void k_select_wave(
+    const double* __restrict__ fkey, const int32_t* __restrict__ fidx,
+    const int32_t* __restrict__ cnt, int cap, int64_t nqry, int L, double* __restrict__ out_key,
+    int32_t* __restrict__ out_idx, int32_t* __restrict__ over_list,
+    int32_t* __restrict__ over_count, int min_count, const int32_t* __restrict__ qlist) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t b = (int64_t)blockIdx.x * 4 + wave;
+  if (b >= nqry) return;  // whole waves leave; no barrier follows
+  const int64_t q = qlist ? (int64_t)qlist[b] : b;
+  const int n = cnt[q];
+  if (n > cap || n < min_count) {
+    if (lane == 0) {
+      const int at = atomicAdd(over_count, 1);
+      if (over_list) over_list[at] = (int32_t)q;
+    }
+    if (lane < L) out_idx[b * L + lane] = -1;
+    return;
+  }
+  auto less = [](uint64_t ka, uint32_t ia, uint64_t kb, uint32_t ib) {
+    return ka < kb || (ka == kb && ia < ib);
+  };
+  constexpr uint64_t kNone = ~0ULL;  // no finite value maps to it
+  uint64_t lk = kNone;
+  uint32_t li = 0xFFFFFFFFu;
+  double lv = 0.0;
+  const double* kb = fkey + q * cap;
+  const int32_t* ib = fidx + q * cap;
+  for (int base = 0; base < n; base += 64) {
+    const int t = base + lane;
+    uint64_t ck = kNone;
+    uint32_t ci = 0xFFFFFFFFu;
+    double cv = 0.0;
+    if (t < n) {
+      const double v = kb[t];
+      if (isfinite(v)) {
+        cv = v;
+        ck = order_key(v, 1);
+        ci = (uint32_t)ib[t];
+      }
+    }
+    const uint64_t tk = wave_shfl_u64(lk, L - 1);
+    const uint32_t ti = (uint32_t)__shfl((int)li, L - 1);
+    if (!less(ck, ci, tk, ti)) {
+      ck = kNone;
+      ci = 0xFFFFFFFFu;
+    }
+    if (__ballot(ck != kNone) == 0) continue;
+    // sort the chunk ascending across the wave
+    for (int size = 2; size <= 64; size <<= 1) {
+      for (int stride = size >> 1; stride > 0; stride >>= 1) {
+        const int partner = lane ^ stride;
+        const uint64_t pk = wave_shfl_u64(ck, partner);
+        const uint32_t pi = (uint32_t)__shfl((int)ci, partner);
+        const double pv = __shfl(cv, partner);
+        const bool lo = (lane & stride) == 0;
+        const bool asc = (lane & size) == 0;
+        const bool mine_less = less(ck, ci, pk, pi);
+        if (mine_less != (lo == asc)) {
+          ck = pk;
+          ci = pi;
+          cv = pv;
+        }
+      }
+    }
+    // the 64 best of list and chunk: elementwise minimum against the
+    // reversed chunk, then a bitonic clean into ascending order
+    {
+      const uint64_t rk = wave_shfl_u64(ck, 63 - lane);
+      const uint32_t ri = (uint32_t)__shfl((int)ci, 63 - lane);
+      const double rv = __shfl(cv, 63 - lane);
+      if (less(rk, ri, lk, li)) {
+        lk = rk;
+        li = ri;
+        lv = rv;
+      }
+    }
+    for (int stride = 32; stride > 0; stride >>= 1) {
+      const int partner = lane ^ stride;
+      const uint64_t pk = wave_shfl_u64(lk, partner);
+      const uint32_t pi = (uint32_t)__shfl((int)li, partner);
+      const double pv = __shfl(lv, partner);
+      const bool lo = (lane & stride) == 0;
+      const bool mine_less = less(lk, li, pk, pi);
+      if (mine_less != lo) {
+        lk = pk;
+        li = pi;
+        lv = pv;
+      }
+    }
+  }
+  if (lane < L) {
+    const bool valid = lk != kNone;
+    out_key[b * L + lane] = valid ? lv : 0.0;
+    out_idx[b * L + lane] = valid ? (int32_t)li : -1;
+  }
+}
+
 // One wave per query: exact FP64 values of the L candidates (the reference's
 // cosine arithmetic), exact order, certificate against the best value any
 // excluded column could have.
@@ -1910,6 +2016,13 @@ int select_lists(sd_context* ctx, const double* fkey, const int32_t* fidx, const
                  int cap, int64_t nqry, int L, double* out_key, int32_t* out_idx,
                  int32_t* over_list, int32_t* over_count, int min_count, const int32_t* qlist) {
   if (nqry <= 0) return SD_OK;
+  if (L <= 64) {
+    hipLaunchKernelGGL(k_select_wave, dim3((unsigned)((nqry + 3) / 4)), dim3(256), 0, ctx->stream,
+                       fkey, fidx, cnt, cap, nqry, L, out_key, out_idx, over_list, over_count,
+                       min_count, qlist);
+    SD_HIP(hipGetLastError());
+    return SD_OK;
+  }
   hipLaunchKernelGGL(k_select, dim3((unsigned)nqry), dim3(256), 0, ctx->stream, fkey, fidx, cnt, cap,
                      nqry, L, out_key, out_idx, over_list, over_count, min_count, qlist);
   SD_HIP(hipGetLastError());
