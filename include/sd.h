@@ -127,7 +127,8 @@ void* sd_context_stream(sd_context* ctx);
 /* ---- autotuning ------------------------------------------------------------ */
 /* With SD_PREC_AUTO each search picks its path from a cost model: the
  * densified MFMA path costs a fixed overhead plus 2 r n_ref per query column
- * (whole query tiles of 256) at the filtered pass's rate; the FP64 merge
+ * (whole query tiles of 256; half that for a self-search, which scores each
+ * pair once) at the filtered pass's rate; the FP64 merge
  * walk costs a smaller overhead plus n_ref (nnz_q + nnz_ref) merge steps per
  * query column.  Both paths give identical results, so the choice never
  * changes an answer.  The re-rank margin m (cosine, euclidean, pearson)

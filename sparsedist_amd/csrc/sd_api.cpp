@@ -509,7 +509,8 @@ int run_knn(sd_context* ctx, sd_matrix* X, sd_matrix* Y, const sd_knn_params* p,
     if (prec == SD_PREC_AUTO &&
         tune_prefer_exact(ctx, kind, nq, nref, kpad,
                           (double)X->nnz / (double)std::max<int64_t>(X->ncol, 1),
-                          (double)ref->nnz / (double)std::max<int64_t>(nref, 1))) {
+                          (double)ref->nnz / (double)std::max<int64_t>(nref, 1),
+                          plan.selfmode && nref >= 128 * 128)) {
       fused_ok_binary = false;
       fused_ok_cosine = false;
       fused_ok_euclid = false;

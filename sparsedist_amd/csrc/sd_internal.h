@@ -302,9 +302,10 @@ inline int64_t dense_qseg_bytes(int kind, int64_t kpad) {
   return kind == kDenseBf16Split ? kpad * 2 : 0;
 }
 // Cost-model choice between the densified path and the FP64 merge walk for
-// a search of nq query columns against nref reference columns.
+// a search of nq query columns against nref reference columns (symmetric: a
+// self-search whose filtered pass scores each pair once).
 bool tune_prefer_exact(const sd_context* ctx, int kind, int64_t nq, int64_t nref, int64_t kpad,
-                       double nnz_q, double nnz_ref);
+                       double nnz_q, double nnz_ref, bool symmetric);
 int fused_list_len(int want);
 // Whether a slab tile of 256 columns of row_bytes each stays within the
 // 32-bit buffer offsets of the slab kernels.

@@ -9,7 +9,9 @@
 // side by side, each at the latency of a dependent load per step).  The
 // densified path of the re-ranked methods adds that floor for its FP64
 // re-rank: one candidate's column per lane (cosine walks the reference
-// column against a bitmap of the query; euclidean and pearson walk both).  Both paths are exact, so the choice
+// column against a bitmap of the query; euclidean and pearson walk both).  A
+// self-search large enough for the filtered pass scores each pair once, half
+// the slab.  Both paths are exact, so the choice
 // changes only the time.  sd_autotune replaces the default rates by ones
 // measured on synthetic inputs of known size.
 #include <algorithm>
@@ -22,7 +24,7 @@
 namespace sd {
 
 bool tune_prefer_exact(const sd_context* ctx, int kind, int64_t nq, int64_t nref, int64_t kpad,
-                       double nnz_q, double nnz_ref) {
+                       double nnz_q, double nnz_ref, bool symmetric) {
   const TuneRates& t = ctx->tune;
   const bool bf16 = kind == kDenseBf16Unit || kind == kDenseBf16Split || kind == kDenseBf16Raw;
   const double rate = kind == kDenseF4Pattern ? t.fp4_ops : bf16 ? t.bf16_ops : t.fp32_ops;
@@ -33,7 +35,9 @@ bool tune_prefer_exact(const sd_context* ctx, int kind, int64_t nq, int64_t nref
   const bool pattern = kind == kDenseF4Pattern || kind == kDenseF32Pattern;
   const bool unit = kind == kDenseBf16Unit || kind == kDenseF32Unit;
   const double rerank = pattern ? 0.0 : (unit ? nnz_ref : nnz_q + nnz_ref) * t.step_latency_s;
-  const double fused = fixed + qcols * (double)nref * 2.0 * kwork / rate + rerank;
+  // a self-search's filtered pass scores each unordered pair once
+  const double slab = qcols * (double)nref * 2.0 * kwork / rate * (symmetric ? 0.5 : 1.0);
+  const double fused = fixed + slab + rerank;
   const double exact =
       t.exact_fixed_s + std::max((double)nq * (double)nref * (nnz_q + nnz_ref) / t.merge_steps,
                                  (nnz_q + nnz_ref) * t.step_latency_s);
