@@ -307,10 +307,18 @@ int fused_candidate_lists(sd_context* ctx, FusedArgs fa, const double** out_key,
     if (xm.phase == 1) {
       // probe of the reverse search: keep its thresholds and buffers for the
       // forward search's filtered pass
-      if (!fa.qlist) {
-        SD_TRY(xm.cnt.ensure(sizeof(int32_t) * (size_t)(nq + 16)));
-        SD_TRY(xm.fkey.ensure(sizeof(double) * (size_t)nq * (size_t)cap + 64));
-        SD_TRY(xm.fidx.ensure(sizeof(int32_t) * (size_t)nq * (size_t)cap + 64));
+      // the buffers stay allocated while the forward search runs; without
+      // room for them the two searches simply run one after the other
+      const bool room = !fa.qlist &&
+                        xm.cnt.ensure(sizeof(int32_t) * (size_t)(nq + 16)) == SD_OK &&
+                        xm.fkey.ensure(sizeof(double) * (size_t)nq * (size_t)cap + 64) == SD_OK &&
+                        xm.fidx.ensure(sizeof(int32_t) * (size_t)nq * (size_t)cap + 64) == SD_OK;
+      if (!room) {
+        xm.cnt.release();
+        xm.fkey.release();
+        xm.fidx.release();
+        clear_error();
+      } else {
         xm.L = L;
         xm.kind = fa.kind;
         xm.prec = fa.prec;
@@ -318,10 +326,9 @@ int fused_candidate_lists(sd_context* ctx, FusedArgs fa, const double** out_key,
         xm.kpad = fa.kpad;
         xm.nqry = nq;
         xm.nref = fa.nref;
-        xm.target = MirrorTarget{thr.as<double>(), cap,       xm.cnt.as<int32_t>(),
+        xm.target = MirrorTarget{thr.as<double>(),    cap,        xm.cnt.as<int32_t>(),
                                  xm.fkey.as<double>(), xm.fidx.as<int32_t>(), fa.ref_deg,
-                                 false,             fa.dot_e0, fa.dot_e1,
-                                 fa.dot_nonneg};
+                                 false,                fa.dot_e0, fa.dot_e1, fa.dot_nonneg};
         xm.ready = true;
       }
       *out_key = nullptr;
