@@ -393,6 +393,17 @@ def cross_main(args, sd, synth):
     log(f"cross auto: {dt * 1e3:.0f} ms for both directions ({nq / dt:.0f} query cols/s); "
         f"certified {st['certified_queries']}, escalated {st['escalated_queries']}, "
         f"fallback {st['fallback_queries']}")
+    # full-size check of the one-pass structure: two independent searches
+    # (each with its own filtered pass) give the same results on every row
+    fw = sd.sparseKNN(dE, dY, k=k, method="cosine", dist=False, verbose=False, context=ctx)
+    rv = sd.sparseKNN(dY, dE, k=k, method="cosine", dist=False, verbose=False, context=ctx)
+    one_pass = dict(
+        forward_identical=bool(np.array_equal(fw.idx, cr["forward"].idx) and np.array_equal(
+            fw.dist.view(np.int64), cr["forward"].dist.view(np.int64))),
+        reverse_identical=bool(np.array_equal(rv.idx, cr["reverse"].idx) and np.array_equal(
+            rv.dist.view(np.int64), cr["reverse"].dist.view(np.int64))))
+    del fw, rv
+    log(f"one filtered pass vs two independent searches: {one_pass}")
     plain = None
     if not args.no_plain:
         cp, dtp, _ = run("fp32")
@@ -429,7 +440,7 @@ def cross_main(args, sd, synth):
                          traffic=None, kernel="k_filter_wide<OpBf16, float> (both directions)",
                          avg_launch_ms=fms, launches=st["filter_launches"]),
         "certified": st["certified_queries"], "escalated": st["escalated_queries"],
-        "fallback": st["fallback_queries"],
+        "fallback": st["fallback_queries"], "one_pass_vs_two_searches": one_pass,
         "plain_fp32": plain, "vs_plain_fp32": (dtp / dt) if plain else None,
         "cpu_baseline": cpu,
     }
