@@ -186,7 +186,7 @@ def main():
             st2 = ctx.stats(reset=True)
             ctx.set_timing(False)
             res[method] = dict(t_knn=t_knn, t_snn=t_snn, stats=st, snn_stats=st2, nnz=g.nnz,
-                               idx=nn.idx)
+                               idx=nn.idx, dist=nn.dist)
         return res
 
     for w in range(args.warmup):
@@ -270,15 +270,37 @@ def main():
         from oracle import oracle as O
         O.build()
         nt = min(16, os.cpu_count() or 1)
-        Xs = X[:, : args.cpu_sample]
+        ns = args.cpu_sample
+        Xs = X[:, :ns]
         t = time.perf_counter()
+        ora = {}
         for method in ("binary", "cosine"):
-            O.knn(Xs, X, k=k, method=method, block_size=8192, nthreads=nt)
+            # k + 1: the self-search row plus the query's own column
+            ora[method] = O.knn(Xs, X, k=k + 1, method=method, block_size=8192, nthreads=nt)
         tc = time.perf_counter() - t
-        cpu = dict(value=2 * args.cpu_sample / tc, unit="query cols/s", cores=nt, kind="port",
-                   sample=f"{args.cpu_sample} query columns x all {nq} reference columns, binary "
-                          f"and cosine k={k}, oracle/sd_oracle.c with OpenMP ({tc:.1f} s)")
-        log(f"cpu baseline: {cpu['value']:.2f} cols/s on {nt} threads")
+        # the timed step's rows of those columns must equal the oracle's with
+        # the query's own column removed
+        parity = {}
+        for method, (oi, od, _) in ora.items():
+            wi = np.empty((ns, k), np.int32)
+            wd = np.empty((ns, k))
+            for r in range(ns):
+                keep = oi[r] != r + 1
+                if keep.all():
+                    keep[-1] = False
+                wi[r] = oi[r][keep][:k]
+                wd[r] = od[r][keep][:k]
+            gi, gd = per[-1][method]["idx"][:ns], per[-1][method]["dist"][:ns]
+            parity[method] = bool(np.array_equal(gi, wi) and
+                                  np.array_equal(np.ascontiguousarray(gd).view(np.int64),
+                                                 wd.view(np.int64)))
+        cpu = dict(value=2 * ns / tc, unit="query cols/s", cores=nt, kind="port",
+                   sample=f"{ns} query columns x all {nq} reference columns, binary "
+                          f"and cosine k={k} (+1 for the query's own column), "
+                          f"oracle/sd_oracle.c with OpenMP ({tc:.1f} s)",
+                   sample_parity=parity)
+        log(f"cpu baseline: {cpu['value']:.2f} cols/s on {nt} threads; GPU rows of the sample "
+            f"equal the oracle's: {parity}")
 
     serving = None
     if pool is not None:
