@@ -256,6 +256,26 @@ def main():
     edges = 2 * nq * k
     snn_s = med("binary", "t_snn") + med("cosine", "t_snn")
 
+    # full-size check of the symmetric pass (outside the timed region): a
+    # cross search against a copy of X runs the full grid of tiles; with the
+    # self rules applied to its k + 1 lists it must equal every timed row
+    full_grid = {}
+    dXc = sd.DeviceMatrix(X, ctx)
+    for method in ("binary", "cosine"):
+        c = sd.sparseKNN(dX, dXc, k=k + 1, method=method, block_size=8192, verbose=False,
+                         context=ctx)
+        n = c.idx.shape[0]
+        hit = c.idx == np.arange(1, n + 1, dtype=np.int32)[:, None]
+        pos = np.where(hit.any(axis=1), hit.argmax(axis=1), k)  # the own column, else the last
+        keep = np.ones(c.idx.shape, bool)
+        keep[np.arange(n), pos] = False
+        wi, wd = c.idx[keep].reshape(n, k), c.dist[keep].reshape(n, k)
+        gi, gd = per[-1][method]["idx"], per[-1][method]["dist"]
+        full_grid[method] = bool(np.array_equal(gi, wi) and np.array_equal(
+            np.ascontiguousarray(gd).view(np.int64), np.ascontiguousarray(wd).view(np.int64)))
+        del c
+    del dXc
+    log(f"symmetric pass vs full grid, all {nq} rows: {full_grid}")
     plain = None
     if not args.no_plain:
         r = step(precision="fp32", timing=True)
@@ -345,6 +365,7 @@ def main():
         "roofline": dominant,
         "roofline_other": roof_cos if dominant is roof_bin else roof_bin,
         "cpu_baseline": cpu,
+        "symmetric_vs_full_grid": full_grid,
         "serving": serving,
     }
     print(json.dumps(out), flush=True)
