@@ -1030,9 +1030,10 @@ __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
         }
       }
   }
-  // Mirror screen (symmetric tiles off the diagonal): reference column rg as
-  // the query, against its own threshold, with this lane's query column as
-  // its candidate.  No self pair lies off the diagonal.
+  // Mirror screen (a self-search's tiles off the diagonal, every tile of a
+  // cross search): reference column rg as a query of the mirrored search,
+  // against its own threshold, with this lane's query column as its
+  // candidate.  No self pair lies off the diagonal, nor in a cross search.
   if constexpr (SYM) {
     if (mirror) {
       // candidates are valid, non-degenerate query columns only.  The flag
