@@ -57,7 +57,8 @@ enum {
   SD_EUCLIDEAN = 2,
   SD_MANHATTAN = 3,
   SD_PEARSON = 4,
-  SD_COVARIANCE = 5
+  SD_COVARIANCE = 5,
+  SD_JS = 6          /* Jensen-Shannon distance: finite, non-negative input */
 };
 
 /* ---- slab arithmetic ------------------------------------------------------ */
@@ -83,8 +84,12 @@ int sd_context_create(int device, sd_context** out);
 int sd_context_destroy(sd_context* ctx);
 
 /* Request cancellation of the running call (reference Progress::check_abort /
- * stop_if_aborted, src/sparseDist.cpp:177-179).  The running call returns
- * SD_ECANCELLED and writes no partial output.  Callable from any thread. */
+ * stop_if_aborted, src/sparseDist.cpp:177-179).  The call polls the request
+ * between kernels, returns SD_ECANCELLED and writes no partial output.  A
+ * request made while no call runs applies to the next call on the context
+ * (a caller's interrupt poll may run before its worker thread enters the
+ * call); every call clears the request when it returns.  Callable from any
+ * thread. */
 int sd_cancel(sd_context* ctx);
 
 /* Kernel statistics since the last reset, measured with HIP events on the
@@ -171,7 +176,7 @@ int sd_matrix_drop_cache(sd_matrix* m);
 /* ---- kNN ------------------------------------------------------------------ */
 typedef struct {
   int32_t k;               /* neighbours requested (>= 1) */
-  int32_t method;          /* SD_BINARY ... SD_COVARIANCE */
+  int32_t method;          /* SD_BINARY ... SD_JS */
   int32_t dist;            /* 1 = distance form, 0 = similarity form */
   int32_t include_self;    /* self-search only (ignored when Y != NULL) */
   int32_t drop_degenerate; /* mask all-zero columns on both sides */
@@ -185,7 +190,8 @@ typedef struct {
  * self-search among X).  Replaces R/sparseKNN.R:110-313 after argument
  * validation, i.e. the block loop over sparseDist() (fastJacc2
  * src/sparseDist.cpp:559-617, fastCosine2 967-1000, fastEuclidean2 833-878,
- * fastManhattan2 734-779, fastCorr2 403-441, fastCov2 481-510) and topKBlock
+ * fastManhattan2 734-779, fastCorr2 403-441, fastCov2 481-510, fastJS2
+ * 1074-1115) and topKBlock
  * (src/topK.cpp:67-149), fused so the slab never leaves the device.
  * idx/dist have room for X.ncol * params.k entries; the result occupies the
  * first X.ncol * (*k_eff) entries.  *warn = 1 when k was clamped to the
@@ -209,9 +215,10 @@ int sd_knn_cross(sd_context* ctx, const sd_matrix* X, const sd_matrix* Y,
 /* ---- one-to-one replacements of the reference's native helpers ------------ */
 /* Dense slab D (X.ncol x Y.ncol, column-major, host memory) in FP64 with the
  * reference's operation order.  Replaces fastJacc2 / fastCosine2 /
- * fastEuclidean2 / fastManhattan2 / fastCorr2 / fastCov2
+ * fastEuclidean2 / fastManhattan2 / fastCorr2 / fastCov2 / fastJS2
  * (src/RcppExports.cpp:80-93, 110-137, 199-242; R/RcppExports.R:24-66) with
- * m = X and m2 = Y. */
+ * m = X and m2 = Y (fastJS2 takes dense matrices; their zero entries are the
+ * ones this CSC does not store). */
 int sd_slab(sd_context* ctx, int32_t method, const sd_matrix* X, const sd_matrix* Y, int32_t dist,
             double* D);
 
@@ -262,10 +269,15 @@ int sd_graph_free(sd_graph* g);
  *
  *  - Admission: a request reserves an estimate of the device memory its work
  *    needs (one chunk's densified columns and candidate buffers, its results
- *    and SNN workspace).  Requests are admitted first come first served while
- *    the reservations fit in budget_bytes; a request that cannot fit even
+ *    and SNN workspace).  Requests are admitted in arrival order while the
+ *    reservations fit in budget_bytes; one that does not fit yet lets later
+ *    requests that fit pass it until it has waited 2 s or been passed 64
+ *    times, after which it is admitted next.  A request that cannot fit even
  *    alone, or that arrives while max_pending requests wait, is refused with
  *    SD_EBUDGET.
+ *  - Memory: each worker reloads its query columns into buffers it keeps,
+ *    and grows its scratch from the device's stream-ordered pool, so items
+ *    do not allocate or free device memory once warm.
  *  - Continuous batching: `slots` workers, each with its own context and
  *    stream, take work items of at most chunk_cols query columns from the
  *    admitted requests, always from the request with the fewest unscheduled
@@ -280,6 +292,9 @@ typedef struct {
   int32_t chunk_cols;    /* query columns per work item (0 = 4096) */
   int64_t budget_bytes;  /* device memory admitted requests may reserve (0 = 16 GiB) */
   int64_t max_pending;   /* requests allowed to wait for admission (0 = unlimited) */
+  int32_t paused;        /* 1: workers take no work until sd_server_resume, so a batch
+                            submitted meanwhile is scheduled as a whole */
+  int32_t reserved;
 } sd_server_params;
 
 typedef struct {
@@ -302,6 +317,8 @@ typedef struct {
   double queued_ms;  /* submit -> admission */
   double run_ms;     /* admission -> finish */
   double total_ms;   /* submit -> finish */
+  int64_t finish_order; /* 1 for the first request the server finished, 2 for the next, ...
+                           (0 while unfinished) */
 } sd_request_info;
 
 typedef struct {
@@ -310,6 +327,7 @@ typedef struct {
   int64_t reserved_bytes;   /* current reservations of admitted requests */
   int64_t query_cols;       /* query columns of completed requests */
   int64_t packed_items;     /* work items that carried columns of several requests */
+  int64_t waiting_callers;  /* callers blocked in sd_server_wait right now */
 } sd_server_stats;
 
 /* Y must stay alive until sd_server_destroy; ctx gives the device. */
@@ -328,8 +346,13 @@ const char* sd_server_error(sd_server* s, int64_t ticket);
 int sd_server_result(sd_server* s, int64_t ticket, int32_t* idx, double* dist, sd_graph** snn);
 /* Forget a finished request. */
 int sd_server_release(sd_server* s, int64_t ticket);
+/* Let the workers of a server created with params.paused = 1 take work. */
+int sd_server_resume(sd_server* s);
 int sd_server_stats_get(sd_server* s, sd_server_stats* out);
-/* Stop the workers (in-flight work is cancelled) and free the server. */
+/* Stop the workers (in-flight work is cancelled) and free the server.
+ * Unfinished requests end with SD_ECANCELLED; callers blocked in
+ * sd_server_wait on other threads return first.  No other call on the server
+ * may start once destroy has begun. */
 int sd_server_destroy(sd_server* s);
 
 /* Number of usable host cores minus one, at least 1 (R/sparseDist.R:221-224

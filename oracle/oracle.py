@@ -19,9 +19,9 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_build", "libsd_oracle.so")
 
-BINARY, COSINE, EUCLIDEAN, MANHATTAN, PEARSON, COVARIANCE = range(6)
+BINARY, COSINE, EUCLIDEAN, MANHATTAN, PEARSON, COVARIANCE, JS = range(7)
 METHODS = {"binary": BINARY, "jaccard": BINARY, "cosine": COSINE, "euclidean": EUCLIDEAN,
-           "manhattan": MANHATTAN, "pearson": PEARSON, "covariance": COVARIANCE}
+           "manhattan": MANHATTAN, "pearson": PEARSON, "covariance": COVARIANCE, "js": JS}
 NA_INTEGER = np.iinfo(np.int32).min
 NA_REAL_BITS = 0x7FF00000000007A2
 

@@ -229,6 +229,49 @@ static double centred_dot(const or_csc* a, int64_t i, double mx, const or_csc* b
   return s;
 }
 
+/* Jensen-Shannon distance (reference fastJS2 and xlogx,
+ * src/sparseDist.cpp:206-210, 1074-1115), over the columns as the dense
+ * vectors the reference takes: per row s = xlogx(p) + xlogx(q) - xlogx(p + q)
+ * + (p + q) log 2, evaluated left to right, with xlogx(x) = x log x and its
+ * NaN (0 log 0) replaced by 0; the rows are summed as Armadillo's accu of a
+ * dense vector does (arrayops::accumulate: even positions into one
+ * accumulator, odd positions into another, then their sum); the distance is
+ * sqrt(0.5 sum) when that is positive, else 0.  Armadillo is not vendored
+ * and its version is not pinned, so this summation order is a restatement of
+ * its published code, not a reference-pinned fact. */
+static double xlogx(double x) {
+  const double r = x * log(x);
+  return (r != r) ? 0.0 : r;
+}
+
+static double js_pair(const or_csc* a, int64_t i, const or_csc* b, int64_t j) {
+  const double log2 = log(2.0);
+  int64_t x = a->p[i], xe = a->p[i + 1], y = b->p[j], ye = b->p[j + 1];
+  double acc1 = 0.0, acc2 = 0.0;
+  for (int64_t r = 0; r < a->nrow; ++r) {
+    double p = 0.0, q = 0.0;
+    if (x < xe && a->i[x] == r) p = a->x[x++];
+    if (y < ye && b->i[y] == r) q = b->x[y++];
+    const double s = xlogx(p) + xlogx(q) - xlogx(p + q) + (p + q) * log2;
+    if (r & 1)
+      acc2 += s;
+    else
+      acc1 += s;
+  }
+  const double v = 0.5 * (acc1 + acc2);
+  return (v > 0.0) ? sqrt(v) : 0.0;
+}
+
+static int js_check(const or_csc* m) {
+  const int64_t nnz = m->p[m->ncol];
+  for (int64_t e = 0; e < nnz; ++e)
+    if (!isfinite(m->x[e]) || m->x[e] < 0.0)
+      return fail(OR_EINVAL,
+                  "fastJS2 requires finite, non-negative input (columns should be normalised to "
+                  "sum to 1 for a true JS divergence).");
+  return OR_OK;
+}
+
 /* ------------------------------------------------------------------------ */
 /* Slab                                                                      */
 /* ------------------------------------------------------------------------ */
@@ -263,6 +306,8 @@ int or_slab(int method, const or_csc* m, const or_csc* m2, int dist, int nthread
     }
     centre_stats(m, ma, qa);
     centre_stats(m2, mb, qb);
+  } else if (method == OR_JS) {
+    if ((rc = js_check(m)) != OR_OK || (rc = js_check(m2)) != OR_OK) return rc;
   } else if (method != OR_BINARY && method != OR_EUCLIDEAN && method != OR_MANHATTAN) {
     return fail(OR_EINVAL, "unknown method %d", method);
   }
@@ -286,6 +331,9 @@ int or_slab(int method, const or_csc* m, const or_csc* m2, int dist, int nthread
           break;
         case OR_MANHATTAN:
           v = man_pair(m, i, m2, j);
+          break;
+        case OR_JS:
+          v = js_pair(m, i, m2, j);
           break;
         case OR_PEARSON: {
           const double s = centred_dot(m, i, ma[i], m2, j, mb[j]);
@@ -578,11 +626,12 @@ static int method_decreasing(int method, int dist, int* dec) {
       return OR_OK;
     case OR_EUCLIDEAN:
     case OR_MANHATTAN:
+    case OR_JS:
       if (!dist)
         return fail(OR_EINVAL,
                     "'dist = FALSE' is only meaningful for methods \"binary\", \"cosine\" and "
                     "\"pearson\"; %s has no similarity form.",
-                    method == OR_EUCLIDEAN ? "euclidean" : "manhattan");
+                    method == OR_EUCLIDEAN ? "euclidean" : method == OR_JS ? "js" : "manhattan");
       *dec = 0;
       return OR_OK;
     default:
@@ -674,7 +723,7 @@ int or_knn(const or_csc* Xin, const or_csc* Yin, int k, int method, int dist, in
             double* at = &D[(from + b) + b * nref];
             if (method == OR_BINARY || method == OR_COSINE || method == OR_PEARSON)
               *at = dist ? 0.0 : 1.0;
-            else if (method == OR_EUCLIDEAN || method == OR_MANHATTAN)
+            else if (method == OR_EUCLIDEAN || method == OR_MANHATTAN || method == OR_JS)
               *at = 0.0;
           }
         }

@@ -46,7 +46,8 @@ enum {
   OR_EUCLIDEAN = 2,
   OR_MANHATTAN = 3,
   OR_PEARSON = 4,
-  OR_COVARIANCE = 5
+  OR_COVARIANCE = 5,
+  OR_JS = 6         /* Jensen-Shannon distance (finite, non-negative input) */
 };
 
 typedef struct {

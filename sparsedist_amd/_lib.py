@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(HERE, "libsd_hip.so")
 
 SD_OK, SD_EINVAL, SD_ENONFINITE, SD_ENOMEM, SD_ENOCAND, SD_ECAP, SD_ECANCELLED, \
     SD_EDEVICE, SD_EUNSUPPORTED, SD_EBUDGET = range(10)
-SD_BINARY, SD_COSINE, SD_EUCLIDEAN, SD_MANHATTAN, SD_PEARSON, SD_COVARIANCE = range(6)
+SD_BINARY, SD_COSINE, SD_EUCLIDEAN, SD_MANHATTAN, SD_PEARSON, SD_COVARIANCE, SD_JS = range(7)
 SD_PREC_AUTO, SD_PREC_EXACT, SD_PREC_FP32, SD_PREC_BF16 = range(4)
 
 # every symbol include/sd.h declares
@@ -26,7 +26,7 @@ EXPORTS = (
     "sd_topk_block", "sd_topk_rows_accum", "sd_snn_jaccard", "sd_snn", "sd_graph_info",
     "sd_graph_copy", "sd_graph_free", "sd_detect_cores", "sd_server_create", "sd_server_submit",
     "sd_server_wait", "sd_server_error", "sd_server_result", "sd_server_release",
-    "sd_server_stats_get", "sd_server_destroy", "sd_autotune",
+    "sd_server_stats_get", "sd_server_destroy", "sd_autotune", "sd_server_resume",
 )
 
 
@@ -67,7 +67,7 @@ class TuneInfo(C.Structure):
 
 class ServerParams(C.Structure):
     _fields_ = [("slots", C.c_int32), ("chunk_cols", C.c_int32), ("budget_bytes", C.c_int64),
-                ("max_pending", C.c_int64)]
+                ("max_pending", C.c_int64), ("paused", C.c_int32), ("reserved", C.c_int32)]
 
 
 class Request(C.Structure):
@@ -79,14 +79,14 @@ class Request(C.Structure):
 class RequestInfo(C.Structure):
     _fields_ = [("done", C.c_int32), ("status", C.c_int32), ("k_eff", C.c_int32),
                 ("warn", C.c_int32), ("queued_ms", C.c_double), ("run_ms", C.c_double),
-                ("total_ms", C.c_double)]
+                ("total_ms", C.c_double), ("finish_order", C.c_int64)]
 
 
 class ServerStats(C.Structure):
     _fields_ = [("submitted", C.c_int64), ("admitted", C.c_int64), ("completed", C.c_int64),
                 ("refused", C.c_int64), ("pending", C.c_int64), ("active", C.c_int64),
                 ("reserved_bytes", C.c_int64), ("query_cols", C.c_int64),
-                ("packed_items", C.c_int64)]
+                ("packed_items", C.c_int64), ("waiting_callers", C.c_int64)]
 
 
 _lib = None
@@ -137,6 +137,7 @@ def load(path: str = LIB_PATH):
         L.sd_server_release.argtypes = [vp, i64]
         L.sd_server_stats_get.argtypes = [vp, C.POINTER(ServerStats)]
         L.sd_server_destroy.argtypes = [vp]
+        L.sd_server_resume.argtypes = [vp]
         L.sd_autotune.argtypes = [vp, i32, C.POINTER(TuneInfo)]
         _lib = L
         return L

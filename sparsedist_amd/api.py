@@ -31,7 +31,8 @@ _METHODS = ("binary", "jaccard", "cosine", "euclidean", "manhattan", "pearson", 
             "covariance")
 _METHOD_CODE = {"binary": _lib.SD_BINARY, "cosine": _lib.SD_COSINE,
                 "euclidean": _lib.SD_EUCLIDEAN, "manhattan": _lib.SD_MANHATTAN,
-                "pearson": _lib.SD_PEARSON, "covariance": _lib.SD_COVARIANCE}
+                "pearson": _lib.SD_PEARSON, "covariance": _lib.SD_COVARIANCE,
+                "js": _lib.SD_JS}
 _PRECISION = {"auto": _lib.SD_PREC_AUTO, "exact": _lib.SD_PREC_EXACT,
               "fp32": _lib.SD_PREC_FP32, "bf16": _lib.SD_PREC_BF16}
 
@@ -223,8 +224,16 @@ def detect_cores() -> int:
 # ---------------------------------------------------------------------------
 # kNN
 # ---------------------------------------------------------------------------
+def _colnames(M):
+    """Column names of a labelled input (a pandas DataFrame), else None: the
+    reference names the result rows after colnames(X) (R/sparseKNN.R:303-312)."""
+    cols = getattr(M, "columns", None)
+    return None if cols is None else [str(c) for c in cols]
+
+
 class KNNResult(dict):
-    """``list(idx, dist)`` with the ``self_search`` attribute."""
+    """``list(idx, dist)`` with the ``self_search`` attribute; ``names`` holds
+    the row names (the query matrix's column names, when it has any)."""
 
     def __init__(self, idx, dist, self_search, names=None):
         super().__init__(idx=idx, dist=dist)
@@ -286,10 +295,7 @@ def sparseKNN(X, Y=None, k=10, method="binary", dist=True, include_self=False,
     k = _chk_count(k, "k")
     block_size = _chk_count(block_size, "block_size")
     method = _match_method(method)
-    dec = _direction(method, dist)
-    if method == "js":
-        raise SparseDistError(_lib.SD_EUNSUPPORTED,
-                              "method \"js\" (dense Jensen-Shannon) is not provided by this library.")
+    _direction(method, dist)
     nq = _ncol(X)
     if nq < 1:
         raise ValueError("'X' must have at least one column.")
@@ -309,14 +315,10 @@ def sparseKNN(X, Y=None, k=10, method="binary", dist=True, include_self=False,
                              idx.ctypes.data, dst.ctypes.data, C.byref(keff), C.byref(warn)))
     ke = keff.value
     if warn.value:
-        ref_n = _ncol(X) if selfmode else _ncol(Y)
-        del ref_n
         warnings.warn(f"k ({k}) exceeds the {ke} available neighbours; returning {ke}.",
                       SparseDistWarning, stacklevel=2)
-    res = KNNResult(idx[: nq * ke].reshape((nq, ke), order="F"),
-                    dst[: nq * ke].reshape((nq, ke), order="F"), selfmode)
-    del dec
-    return res
+    return KNNResult(idx[: nq * ke].reshape((nq, ke), order="F"),
+                     dst[: nq * ke].reshape((nq, ke), order="F"), selfmode, _colnames(X))
 
 
 def sparseKNNCross(X, Y=None, k=10, method="binary", dist=True, drop_degenerate=True,
@@ -336,9 +338,6 @@ def sparseKNNCross(X, Y=None, k=10, method="binary", dist=True, drop_degenerate=
     block_size = _chk_count(block_size, "block_size")
     method = _match_method(method)
     _direction(method, dist)
-    if method == "js":
-        raise SparseDistError(_lib.SD_EUNSUPPORTED,
-                              "method \"js\" (dense Jensen-Shannon) is not provided by this library.")
     if _nrow(X) != _nrow(Y):
         raise ValueError("'X' and 'Y' must have the same number of rows.")
     nq, nr = _ncol(X), _ncol(Y)
@@ -365,9 +364,9 @@ def sparseKNNCross(X, Y=None, k=10, method="binary", dist=True, drop_degenerate=
         warnings.warn(f"k ({k}) exceeds the {kr.value} available neighbours for the reverse "
                       f"direction; returning {kr.value}.", SparseDistWarning, stacklevel=2)
     fwd = KNNResult(fi[: nq * kf.value].reshape((nq, kf.value), order="F"),
-                    fd[: nq * kf.value].reshape((nq, kf.value), order="F"), False)
+                    fd[: nq * kf.value].reshape((nq, kf.value), order="F"), False, _colnames(X))
     rev = KNNResult(ri[: nr * kr.value].reshape((nr, kr.value), order="F"),
-                    rd[: nr * kr.value].reshape((nr, kr.value), order="F"), False)
+                    rd[: nr * kr.value].reshape((nr, kr.value), order="F"), False, _colnames(Y))
     return {"forward": fwd, "reverse": rev}
 
 
@@ -491,6 +490,8 @@ def _graph_to_csc(g, n_rows_cols):
 # ---------------------------------------------------------------------------
 class Server:
     """Serves kNN (+ optional SNN) requests against a resident reference Y.
+    With ``paused=True`` the workers wait for ``resume()``, so requests
+    submitted meanwhile are scheduled as one batch.
 
     Each request equals ``sparseKNN(X, Y, k, method, dist, drop_degenerate)``
     bit for bit; with ``snn=True`` it also carries the shared-neighbour graph
@@ -501,11 +502,12 @@ class Server:
     its own stream, take smallest-request-first.
     """
 
-    def __init__(self, Y, slots=4, chunk_cols=4096, budget_bytes=0, max_pending=0, *,
-                 context: Context | None = None):
+    def __init__(self, Y, slots=4, chunk_cols=4096, budget_bytes=0, max_pending=0,
+                 paused=False, *, context: Context | None = None):
         self.ctx = context or default_context()
         self.Y = Y if isinstance(Y, DeviceMatrix) else DeviceMatrix(Y, self.ctx)
-        prm = _lib.ServerParams(int(slots), int(chunk_cols), int(budget_bytes), int(max_pending))
+        prm = _lib.ServerParams(int(slots), int(chunk_cols), int(budget_bytes), int(max_pending),
+                                int(bool(paused)), 0)
         h = C.c_void_p()
         check(_lib.load().sd_server_create(self.ctx.handle, self.Y.handle, C.byref(prm),
                                            C.byref(h)))
@@ -522,9 +524,6 @@ class Server:
         _chk_flag(drop_degenerate, "drop_degenerate")
         k = _chk_count(k, "k")
         method = _match_method(method)
-        if method == "js":
-            raise SparseDistError(_lib.SD_EUNSUPPORTED,
-                                  "method \"js\" (dense Jensen-Shannon) is not provided by this library.")
         M = _as_csc(X)
         return self.submit_csc(M.shape[0], M.indptr, M.indices, M.data, k=k, method=method,
                                dist=dist, drop_degenerate=drop_degenerate, snn=snn, prune=prune,
@@ -588,6 +587,10 @@ class Server:
         check(_lib.load().sd_server_release(self._h, ticket))
         self._keep.pop(ticket, None)
 
+    def resume(self):
+        """Let the workers of a server created with ``paused=True`` start."""
+        check(_lib.load().sd_server_resume(self._h))
+
     def stats(self) -> dict:
         s = _lib.ServerStats()
         check(_lib.load().sd_server_stats_get(self._h, C.byref(s)))
@@ -619,8 +622,6 @@ def sparseDist2(X, Y, method="binary", dist=True, *, context: Context | None = N
     (reference fastJacc2 / fastCosine2 / fastEuclidean2 / fastManhattan2 /
     fastCorr2 / fastCov2 with m = X, m2 = Y)."""
     method = _match_method(method)
-    if method == "js":
-        raise SparseDistError(_lib.SD_EUNSUPPORTED, "method \"js\" is not provided by this library.")
     ctx = context or default_context()
     dx, _kx = _device(X, ctx)
     dy, _ky = _device(Y, ctx)

@@ -195,10 +195,10 @@ __global__ void k_to_f32(const double* __restrict__ in, int64_t n, float* __rest
 
 int prep_unit(sd_context* ctx, sd_matrix* m) {
   if (m->unit_ready) return SD_OK;
-  SD_TRY(m->unit.alloc(sizeof(double) * (size_t)(m->nnz > 0 ? m->nnz : 1)));
-  SD_TRY(m->scale.alloc(sizeof(double) * (size_t)(m->ncol + 1)));
-  DevBuf sq;
-  SD_TRY(sq.alloc(sizeof(double) * (size_t)(m->ncol + 1)));
+  SD_TRY(m->unit.ensure(sizeof(double) * (size_t)(m->nnz > 0 ? m->nnz : 1)));
+  SD_TRY(m->scale.ensure(sizeof(double) * (size_t)(m->ncol + 1)));
+  DevBuf& sq = m->unit_ssq;
+  SD_TRY(sq.ensure(sizeof(double) * (size_t)(m->ncol + 1)));
   const int64_t nb = (m->ncol + 255) / 256;
   hipLaunchKernelGGL(k_unit_scale, dim3((unsigned)nb), dim3(256), 0, ctx->stream, m->p.as<int64_t>(),
                      m->x.as<double>(), m->ncol, m->scale.as<double>(), sq.as<double>());
@@ -214,10 +214,10 @@ int prep_unit(sd_context* ctx, sd_matrix* m) {
 
 int prep_centre(sd_context* ctx, sd_matrix* m) {
   if (m->centre_ready) return SD_OK;
-  SD_TRY(m->mean.alloc(sizeof(double) * (size_t)(m->ncol + 1)));
-  SD_TRY(m->css.alloc(sizeof(double) * (size_t)(m->ncol + 1)));
-  SD_TRY(m->cmf.alloc(sizeof(float) * (size_t)(m->ncol + 1)));
-  SD_TRY(m->cif.alloc(sizeof(float) * (size_t)(m->ncol + 1)));
+  SD_TRY(m->mean.ensure(sizeof(double) * (size_t)(m->ncol + 1)));
+  SD_TRY(m->css.ensure(sizeof(double) * (size_t)(m->ncol + 1)));
+  SD_TRY(m->cmf.ensure(sizeof(float) * (size_t)(m->ncol + 1)));
+  SD_TRY(m->cif.ensure(sizeof(float) * (size_t)(m->ncol + 1)));
   const int64_t nb = (m->ncol + 255) / 256;
   hipLaunchKernelGGL(k_centre, dim3((unsigned)nb), dim3(256), 0, ctx->stream, m->p.as<int64_t>(),
                      m->x.as<double>(), m->ncol, m->nrow, m->mean.as<double>(), m->css.as<double>());
@@ -232,9 +232,9 @@ int prep_centre(sd_context* ctx, sd_matrix* m) {
 
 int prep_sq(sd_context* ctx, sd_matrix* m) {
   if (m->sq_ready) return SD_OK;
-  SD_TRY(m->sqn.alloc(sizeof(double) * (size_t)(m->ncol + 1)));
+  SD_TRY(m->sqn.ensure(sizeof(double) * (size_t)(m->ncol + 1)));
   const int64_t nb = (m->ncol + 255) / 256;
-  SD_TRY(m->sqf.alloc(sizeof(float) * (size_t)(m->ncol + 1)));
+  SD_TRY(m->sqf.ensure(sizeof(float) * (size_t)(m->ncol + 1)));
   hipLaunchKernelGGL(k_sqnorm, dim3((unsigned)nb), dim3(256), 0, ctx->stream, m->p.as<int64_t>(),
                      m->x.as<double>(), m->ncol, m->sqn.as<double>());
   hipLaunchKernelGGL(k_to_f32, dim3((unsigned)nb), dim3(256), 0, ctx->stream, m->sqn.as<double>(),
@@ -253,7 +253,7 @@ int prep_dense(sd_context* ctx, sd_matrix* m, int kind, int64_t kpad) {
                      (long long)kpad, kind);
   const int64_t row_bytes = dense_row_bytes(kind, kpad);
   m->dense_kpad[kind] = 0;
-  SD_TRY(m->dense[kind].alloc((size_t)m->ncol * (size_t)row_bytes + 256));
+  SD_TRY(m->dense[kind].ensure((size_t)m->ncol * (size_t)row_bytes + 256));
   uint8_t* dst = m->dense[kind].as<uint8_t>();
   const double* vals = m->x.as<double>();
   if (kind == kDenseBf16Unit || kind == kDenseF32Unit) {

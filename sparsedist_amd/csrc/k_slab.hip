@@ -1207,9 +1207,9 @@ __global__ __launch_bounds__(256) void k_merge_lists(const double* __restrict__ 
                                                      const int32_t* __restrict__ idx, int nlists,
                                                      int64_t nqry, int L,
                                                      double* __restrict__ out_key,
-                                                     int32_t* __restrict__ out_idx) {
+                                                     int32_t* __restrict__ out_idx, int64_t q0) {
   __shared__ BlockTopKStorage<512> st;
-  const int64_t q = blockIdx.x;
+  const int64_t q = q0 + blockIdx.x;
   BlockTopK<512> tk(st, L, 1);
   tk.run((int64_t)nlists * L, [&](int64_t t, double& v, int64_t& id) {
     const int64_t s = t / L, e = t % L;
@@ -1257,10 +1257,10 @@ __global__ __launch_bounds__(256) void k_select(const double* __restrict__ fkey,
                                                 int32_t* __restrict__ over_list,
                                                 int32_t* __restrict__ over_count,
                                                 int min_count,
-                                                const int32_t* __restrict__ qlist) {
+                                                const int32_t* __restrict__ qlist, int64_t b0) {
   __shared__ BlockTopKStorage<512> st;
   // list position b; its query's buffer q (listed mode: qlist[b])
-  const int64_t b = blockIdx.x;
+  const int64_t b = b0 + blockIdx.x;
   const int64_t q = qlist ? (int64_t)qlist[b] : b;
   const int n = cnt[q];
   // overflowed, or fewer candidates than the threshold guarantees complete
@@ -2024,9 +2024,12 @@ int select_lists(sd_context* ctx, const double* fkey, const int32_t* fidx, const
     SD_HIP(hipGetLastError());
     return SD_OK;
   }
-  hipLaunchKernelGGL(k_select, dim3((unsigned)nqry), dim3(256), 0, ctx->stream, fkey, fidx, cnt, cap,
-                     nqry, L, out_key, out_idx, over_list, over_count, min_count, qlist);
-  SD_HIP(hipGetLastError());
+  for (int64_t b0 = 0; b0 < nqry; b0 += kMaxItemsPerLaunch) {
+    const int64_t n = std::min(nqry - b0, kMaxItemsPerLaunch);
+    hipLaunchKernelGGL(k_select, dim3((unsigned)n), dim3(256), 0, ctx->stream, fkey, fidx, cnt,
+                       cap, nqry, L, out_key, out_idx, over_list, over_count, min_count, qlist, b0);
+    SD_HIP(hipGetLastError());
+  }
   return SD_OK;
 }
 
@@ -2051,9 +2054,12 @@ int scatter_lists(sd_context* ctx, const double* key, const int32_t* idx, const 
 int merge_lists(sd_context* ctx, const double* key, const int32_t* idx, int nlists, int64_t nqry,
                 int L, double* out_key, int32_t* out_idx) {
   if (nqry <= 0) return SD_OK;
-  hipLaunchKernelGGL(k_merge_lists, dim3((unsigned)nqry), dim3(256), 0, ctx->stream, key, idx,
-                     nlists, nqry, L, out_key, out_idx);
-  SD_HIP(hipGetLastError());
+  for (int64_t q0 = 0; q0 < nqry; q0 += kMaxItemsPerLaunch) {
+    const int64_t n = std::min(nqry - q0, kMaxItemsPerLaunch);
+    hipLaunchKernelGGL(k_merge_lists, dim3((unsigned)n), dim3(256), 0, ctx->stream, key, idx,
+                       nlists, nqry, L, out_key, out_idx, q0);
+    SD_HIP(hipGetLastError());
+  }
   return SD_OK;
 }
 

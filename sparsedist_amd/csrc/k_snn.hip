@@ -17,10 +17,11 @@ namespace {
 
 __global__ void k_validate(const int32_t* __restrict__ idx, int64_t total, int nids,
                            int* __restrict__ bad) {
-  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (e >= total) return;
-  const int32_t v = idx[e];
-  if (v != kNaInt && (v < 1 || v > nids)) atomicOr(bad, 1);
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t v = idx[e];
+    if (v != kNaInt && (v < 1 || v > nids)) atomicOr(bad, 1);
+  }
 }
 
 // One thread per node: gather, insertion-sort, de-duplicate.  self_space:
@@ -76,11 +77,9 @@ __device__ inline int inter_size(const int32_t* a, int na, const int32_t* b, int
 // read per set instead of one dependent load per merge step.  KS == 0: the
 // merge walk over rows of ks entries.
 template <int KS>
-__global__ void k_knn_weights(const int32_t* __restrict__ idx, int n, int k, int ks,
-                              const int32_t* __restrict__ sets, const int32_t* __restrict__ len,
-                              double* __restrict__ w) {
-  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (e >= (int64_t)n * k) return;
+__device__ inline double knn_weight(const int32_t* __restrict__ idx, int n, int ks,
+                                    const int32_t* __restrict__ sets,
+                                    const int32_t* __restrict__ len, int64_t e) {
   const int i = (int)(e % n);
   const int32_t v = idx[e];
   double out = __longlong_as_double((long long)kNaRealBits);
@@ -109,7 +108,16 @@ __global__ void k_knn_weights(const int32_t* __restrict__ idx, int n, int k, int
       out = (double)u / (double)(li + lj - u);
     }
   }
-  w[e] = out;
+  return out;
+}
+
+template <int KS>
+__global__ void k_knn_weights(const int32_t* __restrict__ idx, int n, int k, int ks,
+                              const int32_t* __restrict__ sets, const int32_t* __restrict__ len,
+                              double* __restrict__ w) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < (int64_t)n * k;
+       e += (int64_t)gridDim.x * blockDim.x)
+    w[e] = knn_weight<KS>(idx, n, ks, sets, len, e);
 }
 
 // Kept kNN entries (R/sparseSNN.R:226: !is.na(w) & !is.na(idx) & w > prune)
@@ -117,39 +125,41 @@ __global__ void k_knn_weights(const int32_t* __restrict__ idx, int n, int k, int
 __global__ void k_knn_emit(const int32_t* __restrict__ idx, const double* __restrict__ w, int n,
                            int k, double prune, int symmetrise, uint64_t* __restrict__ keys,
                            double* __restrict__ vals, unsigned long long* __restrict__ count) {
-  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (e >= (int64_t)n * k) return;
-  const int32_t v = idx[e];
-  const double x = w[e];
-  if (v == kNaInt || isnan(x) || !(x > prune)) return;
-  const uint64_t i = (uint64_t)(e % n), j = (uint64_t)(v - 1);
-  const unsigned long long at = atomicAdd(count, symmetrise ? 2ULL : 1ULL);
-  keys[at] = (j << 32) | i;
-  vals[at] = x;
-  if (symmetrise) {
-    keys[at + 1] = (i << 32) | j;
-    vals[at + 1] = x;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < (int64_t)n * k;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t v = idx[e];
+    const double x = w[e];
+    if (v == kNaInt || isnan(x) || !(x > prune)) continue;
+    const uint64_t i = (uint64_t)(e % n), j = (uint64_t)(v - 1);
+    const unsigned long long at = atomicAdd(count, symmetrise ? 2ULL : 1ULL);
+    keys[at] = (j << 32) | i;
+    vals[at] = x;
+    if (symmetrise) {
+      keys[at + 1] = (i << 32) | j;
+      vals[at + 1] = x;
+    }
   }
 }
 
 __global__ void k_run_heads(const uint64_t* __restrict__ keys, int64_t m,
                             int64_t* __restrict__ flag) {
-  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (e >= m) return;
-  flag[e] = (e == 0 || keys[e] != keys[e - 1]) ? 1 : 0;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < m;
+       e += (int64_t)gridDim.x * blockDim.x)
+    flag[e] = (e == 0 || keys[e] != keys[e - 1]) ? 1 : 0;
 }
 
 __global__ void k_scatter_heads(const uint64_t* __restrict__ keys, const double* __restrict__ vals,
                                 const int64_t* __restrict__ pos, int64_t m,
                                 int32_t* __restrict__ rows, double* __restrict__ out_vals,
                                 uint64_t* __restrict__ out_keys) {
-  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (e >= m) return;
-  if (e == 0 || keys[e] != keys[e - 1]) {
-    const int64_t at = pos[e];
-    rows[at] = (int32_t)(keys[e] & 0xFFFFFFFFu);
-    out_vals[at] = vals[e];
-    out_keys[at] = keys[e];
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < m;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    if (e == 0 || keys[e] != keys[e - 1]) {
+      const int64_t at = pos[e];
+      rows[at] = (int32_t)(keys[e] & 0xFFFFFFFFu);
+      out_vals[at] = vals[e];
+      out_keys[at] = keys[e];
+    }
   }
 }
 
@@ -201,8 +211,9 @@ __global__ void k_pair_count(const int64_t* __restrict__ rp, int n, int64_t* __r
 // One workgroup per m: every pair (x < y) of positions in R(m); thread x
 // writes its pairs at x*(2r-x-1)/2 + (y-x-1) past the node's offset.
 __global__ void k_pair_emit(const int64_t* __restrict__ rp, const int32_t* __restrict__ rv, int n,
-                            const int64_t* __restrict__ poff, uint64_t* __restrict__ keys) {
-  const int m = blockIdx.x;
+                            const int64_t* __restrict__ poff, uint64_t* __restrict__ keys,
+                            int64_t m0) {
+  const int64_t m = m0 + blockIdx.x;
   const int64_t b0 = rp[m], r = rp[m + 1] - b0;
   const int64_t base = poff[m];
   for (int64_t x = threadIdx.x; x < r; x += blockDim.x) {
@@ -227,20 +238,21 @@ __global__ void k_shared_emit(const uint64_t* __restrict__ pk, int64_t m,
                               const int32_t* __restrict__ len, double prune,
                               uint64_t* __restrict__ keys, double* __restrict__ vals,
                               unsigned long long* __restrict__ count) {
-  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (e >= m) return;
-  if (e > 0 && pk[e] == pk[e - 1]) return;
-  int64_t end = e + 1;
-  while (end < m && pk[end] == pk[e]) ++end;
-  const int u = (int)(end - e);
-  const uint64_t a = pk[e] >> 32, b = pk[e] & 0xFFFFFFFFu;
-  const double w = (double)u / (double)(len[a] + len[b] - u);
-  if (!(w > prune)) return;
-  const unsigned long long at = atomicAdd(count, 2ULL);
-  keys[at] = (b << 32) | a;  // row a, column b
-  vals[at] = w;
-  keys[at + 1] = (a << 32) | b;
-  vals[at + 1] = w;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < m;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    if (e > 0 && pk[e] == pk[e - 1]) continue;
+    int64_t end = e + 1;
+    while (end < m && pk[end] == pk[e]) ++end;
+    const int u = (int)(end - e);
+    const uint64_t a = pk[e] >> 32, b = pk[e] & 0xFFFFFFFFu;
+    const double w = (double)u / (double)(len[a] + len[b] - u);
+    if (!(w > prune)) continue;
+    const unsigned long long at = atomicAdd(count, 2ULL);
+    keys[at] = (b << 32) | a;  // row a, column b
+    vals[at] = w;
+    keys[at + 1] = (a << 32) | b;
+    vals[at + 1] = w;
+  }
 }
 
 int sort_pairs(sd_context* ctx, DevBuf& kin, DevBuf& vin, DevBuf& kout, DevBuf& vout, int64_t m,
@@ -280,7 +292,7 @@ int assemble_csc(sd_context* ctx, DevBuf& keys, DevBuf& vals, int64_t m, int64_t
   DevBuf& pos = ctx->snn_scratch[12];
   SD_TRY(flag.ensure(sizeof(int64_t) * (size_t)m));
   SD_TRY(pos.ensure(sizeof(int64_t) * (size_t)(m + 1)));
-  const unsigned gb = (unsigned)((m + 255) / 256);
+  const unsigned gb = stride_grid(m);
   hipLaunchKernelGGL(k_run_heads, dim3(gb), dim3(256), 0, ctx->stream, keys.as<uint64_t>(), m,
                      flag.as<int64_t>());
   size_t bytes = 0;
@@ -325,7 +337,7 @@ int snn_build(sd_context* ctx, const int32_t* d_idx, int n, int k, int include_s
   DevBuf& bad = ctx->snn_scratch[1];
   SD_TRY(bad.ensure(sizeof(int)));
   SD_HIP(hipMemsetAsync(bad.ptr, 0, sizeof(int), ctx->stream));
-  hipLaunchKernelGGL(k_validate, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, ctx->stream,
+  hipLaunchKernelGGL(k_validate, dim3(stride_grid(total)), dim3(256), 0, ctx->stream,
                      d_idx, total, nids, bad.as<int>());
   int hbad = 0;
   SD_HIP(hipMemcpyAsync(&hbad, bad.ptr, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
@@ -354,7 +366,7 @@ int snn_build(sd_context* ctx, const int32_t* d_idx, int n, int k, int include_s
       SD_TRY(w.ensure(sizeof(double) * (size_t)total));
       dw = w.as<double>();
     }
-    const unsigned ge = (unsigned)((total + 255) / 256);
+    const unsigned ge = stride_grid(total);
     switch (ks) {
 #define SD_WEIGHTS(KS)                                                                   \
   case KS:                                                                               \
@@ -457,9 +469,13 @@ int snn_build(sd_context* ctx, const int32_t* d_idx, int n, int k, int include_s
   DevBuf& pk2 = ctx->snn_scratch[22];
   SD_TRY(pk.ensure(sizeof(uint64_t) * (size_t)npairs + 16));
   SD_TRY(pk2.ensure(sizeof(uint64_t) * (size_t)npairs + 16));
-  hipLaunchKernelGGL(k_pair_emit, dim3((unsigned)nids), dim3(256), 0, ctx->stream, rp.as<int64_t>(),
-                     rv.as<int32_t>(), nids, poff.as<int64_t>(), pk.as<uint64_t>());
-  SD_HIP(hipGetLastError());
+  for (int64_t m0 = 0; m0 < nids; m0 += kMaxItemsPerLaunch) {
+    const int64_t nm = std::min<int64_t>(nids - m0, kMaxItemsPerLaunch);
+    hipLaunchKernelGGL(k_pair_emit, dim3((unsigned)nm), dim3(256), 0, ctx->stream,
+                       rp.as<int64_t>(), rv.as<int32_t>(), nids, poff.as<int64_t>(),
+                       pk.as<uint64_t>(), m0);
+    SD_HIP(hipGetLastError());
+  }
   bytes = 0;
   SD_HIP(rocprim::radix_sort_keys(nullptr, bytes, pk.as<uint64_t>(), pk2.as<uint64_t>(),
                                   (size_t)npairs, 0, end_bit, ctx->stream));
@@ -475,7 +491,7 @@ int snn_build(sd_context* ctx, const int32_t* d_idx, int n, int k, int include_s
   SD_TRY(vals.ensure(sizeof(double) * (size_t)npairs * 2 + 16));
   SD_TRY(cnt.ensure(sizeof(unsigned long long)));
   SD_HIP(hipMemsetAsync(cnt.ptr, 0, sizeof(unsigned long long), ctx->stream));
-  hipLaunchKernelGGL(k_shared_emit, dim3((unsigned)((npairs + 255) / 256)), dim3(256), 0,
+  hipLaunchKernelGGL(k_shared_emit, dim3(stride_grid(npairs)), dim3(256), 0,
                      ctx->stream, pk2.as<uint64_t>(), npairs, len.as<int32_t>(), prune,
                      keys.as<uint64_t>(), vals.as<double>(), cnt.as<unsigned long long>());
   unsigned long long m = 0;

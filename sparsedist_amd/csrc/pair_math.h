@@ -116,6 +116,38 @@ __device__ inline double man_value(const int32_t* __restrict__ ra, const double*
   return num;
 }
 
+// Jensen-Shannon distance (reference fastJS2 and xlogx,
+// src/sparseDist.cpp:206-210, 1074-1115).  Per row
+// s = xlogx(p) + xlogx(q) - xlogx(p + q) + (p + q) log 2 in that order, with
+// xlogx(x) = x log x and its NaN (0 log 0) taken as 0; the rows are summed
+// the way Armadillo's accu sums a dense vector (even rows into one
+// accumulator, odd rows into the other, then the two added), and the
+// distance is sqrt(0.5 sum) when positive, else 0.  A row stored in neither
+// column gives s = +0, which leaves either accumulator unchanged, so the walk
+// visits only the union of stored rows.  log is the device library's FP64
+// log, which may differ from the host C library's in the last bit.
+__device__ inline double js_xlogx(double x) {
+  const double r = x * log(x);
+  return (r != r) ? 0.0 : r;
+}
+__device__ inline double js_value(const int32_t* __restrict__ ra, const double* __restrict__ xa,
+                                  int64_t na, const int32_t* __restrict__ rb,
+                                  const double* __restrict__ xb, int64_t nb) {
+  const double log2 = 0x1.62e42fefa39efp-1;  // log(2), correctly rounded
+  int64_t a = 0, b = 0;
+  double acc[2] = {0.0, 0.0};
+  while (a < na || b < nb) {
+    const int32_t x = a < na ? ra[a] : INT32_MAX, y = b < nb ? rb[b] : INT32_MAX;
+    const int32_t r = x < y ? x : y;
+    const double p = (x == r) ? xa[a++] : 0.0;
+    const double q = (y == r) ? xb[b++] : 0.0;
+    const double t = js_xlogx(p) + js_xlogx(q) - js_xlogx(p + q) + (p + q) * log2;
+    acc[r & 1] += t;
+  }
+  const double v = 0.5 * (acc[0] + acc[1]);
+  return (v > 0.0) ? sqrt(v) : 0.0;
+}
+
 // Centred cross-product over all nrow rows (this project's pearson and
 // covariance definition; stored-both and stored-one-side terms in ascending
 // row order, then the (nrow - |union|) rows stored in neither column).
@@ -172,6 +204,8 @@ __device__ inline double exact_pair(const ExactArgs& A, int64_t i, const int32_t
       return euc_value(ra, A.ref.x + a0, na, qr, qv, qn);
     case SD_MANHATTAN:
       return man_value(ra, A.ref.x + a0, na, qr, qv, qn);
+    case SD_JS:
+      return js_value(ra, A.ref.x + a0, na, qr, qv, qn);
     case SD_PEARSON: {
       const double c = pear_value(A, i, qr, qv, qn, j);
       const double v = A.dist ? (1.0 - c) : c;

@@ -31,7 +31,7 @@ void clear_error() { g_err.clear(); }
 int DevBuf::alloc(size_t n) {
   release();
   if (n == 0) n = 16;
-  hipError_t e = hipMalloc(&ptr, n);
+  hipError_t e = pool ? hipMallocAsync(&ptr, n, pool) : hipMalloc(&ptr, n);
   if (e != hipSuccess) {
     ptr = nullptr;
     (void)hipGetLastError();
@@ -48,7 +48,7 @@ int DevBuf::ensure(size_t n) {
 }
 
 void DevBuf::release() {
-  if (ptr) (void)hipFree(ptr);
+  if (ptr) (void)(pool ? hipFreeAsync(ptr, pool) : hipFree(ptr));
   ptr = nullptr;
   bytes = 0;
 }
@@ -66,6 +66,108 @@ KernelTimer::~KernelTimer() {
     if (hipEventElapsedTime(&ms, ctx->ev_a, ctx->ev_b) == hipSuccess) *acc += ms;
     *count += 1;
   }
+}
+
+int context_use_pool(sd_context* ctx) {
+  hipMemPool_t pool = nullptr;
+  SD_HIP(hipDeviceGetDefaultMemPool(&pool, ctx->device));
+  // keep freed blocks in the pool instead of returning them at each
+  // synchronisation, so a worker's buffers are reused without new allocations
+  uint64_t keep = UINT64_MAX;
+  SD_HIP(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep));
+  for (auto& b : ctx->scratch) b.pool = ctx->stream;
+  for (auto& b : ctx->snn_scratch) b.pool = ctx->stream;
+  for (DevBuf* b : {&ctx->xm.thr, &ctx->xm.cnt, &ctx->xm.fkey, &ctx->xm.fidx})
+    b->pool = ctx->stream;
+  return SD_OK;
+}
+
+void matrix_use_pool(sd_matrix* m, hipStream_t stream) {
+  for (DevBuf* b : {&m->p, &m->i, &m->x, &m->deg, &m->unit, &m->scale, &m->unit_ssq, &m->mean,
+                    &m->css, &m->cmf, &m->cif, &m->sqn, &m->sqf})
+    b->pool = stream;
+  for (auto& b : m->dense) b.pool = stream;
+}
+
+// Validate a CSC matrix, drop explicit zeros, compute the column flags and
+// copy it into m's device buffers (grown only when too small), discarding
+// every layout derived from earlier contents.
+int matrix_load(sd_context* ctx, sd_matrix* m, int64_t nrow, int64_t ncol, const int64_t* colptr,
+                const int32_t* rowind, const double* values) {
+  if (!colptr) return set_error(SD_EINVAL, "NULL argument");
+  if (nrow < 0 || ncol < 0) return set_error(SD_EINVAL, "negative dimensions");
+  if (nrow >= ((int64_t)1 << 31)) return set_error(SD_EINVAL, "nrow must be below 2^31");
+  if (colptr[0] != 0) return set_error(SD_EINVAL, "colptr[0] must be 0");
+  const int64_t nnz_in = colptr[ncol];
+  if (nnz_in > 0 && (!rowind || !values)) return set_error(SD_EINVAL, "NULL argument");
+  // validate, drop explicit zeros, flags (one host pass)
+  std::vector<int64_t> p((size_t)ncol + 1);
+  std::vector<int32_t> ri;
+  std::vector<double> vx;
+  ri.reserve((size_t)nnz_in);
+  vx.reserve((size_t)nnz_in);
+  std::vector<uint8_t> hdeg((size_t)ncol, 0);
+  bool any_nonfinite = false, any_negative = false;
+  double abs_min = INFINITY, abs_max = 0.0;
+  p[0] = 0;
+  for (int64_t j = 0; j < ncol; ++j) {
+    if (colptr[j + 1] < colptr[j]) return set_error(SD_EINVAL, "colptr must be non-decreasing");
+    int32_t prev = -1;
+    for (int64_t e = colptr[j]; e < colptr[j + 1]; ++e) {
+      const int32_t r = rowind[e];
+      if (r < 0 || r >= nrow || r <= prev)
+        return set_error(SD_EINVAL,
+                         "row indices must be in [0, nrow) and strictly ascending per column");
+      prev = r;
+      const double v = values[e];
+      if (v == 0.0) continue;
+      if (!std::isfinite(v)) {
+        any_nonfinite = true;
+      } else {
+        abs_min = std::min(abs_min, std::fabs(v));
+        abs_max = std::max(abs_max, std::fabs(v));
+      }
+      if (v < 0.0) any_negative = true;
+      ri.push_back(r);
+      vx.push_back(v);
+    }
+    p[(size_t)j + 1] = (int64_t)ri.size();
+    // colSums(abs(.)) == 0 <=> no stored non-zero value (NaN is not zero)
+    hdeg[(size_t)j] = (p[(size_t)j + 1] == p[(size_t)j]) ? 1 : 0;
+  }
+  const int64_t nnz = (int64_t)ri.size();
+  SD_TRY(m->p.ensure(sizeof(int64_t) * (size_t)(ncol + 1)));
+  SD_TRY(m->i.ensure(sizeof(int32_t) * (size_t)(nnz + 1)));
+  SD_TRY(m->x.ensure(sizeof(double) * (size_t)(nnz + 1)));
+  SD_TRY(m->deg.ensure((size_t)ncol + 16));
+  SD_HIP(hipMemcpyAsync(m->p.ptr, p.data(), sizeof(int64_t) * (size_t)(ncol + 1),
+                        hipMemcpyHostToDevice, ctx->stream));
+  if (nnz) {
+    SD_HIP(hipMemcpyAsync(m->i.ptr, ri.data(), sizeof(int32_t) * (size_t)nnz,
+                          hipMemcpyHostToDevice, ctx->stream));
+    SD_HIP(hipMemcpyAsync(m->x.ptr, vx.data(), sizeof(double) * (size_t)nnz,
+                          hipMemcpyHostToDevice, ctx->stream));
+  }
+  if (ncol)
+    SD_HIP(hipMemcpyAsync(m->deg.ptr, hdeg.data(), (size_t)ncol, hipMemcpyHostToDevice,
+                          ctx->stream));
+  SD_HIP(hipStreamSynchronize(ctx->stream));
+  m->ctx = ctx;
+  m->device = ctx->device;
+  m->nrow = nrow;
+  m->ncol = ncol;
+  m->nnz = nnz;
+  m->hp = std::move(p);
+  m->hdeg = std::move(hdeg);
+  m->any_nonfinite = any_nonfinite;
+  m->any_negative = any_negative;
+  m->abs_min = abs_min;
+  m->abs_max = abs_max;
+  m->unit_ready = m->centre_ready = m->sq_ready = false;
+  m->tuned_margin = 0;
+  m->raw_split = false;
+  for (int t = 0; t < kDenseKinds; ++t) m->dense_kpad[t] = 0;
+  return SD_OK;
 }
 
 }  // namespace sd
@@ -97,11 +199,14 @@ int method_direction(int method, int dist, int* dec) {
       return SD_OK;
     case SD_EUCLIDEAN:
     case SD_MANHATTAN:
+    case SD_JS:
       if (!dist)
         return set_error(SD_EINVAL,
                          "'dist = FALSE' is only meaningful for methods \"binary\", \"cosine\" and "
                          "\"pearson\"; %s has no similarity form.",
-                         method == SD_EUCLIDEAN ? "euclidean" : "manhattan");
+                         method == SD_EUCLIDEAN ? "euclidean"
+                         : method == SD_JS      ? "js"
+                                                : "manhattan");
       *dec = 0;
       return SD_OK;
     default:
@@ -123,6 +228,10 @@ int prep_method(sd_context* ctx, sd_matrix* m, int method) {
     return prep_unit(ctx, m);
   }
   if (method == SD_PEARSON || method == SD_COVARIANCE) return prep_centre(ctx, m);
+  if (method == SD_JS && (m->any_nonfinite || m->any_negative))
+    return set_error(SD_EINVAL,
+                     "fastJS2 requires finite, non-negative input (columns should be normalised "
+                     "to sum to 1 for a true JS divergence).");
   return SD_OK;
 }
 
@@ -158,6 +267,9 @@ int exact_knn_columns(sd_context* ctx, const ExactArgs& a, const int32_t* d_qcol
   DevBuf& D = ctx->scratch[0];
   SD_TRY(D.ensure(sizeof(double) * (size_t)nref * (size_t)nb + 64));
   for (int64_t c0 = 0; c0 < ncols; c0 += nb) {
+    // the host waits for each block before queueing the next, so a cancel
+    // request stops the search within one block
+    if (c0 > 0) SD_HIP(hipStreamSynchronize(ctx->stream));
     SD_TRY(check_cancel(ctx));
     const int64_t n = std::min(nb, ncols - c0);
     SD_TRY(exact_slab(ctx, a, d_qcols + c0, n, D.as<double>(), nref, 1));
@@ -760,6 +872,9 @@ template <class F>
 int guarded(F&& f) {
   try {
     clear_error();
+    // a HIP error left on this thread by other code would otherwise surface
+    // at this call's first launch check
+    (void)hipGetLastError();
     return f();
   } catch (const std::bad_alloc&) {
     return set_error(SD_ENOMEM, "host allocation failed");
@@ -767,6 +882,18 @@ int guarded(F&& f) {
     return set_error(SD_EDEVICE, "unexpected internal error");
   }
 }
+
+// A cancellable call on a context.  A cancel request stays set until the
+// call it reaches returns: one sent just before the call starts (the caller's
+// interrupt poll can win the race with its worker thread) still stops it, and
+// the next call starts clean.
+struct CancelScope {
+  sd_context* ctx;
+  explicit CancelScope(sd_context* c) : ctx(c) {}
+  ~CancelScope() {
+    if (ctx) ctx->cancel.store(0);
+  }
+};
 
 }  // namespace
 
@@ -805,7 +932,11 @@ int sd_context_destroy(sd_context* ctx) {
   if (!ctx) return SD_OK;
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
+  // pooled buffers are freed on the stream, so before it goes
   for (auto& b : ctx->scratch) b.release();
+  for (auto& b : ctx->snn_scratch) b.release();
+  for (DevBuf* b : {&ctx->xm.thr, &ctx->xm.cnt, &ctx->xm.fkey, &ctx->xm.fidx}) b->release();
+  (void)hipStreamSynchronize(ctx->stream);
   if (ctx->ev_a) (void)hipEventDestroy(ctx->ev_a);
   if (ctx->ev_b) (void)hipEventDestroy(ctx->ev_b);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -837,94 +968,35 @@ void* sd_context_stream(sd_context* ctx) { return ctx ? (void*)ctx->stream : nul
 int sd_matrix_upload_csc(sd_context* ctx, int64_t nrow, int64_t ncol, const int64_t* colptr,
                          const int32_t* rowind, const double* values, sd_matrix** out) {
   return guarded([&]() -> int {
-    if (!ctx || !out || !colptr) return set_error(SD_EINVAL, "NULL argument");
-    if (nrow < 0 || ncol < 0) return set_error(SD_EINVAL, "negative dimensions");
-    if (nrow >= ((int64_t)1 << 31)) return set_error(SD_EINVAL, "nrow must be below 2^31");
+    if (!ctx || !out) return set_error(SD_EINVAL, "NULL argument");
     SD_HIP(hipSetDevice(ctx->device));
-    if (colptr[0] != 0) return set_error(SD_EINVAL, "colptr[0] must be 0");
-    const int64_t nnz_in = colptr[ncol];
-    // validate, drop explicit zeros, flags (one host pass)
-    std::vector<int64_t> p((size_t)ncol + 1);
-    std::vector<int32_t> ri;
-    std::vector<double> vx;
-    ri.reserve((size_t)nnz_in);
-    vx.reserve((size_t)nnz_in);
     auto* m = new sd_matrix();
-    m->ctx = ctx;
-    m->nrow = nrow;
-    m->ncol = ncol;
-    m->hdeg.assign((size_t)ncol, 0);
-    p[0] = 0;
-    for (int64_t j = 0; j < ncol; ++j) {
-      if (colptr[j + 1] < colptr[j]) {
-        delete m;
-        return set_error(SD_EINVAL, "colptr must be non-decreasing");
-      }
-      int32_t prev = -1;
-      for (int64_t e = colptr[j]; e < colptr[j + 1]; ++e) {
-        const int32_t r = rowind[e];
-        if (r < 0 || r >= nrow || r <= prev) {
-          delete m;
-          return set_error(SD_EINVAL,
-                           "row indices must be in [0, nrow) and strictly ascending per column");
-        }
-        prev = r;
-        const double v = values[e];
-        if (v == 0.0) continue;
-        if (!std::isfinite(v)) {
-          m->any_nonfinite = true;
-        } else {
-          m->abs_min = std::min(m->abs_min, std::fabs(v));
-          m->abs_max = std::max(m->abs_max, std::fabs(v));
-        }
-        if (v < 0.0) m->any_negative = true;
-        ri.push_back(r);
-        vx.push_back(v);
-      }
-      p[(size_t)j + 1] = (int64_t)ri.size();
-      // colSums(abs(.)) == 0 <=> no stored non-zero value (NaN is not zero)
-      m->hdeg[(size_t)j] = (p[(size_t)j + 1] == p[(size_t)j]) ? 1 : 0;
-    }
-    m->nnz = (int64_t)ri.size();
-    int rc = SD_OK;
-    if ((rc = m->p.alloc(sizeof(int64_t) * (size_t)(ncol + 1))) != SD_OK ||
-        (rc = m->i.alloc(sizeof(int32_t) * (size_t)(m->nnz + 1))) != SD_OK ||
-        (rc = m->x.alloc(sizeof(double) * (size_t)(m->nnz + 1))) != SD_OK ||
-        (rc = m->deg.alloc((size_t)ncol + 16)) != SD_OK) {
+    const int rc = matrix_load(ctx, m, nrow, ncol, colptr, rowind, values);
+    if (rc != SD_OK) {
       delete m;
       return rc;
     }
-    hipError_t e = hipMemcpy(m->p.ptr, p.data(), sizeof(int64_t) * (size_t)(ncol + 1),
-                             hipMemcpyHostToDevice);
-    if (e == hipSuccess && m->nnz)
-      e = hipMemcpy(m->i.ptr, ri.data(), sizeof(int32_t) * (size_t)m->nnz, hipMemcpyHostToDevice);
-    if (e == hipSuccess && m->nnz)
-      e = hipMemcpy(m->x.ptr, vx.data(), sizeof(double) * (size_t)m->nnz, hipMemcpyHostToDevice);
-    if (e == hipSuccess && ncol)
-      e = hipMemcpy(m->deg.ptr, m->hdeg.data(), (size_t)ncol, hipMemcpyHostToDevice);
-    if (e != hipSuccess) {
-      delete m;
-      return set_error(SD_EDEVICE, "upload failed: %s", hipGetErrorString(e));
-    }
-    m->hp = std::move(p);
     *out = m;
     return SD_OK;
   });
 }
 
+// Freeing waits for work that may still read the buffers: hipFree
+// synchronises the device, and pooled buffers are freed in order on their
+// stream.  The matrix's own context may be gone already, so it is not used.
 int sd_matrix_free(sd_matrix* m) {
   if (!m) return SD_OK;
-  (void)hipSetDevice(m->ctx->device);
-  (void)hipStreamSynchronize(m->ctx->stream);
+  (void)hipSetDevice(m->device);
   delete m;
   return SD_OK;
 }
 
 int sd_matrix_drop_cache(sd_matrix* m) {
   if (!m) return SD_OK;
-  (void)hipStreamSynchronize(m->ctx->stream);
+  (void)hipSetDevice(m->device);
   m->unit.release();
   m->scale.release();
+  m->unit_ssq.release();
   m->unit_ready = false;
   m->mean.release();
   m->css.release();
@@ -947,7 +1019,7 @@ int sd_knn(sd_context* ctx, const sd_matrix* X, const sd_matrix* Y, const sd_knn
     if (!ctx || !X || !params || !idx || !dist || !k_eff || !warn)
       return set_error(SD_EINVAL, "NULL argument");
     SD_HIP(hipSetDevice(ctx->device));
-    ctx->cancel.store(0);
+    CancelScope scope(ctx);
     KnnPlan plan;
     int64_t avail = 0;
     SD_TRY(plan_knn(X, Y, params, &plan, &avail));
@@ -966,7 +1038,7 @@ int sd_knn_cross(sd_context* ctx, const sd_matrix* X, const sd_matrix* Y,
         !k_rev || !warn)
       return set_error(SD_EINVAL, "NULL argument");
     SD_HIP(hipSetDevice(ctx->device));
-    ctx->cancel.store(0);
+    CancelScope scope(ctx);
     if (params->k < 1)
       return set_error(SD_EINVAL,
                        "'k' must be a positive integer no larger than .Machine$integer.max.");
@@ -1017,9 +1089,16 @@ int sd_knn_cross(sd_context* ctx, const sd_matrix* X, const sd_matrix* Y,
       ctx->stats.fused_calls = fused;                               // counted when it resumes
       xm.phase = 2;
     }
-    SD_TRY(run_knn(ctx, Xm, Ym, params, pf, fwd_idx, fwd_dist));
+    // the forward results wait in host memory until the reverse search has
+    // succeeded, so a cancelled or failed call leaves every output untouched
+    std::vector<int32_t> fi((size_t)X->ncol * (size_t)pf.k);
+    std::vector<double> fd((size_t)X->ncol * (size_t)pf.k);
+    SD_TRY(run_knn(ctx, Xm, Ym, params, pf, fi.data(), fd.data()));
     if (xm.phase == 2) xm.phase = 3;
-    return run_knn(ctx, Ym, Xm, params, pr, rev_idx, rev_dist);
+    SD_TRY(run_knn(ctx, Ym, Xm, params, pr, rev_idx, rev_dist));
+    std::copy(fi.begin(), fi.end(), fwd_idx);
+    std::copy(fd.begin(), fd.end(), fwd_dist);
+    return SD_OK;
   });
 }
 
@@ -1032,7 +1111,7 @@ int sd_slab(sd_context* ctx, int32_t method, const sd_matrix* X, const sd_matrix
       return set_error(SD_EINVAL, "Mismatched row dimensions of m (%lld) and m2 (%lld).",
                        (long long)X->nrow, (long long)Y->nrow);
     int dec;
-    if (method == SD_EUCLIDEAN || method == SD_MANHATTAN) dist = 1;
+    if (method == SD_EUCLIDEAN || method == SD_MANHATTAN || method == SD_JS) dist = 1;
     SD_TRY(method_direction(method, dist, &dec));
     sd_matrix* m = const_cast<sd_matrix*>(X);
     sd_matrix* m2 = const_cast<sd_matrix*>(Y);
@@ -1154,7 +1233,7 @@ int sd_snn(sd_context* ctx, const int32_t* idx, int32_t n, int32_t k, int32_t in
       return set_error(SD_EINVAL, "'prune' must be a finite number between 0 and 1.");
     if (edges != 0 && edges != 1) return set_error(SD_EINVAL, "edges must be 0 (knn) or 1 (shared)");
     SD_HIP(hipSetDevice(ctx->device));
-    ctx->cancel.store(0);
+    CancelScope scope(ctx);
     DevBuf& di = ctx->snn_scratch[0];
     const int32_t* d_idx = idx;
     if (!idx_on_device) {

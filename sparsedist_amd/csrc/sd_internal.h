@@ -42,15 +42,27 @@ void clear_error();
 // ---------------------------------------------------------------------------
 // device buffers
 // ---------------------------------------------------------------------------
+// A device allocation.  With `pool` set (the worker contexts of a server),
+// memory comes from the device's stream-ordered pool on that stream
+// (hipMallocAsync / hipFreeAsync), so growing a buffer never synchronises
+// the device the way hipFree does; otherwise hipMalloc / hipFree.
 struct DevBuf {
   void* ptr = nullptr;
   size_t bytes = 0;
+  hipStream_t pool = nullptr;
   DevBuf() = default;
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
-  DevBuf(DevBuf&& o) noexcept : ptr(o.ptr), bytes(o.bytes) { o.ptr = nullptr; o.bytes = 0; }
+  DevBuf(DevBuf&& o) noexcept : ptr(o.ptr), bytes(o.bytes), pool(o.pool) {
+    o.ptr = nullptr;
+    o.bytes = 0;
+  }
   DevBuf& operator=(DevBuf&& o) noexcept {
-    if (this != &o) { release(); ptr = o.ptr; bytes = o.bytes; o.ptr = nullptr; o.bytes = 0; }
+    if (this != &o) {
+      release();
+      ptr = o.ptr; bytes = o.bytes; pool = o.pool;
+      o.ptr = nullptr; o.bytes = 0;
+    }
     return *this;
   }
   ~DevBuf() { release(); }
@@ -142,7 +154,10 @@ struct sd_context {
 };
 
 struct sd_matrix {
+  // the context it was loaded on; the matrix may outlive it, so only the
+  // device ordinal is used afterwards
   sd_context* ctx = nullptr;
+  int device = 0;
   // Held while derived layouts are built and until they are complete on the
   // building stream, so contexts on other streams (server workers) that
   // share the matrix never read a half-built layout.
@@ -157,7 +172,7 @@ struct sd_matrix {
   sd::DevBuf deg;                  // uint8 per column: all stored values == 0
   // cosine: unit-normalised values (nnz) and per-column max |v|
   bool unit_ready = false;
-  sd::DevBuf unit, scale;
+  sd::DevBuf unit, scale, unit_ssq;
   // pearson / covariance: per-column mean and centred sum of squares, and
   // their FP32 forms for the slab epilogue (sqrt(nrow) mean, 1 / sqrt(css))
   bool centre_ready = false;
@@ -199,6 +214,17 @@ namespace sd {
 enum DenseKind { kDenseF4Pattern = 0, kDenseBf16Unit = 1, kDenseBf16Split = 2, kDenseF32Unit = 3,
                  kDenseF32Raw = 4, kDenseF32Pattern = 5, kDenseBf16Raw = 6, kDenseKinds = 7 };
 static_assert(sizeof(sd_matrix::dense) / sizeof(DevBuf) == kDenseKinds, "one layout slot per kind");
+
+// Serve every device buffer of ctx (and of matrices loaded on it with
+// matrix_load into buffers given the same pool) from the device's
+// stream-ordered pool on ctx's stream, which keeps freed memory for reuse.
+int context_use_pool(sd_context* ctx);
+void matrix_use_pool(sd_matrix* m, hipStream_t stream);
+
+// Validate a host CSC matrix and copy it into m (device buffers grown only
+// when too small; derived layouts of earlier contents are discarded).
+int matrix_load(sd_context* ctx, sd_matrix* m, int64_t nrow, int64_t ncol, const int64_t* colptr,
+                const int32_t* rowind, const double* values);
 
 // ---- prep (k_prep.hip) ------------------------------------------------------
 int prep_flags(sd_context* ctx, sd_matrix* m);                 // deg, nonfinite, negative
@@ -351,6 +377,17 @@ int topk_rows(sd_context* ctx, const double* D, int64_t nrow, int64_t ncol, int 
 // ("shared") applies and a node is never its own neighbour.
 int snn_build(sd_context* ctx, const int32_t* d_idx, int n, int k, int include_self, double prune,
               int symmetrise, int edges, sd_graph* g, double* d_w_out, int nids = 0);
+
+// One-workgroup-per-item launches are cut into grids of at most this many
+// workgroups: a dispatch packet counts its work-items in 32 bits, so a 1-D
+// grid of 256-thread workgroups must stay below 2^24 of them.
+constexpr int64_t kMaxItemsPerLaunch = (int64_t)1 << 22;
+// Grid of an elementwise kernel that strides over n items (capped; the
+// kernels loop).
+inline unsigned stride_grid(int64_t n) {
+  const int64_t b = (n + 255) / 256;
+  return (unsigned)(b < 1 ? 1 : (b > (1 << 20) ? (1 << 20) : b));
+}
 
 // ---- timing helpers ------------------------------------------------------------
 struct KernelTimer {

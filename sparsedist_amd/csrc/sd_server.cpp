@@ -53,11 +53,13 @@ struct Request {
   int snn = 0;
   double prune = 0.0;
   int64_t reserve = 0;  // bytes reserved while admitted
+  int bypassed = 0;     // requests admitted ahead of it while it waited
   // progress (guarded by the server mutex)
   int64_t next_col = 0;   // first unscheduled column
   int64_t done_cols = 0;  // columns whose results are in place
   int inflight = 0;       // items taken and not yet returned
   bool admitted = false, finished = false;
+  int64_t finish_order = 0;
   int status = SD_OK;
   std::string error;
   int32_t k_eff = 0, warn = 0;
@@ -78,10 +80,16 @@ struct sd_server {
   int64_t budget = 0;
   int64_t max_pending = 0;
   std::vector<sd_context*> ctx;
+  // per slot: the query matrix of its current item, reloaded in place so that
+  // items allocate no device memory once its buffers have grown
+  std::vector<sd_matrix*> qmat;
   std::vector<std::thread> workers;
   std::mutex mu;
-  std::condition_variable cv_work, cv_done;
+  std::condition_variable cv_work, cv_done, cv_gone;
   bool stop = false;
+  bool paused = false;
+  int64_t finished_count = 0;
+  int waiters = 0;  // callers blocked in sd_server_wait (destroy waits for them)
   int64_t next_ticket = 1;
   std::map<int64_t, std::unique_ptr<Request>> all;
   std::deque<Request*> pending;
@@ -113,22 +121,37 @@ int64_t reserve_estimate(const sd_server* s, const Request& r) {
   return b + (1 << 20);
 }
 
-// Admit waiting requests in arrival order while their reservations fit.
+// A waiting request that does not fit lets later requests that fit pass it
+// until it has waited kAgeMs or been passed kMaxBypass times; from then on
+// nothing behind it is admitted first, so no request waits for ever.
+constexpr double kAgeMs = 2000.0;
+constexpr int kMaxBypass = 64;
+
+// Admit waiting requests in arrival order while their reservations fit,
+// letting fitting requests pass blocked ones that have not aged yet.
 void admit_locked(sd_server* s) {
-  while (!s->pending.empty()) {
-    Request* r = s->pending.front();
-    if (s->reserved + r->reserve > s->budget) break;
-    s->pending.pop_front();
-    s->reserved += r->reserve;
-    r->admitted = true;
-    r->t_admit = Clock::now();
-    s->active.push_back(r);
-    ++s->st.admitted;
+  const auto now = Clock::now();
+  for (auto it = s->pending.begin(); it != s->pending.end();) {
+    Request* r = *it;
+    if (s->reserved + r->reserve <= s->budget) {
+      // every request still waiting ahead of r is passed once more
+      for (auto jt = s->pending.begin(); jt != it; ++jt) ++(*jt)->bypassed;
+      it = s->pending.erase(it);
+      s->reserved += r->reserve;
+      r->admitted = true;
+      r->t_admit = now;
+      s->active.push_back(r);
+      ++s->st.admitted;
+      continue;
+    }
+    if (r->bypassed >= kMaxBypass || ms_between(r->t_submit, now) >= kAgeMs) break;
+    ++it;
   }
 }
 
 void finish_locked(sd_server* s, Request* r) {
   r->finished = true;
+  r->finish_order = ++s->finished_count;
   r->t_done = Clock::now();
   s->reserved -= r->reserve;
   s->active.erase(std::remove(s->active.begin(), s->active.end(), r), s->active.end());
@@ -170,7 +193,7 @@ bool same_search(const sd_knn_params& a, const sd_knn_params& b) {
 // slot's context; each piece's results go to its own request.  A query's
 // result does not depend on the other columns of the search, so packing
 // changes no answer.
-int run_segs(sd_server* s, sd_context* ctx, const std::vector<Seg>& segs) {
+int run_segs(sd_server* s, sd_context* ctx, sd_matrix* X, const std::vector<Seg>& segs) {
   const Request* r0 = segs[0].r;
   int64_t nc = 0, nnz = 0;
   for (const Seg& g : segs) {
@@ -202,17 +225,14 @@ int run_segs(sd_server* s, sd_context* ctx, const std::vector<Seg>& segs) {
     rows = ri.data();
     vals = vx.data();
   }
-  sd_matrix* X = nullptr;
-  SD_TRY(sd_matrix_upload_csc(ctx, r0->nrow, nc, p.data(), rows, vals, &X));
+  SD_TRY(sd::matrix_load(ctx, X, r0->nrow, nc, p.data(), rows, vals));
   const int32_t k = r0->knn.k;
   std::vector<int32_t> idx((size_t)nc * k);
   std::vector<double> dist((size_t)nc * k);
   int32_t keff = 0, warn = 0;
   sd_knn_params prm = r0->knn;
   prm.include_self = 0;
-  int rc = sd_knn(ctx, X, s->Y, &prm, idx.data(), dist.data(), &keff, &warn);
-  sd_matrix_free(X);
-  if (rc != SD_OK) return rc;
+  SD_TRY(sd_knn(ctx, X, s->Y, &prm, idx.data(), dist.data(), &keff, &warn));
   // columns [c0, c1) of each request belong to this item alone, so no lock
   // is needed here; k_eff is a property of (Y, k) and equal for every item.
   int64_t off = 0;
@@ -243,8 +263,8 @@ int run_snn(sd_server* s, sd_context* ctx, Request* r) {
   const int32_t k = r->k_eff;
   if (n >= ((int64_t)1 << 31) || s->Y->ncol >= ((int64_t)1 << 31) || k < 1)
     return sd::set_error(SD_EINVAL, "SNN needs fewer than 2^31 columns and k >= 1");
-  sd::DevBuf di;
-  SD_TRY(di.alloc(sizeof(int32_t) * (size_t)n * (size_t)k));
+  sd::DevBuf& di = ctx->snn_scratch[0];
+  SD_TRY(di.ensure(sizeof(int32_t) * (size_t)n * (size_t)k));
   SD_HIP(hipMemcpyAsync(di.ptr, r->idx.data(), sizeof(int32_t) * (size_t)n * (size_t)k,
                         hipMemcpyHostToDevice, ctx->stream));
   auto* g = new sd_graph();
@@ -278,13 +298,14 @@ Request* pick_compatible_locked(sd_server* s, const std::vector<Seg>& segs) {
 
 void worker(sd_server* s, int slot) {
   sd_context* ctx = s->ctx[slot];
+  sd_matrix* X = s->qmat[slot];
   (void)hipSetDevice(s->device);
   constexpr int64_t kTile = 256;  // query columns of one slab tile
   for (;;) {
     std::vector<Seg> segs;
     {
       std::unique_lock<std::mutex> lk(s->mu);
-      s->cv_work.wait(lk, [&] { return s->stop || pick_locked(s) != nullptr; });
+      s->cv_work.wait(lk, [&] { return s->stop || (!s->paused && pick_locked(s) != nullptr); });
       if (s->stop) return;
       Request* r = pick_locked(s);
       const int64_t c0 = r->next_col, c1 = std::min(r->ncol, c0 + s->chunk);
@@ -311,15 +332,17 @@ void worker(sd_server* s, int slot) {
     // (say, non-finite input of one request) is charged to its own request
     std::vector<int> rcs(segs.size(), SD_OK);
     std::vector<std::string> msgs(segs.size());
-    int rc = guarded_call([&] { return run_segs(s, ctx, segs); });
-    if (rc != SD_OK && segs.size() > 1) {
+    int rc = guarded_call([&] { return run_segs(s, ctx, X, segs); });
+    if (rc != SD_OK && rc != SD_ECANCELLED && segs.size() > 1) {
       for (size_t i = 0; i < segs.size(); ++i) {
-        rcs[i] = guarded_call([&] { return run_segs(s, ctx, {segs[i]}); });
+        rcs[i] = guarded_call([&] { return run_segs(s, ctx, X, {segs[i]}); });
         if (rcs[i] != SD_OK) msgs[i] = sd_last_error();
       }
     } else if (rc != SD_OK) {
-      rcs[0] = rc;
-      msgs[0] = sd_last_error();
+      for (size_t i = 0; i < segs.size(); ++i) {
+        rcs[i] = rc;
+        msgs[i] = sd_last_error();
+      }
     }
     for (size_t i = 0; i < segs.size(); ++i) {
       Request* r = segs[i].r;
@@ -379,14 +402,27 @@ int sd_server_create(sd_context* ctx, const sd_matrix* Y, const sd_server_params
     s->chunk = p.chunk_cols ? p.chunk_cols : 4096;
     s->budget = p.budget_bytes ? p.budget_bytes : ((int64_t)16 << 30);
     s->max_pending = p.max_pending;
+    s->paused = p.paused != 0;
+    auto undo = [&] {
+      for (sd_matrix* m : s->qmat) sd_matrix_free(m);
+      for (sd_context* d : s->ctx) sd_context_destroy(d);
+    };
     for (int i = 0; i < s->slots; ++i) {
       sd_context* c = nullptr;
-      const int rc = sd_context_create(s->device, &c);
+      int rc = sd_context_create(s->device, &c);
+      if (rc == SD_OK) {
+        s->ctx.push_back(c);
+        rc = sd::context_use_pool(c);
+      }
       if (rc != SD_OK) {
-        for (sd_context* d : s->ctx) sd_context_destroy(d);
+        undo();
         return rc;
       }
-      s->ctx.push_back(c);
+      auto* m = new sd_matrix();
+      m->ctx = c;
+      m->device = c->device;
+      sd::matrix_use_pool(m, c->stream);
+      s->qmat.push_back(m);
     }
     for (int i = 0; i < s->slots; ++i) s->workers.emplace_back(worker, s.get(), i);
     *out = s.release();
@@ -471,6 +507,15 @@ int sd_server_wait(sd_server* s, int64_t ticket, double timeout_s, sd_request_in
     Request* r = find_locked(s, ticket);
     if (!r) return sd::set_error(SD_EINVAL, "unknown ticket %lld", (long long)ticket);
     auto ready = [&] { return r->finished; };
+    // sd_server_destroy finishes every request and then waits until no
+    // caller is left in here before it frees the server
+    ++s->waiters;
+    struct Leave {
+      sd_server* s;
+      ~Leave() {
+        if (--s->waiters == 0) s->cv_gone.notify_all();
+      }
+    } leave{s};
     if (timeout_s < 0)
       s->cv_done.wait(lk, ready);
     else
@@ -480,6 +525,7 @@ int sd_server_wait(sd_server* s, int64_t ticket, double timeout_s, sd_request_in
     info->status = r->status;
     info->k_eff = r->k_eff;
     info->warn = r->warn;
+    info->finish_order = r->finish_order;
     if (r->admitted) info->queued_ms = ms_between(r->t_submit, r->t_admit);
     if (r->finished) {
       info->run_ms = ms_between(r->t_admit, r->t_done);
@@ -538,6 +584,17 @@ int sd_server_stats_get(sd_server* s, sd_server_stats* out) {
   out->pending = (int64_t)s->pending.size();
   out->active = (int64_t)s->active.size();
   out->reserved_bytes = s->reserved;
+  out->waiting_callers = s->waiters;
+  return SD_OK;
+}
+
+int sd_server_resume(sd_server* s) {
+  if (!s) return sd::set_error(SD_EINVAL, "NULL argument");
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->paused = false;
+  }
+  s->cv_work.notify_all();
   return SD_OK;
 }
 
@@ -551,20 +608,26 @@ int sd_server_destroy(sd_server* s) {
   s->cv_work.notify_all();
   for (auto& t : s->workers) t.join();
   {
-    std::lock_guard<std::mutex> lk(s->mu);
+    std::unique_lock<std::mutex> lk(s->mu);
     for (auto& kv : s->all) {
       Request* r = kv.second.get();
       if (!r->finished) {
         r->finished = true;
+        r->finish_order = ++s->finished_count;
+        r->t_done = Clock::now();
         if (r->status == SD_OK) {
           r->status = SD_ECANCELLED;
           r->error = "Computation interrupted by the user.";
         }
       }
-      if (r->graph) sd_graph_free(r->graph);
     }
     s->cv_done.notify_all();
+    // callers blocked in sd_server_wait read their request once more
+    s->cv_gone.wait(lk, [&] { return s->waiters == 0; });
+    for (auto& kv : s->all)
+      if (kv.second->graph) sd_graph_free(kv.second->graph);
   }
+  for (sd_matrix* m : s->qmat) sd_matrix_free(m);
   for (sd_context* c : s->ctx) sd_context_destroy(c);
   delete s;
   return SD_OK;

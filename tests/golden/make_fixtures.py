@@ -130,6 +130,27 @@ fixtures = {
         "prune_values": [0.0, 0.2, 0.25, 0.5, 2 / 3],
         "prune_one_empty": True,
     },
+    "js_standard_metric": {
+        "source": "tests/testthat/test-sparseDist.R:100-115",
+        # two point masses: d(a, b) = sqrt(log 2), the metric's upper bound
+        "P": cols([1, 0], [0, 1]),
+        "expect_d12": "sqrt(log(2))",
+        "bound": "sqrt(log(2)) + 1e-12 for normalised columns",
+    },
+    "js_invalid_input": {
+        "source": "tests/testthat/test-sparseDist.R:117-124",
+        "cases": [{"M": [[-1.0, 2.0], [3.0, 4.0]], "error": "non-negative"},
+                  {"M": [[None, 2.0], [3.0, 4.0]], "error": "finite"}],
+    },
+    "js_self_exact_zero": {
+        "source": "tests/testthat/test-sparseKNN.R:271-277",
+        "k": 2, "include_self": True,
+        "expect": "dist[, 1] identical to 0 and idx[, 1] the column itself",
+    },
+    "distance_only_methods": {
+        "source": "tests/testthat/test-sparseKNN.R:209-217",
+        "methods": ["euclidean", "manhattan", "js"], "error": "only meaningful",
+    },
 }
 
 if __name__ == "__main__":

@@ -1,12 +1,16 @@
 """The request loop (sd_server): each result is the same alone and in the mix,
 small requests overtake large ones, admission honours the memory budget.
 
-kNN results are compared bit for bit with one-shot sparseKNN(X, Y) calls (the
-path the parity tests pin against the oracle).  The SNN graph of a request is
-checked against a brute force over all query pairs: queries i and j are
-joined with weight |S_i & S_j| / |S_i | S_j| when that exceeds prune, where
-S_i is the set of i's neighbours in Y.
+kNN results are compared bit for bit with the CPU oracle and with one-shot
+sparseKNN(X, Y) calls.  The SNN graph of a request is checked against a brute
+force over all query pairs: queries i and j are joined with weight
+|S_i & S_j| / |S_i | S_j| when that exceeds prune, where S_i is the set of
+i's neighbours in Y.
 """
+import os
+import threading
+import time
+
 import numpy as np
 import pytest
 import scipy.sparse as sp
@@ -18,6 +22,16 @@ from conftest import bits
 pytestmark = pytest.mark.gpu
 
 PRUNE = 1 / 15
+NT = min(16, os.cpu_count() or 1)
+
+
+def same_as_oracle(oracle_lib, res, X, Y, k, method, cols=None):
+    """Served rows (all, or the listed query columns) equal the oracle's."""
+    if cols is None:
+        cols = np.arange(X.shape[1])
+    oi, od, _ = oracle_lib.knn(X[:, cols], Y, k=k, method=method, nthreads=NT)
+    assert np.array_equal(res.idx[cols], oi), (method, k)
+    assert np.array_equal(bits(res.dist[cols]), bits(od)), (method, k)
 
 
 def brute_shared_snn(idx, prune):
@@ -60,7 +74,7 @@ def ref_and_requests():
     return Y, reqs
 
 
-def test_server_same_alone_and_in_mix(ref_and_requests, gpu_ctx):
+def test_server_same_alone_and_in_mix(ref_and_requests, oracle_lib, gpu_ctx):
     Y, reqs = ref_and_requests
     dY = sd.DeviceMatrix(Y)
     with sd.Server(dY, slots=4, chunk_cols=256) as srv:
@@ -71,6 +85,7 @@ def test_server_same_alone_and_in_mix(ref_and_requests, gpu_ctx):
     assert st["completed"] == len(reqs) and st["pending"] == 0 and st["active"] == 0
     assert st["reserved_bytes"] == 0
     for r, m in zip(reqs, mixed):
+        same_as_oracle(oracle_lib, m, r["X"], Y, r["k"], r["method"])
         # one-shot call on the user's context
         alone = sd.sparseKNN(r["X"], dY, k=r["k"], method=r["method"], verbose=False)
         assert np.array_equal(m.idx, alone.idx)
@@ -88,22 +103,118 @@ def test_server_same_alone_and_in_mix(ref_and_requests, gpu_ctx):
             same_graph(a["snn"], m["snn"])
 
 
-def test_small_request_overtakes_large(gpu_ctx):
+def test_small_request_overtakes_large(oracle_lib, gpu_ctx):
+    """A large request (47 work items) and a small one submitted after it to
+    a paused one-slot server: once resumed, the slot takes the request with
+    the fewest unscheduled columns first, so the small one finishes first."""
     Y = synth.nb_log1p(300, 5000, 0.05, 72)
     big = synth.nb_log1p(300, 6000, 0.05, 73)
     small = synth.nb_log1p(300, 5, 0.05, 74)
-    with sd.Server(Y, slots=1, chunk_cols=128) as srv:
+    with sd.Server(Y, slots=1, chunk_cols=128, paused=True) as srv:
         tb = srv.submit(big, k=10, method="cosine")
         ts = srv.submit(small, k=10, method="cosine")
-        info_s = srv.wait(ts)
-        info_b_then = srv.wait(tb, timeout=0)
+        srv.resume()
         rb = srv.result(tb)
         rs = srv.result(ts)
-    assert info_s["done"] == 1 and info_s["status"] == 0
-    assert info_b_then["done"] == 0  # the large one was still running
-    alone = sd.sparseKNN(small, Y, k=10, method="cosine", verbose=False)
-    assert np.array_equal(rs.idx, alone.idx)
-    assert rb.idx.shape == (6000, 10)
+    assert rs["info"]["finish_order"] == 1 and rb["info"]["finish_order"] == 2
+    same_as_oracle(oracle_lib, rs, small, Y, 10, "cosine")
+    same_as_oracle(oracle_lib, rb, big, Y, 10, "cosine", cols=np.arange(0, 6000, 37))
+
+
+def test_admission_lets_fitting_requests_pass(oracle_lib, gpu_ctx):
+    """A request whose reservation does not fit yet does not hold back a later
+    one that fits (it is admitted once the budget frees, here when the server
+    resumes and the first request completes)."""
+    Y = synth.nb_log1p(200, 3000, 0.05, 86)
+    first = synth.nb_log1p(200, 30, 0.05, 87)
+    big = synth.nb_log1p(200, 3000, 0.05, 88)
+    tiny = synth.nb_log1p(200, 3, 0.05, 89)
+    reserve = {}
+    with sd.Server(Y, slots=1, paused=True) as probe:  # each request's reservation
+        for name, X, snn in (("first", first, False), ("big", big, True), ("tiny", tiny, False)):
+            before = probe.stats()["reserved_bytes"]
+            probe.submit(X, k=20, method="binary", snn=snn, prune=PRUNE)
+            reserve[name] = probe.stats()["reserved_bytes"] - before
+    # the first and the tiny request fit together, the big one fits alone but
+    # not beside the first
+    budget = max(reserve["big"], reserve["first"] + reserve["tiny"])
+    assert reserve["first"] + reserve["big"] > budget
+    with sd.Server(Y, slots=1, budget_bytes=budget, paused=True) as srv:
+        t1 = srv.submit(first, k=20, method="binary")
+        tb = srv.submit(big, k=20, method="binary", snn=True, prune=PRUNE)
+        tt = srv.submit(tiny, k=20, method="binary")
+        st = srv.stats()
+        assert st["admitted"] == 2 and st["pending"] == 1, st
+        srv.resume()
+        r1, rb, rt = srv.result(t1), srv.result(tb), srv.result(tt)
+    same_as_oracle(oracle_lib, r1, first, Y, 20, "binary")
+    same_as_oracle(oracle_lib, rt, tiny, Y, 20, "binary")
+    same_as_oracle(oracle_lib, rb, big, Y, 20, "binary", cols=np.arange(0, 3000, 29))
+    assert rb["snn"].shape == (3000, 3000)
+
+
+def test_destroy_while_a_caller_waits(gpu_ctx):
+    """sd_server_destroy while another thread is blocked in sd_server_wait:
+    the waiter returns (its request cancelled) before the server is freed."""
+    Y = synth.nb_log1p(200, 1000, 0.05, 93)
+    X = synth.nb_log1p(200, 50, 0.05, 94)
+    srv = sd.Server(Y, slots=1, paused=True)
+    t = srv.submit(X, k=5, method="cosine")
+    out = {}
+
+    def waiter():
+        out["info"] = srv.wait(t)
+
+    th = threading.Thread(target=waiter)
+    th.start()
+    deadline = time.time() + 30
+    while srv.stats()["waiting_callers"] < 1:
+        assert time.time() < deadline, "the waiter never blocked"
+        time.sleep(0.01)
+    srv.close()
+    th.join(timeout=30)
+    assert not th.is_alive()
+    assert out["info"]["done"] == 1 and out["info"]["status"] == _lib.SD_ECANCELLED
+
+
+def test_close_with_work_in_flight(gpu_ctx):
+    """Closing a server whose slots are busy cancels the work (polled between
+    kernels) and returns."""
+    Y = synth.nb_log1p(2000, 20000, 0.05, 95)
+    srv = sd.Server(Y, slots=2, chunk_cols=512)
+    ts = [srv.submit(synth.nb_log1p(2000, 4000, 0.05, 96 + i), k=10, method="cosine")
+          for i in range(3)]
+    time.sleep(0.2)
+    t0 = time.perf_counter()
+    srv.close()
+    assert time.perf_counter() - t0 < 60
+    del ts
+
+
+def test_serving_workload_reference(oracle_lib, gpu_ctx):
+    """The serving workload's reference rows and density (20000 rows, d =
+    0.05, NB/log1p; 20000 columns here, so every work item runs the pre-pass
+    and filtered pass): a mix of request sizes, methods and k served
+    concurrently on four slots; sampled rows of every request equal the
+    oracle's, and the SNN graphs of the small SNN requests equal the brute
+    force."""
+    Y = synth.nb_log1p(20000, 20000, 0.05, 531)
+    dY = sd.DeviceMatrix(Y, gpu_ctx)
+    sizes = [1, 6, 45, 300, 2100, 5000, 12, 700]
+    reqs = [dict(X=synth.nb_log1p(20000, n, 0.05, 540 + t), k=(10, 20, 30)[t % 3],
+                 method=("cosine", "binary")[t % 2], snn=t % 2 == 0) for t, n in enumerate(sizes)]
+    with sd.Server(dY, slots=4, chunk_cols=4096, paused=True) as srv:
+        ts = [srv.submit(r["X"], k=r["k"], method=r["method"], snn=r["snn"], prune=PRUNE)
+              for r in reqs]
+        srv.resume()
+        res = [srv.result(t) for t in ts]
+    rng = np.random.default_rng(532)
+    for r, m in zip(reqs, res):
+        n = r["X"].shape[1]
+        cols = np.sort(rng.choice(n, min(n, 24), replace=False))
+        same_as_oracle(oracle_lib, m, r["X"], Y, r["k"], r["method"], cols=cols)
+        if r["snn"] and n <= 700:
+            same_graph(m["snn"], brute_shared_snn(m.idx, PRUNE))
 
 
 def test_budget_admission(gpu_ctx):

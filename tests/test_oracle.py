@@ -366,3 +366,83 @@ def test_fixture_shared_awkward(oracle_lib):
         assert np.array_equal(got, want)
     p, ri, x = oracle_lib.snn_graph(idx, include_self=True, prune=1.0, edges="shared")
     assert p[-1] == 0
+
+
+def _dense_js(P, Q):
+    """sqrt(0.5 * sum(xlogx(p) + xlogx(q) - xlogx(p + q) + (p + q) log 2)) per
+    column pair, numpy on dense arrays (0 log 0 = 0)."""
+    def xlogx(x):
+        with np.errstate(divide="ignore", invalid="ignore"):
+            return np.where(x > 0, x * np.log(np.where(x > 0, x, 1.0)), 0.0)
+    out = np.empty((P.shape[1], Q.shape[1]))
+    for i in range(P.shape[1]):
+        p = P[:, i:i + 1]
+        s = xlogx(p) + xlogx(Q) - xlogx(p + Q) + (p + Q) * np.log(2.0)
+        v = 0.5 * s.sum(axis=0)
+        out[i] = np.where(v > 0, np.sqrt(np.maximum(v, 0)), 0.0)
+    return out
+
+
+def test_js_matches_scipy_on_normalised_columns(oracle_lib):
+    """On columns that sum to 1 the reference's sum is the Jensen-Shannon
+    divergence, which scipy computes another way (KL divergences to the
+    midpoint distribution)."""
+    from scipy.spatial.distance import jensenshannon
+    A = rand_sparse(50, 13, 0.3, 21).toarray()
+    B = rand_sparse(50, 9, 0.3, 22).toarray()
+    A[:, 4] = 0.0
+    A[:, 5] = B[:, 2] * 3.0  # same distribution, other scale
+    for M in (A, B):
+        s = M.sum(axis=0)
+        M[:, s > 0] /= s[s > 0]
+    got = oracle_lib.slab("js", sp.csc_matrix(A), sp.csc_matrix(B))
+    for i in range(A.shape[1]):
+        for j in range(B.shape[1]):
+            if A[:, i].sum() == 0 or B[:, j].sum() == 0:
+                continue
+            assert abs(got[i, j] - jensenshannon(A[:, i], B[:, j])) < 1e-7, (i, j)
+    # A[:, 5] is B[:, 2]: distance 0 up to rounding of the divergence sum
+    assert got[5, 2] < 1e-7
+    assert np.all(got <= np.sqrt(np.log(2.0)) + 1e-12) and np.all(got >= 0)
+
+
+def test_js_matches_dense_closed_form(oracle_lib):
+    """Unnormalised non-negative columns against the dense numpy form; a
+    dropped term or a wrong sign would differ by far more than rounding."""
+    A = rand_sparse(40, 11, 0.35, 23)
+    B = rand_sparse(40, 7, 0.35, 24)
+    got = oracle_lib.slab("js", A, B)
+    np.testing.assert_allclose(got, _dense_js(A.toarray(), B.toarray()), rtol=1e-12, atol=1e-14)
+
+
+def test_fixture_js(oracle_lib):
+    f = FIX["js_standard_metric"]
+    P = from_cols(f["P"])
+    d = oracle_lib.slab("js", P, P)
+    assert d[0, 1] == np.sqrt(np.log(2.0)) and d[1, 0] == np.sqrt(np.log(2.0))
+    for case in FIX["js_invalid_input"]["cases"]:
+        M = np.array([[np.nan if v is None else v for v in row] for row in case["M"]])
+        with pytest.raises(Exception, match=case["error"]):
+            oracle_lib.slab("js", sp.csc_matrix(M), sp.csc_matrix(M))
+    f = FIX["js_self_exact_zero"]
+    rng = np.random.default_rng(25)
+    P = np.abs(rng.standard_normal((12, 6)))
+    P /= P.sum(axis=0)
+    idx, dst, _ = oracle_lib.knn(sp.csc_matrix(P), k=f["k"], method="js",
+                                 include_self=f["include_self"])
+    assert (bits(dst[:, 0]) == bits(np.zeros(6))).all()
+    assert idx[:, 0].tolist() == list(range(1, 7))
+    for m in FIX["distance_only_methods"]["methods"]:
+        with pytest.raises(Exception, match=FIX["distance_only_methods"]["error"]):
+            oracle_lib.knn(sp.csc_matrix(P), k=2, method=m, dist=False)
+
+
+def test_js_knn_matches_full_matrix(oracle_lib):
+    """The kNN driver's JS lists are the brute-force ranking of the dense
+    closed form (values differ from it only by rounding)."""
+    A = rand_sparse(30, 25, 0.4, 26)
+    idx, dst, _ = oracle_lib.knn(A, k=4, method="js")
+    full = _dense_js(A.toarray(), A.toarray())
+    ridx, _ = brute.ref_topk(full, 4, False, True)
+    assert np.array_equal(idx, ridx)
+    np.testing.assert_allclose(dst, np.take_along_axis(full.T, idx - 1, 1), rtol=1e-12)
