@@ -242,6 +242,20 @@ int sd_topk_rows_accum(sd_context* ctx, const double* D, int64_t nrow, int64_t n
 int sd_snn_jaccard(sd_context* ctx, const int32_t* idx, int32_t n, int32_t k,
                    int32_t include_self, double* w);
 
+/* sd_snn_jaccard without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid allocation).
+ * d_idx (n x k int32, 1-based, NA_integer_ padding) and d_w (n x k double)
+ * are DEVICE pointers; the weights are those of sd_snn_jaccard bit for bit.
+ * The neighbour sets live in a static device array of 64 MiB per GPU, held
+ * until the process ends: n * (ks + 1) * 4 + 768 bytes must fit in it, with
+ * ks = k + 1 rounded up to a multiple of 4 (n <= 671,000 at k = 20);
+ * larger inputs are rejected with SD_EINVAL before any work.  All calls on
+ * one GPU share that array, so calls that overlap on one GPU race.  Runs on
+ * ctx's stream and returns when the weights are written. */
+int sd_snn_jaccard_noalloc(sd_context* ctx, const int32_t* d_idx, int32_t n, int32_t k,
+                           int32_t include_self, double* d_w);
+
 /* The SNN graph as an n x n sparse matrix on the device.  edges: 0 = "knn"
  * (weights over the kNN edge set, strict w > prune, duplicates collapsed,
  * optional union symmetrisation; R/sparseSNN.R:221-250), 1 = "shared"

@@ -27,6 +27,7 @@ EXPORTS = (
     "sd_graph_copy", "sd_graph_free", "sd_detect_cores", "sd_server_create", "sd_server_submit",
     "sd_server_wait", "sd_server_error", "sd_server_result", "sd_server_release",
     "sd_server_stats_get", "sd_server_destroy", "sd_autotune", "sd_server_resume",
+    "sd_snn_jaccard_noalloc",
 )
 
 
@@ -123,6 +124,7 @@ def load(path: str = LIB_PATH):
         L.sd_topk_block.argtypes = [vp, vp, i64, i64, i32, i32, vp, vp, vp]
         L.sd_topk_rows_accum.argtypes = [vp, vp, i64, i64, i32, i32, vp, vp, i64, vp]
         L.sd_snn_jaccard.argtypes = [vp, vp, i32, i32, i32, vp]
+        L.sd_snn_jaccard_noalloc.argtypes = [vp, vp, i32, i32, i32, vp]
         L.sd_snn.argtypes = [vp, vp, i32, i32, i32, dbl, i32, i32, i32, C.POINTER(vp)]
         L.sd_graph_info.argtypes = [vp, C.POINTER(i64), C.POINTER(i64)]
         L.sd_graph_copy.argtypes = [vp, vp, vp, vp]

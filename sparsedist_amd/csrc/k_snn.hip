@@ -7,6 +7,7 @@
 // assembled as (column << 32 | row) keys sorted on the device with rocPRIM,
 // so the result comes out directly in dgCMatrix (CSC) order.
 #include <cstring>
+#include <mutex>
 
 #include <rocprim/rocprim.hpp>
 
@@ -324,6 +325,34 @@ int assemble_csc(sd_context* ctx, DevBuf& keys, DevBuf& vals, int64_t m, int64_t
   return SD_OK;
 }
 
+// The weight kernel for rows of ks set entries (register form for ks <= 32,
+// merge walk beyond).
+int launch_knn_weights(sd_context* ctx, const int32_t* d_idx, int n, int k, int ks,
+                       const int32_t* sets, const int32_t* len, double* dw) {
+  const unsigned ge = stride_grid((int64_t)n * k);
+  switch (ks) {
+#define SD_WEIGHTS(KS)                                                                        \
+  case KS:                                                                                    \
+    hipLaunchKernelGGL(k_knn_weights<KS>, dim3(ge), dim3(256), 0, ctx->stream, d_idx, n, k, ks, \
+                       sets, len, dw);                                                        \
+    break;
+    SD_WEIGHTS(4)
+    SD_WEIGHTS(8)
+    SD_WEIGHTS(12)
+    SD_WEIGHTS(16)
+    SD_WEIGHTS(20)
+    SD_WEIGHTS(24)
+    SD_WEIGHTS(28)
+    SD_WEIGHTS(32)
+#undef SD_WEIGHTS
+    default:
+      hipLaunchKernelGGL(k_knn_weights<0>, dim3(ge), dim3(256), 0, ctx->stream, d_idx, n, k, ks,
+                         sets, len, dw);
+  }
+  SD_HIP(hipGetLastError());
+  return SD_OK;
+}
+
 }  // namespace
 
 int snn_build(sd_context* ctx, const int32_t* d_idx, int n, int k, int include_self, double prune,
@@ -367,26 +396,7 @@ int snn_build(sd_context* ctx, const int32_t* d_idx, int n, int k, int include_s
       dw = w.as<double>();
     }
     const unsigned ge = stride_grid(total);
-    switch (ks) {
-#define SD_WEIGHTS(KS)                                                                   \
-  case KS:                                                                               \
-    hipLaunchKernelGGL(k_knn_weights<KS>, dim3(ge), dim3(256), 0, ctx->stream, d_idx, n, k, \
-                       ks, sets.as<int32_t>(), len.as<int32_t>(), dw);                   \
-    break;
-      SD_WEIGHTS(4)
-      SD_WEIGHTS(8)
-      SD_WEIGHTS(12)
-      SD_WEIGHTS(16)
-      SD_WEIGHTS(20)
-      SD_WEIGHTS(24)
-      SD_WEIGHTS(28)
-      SD_WEIGHTS(32)
-#undef SD_WEIGHTS
-      default:
-        hipLaunchKernelGGL(k_knn_weights<0>, dim3(ge), dim3(256), 0, ctx->stream, d_idx, n, k,
-                           ks, sets.as<int32_t>(), len.as<int32_t>(), dw);
-    }
-    SD_HIP(hipGetLastError());
+    SD_TRY(launch_knn_weights(ctx, d_idx, n, k, ks, sets.as<int32_t>(), len.as<int32_t>(), dw));
     if (!g) {
       SD_HIP(hipStreamSynchronize(ctx->stream));
       return SD_OK;
@@ -506,3 +516,96 @@ int snn_build(sd_context* ctx, const int32_t* d_idx, int n, int k, int include_s
 }
 
 }  // namespace sd
+
+// ---------------------------------------------------------------------------
+// Allocation-free SNN weights (sd_snn_jaccard_noalloc).
+//
+// Its scratch -- the neighbour sets (n rows of ks entries), their sizes and
+// the validation flag -- lives in `scratch`, one array per GPU that the
+// library holds until the process ends.  In host code `scratch` only names
+// the symbol: its device address on each GPU comes from hipGetSymbolAddress
+// (once per device, with that device current, when a context is created) and
+// only that address is ever passed to kernels.  Every call on a GPU shares
+// the array, so calls that overlap on one GPU race; each call clears the
+// flag it needs at zero and writes every set before reading it.
+// ---------------------------------------------------------------------------
+namespace sd {
+constexpr size_t kNoAllocScratchBytes = (size_t)64 << 20;
+constexpr int kNoAllocMaxDevices = 64;
+}  // namespace sd
+
+static __device__ __align__(256) uint8_t scratch[sd::kNoAllocScratchBytes];
+
+namespace sd {
+namespace {
+
+std::mutex g_noalloc_mu;
+void* g_noalloc_ptr[kNoAllocMaxDevices] = {};
+
+__global__ void k_clear_flag(int* __restrict__ flag) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *flag = 0;
+}
+
+size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+}  // namespace
+
+int noalloc_scratch_setup(int device, void** out) {
+  if (device < 0 || device >= kNoAllocMaxDevices)
+    return set_error(SD_EINVAL, "device %d is beyond the %d this library supports", device,
+                     kNoAllocMaxDevices);
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  if (!g_noalloc_ptr[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return set_error(SD_EDEVICE, "scratch address on device %d: %s", device,
+                       hipGetErrorString(e));
+    g_noalloc_ptr[device] = p;
+  }
+  *out = g_noalloc_ptr[device];
+  return SD_OK;
+}
+
+}  // namespace sd
+
+extern "C" int sd_snn_jaccard_noalloc(sd_context* ctx, const int32_t* d_idx, int32_t n, int32_t k,
+                                      int32_t include_self, double* d_w) {
+  using namespace sd;
+  clear_error();
+  if (!ctx || !d_idx || !d_w) return set_error(SD_EINVAL, "NULL argument");
+  if (n < 1 || k < 1) return set_error(SD_EINVAL, "'idx' must have at least one row and column.");
+  const int ks = (k + 1 + 3) & ~3;  // set row stride, as snn_build
+  const size_t sets_b = align256(sizeof(int32_t) * (size_t)n * (size_t)ks);
+  const size_t len_b = align256(sizeof(int32_t) * (size_t)n);
+  const size_t need = sets_b + len_b + 256;
+  if (need > kNoAllocScratchBytes)
+    return set_error(SD_EINVAL,
+                     "sd_snn_jaccard_noalloc: n = %d, k = %d needs %zu bytes of scratch, more "
+                     "than its %zu",
+                     n, k, need, kNoAllocScratchBytes);
+  if (!ctx->noalloc_scratch) return set_error(SD_EDEVICE, "context has no scratch address");
+  (void)hipGetLastError();
+  SD_HIP(hipSetDevice(ctx->device));
+  uint8_t* base = static_cast<uint8_t*>(ctx->noalloc_scratch);
+  int32_t* sets = reinterpret_cast<int32_t*>(base);
+  int32_t* len = reinterpret_cast<int32_t*>(base + sets_b);
+  int* flag = reinterpret_cast<int*>(base + sets_b + len_b);
+  const int64_t total = (int64_t)n * k;
+  // validate before any kernel reads a set through an index
+  hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(64), 0, ctx->stream, flag);
+  hipLaunchKernelGGL(k_validate, dim3(stride_grid(total)), dim3(256), 0, ctx->stream, d_idx, total,
+                     n, flag);
+  SD_HIP(hipGetLastError());
+  int hbad = 0;
+  SD_HIP(hipMemcpyAsync(&hbad, flag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  SD_HIP(hipStreamSynchronize(ctx->stream));
+  if (hbad)
+    return set_error(SD_EINVAL, "'idx' entries must be NA or 1-based row indices in [1, nrow(idx)].");
+  hipLaunchKernelGGL(k_build_sets, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream,
+                     d_idx, n, k, ks, include_self, 1, sets, len);
+  SD_HIP(hipGetLastError());
+  SD_TRY(launch_knn_weights(ctx, d_idx, n, k, ks, sets, len, d_w));
+  SD_HIP(hipStreamSynchronize(ctx->stream));
+  return SD_OK;
+}

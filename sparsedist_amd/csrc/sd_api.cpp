@@ -923,6 +923,11 @@ int sd_context_create(int device, sd_context** out) {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
       c->cu_count = prop.multiProcessorCount;
+    const int rc = noalloc_scratch_setup(device, &c->noalloc_scratch);
+    if (rc != SD_OK) {
+      sd_context_destroy(c);
+      return rc;
+    }
     *out = c;
     return SD_OK;
   });

@@ -151,6 +151,9 @@ struct sd_context {
   sd::DevBuf snn_scratch[24];
   sd::CrossMirror xm;
   std::vector<uint8_t> host_scratch;
+  // device address of the allocation-free entry point's scratch array on
+  // this context's device (resolved when the context is created)
+  void* noalloc_scratch = nullptr;
 };
 
 struct sd_matrix {
@@ -388,6 +391,10 @@ inline unsigned stride_grid(int64_t n) {
   const int64_t b = (n + 255) / 256;
   return (unsigned)(b < 1 ? 1 : (b > (1 << 20) ? (1 << 20) : b));
 }
+
+// Device address of the allocation-free SNN entry point's static scratch
+// array on `device` (the current device), resolved once per device.
+int noalloc_scratch_setup(int device, void** out);
 
 // ---- timing helpers ------------------------------------------------------------
 struct KernelTimer {
