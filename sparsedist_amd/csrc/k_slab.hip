@@ -839,9 +839,37 @@ __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
   // barrier, after which every wave has read CUR and chunk c + 1 is visible;
   // refill CUR with chunk c + NS and request chunk c + 1's step 0 while the
   // last step's MFMAs run.
-#define SD_WIDE_STEPS(CUR)                                                      load_frag(f1, CUR, 1);                                                        __builtin_amdgcn_sched_barrier(0);                                            mma_frag(f0);                                                                 __builtin_amdgcn_sched_barrier(0);                                            if constexpr (STEPS == 4) {                                                     load_frag(f0, CUR, 2);                                                        __builtin_amdgcn_sched_barrier(0);                                            mma_frag(f1);                                                                 __builtin_amdgcn_sched_barrier(0);                                            load_frag(f1, CUR, 3);                                                        __builtin_amdgcn_sched_barrier(0);                                            mma_frag(f0);                                                                 __builtin_amdgcn_sched_barrier(0);                                          }
-#define SD_WIDE_CHUNK(CUR, NXT, C)                                              {                                                                               SD_WIDE_STEPS(CUR)                                                            wait_vm<(NS - 2) * INST>();                                                   lds_barrier();                                                                issue(CUR, (C) + NS < last ? (C) + NS : last);                                load_frag(f0, NXT, 0);                                                        __builtin_amdgcn_sched_barrier(0);                                            mma_frag(f1);                                                                 __builtin_amdgcn_sched_barrier(0);                                          }
-#define SD_WIDE_LAST(CUR)                                                       {                                                                               SD_WIDE_STEPS(CUR)                                                            mma_frag(f1);                                                               }
+#define SD_WIDE_STEPS(CUR)              \
+  load_frag(f1, CUR, 1);                \
+  __builtin_amdgcn_sched_barrier(0);    \
+  mma_frag(f0);                         \
+  __builtin_amdgcn_sched_barrier(0);    \
+  if constexpr (STEPS == 4) {           \
+    load_frag(f0, CUR, 2);              \
+    __builtin_amdgcn_sched_barrier(0);  \
+    mma_frag(f1);                       \
+    __builtin_amdgcn_sched_barrier(0);  \
+    load_frag(f1, CUR, 3);              \
+    __builtin_amdgcn_sched_barrier(0);  \
+    mma_frag(f0);                       \
+    __builtin_amdgcn_sched_barrier(0);  \
+  }
+#define SD_WIDE_CHUNK(CUR, NXT, C)                  \
+  {                                                 \
+    SD_WIDE_STEPS(CUR)                              \
+    wait_vm<(NS - 2) * INST>();                     \
+    lds_barrier();                                  \
+    issue(CUR, (C) + NS < last ? (C) + NS : last);  \
+    load_frag(f0, NXT, 0);                          \
+    __builtin_amdgcn_sched_barrier(0);              \
+    mma_frag(f1);                                   \
+    __builtin_amdgcn_sched_barrier(0);              \
+  }
+#define SD_WIDE_LAST(CUR)  \
+  {                        \
+    SD_WIDE_STEPS(CUR)     \
+    mma_frag(f1);          \
+  }
   int64_t c = 0;
   if constexpr (NS == 4) {
     for (; c + 4 < nchunks; c += 4) {
