@@ -9,6 +9,11 @@ X is resident in device memory before the timed region, and every step
 starts from a cold matrix (derived layouts dropped, so densification and
 normalisation are timed).
 
+After the timed steps (outside the timed region): full-size checks against
+a full-grid search and the oracle's SNN graphs, the plain FP32 path, the
+CPU baseline (the oracle on a seeded sample of query columns) and the
+serving run on the serving workload's resident 20,000 x 500,000 reference.
+
 Prints one JSON line (rank 0, N = 1).
 """
 import argparse
@@ -109,9 +114,15 @@ def main():
     ap.add_argument("--k", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--cpu-sample", type=int, default=48)
+    ap.add_argument("--cpu-sample", type=int, default=48,
+                    help="query columns of the CPU baseline's sample (a seeded draw over all "
+                         "columns)")
+    ap.add_argument("--cpu-reps", type=int, default=3)
     ap.add_argument("--serve-window", type=float, default=4.0,
                     help="seconds of request arrivals in the serving run (0: skip it)")
+    ap.add_argument("--serve-window-plain", type=float, default=1.0,
+                    help="seconds of arrivals for the plain FP32 serving run (its requests "
+                         "take far longer to drain)")
     ap.add_argument("--serve", action="store_true",
                     help="serving benchmark only, on the 20000 x 500000 resident reference")
     ap.add_argument("--sweep", action="store_true",
@@ -160,10 +171,15 @@ def main():
     t0 = time.time()
     X = synth.nb_log1p(args.nrow, args.ncol, args.density, seed=20)
     log(f"generated X {X.shape} nnz={X.nnz} in {time.time() - t0:.1f}s")
-    pool = None
+    pool = Y5 = None
     if args.serve_window > 0:
-        # query pool for the serving run, generated before the GPU is touched
-        pool = synth.nb_log1p_parallel(args.nrow, SERVE_MAX_COLS, args.density, 21, workers)
+        # the serving workload's resident reference and query pool, generated
+        # before the GPU is touched
+        t0 = time.time()
+        Y5 = synth.nb_log1p_parallel(20_000, 500_000, 0.05, 50, workers)
+        pool = synth.nb_log1p_parallel(20_000, SERVE_MAX_COLS, 0.05, 51, workers)
+        log(f"generated the serving reference {Y5.shape} nnz={Y5.nnz} and the query pool "
+            f"in {time.time() - t0:.1f}s")
     ctx = sd.default_context()
     dX = sd.DeviceMatrix(X, ctx)
     nq = args.ncol
@@ -186,7 +202,7 @@ def main():
             st2 = ctx.stats(reset=True)
             ctx.set_timing(False)
             res[method] = dict(t_knn=t_knn, t_snn=t_snn, stats=st, snn_stats=st2, nnz=g.nnz,
-                               idx=nn.idx, dist=nn.dist)
+                               idx=nn.idx, dist=nn.dist, graph=g)
         return res
 
     for w in range(args.warmup):
@@ -285,54 +301,45 @@ def main():
                      idx_identical=bool(same))
         log(f"plain fp32 path: binary {plain['binary_ms']:.1f} ms cosine {plain['cosine_ms']:.1f} ms")
 
+    # full-size SNN check (outside the timed region): both graphs of the last
+    # timed step equal the oracle's CSC (structure and weight bits)
+    from oracle import oracle as O
+    O.build()
+    nt = min(16, os.cpu_count() or 1)
+    snn_parity = {}
+    for method in ("binary", "cosine"):
+        g = per[-1][method]["graph"]
+        p_, ri_, x_ = O.snn_graph(per[-1][method]["idx"], include_self=True, prune=1 / 15,
+                                  symmetrise=True, nthreads=nt)
+        snn_parity[method] = bool(np.array_equal(g.indptr, p_) and np.array_equal(g.indices, ri_)
+                                  and np.array_equal(g.data.view(np.int64), x_.view(np.int64)))
+    log(f"SNN graphs of all {nq} columns equal the oracle's: {snn_parity}")
+
     cpu = None
     if not args.no_cpu:
-        from oracle import oracle as O
-        O.build()
-        nt = min(16, os.cpu_count() or 1)
-        ns = args.cpu_sample
-        Xs = X[:, :ns]
-        t = time.perf_counter()
-        ora = {}
-        for method in ("binary", "cosine"):
-            # k + 1: the self-search row plus the query's own column
-            ora[method] = O.knn(Xs, X, k=k + 1, method=method, block_size=8192, nthreads=nt)
-        tc = time.perf_counter() - t
-        # the timed step's rows of those columns must equal the oracle's with
-        # the query's own column removed
-        parity = {}
-        for method, (oi, od, _) in ora.items():
-            wi = np.empty((ns, k), np.int32)
-            wd = np.empty((ns, k))
-            for r in range(ns):
-                keep = oi[r] != r + 1
-                if keep.all():
-                    keep[-1] = False
-                wi[r] = oi[r][keep][:k]
-                wd[r] = od[r][keep][:k]
-            gi, gd = per[-1][method]["idx"][:ns], per[-1][method]["dist"][:ns]
-            parity[method] = bool(np.array_equal(gi, wi) and
-                                  np.array_equal(np.ascontiguousarray(gd).view(np.int64),
-                                                 wd.view(np.int64)))
-        cpu = dict(value=2 * ns / tc, unit="query cols/s", cores=nt, kind="port",
-                   sample=f"{ns} query columns x all {nq} reference columns, binary "
-                          f"and cosine k={k} (+1 for the query's own column), "
-                          f"oracle/sd_oracle.c with OpenMP ({tc:.1f} s)",
-                   sample_parity=parity)
-        log(f"cpu baseline: {cpu['value']:.2f} cols/s on {nt} threads; GPU rows of the sample "
-            f"equal the oracle's: {parity}")
+        cpu = cpu_baseline(args, O, X, per[-1], nt)
+        log(f"cpu baseline: {cpu['value']:.2f} cols/s on {nt} threads (median of "
+            f"{args.cpu_reps}), {cpu['one_thread']['value']:.2f} cols/s on 1 thread; GPU rows of "
+            f"the sample equal the oracle's: {cpu['sample_parity']}")
 
     serving = None
     if pool is not None:
+        dX.drop_cache()
+        dY5 = sd.DeviceMatrix(Y5, ctx)
         arr = serve_arrivals(args.serve_window, 22, pool.shape[1])
-        runs = [serve_run(sd, dX, pool, arr, "auto")]
+        runs = [serve_run(sd, dY5, pool, arr, "auto")]
         if not args.no_plain:
-            runs.append(serve_run(sd, dX, pool, arr, "fp32"))
-        serving = dict(reference=f"the count matrix X ({args.nrow} x {nq}) resident",
-                       stream=f"Poisson {SERVE_RATE:.0f} req/s for {args.serve_window:.0f} s, "
-                              "sizes log-uniform on [1, 50000] columns, cosine/binary, "
-                              "k in {10, 20, 30}, SNN on half; 4 slots, 4096-column work "
-                              "items, 16 waiting requests at most",
+            arr_p = serve_arrivals(args.serve_window_plain, 22, pool.shape[1])
+            runs.append(serve_run(sd, dY5, pool, arr_p, "fp32"))
+        dY5.free()
+        serving = dict(reference=f"Y {Y5.shape[0]} x {Y5.shape[1]} d=0.05 NB/log1p "
+                                 f"(nnz {Y5.nnz}) resident",
+                       stream=f"Poisson {SERVE_RATE:.0f} req/s, sizes log-uniform on [1, 50000] "
+                              "columns, cosine/binary, k in {10, 20, 30}, SNN on half; 4 slots, "
+                              "4096-column work items, 16 waiting requests at most; "
+                              f"{args.serve_window:.0f} s of arrivals for the fast path, "
+                              f"{args.serve_window_plain:.0f} s (the same stream's first "
+                              "arrivals) for plain FP32",
                        runs=runs)
         for r in runs:
             log(f"serving {r['precision']}: served {r['served']}/{r['offered']} p50 "
@@ -366,9 +373,73 @@ def main():
         "roofline_other": roof_cos if dominant is roof_bin else roof_bin,
         "cpu_baseline": cpu,
         "symmetric_vs_full_grid": full_grid,
+        "snn_vs_oracle": snn_parity,
         "serving": serving,
     }
     print(json.dumps(out), flush=True)
+
+
+def self_rows(oi, od, cols, k):
+    """Self-search rows (k neighbours) of query columns cols from an oracle
+    search of X[:, cols] against X with k + 1 neighbours: the query's own
+    column leaves its list."""
+    wi = np.empty((len(cols), k), np.int32)
+    wd = np.empty((len(cols), k))
+    for r, c in enumerate(cols):
+        keep = oi[r] != c + 1
+        if keep.all():
+            keep[-1] = False
+        wi[r] = oi[r][keep][:k]
+        wd[r] = od[r][keep][:k]
+    return wi, wd
+
+
+def cpu_baseline(args, O, X, last, nt):
+    """The oracle (C + OpenMP restatement of the reference's slab-then-top-k
+    loop) on a seeded sample of query columns spread over all of X, against
+    every column of X, both methods: median and MAD over args.cpu_reps
+    repetitions on nt threads, one repetition on one thread (a smaller
+    sample), and the slab-kernel / top-k / end-to-end split.  The timed
+    step's GPU rows of the sample must equal the oracle's."""
+    nq, k, ns = X.shape[1], args.k, args.cpu_sample
+    cols = np.sort(np.random.default_rng(23).choice(nq, ns, replace=False))
+    Xs = X[:, cols]
+    reps, phases, parity = [], [], {}
+    for rep in range(args.cpu_reps):
+        t = time.perf_counter()
+        slab = topk = 0.0
+        for method in ("binary", "cosine"):
+            oi, od, _ = O.knn(Xs, X, k=k + 1, method=method, block_size=8192, nthreads=nt)
+            a, b = O.last_phases()
+            slab, topk = slab + a, topk + b
+            if rep == 0:
+                wi, wd = self_rows(oi, od, cols, k)
+                gi, gd = last[method]["idx"][cols], last[method]["dist"][cols]
+                parity[method] = bool(np.array_equal(gi, wi) and np.array_equal(
+                    np.ascontiguousarray(gd).view(np.int64), wd.view(np.int64)))
+        reps.append(time.perf_counter() - t)
+        phases.append((slab, topk))
+        log(f"cpu baseline rep {rep + 1}/{args.cpu_reps}: {reps[-1]:.1f} s")
+    med = float(np.median(reps))
+    mad = float(np.median(np.abs(np.array(reps) - med)))
+    n1 = max(1, ns // 6)
+    t = time.perf_counter()
+    for method in ("binary", "cosine"):
+        O.knn(Xs[:, :n1], X, k=k + 1, method=method, block_size=8192, nthreads=1)
+    t1 = time.perf_counter() - t
+    slab_m = float(np.median([p[0] for p in phases]))
+    topk_m = float(np.median([p[1] for p in phases]))
+    return dict(value=2 * ns / med, unit="query cols/s", cores=nt, kind="port",
+                sample=f"{ns} query columns (seeded draw over all {nq}) x all {nq} reference "
+                       f"columns, binary and cosine k={k} (+1 for the query's own column), "
+                       f"oracle/sd_oracle.c with OpenMP; median of {args.cpu_reps} "
+                       f"repetitions ({med:.1f} s, MAD {mad:.2f} s)",
+                seconds_median=med, seconds_mad=mad, seconds_reps=reps,
+                phases={"slab_kernel_s": slab_m, "topk_s": topk_m,
+                        "end_to_end_s": med},
+                one_thread=dict(value=2 * n1 / t1, unit="query cols/s", cores=1,
+                                sample=f"{n1} of the sampled columns, both methods ({t1:.1f} s)"),
+                sample_parity=parity)
 
 
 def serve_main(args, sd, synth, workers):
@@ -465,13 +536,21 @@ def cross_main(args, sd, synth):
         O.build()
         nt = min(16, os.cpu_count() or 1)
         ns = 16
+        cols = np.sort(np.random.default_rng(62).choice(E.shape[1], ns, replace=False))
         t = time.perf_counter()
-        O.knn(sp.csc_matrix(E[:, :ns].astype(np.float64)), sp.csc_matrix(Y.astype(np.float64)),
-              k=k, method="cosine", dist=False, nthreads=nt)
+        oi, od, _ = O.knn(sp.csc_matrix(E[:, cols].astype(np.float64)),
+                          sp.csc_matrix(Y.astype(np.float64)), k=k, method="cosine", dist=False,
+                          nthreads=nt)
         tc = time.perf_counter() - t
+        # the timed search's forward rows of the sample equal the oracle's
+        parity = bool(np.array_equal(cr["forward"].idx[cols], oi) and np.array_equal(
+            np.ascontiguousarray(cr["forward"].dist[cols]).view(np.int64), od.view(np.int64)))
         cpu = dict(value=ns / tc, unit="query cols/s", cores=nt, kind="port",
-                   sample=f"{ns} E columns x all 1,000,000 Y columns (forward direction), "
-                          f"oracle/sd_oracle.c with OpenMP ({tc:.1f} s)")
+                   sample=f"{ns} E columns (seeded draw) x all 1,000,000 Y columns (forward "
+                          f"direction), oracle/sd_oracle.c with OpenMP ({tc:.1f} s)",
+                   sample_parity=parity)
+        log(f"cpu baseline {ns / tc:.2f} cols/s; GPU forward rows of the sample equal the "
+            f"oracle's: {parity}")
     out = {
         "metric": "exact cross-kNN query cols/s (k=10, both directions)",
         "value": nq / dt, "unit": "query cols/s", "n_gpus": 1, "higher_is_better": True,

@@ -141,6 +141,13 @@ def knn(X, Y=None, k=10, method="binary", dist=True, include_self=False, drop_de
             dst[: nq * ke].reshape((nq, ke), order="F"), bool(warn.value))
 
 
+def last_phases():
+    """(slab seconds, top-k seconds) of the last knn() call on this thread."""
+    a, b = C.c_double(0.0), C.c_double(0.0)
+    lib().or_last_phases(C.byref(a), C.byref(b))
+    return a.value, b.value
+
+
 def knn_cross(X, Y, k=10, method="binary", dist=True, drop_degenerate=True, block_size=256,
               nthreads=1):
     a, ka = _csc(X)

@@ -10,6 +10,7 @@
 #include "sd_oracle.h"
 
 #include <math.h>
+#include <time.h>
 #include <stdarg.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -41,6 +42,20 @@ static double na_real(void) {
 }
 
 static int nthreads_of(int n) { return n < 1 ? 1 : n; }
+
+/* Seconds the last or_knn call on this thread spent in its slab kernel and
+ * in its top-k (the rest of its time is driver work: zero dropping, masks,
+ * copies), for the phase split of the CPU baseline. */
+static __thread double g_slab_s, g_topk_s;
+static double now_s(void) {
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+void or_last_phases(double* slab_s, double* topk_s) {
+  *slab_s = g_slab_s;
+  *topk_s = g_topk_s;
+}
 
 /* ------------------------------------------------------------------------ */
 /* Per-column helpers                                                        */
@@ -657,6 +672,7 @@ int or_knn(const or_csc* Xin, const or_csc* Yin, int k, int method, int dist, in
            int* k_eff, int* warn) {
   /* reference sparseKNN, R/sparseKNN.R:110-313 (after argument validation) */
   *warn = 0;
+  g_slab_s = g_topk_s = 0.0;
   if (k < 1) return fail(OR_EINVAL, "'k' must be a positive integer no larger than .Machine$integer.max.");
   if (block_size < 1)
     return fail(OR_EINVAL, "'block_size' must be a positive integer no larger than .Machine$integer.max.");
@@ -713,7 +729,9 @@ int or_knn(const or_csc* Xin, const or_csc* Yin, int k, int method, int dist, in
         const int64_t nb = to - from;
         or_csc blk;
         col_block(&X, from, to, pb, &blk);
+        double t0 = now_s();
         rc = or_slab(method, ref, &blk, dist, nthreads, D);
+        g_slab_s += now_s() - t0;
         if (rc != OR_OK) break;
         for (int64_t b = 0; b < nb; ++b)
           self_row[b] = (selfmode && !include_self) ? (int32_t)(from + b + 1) : 0;
@@ -730,7 +748,9 @@ int or_knn(const or_csc* Xin, const or_csc* Yin, int k, int method, int dist, in
         for (int64_t i = 0; i < nref; ++i)
           if (deg_ref[i])
             for (int64_t b = 0; b < nb; ++b) D[i + b * nref] = NA;
+        t0 = now_s();
         rc = or_topk_block(D, nref, nb, k, dec, self_row, nthreads, pidx, pval);
+        g_topk_s += now_s() - t0;
         if (rc != OR_OK) break;
         for (int t = 0; t < k; ++t)
           for (int64_t b = 0; b < nb; ++b) {

@@ -81,6 +81,10 @@ int or_knn(const or_csc* X, const or_csc* Y, int k, int method, int dist, int in
            int drop_degenerate, int64_t block_size, int nthreads, int32_t* idx, double* dst,
            int* k_eff, int* warn);
 
+/* Seconds the last or_knn call on the calling thread spent in its slab
+ * kernel and in its top-k reduction. */
+void or_last_phases(double* slab_s, double* topk_s);
+
 /* Both directions from one slab per block (reference sparseKNNCross).
  * warn bit 0: forward clamped, bit 1: reverse clamped.  Error OR_ENOCAND
  * reports the direction in or_last_error. */
