@@ -122,6 +122,9 @@ typedef struct {
                                  unordered pair once */
   int64_t buffer_escalations; /* searches whose wider second lists came from the filtered
                                  pass's candidate buffers instead of a second slab pass */
+  int64_t escalation_list_queries; /* of those, queries whose buffer could not vouch for the
+                                      wider list (too few entries, or overflowed) and took a
+                                      list-mode pass of their own */
 } sd_stats;
 int sd_context_stats(sd_context* ctx, sd_stats* out, int reset);
 /* Enable/disable event timing of kernels (off by default). */

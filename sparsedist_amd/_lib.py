@@ -55,7 +55,8 @@ class Stats(C.Structure):
                 ("filter_ms", C.c_double), ("filter_launches", C.c_int64),
                 ("fused_calls", C.c_int64), ("exact_calls", C.c_int64),
                 ("last_margin", C.c_int64), ("slab_products", C.c_int64),
-                ("filter_pairs", C.c_int64), ("buffer_escalations", C.c_int64)]
+                ("filter_pairs", C.c_int64), ("buffer_escalations", C.c_int64),
+                ("escalation_list_queries", C.c_int64)]
 
 
 class TuneInfo(C.Structure):

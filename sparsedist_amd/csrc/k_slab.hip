@@ -151,6 +151,14 @@ __device__ inline float pear_key(const KeyWeights& w, float qa, float qc, float 
   return r + (((w.w1 * (fabsf(x) * sc) + w.w2 * gy) + kPearRound * (fabsf(x) + fabsf(t)) * sc) +
               kEucAbs);
 }
+// The next float above a finite t (t itself for +inf and NaN).
+__device__ inline float next_float_up(float t) {
+  if (!(t < INFINITY)) return t;
+  if (t == 0.f) return __int_as_float(1);
+  const int b = __float_as_int(t);
+  return __int_as_float(t > 0.f ? b + 1 : b - 1);
+}
+
 // Split layout's query offset: segment 1 <-> 2.  Written as arithmetic, not
 // selects, so that the K loop stays one basic block (a branch there makes the
 // wait-count pass merge states at the join and wait for every load), and
@@ -675,7 +683,7 @@ __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
   __shared__ __align__(16) int32_t s_rnnz[TM];
   __shared__ float s_raux[TM];
   __shared__ float s_raux2[TM];
-  __shared__ uint8_t s_rdeg[TM];
+  __shared__ __align__(16) uint8_t s_rdeg[TM];
   __shared__ __align__(16) float s_rthr[SYM ? TM : 4];  // mirror screen thresholds
   __shared__ double s_rthrd[SYM ? TM : 1];                // mirror exact thresholds
   __shared__ int32_t s_mcnt[SYM ? TM : 1];                // mirror hits per reference
@@ -904,6 +912,9 @@ __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
   // only on those that pass, and appends them with one atomic per (lane,
   // query) after counting, so a wave never waits on a memory round trip per
   // candidate.
+  // Cosine keys are the slab values themselves, so a screen against the
+  // least float at or above each threshold is the exact test.
+  constexpr bool kExactScreen = MK == kKeyScore && sizeof(K) != 8;
   // the lane's two queries: 64 wn + 32 j + n
   int64_t qp[2], qc[2];
   bool qv[2];
@@ -922,9 +933,15 @@ __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
     kw[j] = MK == kKeyPearson ? pear_weights(qa[j], qa2[j], qn[j], P.dot_e0, P.dot_e1, P.dot_nonneg)
                               : euc_weights(qn[j], P.dot_e0, P.dot_e1, P.dot_nonneg);
     thr_d[j] = qv[j] ? P.thr[qp[j]] : INFINITY;
-    // float screen a little below the threshold; the exact test follows
     const float t = (float)thr_d[j];
-    thr_f[j] = t - fabsf(t) * 9.5367431640625e-07f - 1e-30f;
+    if constexpr (kExactScreen) {
+      // the least float at or above the threshold: for a float key x,
+      // x >= thr_f exactly when (double) x >= thr_d
+      thr_f[j] = ((double)t < thr_d[j]) ? next_float_up(t) : t;
+    } else {
+      // float screen a little below the threshold; the exact test follows
+      thr_f[j] = t - fabsf(t) * 9.5367431640625e-07f - 1e-30f;
+    }
   }
   const int64_t rg0 = r_base + wm * 128 + 4 * h;
   // exact test of one value: reference rg (tile row rloc, ni stored entries)
@@ -994,11 +1011,21 @@ __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
       const float ras[4] = {ra4.x, ra4.y, ra4.z, ra4.w};
       const float rcs[4] = {rc4.x, rc4.y, rc4.z, rc4.w};
 #pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int r = 4 * q4 + e;
+      for (int j = 0; j < 2; ++j) {
+        bool pass[4];
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
-          if (screen(acc[b][j][r], j, nis[e], ras[e], rcs[e])) pm[j] |= 1ull << (16 * b + r);
+        for (int e = 0; e < 4; ++e) pass[e] = screen(acc[b][j][4 * q4 + e], j, nis[e], ras[e], rcs[e]);
+        // A compare per value, its lane masks ORed on the scalar unit; the
+        // per-lane bits are set only for groups in which some lane of the wave
+        // passed (about one group in four at the thresholds' density), so most
+        // values cost one VALU compare instead of compare, select and or.
+        if (__builtin_expect(__builtin_amdgcn_ballot_w64(pass[0] | pass[1] | pass[2] | pass[3]) != 0,
+                             0)) {
+          uint32_t bits = 0u;
+#pragma unroll
+          for (int e = 0; e < 4; ++e) bits |= pass[e] ? 1u << e : 0u;
+          pm[j] |= (uint64_t)bits << (16 * b + 4 * q4);
+        }
       }
       // one group at a time: hoisting every group's LDS reads would hold
       // 16 x 4 aux values in registers next to the accumulators
@@ -1018,22 +1045,60 @@ __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
   // pass 2: exact test of the marked values, then one atomic per query to
   // reserve buffer slots, then the writes
   int cnt[2] = {0, 0};
+  if constexpr (kExactScreen) {
+    // the marks are the exact test already; what remains is to drop
+    // degenerate and padding references (flag s_rdeg) and, without
+    // include_self, the query's own column.  A degenerate column's slab value
+    // is 0, so the flags matter only where a threshold is at most 0 or the
+    // tile runs past the last reference (wave-uniform, rare).
+    if (__builtin_amdgcn_ballot_w64(thr_f[0] <= 0.f || thr_f[1] <= 0.f) != 0 ||
+        r_base + TM > P.nref) {
+      uint64_t dm = 0ull;
 #pragma unroll
-  for (int b = 0; b < 4; ++b)
+      for (int b = 0; b < 4; ++b)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      uint32_t m = (uint32_t)(pm[j] >> (16 * b)) & 0xFFFFu, keep = 0u;
-      while (m) {
-        const int r = __builtin_ctz(m);
-        m &= m - 1u;
-        const int rloc = wm * 128 + b * 32 + 8 * (r >> 2) + 4 * h + (r & 3);
-        const int64_t rg = rg0 + b * 32 + 8 * (r >> 2) + (r & 3);
-        double kd;
-        if (exact(value_of(b, j, r), j, rloc, s_rnnz[rloc], rg, kd)) keep |= 1u << r;
-      }
-      pm[j] = (pm[j] & ~(0xFFFFull << (16 * b))) | ((uint64_t)keep << (16 * b));
-      cnt[j] += __popc(keep);
+        for (int q4 = 0; q4 < 4; ++q4) {
+          const uint32_t d4 =
+              *reinterpret_cast<const uint32_t*>(&s_rdeg[wm * 128 + b * 32 + 8 * q4 + 4 * h]);
+#pragma unroll
+          for (int e = 0; e < 4; ++e)
+            if ((d4 >> (8 * e)) & 0xFFu) dm |= 1ull << (16 * b + 4 * q4 + e);
+        }
+      pm[0] &= ~dm;
+      pm[1] &= ~dm;
     }
+    if (P.self_mode) {
+      // the own column goes without include_self, and when it is degenerate
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const int64_t u = qc[j] - r_base - wm * 128;
+        if (u >= 0 && u < 128 && (((u & 31) >> 2) & 1) == h) {
+          const int w = (int)(u & 31);
+          if (!P.include_self || s_rdeg[wm * 128 + (int)u])
+            pm[j] &= ~(1ull << (16 * (int)(u >> 5) + (((w >> 3) << 2) | (w & 3))));
+        }
+      }
+    }
+    cnt[0] = __popcll(pm[0]);
+    cnt[1] = __popcll(pm[1]);
+  } else {
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        uint32_t m = (uint32_t)(pm[j] >> (16 * b)) & 0xFFFFu, keep = 0u;
+        while (m) {
+          const int r = __builtin_ctz(m);
+          m &= m - 1u;
+          const int rloc = wm * 128 + b * 32 + 8 * (r >> 2) + 4 * h + (r & 3);
+          const int64_t rg = rg0 + b * 32 + 8 * (r >> 2) + (r & 3);
+          double kd;
+          if (exact(value_of(b, j, r), j, rloc, s_rnnz[rloc], rg, kd)) keep |= 1u << r;
+        }
+        pm[j] = (pm[j] & ~(0xFFFFull << (16 * b))) | ((uint64_t)keep << (16 * b));
+        cnt[j] += __popc(keep);
+      }
+  }
   if (cnt[0] + cnt[1] != 0) {
     int at[2];
 #pragma unroll
@@ -1292,11 +1357,12 @@ __global__ __launch_bounds__(256) void k_select(const double* __restrict__ fkey,
   const int64_t q = qlist ? (int64_t)qlist[b] : b;
   const int n = cnt[q];
   // overflowed, or fewer candidates than the threshold guarantees complete
-  // lists for: the query is searched again by a full list pass
+  // lists for: the query is searched again by a full list pass.  The list
+  // records the query (or, for a query list, the list position b).
   if (n > cap || n < min_count) {
     if (threadIdx.x == 0) {
       const int at = atomicAdd(over_count, 1);
-      if (over_list) over_list[at] = (int32_t)q;
+      if (over_list) over_list[at] = (int32_t)(qlist ? b : q);
     }
     for (int t = threadIdx.x; t < L; t += blockDim.x) out_idx[b * L + t] = -1;
     return;
@@ -1321,6 +1387,14 @@ __global__ void k_threshold(const double* __restrict__ key, const int32_t* __res
   const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (q >= nqry) return;
   thr[q] = (idx[q * L + L - 1] >= 0) ? key[q * L + L - 1] : -INFINITY;
+}
+
+// out[j] = src[pos[j]] (query ids of listed positions).
+__global__ void k_gather_i32(const int32_t* __restrict__ src, const int32_t* __restrict__ pos,
+                             int64_t n, int32_t* __restrict__ out) {
+  for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n;
+       j += (int64_t)gridDim.x * blockDim.x)
+    out[j] = src[pos[j]];
 }
 
 // Copy list rows j into rows rows[j] of the main lists.
@@ -1368,7 +1442,7 @@ __global__ __launch_bounds__(256) void k_select_wave(
   if (n > cap || n < min_count) {
     if (lane == 0) {
       const int at = atomicAdd(over_count, 1);
-      if (over_list) over_list[at] = (int32_t)q;
+      if (over_list) over_list[at] = (int32_t)(qlist ? b : q);  // as k_select
     }
     if (lane < L) out_idx[b * L + lane] = -1;
     return;
@@ -2075,6 +2149,13 @@ int scatter_lists(sd_context* ctx, const double* key, const int32_t* idx, const 
   if (n <= 0) return SD_OK;
   hipLaunchKernelGGL(k_scatter_lists, dim3((unsigned)((n * L + 255) / 256)), dim3(256), 0,
                      ctx->stream, key, idx, rows, n, L, out_key, out_idx);
+  SD_HIP(hipGetLastError());
+  return SD_OK;
+}
+
+int gather_i32(sd_context* ctx, const int32_t* src, const int32_t* pos, int64_t n, int32_t* out) {
+  if (n <= 0) return SD_OK;
+  hipLaunchKernelGGL(k_gather_i32, dim3(stride_grid(n)), dim3(256), 0, ctx->stream, src, pos, n, out);
   SD_HIP(hipGetLastError());
   return SD_OK;
 }

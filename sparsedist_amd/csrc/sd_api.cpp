@@ -380,10 +380,18 @@ int fused_candidate_lists(sd_context* ctx, FusedArgs fa, const double** out_key,
     return list_pass(fa, ctx->scratch[3], ctx->scratch[4], ctx->scratch[5], ctx->scratch[6],
                      out_key, out_idx);
   }
-  const int64_t stride = rtiles >= 4 * 128 ? 128 : rtiles >= 4 * 64 ? 64 : 32;
-  const int Lp = stride == 128 ? fused_list_len((L + 3) / 4)
-                 : stride == 64 ? fused_list_len((L + 1) / 2)
-                                : L;
+  // Every candidate costs the filtered pass per-tile epilogue work (exact
+  // tests, appends) that a long K amortises and a short K does not, while a
+  // sampled pair costs the pre-pass K operations.  Short rows (embeddings of
+  // a few hundred dimensions) therefore take a denser sample at a low rank:
+  // every 32nd tile at rank ~L/4 leaves about 8 x 32 = 256 candidates per
+  // query instead of about 1000.
+  const bool short_rows = dense_row_bytes(fa.kind, fa.kpad) <= 2048;
+  const int64_t stride = short_rows ? 32 : rtiles >= 4 * 128 ? 128 : rtiles >= 4 * 64 ? 64 : 32;
+  const int Lp = short_rows      ? fused_list_len(std::max((L + 3) / 4, 8))
+                 : stride == 128 ? fused_list_len((L + 3) / 4)
+                 : stride == 64  ? fused_list_len((L + 1) / 2)
+                                 : L;
   // room for three times the expected candidates (the count above the
   // Lp-th of a sample varies by about 1/sqrt(Lp))
   int cap = (int)std::min<int64_t>(4096, 3 * Lp * stride + 256);
@@ -764,23 +772,58 @@ int run_knn(sd_context* ctx, sd_matrix* X, sd_matrix* Y, const sd_knn_params* p,
         const double* k2 = nullptr;
         const int32_t* i2 = nullptr;
         if (cb.fkey) {
-          // the wider lists straight from the filtered pass's buffers, when
-          // every failed query's buffer is complete for Lmax
+          // the wider lists straight from the filtered pass's buffers; a
+          // failed query whose buffer is not complete for Lmax (fewer
+          // entries, or overflowed) gets its list from a list-mode pass over
+          // every reference tile instead, placed at its position
           SD_TRY(mk2.ensure(sizeof(double) * (size_t)nfail * Lmax + 64));
           SD_TRY(mi2.ensure(sizeof(int32_t) * (size_t)nfail * Lmax + 64));
           DevBuf& bc = ctx->scratch[15];
+          DevBuf& bpos = ctx->scratch[20];
           SD_TRY(bc.ensure(64));
+          SD_TRY(bpos.ensure(sizeof(int32_t) * (size_t)(nfail + 16)));
           SD_HIP(hipMemsetAsync(bc.ptr, 0, sizeof(int32_t), ctx->stream));
           SD_TRY(select_lists(ctx, cb.fkey, cb.fidx, cb.cnt, cb.cap, nfail, Lmax, mk2.as<double>(),
-                              mi2.as<int32_t>(), nullptr, bc.as<int32_t>(), Lmax,
+                              mi2.as<int32_t>(), bpos.as<int32_t>(), bc.as<int32_t>(), Lmax,
                               fl.as<int32_t>()));
           int32_t nbad = 0;
           SD_HIP(hipMemcpyAsync(&nbad, bc.ptr, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
           SD_HIP(hipStreamSynchronize(ctx->stream));
-          if (nbad == 0) {
-            k2 = mk2.as<double>();
-            i2 = mi2.as<int32_t>();
+          ctx->stats.escalation_list_queries += nbad;
+          if (nbad > 0) {
+            DevBuf& bq = ctx->scratch[21];
+            SD_TRY(bq.ensure(sizeof(int32_t) * (size_t)(nbad + 16)));
+            SD_TRY(gather_i32(ctx, fl.as<int32_t>(), bpos.as<int32_t>(), nbad, bq.as<int32_t>()));
+            FusedArgs fb = fa;
+            fb.qlist = bq.as<int32_t>();
+            fb.nqry = nbad;
+            fb.L = Lmax;
+            fb.tile_stride = 1;
+            const int64_t qt2 = (nbad + 255) / 256;
+            fb.splits = (int)std::min<int64_t>(round_up((2 * (int64_t)ctx->cu_count + qt2 - 1) / qt2, 8), 4096);
+            const int nl2 = fused_effective_splits(fb);
+            DevBuf& ck2 = ctx->scratch[22];
+            DevBuf& ci2 = ctx->scratch[23];
+            DevBuf& ck3 = ctx->scratch[24];
+            DevBuf& ci3 = ctx->scratch[25];
+            SD_TRY(ck2.ensure(sizeof(double) * (size_t)nl2 * (size_t)nbad * Lmax + 64));
+            SD_TRY(ci2.ensure(sizeof(int32_t) * (size_t)nl2 * (size_t)nbad * Lmax + 64));
+            SD_TRY(fused_slab_topl(ctx, fb, ck2.as<double>(), ci2.as<int32_t>()));
+            const double* lk = ck2.as<double>();
+            const int32_t* li = ci2.as<int32_t>();
+            if (nl2 > 1) {
+              SD_TRY(ck3.ensure(sizeof(double) * (size_t)nbad * Lmax + 64));
+              SD_TRY(ci3.ensure(sizeof(int32_t) * (size_t)nbad * Lmax + 64));
+              SD_TRY(merge_lists(ctx, lk, li, nl2, nbad, Lmax, ck3.as<double>(),
+                                 ci3.as<int32_t>()));
+              lk = ck3.as<double>();
+              li = ci3.as<int32_t>();
+            }
+            SD_TRY(scatter_lists(ctx, lk, li, bpos.as<int32_t>(), nbad, Lmax, mk2.as<double>(),
+                                 mi2.as<int32_t>()));
           }
+          k2 = mk2.as<double>();
+          i2 = mi2.as<int32_t>();
         }
         if (!k2) {
           // second fused pass over just those columns with the widest list

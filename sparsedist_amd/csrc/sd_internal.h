@@ -146,7 +146,7 @@ struct sd_context {
   hipEvent_t ev_a = nullptr, ev_b = nullptr;
   int cu_count = 256;
   // scratch reused across calls (grow-only)
-  sd::DevBuf scratch[24];
+  sd::DevBuf scratch[28];
   // the SNN builder's temporaries, one slot each (grow-only)
   sd::DevBuf snn_scratch[24];
   sd::CrossMirror xm;
@@ -347,7 +347,8 @@ int fused_filter_pass(sd_context* ctx, const FusedArgs& a, const double* thr, in
 // candidates (buffer overflow) or fewer than min_count (the threshold cannot
 // vouch for complete lists) are counted in over_count and, when over_list is
 // given, listed there for a full pass.  With qlist, list position b takes
-// query qlist[b]'s buffer (outputs by position).
+// query qlist[b]'s buffer (outputs by position) and over_list records the
+// positions.
 int select_lists(sd_context* ctx, const double* fkey, const int32_t* fidx, const int32_t* cnt,
                  int cap, int64_t nqry, int L, double* out_key, int32_t* out_idx,
                  int32_t* over_list, int32_t* over_count, int min_count,
@@ -356,6 +357,8 @@ int list_thresholds(sd_context* ctx, const double* key, const int32_t* idx, int6
                     double* thr);
 int scatter_lists(sd_context* ctx, const double* key, const int32_t* idx, const int32_t* rows,
                   int64_t n, int L, double* out_key, int32_t* out_idx);
+// out[j] = src[pos[j]] for j < n (device arrays).
+int gather_i32(sd_context* ctx, const int32_t* src, const int32_t* pos, int64_t n, int32_t* out);
 int fused_effective_splits(const FusedArgs& a);
 int merge_lists(sd_context* ctx, const double* key, const int32_t* idx, int nlists, int64_t nqry,
                 int L, double* out_key, int32_t* out_idx);

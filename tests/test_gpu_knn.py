@@ -648,8 +648,10 @@ def test_sparse_sample_threshold_redo(oracle_lib, gpu_ctx, method, copies, nref)
     threshold can exceed the true L-th key: here copies of the first query
     fill the first sampled tile beyond that rank, so the sample's key at that
     rank is the top score and fewer than L candidates pass; the query is
-    searched again in full and every result still matches the oracle."""
-    Y = synth.nb_log1p(600, nref, 0.05, 43)
+    searched again in full and every result still matches the oracle.  The
+    rows are long enough (5000: 2.5 KB of FP4 patterns per column) for these
+    sampling tiers; rows of 2 KB or less take a denser sample instead."""
+    Y = synth.nb_log1p(5000, nref, 0.05, 43)
     Yd = Y.tolil()
     base = Y[:, 100]
     for c in range(copies):
@@ -662,3 +664,31 @@ def test_sparse_sample_threshold_redo(oracle_lib, gpu_ctx, method, copies, nref)
     o = oracle_lib.knn(X, Y, k=20, method=method, nthreads=NT)
     same((g.idx, g.dist), o, method)
     assert st["overflow_queries"] >= 1
+
+
+def test_escalation_with_incomplete_buffers(oracle_lib, gpu_ctx):
+    """Short rows take a dense threshold sample at a low rank (every 32nd
+    tile of 128 references, rank 16 for L = 40).  Sixteen near-copies of
+    query column 5000 in the first sampled tile put its threshold at the
+    copies' level, and thirty more copies in an unsampled tile leave its
+    buffer with 47 entries: at least L but fewer than 64.  Copies equal to
+    within BF16 rounding make its certificate fail, so its wider list comes
+    from a list-mode pass of its own, written back at its position in the
+    list of failed queries.  That query is the 151st query column while few
+    others fail, so its position and its column differ.  Every row must
+    match the oracle."""
+    rng = np.random.default_rng(46)
+    Y = synth.nb_log1p(300, 20000, 0.05, 47).tolil()
+    base = Y[:, 5000].toarray().ravel()
+    for c in list(range(16)) + list(range(128, 158)):
+        Y[:, c] = (base * (1 + 1e-7 * rng.standard_normal(base.size)))[:, None]
+    Y = Y.tocsc()
+    qcols = list(range(200, 350)) + [5000]
+    X = Y[:, qcols]
+    gpu_ctx.stats(reset=True)
+    g = sd.sparseKNN(X, Y, k=20, method="cosine", precision="bf16", verbose=False)
+    st = gpu_ctx.stats(reset=True)
+    o = oracle_lib.knn(X, Y, k=20, method="cosine", nthreads=NT)
+    same((g.idx, g.dist), o, "incomplete escalation buffers")
+    assert st["buffer_escalations"] == 1 and st["escalation_list_queries"] >= 1, st
+    assert st["escalated_queries"] < 100, st
