@@ -667,7 +667,7 @@ def test_sparse_sample_threshold_redo(oracle_lib, gpu_ctx, method, copies, nref)
 
 
 def test_escalation_with_incomplete_buffers(oracle_lib, gpu_ctx):
-    """Short rows take a dense threshold sample at a low rank (every 32nd
+    """Short rows take a dense threshold sample at a low rank (every 64th
     tile of 128 references, rank 16 for L = 40).  Sixteen near-copies of
     query column 5000 in the first sampled tile put its threshold at the
     copies' level, and thirty more copies in an unsampled tile leave its
