@@ -125,10 +125,19 @@ typedef struct {
   int64_t escalation_list_queries; /* of those, queries whose buffer could not vouch for the
                                       wider list (too few entries, or overflowed) and took a
                                       list-mode pass of their own */
+  int64_t query_blocks;       /* blocks of query columns the fused searches ran in (one per
+                                 search unless its buffers exceeded the buffer budget) */
 } sd_stats;
 int sd_context_stats(sd_context* ctx, sd_stats* out, int reset);
 /* Enable/disable event timing of kernels (off by default). */
 int sd_context_set_timing(sd_context* ctx, int on);
+/* Device bytes one fused search may hold in candidate buffers and lists
+ * (about 50 KB per query column at most).  A search whose query columns need
+ * more runs them in blocks, each a whole search of its own with identical
+ * results; 0 (the default) means half the device memory free at the call.
+ * No counterpart in the reference (which keeps one dense block_size x n_ref
+ * slab per pass, R/sparseKNN.R:68 block_size). */
+int sd_context_set_buffer_budget(sd_context* ctx, int64_t bytes);
 /* The context's hipStream_t, for callers that order their own work. */
 void* sd_context_stream(sd_context* ctx);
 

@@ -75,6 +75,11 @@ class Context:
     def set_timing(self, on: bool):
         check(_lib.load().sd_context_set_timing(self._h, int(bool(on))))
 
+    def set_buffer_budget(self, nbytes: int):
+        """Device bytes one fused search may hold in candidate buffers (0: half
+        the free memory); more query columns than fit run in blocks."""
+        check(_lib.load().sd_context_set_buffer_budget(self._h, int(nbytes)))
+
     def stats(self, reset: bool = False) -> dict:
         s = _lib.Stats()
         check(_lib.load().sd_context_stats(self._h, C.byref(s), int(bool(reset))))

@@ -37,7 +37,7 @@ __global__ __launch_bounds__(256) void k_exact_slab(ExactArgs A, const int32_t* 
   double v;
   if (mask && A.ref_deg && A.ref_deg[i]) {
     v = __longlong_as_double((long long)kNaRealBits);
-  } else if (mask && A.self_mode && i == j) {
+  } else if (mask && A.self_mode && i == j + A.self_off) {
     if (A.include_self) {
       // exact self value restored (R/sparseKNN.R:270-279)
       if (A.method == SD_BINARY || A.method == SD_COSINE || A.method == SD_PEARSON)

@@ -191,6 +191,7 @@ struct FusedKernelArgs {
   const float* qry_aux2;
   const uint8_t* ref_deg;
   int self_mode, include_self, method;
+  int64_t self_off;     // self mode: query column j is reference column j + self_off
   int splits, qtiles, tiles_per_split;
   int64_t ntiles;       // list mode: virtual tiles (every tile_stride-th reference tile)
   int tile_stride;
@@ -581,7 +582,7 @@ __global__ __launch_bounds__(WM * 256, 1) void k_fused(FusedKernelArgs P) {
           const int b = b0 + (v >> 4), rr = v & 15;
           const int rloc = rloc_base + b * 32 + 8 * (rr >> 2) + 4 * hh + (rr & 3);
           const int64_t rg = r_base + rloc;
-          if (s_rdeg[rloc] || (P.self_mode && rg == my_q && !P.include_self)) continue;
+          if (s_rdeg[rloc] || (P.self_mode && rg == my_q + P.self_off && !P.include_self)) continue;
           const uint32_t bits = xch[tid * 33 + v];
           K key;
           if constexpr (sizeof(K) == 8) {
@@ -948,7 +949,7 @@ __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
   // for query j; lanes of invalid queries have an infinite threshold
   auto exact = [&](const float x, int j, int rloc, int ni, int64_t rg, double& kd) -> bool {
     if (s_rdeg[rloc]) return false;
-    const bool self = P.self_mode && rg == qc[j];
+    const bool self = P.self_mode && rg == qc[j] + P.self_off;
     if (self && !P.include_self) return false;
     if constexpr (sizeof(K) == 8) {
       const uint32_t cc = (uint32_t)x;
@@ -1035,7 +1036,7 @@ __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
   if (P.self_mode && P.include_self) {
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
-      const int64_t u = qc[j] - r_base - wm * 128;  // -1 - ... for invalid queries
+      const int64_t u = qv[j] ? qc[j] + P.self_off - r_base - wm * 128 : -1;
       if (u >= 0 && u < 128 && (((u & 31) >> 2) & 1) == h) {
         const int w = (int)(u & 31);
         pm[j] |= 1ull << (16 * (int)(u >> 5) + (((w >> 3) << 2) | (w & 3)));
@@ -1071,7 +1072,7 @@ __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
       // the own column goes without include_self, and when it is degenerate
 #pragma unroll
       for (int j = 0; j < 2; ++j) {
-        const int64_t u = qc[j] - r_base - wm * 128;
+        const int64_t u = qv[j] ? qc[j] + P.self_off - r_base - wm * 128 : -1;
         if (u >= 0 && u < 128 && (((u & 31) >> 2) & 1) == h) {
           const int w = (int)(u & 31);
           if (!P.include_self || s_rdeg[wm * 128 + (int)u])
@@ -1324,10 +1325,11 @@ __global__ __launch_bounds__(256) void k_merge_lists(const double* __restrict__ 
 __global__ void k_finish_exact(const double* __restrict__ key, const int32_t* __restrict__ idx,
                                int64_t nqry, int L, int k, int dist,
                                const uint8_t* __restrict__ qdeg, int32_t* __restrict__ out_idx,
-                               double* __restrict__ out_val) {
-  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (e >= nqry * k) return;
-  const int64_t q = e % nqry, t = e / nqry;
+                               double* __restrict__ out_val, int64_t out_ld) {
+  const int64_t e0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (e0 >= nqry * k) return;
+  const int64_t q = e0 % nqry, t = e0 / nqry;
+  const int64_t e = t * out_ld + q;  // output entry (q, t)
   const int32_t i = idx[q * L + t];
   if (i < 0 || (qdeg && qdeg[q])) {
     out_idx[e] = kNaInt;
@@ -1590,7 +1592,7 @@ __global__ __launch_bounds__(256) void k_rerank(RerankArgs R) {
   double c = -2.0, v = 0.0;
   const bool valid = ci >= 0 && !deg_q;
   if (valid) {
-    if (R.ex.self_mode && R.ex.include_self && ci == jq) {
+    if (R.ex.self_mode && R.ex.include_self && ci == jq + R.ex.self_off) {
       c = 1.0;  // restored self value (R/sparseKNN.R:270-279)
     } else {
       const int64_t a0 = R.ex.ref.p[ci], na = R.ex.ref.p[ci + 1] - a0;
@@ -1772,7 +1774,7 @@ __global__ __launch_bounds__(256) void k_rerank_exact(RerankArgs R) {
       c = euc_num(R.ex.ref.i + a0, R.ex.ref.x + a0, na, qr, qv, qn);
       v = sqrt(c);
     } else {
-      if (R.ex.self_mode && R.ex.include_self && ci == jq)
+      if (R.ex.self_mode && R.ex.include_self && ci == jq + R.ex.self_off)
         c = 1.0;  // restored self value (R/sparseKNN.R:270-279)
       else
         c = pear_value(R.ex, ci, qr, qv, qn, jq);
@@ -1967,6 +1969,7 @@ int fused_slab_topl(sd_context* ctx, const FusedArgs& a, double* cand_key, int32
   P.dot_nonneg = a.dot_nonneg;
   P.ref_deg = a.ref_deg;
   P.self_mode = a.self_mode;
+  P.self_off = a.self_off;
   P.include_self = a.include_self;
   P.method = a.method;
   constexpr int WM = 1;
@@ -2032,6 +2035,7 @@ int fused_filter_pass(sd_context* ctx, const FusedArgs& a, const double* thr, in
   P.dot_nonneg = a.dot_nonneg;
   P.ref_deg = a.ref_deg;
   P.self_mode = a.self_mode;
+  P.self_off = a.self_off;
   P.include_self = a.include_self;
   P.method = a.method;
   P.qtiles = (int)((a.nqry + 255) / 256);
@@ -2173,11 +2177,12 @@ int merge_lists(sd_context* ctx, const double* key, const int32_t* idx, int nlis
 }
 
 int finish_exact_lists(sd_context* ctx, const double* key, const int32_t* idx, int64_t nqry, int L,
-                       int k, int dist, const uint8_t* qdeg, int32_t* out_idx, double* out_val) {
+                       int k, int dist, const uint8_t* qdeg, int32_t* out_idx, double* out_val,
+                       int64_t out_ld) {
   const int64_t n = nqry * k;
   if (n <= 0) return SD_OK;
   hipLaunchKernelGGL(k_finish_exact, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream,
-                     key, idx, nqry, L, k, dist, qdeg, out_idx, out_val);
+                     key, idx, nqry, L, k, dist, qdeg, out_idx, out_val, out_ld);
   SD_HIP(hipGetLastError());
   return SD_OK;
 }

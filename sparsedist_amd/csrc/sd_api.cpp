@@ -575,6 +575,28 @@ bool dot_bound(const sd_matrix* ref, const sd_matrix* X, int kind, DotBound* out
   return true;
 }
 
+// Query columns per block of a fused search.  Per query the search holds up
+// to 4096 filtered-pass candidates, the pre-pass's split lists (at most 8 of
+// rank <= L, before and after merging), the selected lists and the wider
+// escalation lists, 12 bytes an entry (FP64 key, int32 index); a budget --
+// the context's, else half the free device memory -- divided by that gives
+// the block, a multiple of the 256-column query tile.
+int64_t fused_query_block(sd_context* ctx, int64_t nq, int L) {
+  int64_t budget = ctx->buffer_budget;
+  if (budget <= 0) {
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
+      (void)hipGetLastError();
+      return nq;
+    }
+    budget = (int64_t)(free_b / 2);
+  }
+  const int64_t Lmax = fused_list_len(64);
+  const int64_t per_query = 12 * (4096 + 16 * (int64_t)L + (int64_t)L + 3 * Lmax) + 64;
+  const int64_t qb = std::max<int64_t>(256, budget / per_query / 256 * 256);
+  return std::min(qb, nq);
+}
+
 int run_knn(sd_context* ctx, sd_matrix* X, sd_matrix* Y, const sd_knn_params* p, const KnnPlan& plan,
             int32_t* h_idx, double* h_val) {
   sd_matrix* ref = plan.selfmode ? X : Y;
@@ -590,7 +612,7 @@ int run_knn(sd_context* ctx, sd_matrix* X, sd_matrix* Y, const sd_knn_params* p,
   SD_TRY(check_cancel(ctx));
 
   const uint8_t* ref_deg = p->drop_degenerate ? ref->deg.as<uint8_t>() : nullptr;
-  const uint8_t* qry_deg = p->drop_degenerate ? X->deg.as<uint8_t>() : nullptr;
+  const uint8_t* qry_deg_all = p->drop_degenerate ? X->deg.as<uint8_t>() : nullptr;
 
   DevBuf& out_idx = ctx->scratch[1];
   DevBuf& out_val = ctx->scratch[2];
@@ -697,189 +719,232 @@ int run_knn(sd_context* ctx, sd_matrix* X, sd_matrix* Y, const sd_knn_params* p,
     }
     fa.L = fused_list_len(method == SD_BINARY ? k : k + margin);
     const int L = fa.L;
-    const double* lkey = nullptr;
-    const int32_t* lidx = nullptr;
-    CandBuffers cb;
-    SD_TRY(fused_candidate_lists(ctx, fa, &lkey, &lidx, &cb));
-    if (ctx->xm.phase == 1) return SD_OK;  // probe of a cross pair's reverse search
-    if (method == SD_BINARY) {
-      SD_TRY(finish_exact_lists(ctx, lkey, lidx, nq, L, k, p->dist,
-                                p->drop_degenerate ? X->deg.as<uint8_t>() : nullptr,
-                                out_idx.as<int32_t>(), out_val.as<double>()));
-    } else {
-      // cosine / euclidean / pearson: FP64 re-rank with certificate; queries
-      // it cannot clear get a second fused pass over just those columns with
-      // the widest list, and only what is still uncertified goes to the exact
-      // FP64 slab.
-      auto rerank = [&](const RerankArgs& RA) {
-        return fused_ok_euclid    ? rerank_euclid(ctx, RA)
-               : fused_ok_pearson ? rerank_pearson(ctx, RA)
-                                  : rerank_cosine(ctx, RA);
-      };
-      DevBuf& fl = ctx->scratch[7];
-      DevBuf& fl2 = ctx->scratch[8];
-      SD_TRY(fl.ensure(sizeof(int32_t) * (size_t)(nq + 16)));
-      int32_t* fail_count = fl.as<int32_t>() + nq;
-      SD_HIP(hipMemsetAsync(fail_count, 0, sizeof(int32_t), ctx->stream));
-      RerankArgs R;
-      R.ex = ea;
-      R.cand_key = lkey;
-      R.cand_idx = lidx;
-      R.nqry = nq;
-      R.L = L;
-      R.k = k;
-      R.decreasing = plan.dec;
-      R.nonneg = !ref->any_negative && !X->any_negative;
-      R.in_round = use_f32 ? std::ldexp(1.0, -24) : std::ldexp(1.0, -8) + std::ldexp(1.0, -23);
-      R.acc_round = std::ldexp(1.0, -23);
-      R.qdeg = qry_deg;
-      R.out_idx = out_idx.as<int32_t>();
-      R.out_val = out_val.as<double>();
-      R.fail_list = fl.as<int32_t>();
-      R.fail_count = fail_count;
-      R.qlist = nullptr;
-      R.out_ld = nq;
-      R.map_words = rerank_map_words(ref->nrow);
-      SD_TRY(rerank(R));
-      int32_t nfail = 0;
-      SD_HIP(hipMemcpyAsync(&nfail, fail_count, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
-      SD_HIP(hipStreamSynchronize(ctx->stream));
-      const int32_t* fail_ids = fl.as<int32_t>();
-      ctx->stats.last_margin = margin;
-      if (prec == SD_PREC_AUTO && p->margin <= 0) {
-        // adapt this reference's margin.  After a filtered pass the wider
-        // lists come from the candidate buffers and a failed query costs only
-        // its second re-rank (64 lanes instead of L): widen when more than 15%
-        // fail, narrow by 4 (down to 8) when fewer than 1% do.  Otherwise a
-        // failed query costs a second slab pass: widen above 1%, narrow by 8
-        // (down to 16) only after a clean search.
-        std::lock_guard<std::mutex> lk(ref->prep_mu);
-        if (cb.fkey) {
-          if ((int64_t)nfail * 100 > 15 * nq)
-            ref->tuned_margin = std::min(2 * margin, 48);
-          else if ((int64_t)nfail * 100 < nq)
-            ref->tuned_margin = std::max(margin - 4, 8);
-        } else {
-          if ((int64_t)nfail * 100 > nq)
-            ref->tuned_margin = std::min(2 * margin, 48);
-          else if (nfail == 0)
-            ref->tuned_margin = std::max(margin - 8, 16);
-        }
+    // Candidate buffers and lists grow with the query count; past the
+    // context's buffer budget the queries run in blocks of columns, each a
+    // whole search of its own (a blocked self-search scores every pair from
+    // both sides, and a blocked cross search does not share its pass).
+    const int64_t qb = fused_query_block(ctx, nq, L);
+    if (qb < nq) {
+      CrossMirror& xm = ctx->xm;
+      if (xm.phase == 1) return SD_OK;  // no shared pass for this cross pair
+      if (xm.ready) {
+        xm.ready = false;
+        xm.cnt.release();
+        xm.fkey.release();
+        xm.fidx.release();
       }
-      const int Lmax = fused_list_len(64);
-      if (nfail > 0 && L < Lmax) {
-        SD_TRY(check_cancel(ctx));
-        DevBuf& mk2 = ctx->scratch[11];
-        DevBuf& mi2 = ctx->scratch[12];
-        const double* k2 = nullptr;
-        const int32_t* i2 = nullptr;
-        if (cb.fkey) {
-          // the wider lists straight from the filtered pass's buffers; a
-          // failed query whose buffer is not complete for Lmax (fewer
-          // entries, or overflowed) gets its list from a list-mode pass over
-          // every reference tile instead, placed at its position
-          SD_TRY(mk2.ensure(sizeof(double) * (size_t)nfail * Lmax + 64));
-          SD_TRY(mi2.ensure(sizeof(int32_t) * (size_t)nfail * Lmax + 64));
-          DevBuf& bc = ctx->scratch[15];
-          DevBuf& bpos = ctx->scratch[20];
-          SD_TRY(bc.ensure(64));
-          SD_TRY(bpos.ensure(sizeof(int32_t) * (size_t)(nfail + 16)));
-          SD_HIP(hipMemsetAsync(bc.ptr, 0, sizeof(int32_t), ctx->stream));
-          SD_TRY(select_lists(ctx, cb.fkey, cb.fidx, cb.cnt, cb.cap, nfail, Lmax, mk2.as<double>(),
-                              mi2.as<int32_t>(), bpos.as<int32_t>(), bc.as<int32_t>(), Lmax,
-                              fl.as<int32_t>()));
-          int32_t nbad = 0;
-          SD_HIP(hipMemcpyAsync(&nbad, bc.ptr, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
-          SD_HIP(hipStreamSynchronize(ctx->stream));
-          ctx->stats.escalation_list_queries += nbad;
-          if (nbad > 0) {
-            DevBuf& bq = ctx->scratch[21];
-            SD_TRY(bq.ensure(sizeof(int32_t) * (size_t)(nbad + 16)));
-            SD_TRY(gather_i32(ctx, fl.as<int32_t>(), bpos.as<int32_t>(), nbad, bq.as<int32_t>()));
-            FusedArgs fb = fa;
-            fb.qlist = bq.as<int32_t>();
-            fb.nqry = nbad;
-            fb.L = Lmax;
-            fb.tile_stride = 1;
-            const int64_t qt2 = (nbad + 255) / 256;
-            fb.splits = (int)std::min<int64_t>(round_up((2 * (int64_t)ctx->cu_count + qt2 - 1) / qt2, 8), 4096);
-            const int nl2 = fused_effective_splits(fb);
-            DevBuf& ck2 = ctx->scratch[22];
-            DevBuf& ci2 = ctx->scratch[23];
-            DevBuf& ck3 = ctx->scratch[24];
-            DevBuf& ci3 = ctx->scratch[25];
-            SD_TRY(ck2.ensure(sizeof(double) * (size_t)nl2 * (size_t)nbad * Lmax + 64));
-            SD_TRY(ci2.ensure(sizeof(int32_t) * (size_t)nl2 * (size_t)nbad * Lmax + 64));
-            SD_TRY(fused_slab_topl(ctx, fb, ck2.as<double>(), ci2.as<int32_t>()));
-            const double* lk = ck2.as<double>();
-            const int32_t* li = ci2.as<int32_t>();
-            if (nl2 > 1) {
-              SD_TRY(ck3.ensure(sizeof(double) * (size_t)nbad * Lmax + 64));
-              SD_TRY(ci3.ensure(sizeof(int32_t) * (size_t)nbad * Lmax + 64));
-              SD_TRY(merge_lists(ctx, lk, li, nl2, nbad, Lmax, ck3.as<double>(),
-                                 ci3.as<int32_t>()));
-              lk = ck3.as<double>();
-              li = ci3.as<int32_t>();
-            }
-            SD_TRY(scatter_lists(ctx, lk, li, bpos.as<int32_t>(), nbad, Lmax, mk2.as<double>(),
-                                 mi2.as<int32_t>()));
+    }
+    ctx->stats.query_blocks += (nq + qb - 1) / qb;
+    const FusedArgs fa_all = fa;
+    const ExactArgs ea_all = ea;
+    const int64_t nq_total = nq;
+    const size_t qrow = (size_t)dense_row_bytes(kind, kpad);
+    for (int64_t q0 = 0; q0 < nq_total; q0 += qb) {
+      if (q0 > 0) SD_TRY(check_cancel(ctx));
+      // block views: query columns [q0, q0 + nq) of X; results at column q0
+      const int64_t nq = std::min(qb, nq_total - q0);
+      FusedArgs fa = fa_all;
+      ExactArgs ea = ea_all;
+      if (q0 > 0 || nq < nq_total) {
+        fa.nqry = nq;
+        fa.qry_dense = (const char*)fa_all.qry_dense + (size_t)q0 * qrow;
+        fa.qry_p = fa_all.qry_p + q0;
+        if (fa_all.qry_aux) fa.qry_aux = fa_all.qry_aux + q0;
+        if (fa_all.qry_aux2) fa.qry_aux2 = fa_all.qry_aux2 + q0;
+        fa.self_off = q0;
+        ea.qry.ncol = nq;
+        ea.qry.p = ea_all.qry.p + q0;
+        if (ea_all.qry_scale) ea.qry_scale = ea_all.qry_scale + q0;
+        if (ea_all.qry_mean) ea.qry_mean = ea_all.qry_mean + q0;
+        if (ea_all.qry_css) ea.qry_css = ea_all.qry_css + q0;
+        ea.self_off = q0;
+      }
+      const uint8_t* qry_deg = qry_deg_all ? qry_deg_all + q0 : nullptr;
+      int32_t* out_i = out_idx.as<int32_t>() + q0;
+      double* out_v = out_val.as<double>() + q0;
+      const double* lkey = nullptr;
+      const int32_t* lidx = nullptr;
+      CandBuffers cb;
+      SD_TRY(fused_candidate_lists(ctx, fa, &lkey, &lidx, &cb));
+      if (ctx->xm.phase == 1) return SD_OK;  // probe of a cross pair's reverse search
+      if (method == SD_BINARY) {
+        SD_TRY(finish_exact_lists(ctx, lkey, lidx, nq, L, k, p->dist,
+                                  qry_deg, out_i, out_v, nq_total));
+      } else {
+        // cosine / euclidean / pearson: FP64 re-rank with certificate; queries
+        // it cannot clear get a second fused pass over just those columns with
+        // the widest list, and only what is still uncertified goes to the exact
+        // FP64 slab.
+        auto rerank = [&](const RerankArgs& RA) {
+          return fused_ok_euclid    ? rerank_euclid(ctx, RA)
+                 : fused_ok_pearson ? rerank_pearson(ctx, RA)
+                                    : rerank_cosine(ctx, RA);
+        };
+        DevBuf& fl = ctx->scratch[7];
+        DevBuf& fl2 = ctx->scratch[8];
+        SD_TRY(fl.ensure(sizeof(int32_t) * (size_t)(nq + 16)));
+        int32_t* fail_count = fl.as<int32_t>() + nq;
+        SD_HIP(hipMemsetAsync(fail_count, 0, sizeof(int32_t), ctx->stream));
+        RerankArgs R;
+        R.ex = ea;
+        R.cand_key = lkey;
+        R.cand_idx = lidx;
+        R.nqry = nq;
+        R.L = L;
+        R.k = k;
+        R.decreasing = plan.dec;
+        R.nonneg = !ref->any_negative && !X->any_negative;
+        R.in_round = use_f32 ? std::ldexp(1.0, -24) : std::ldexp(1.0, -8) + std::ldexp(1.0, -23);
+        R.acc_round = std::ldexp(1.0, -23);
+        R.qdeg = qry_deg;
+        R.out_idx = out_i;
+        R.out_val = out_v;
+        R.fail_list = fl.as<int32_t>();
+        R.fail_count = fail_count;
+        R.qlist = nullptr;
+        R.out_ld = nq_total;
+        R.map_words = rerank_map_words(ref->nrow);
+        SD_TRY(rerank(R));
+        int32_t nfail = 0;
+        SD_HIP(hipMemcpyAsync(&nfail, fail_count, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+        SD_HIP(hipStreamSynchronize(ctx->stream));
+        const int32_t* fail_ids = fl.as<int32_t>();
+        ctx->stats.last_margin = margin;
+        if (prec == SD_PREC_AUTO && p->margin <= 0) {
+          // adapt this reference's margin.  After a filtered pass the wider
+          // lists come from the candidate buffers and a failed query costs only
+          // its second re-rank (64 lanes instead of L): widen when more than 15%
+          // fail, narrow by 4 (down to 8) when fewer than 1% do.  Otherwise a
+          // failed query costs a second slab pass: widen above 1%, narrow by 8
+          // (down to 16) only after a clean search.
+          std::lock_guard<std::mutex> lk(ref->prep_mu);
+          if (cb.fkey) {
+            if ((int64_t)nfail * 100 > 15 * nq)
+              ref->tuned_margin = std::min(2 * margin, 48);
+            else if ((int64_t)nfail * 100 < nq)
+              ref->tuned_margin = std::max(margin - 4, 8);
+          } else {
+            if ((int64_t)nfail * 100 > nq)
+              ref->tuned_margin = std::min(2 * margin, 48);
+            else if (nfail == 0)
+              ref->tuned_margin = std::max(margin - 8, 16);
           }
-          k2 = mk2.as<double>();
-          i2 = mi2.as<int32_t>();
         }
-        if (!k2) {
-          // second fused pass over just those columns with the widest list
-          FusedArgs fb = fa;
-          fb.qlist = fl.as<int32_t>();
-          fb.nqry = nfail;
-          fb.L = Lmax;
-          const int64_t qt2 = (nfail + 255) / 256;
-          fb.splits = (int)std::min<int64_t>(round_up((2 * (int64_t)ctx->cu_count + qt2 - 1) / qt2, 8), 4096);
-          const int nl2 = fused_effective_splits(fb);
-          DevBuf& ck2 = ctx->scratch[9];
-          DevBuf& ci2 = ctx->scratch[10];
-          SD_TRY(ck2.ensure(sizeof(double) * (size_t)nl2 * (size_t)nfail * Lmax + 64));
-          SD_TRY(ci2.ensure(sizeof(int32_t) * (size_t)nl2 * (size_t)nfail * Lmax + 64));
-          SD_TRY(fused_slab_topl(ctx, fb, ck2.as<double>(), ci2.as<int32_t>()));
-          k2 = ck2.as<double>();
-          i2 = ci2.as<int32_t>();
-          if (nl2 > 1) {
+        const int Lmax = fused_list_len(64);
+        if (nfail > 0 && L < Lmax) {
+          SD_TRY(check_cancel(ctx));
+          DevBuf& mk2 = ctx->scratch[11];
+          DevBuf& mi2 = ctx->scratch[12];
+          const double* k2 = nullptr;
+          const int32_t* i2 = nullptr;
+          if (cb.fkey) {
+            // the wider lists straight from the filtered pass's buffers; a
+            // failed query whose buffer is not complete for Lmax (fewer
+            // entries, or overflowed) gets its list from a list-mode pass over
+            // every reference tile instead, placed at its position
             SD_TRY(mk2.ensure(sizeof(double) * (size_t)nfail * Lmax + 64));
             SD_TRY(mi2.ensure(sizeof(int32_t) * (size_t)nfail * Lmax + 64));
-            SD_TRY(merge_lists(ctx, k2, i2, nl2, nfail, Lmax, mk2.as<double>(), mi2.as<int32_t>()));
+            DevBuf& bc = ctx->scratch[15];
+            DevBuf& bpos = ctx->scratch[20];
+            SD_TRY(bc.ensure(64));
+            SD_TRY(bpos.ensure(sizeof(int32_t) * (size_t)(nfail + 16)));
+            SD_HIP(hipMemsetAsync(bc.ptr, 0, sizeof(int32_t), ctx->stream));
+            SD_TRY(select_lists(ctx, cb.fkey, cb.fidx, cb.cnt, cb.cap, nfail, Lmax, mk2.as<double>(),
+                                mi2.as<int32_t>(), bpos.as<int32_t>(), bc.as<int32_t>(), Lmax,
+                                fl.as<int32_t>()));
+            int32_t nbad = 0;
+            SD_HIP(hipMemcpyAsync(&nbad, bc.ptr, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+            SD_HIP(hipStreamSynchronize(ctx->stream));
+            ctx->stats.escalation_list_queries += nbad;
+            if (nbad > 0) {
+              DevBuf& bq = ctx->scratch[21];
+              SD_TRY(bq.ensure(sizeof(int32_t) * (size_t)(nbad + 16)));
+              SD_TRY(gather_i32(ctx, fl.as<int32_t>(), bpos.as<int32_t>(), nbad, bq.as<int32_t>()));
+              FusedArgs fb = fa;
+              fb.qlist = bq.as<int32_t>();
+              fb.nqry = nbad;
+              fb.L = Lmax;
+              fb.tile_stride = 1;
+              const int64_t qt2 = (nbad + 255) / 256;
+              fb.splits = (int)std::min<int64_t>(round_up((2 * (int64_t)ctx->cu_count + qt2 - 1) / qt2, 8), 4096);
+              const int nl2 = fused_effective_splits(fb);
+              DevBuf& ck2 = ctx->scratch[22];
+              DevBuf& ci2 = ctx->scratch[23];
+              DevBuf& ck3 = ctx->scratch[24];
+              DevBuf& ci3 = ctx->scratch[25];
+              SD_TRY(ck2.ensure(sizeof(double) * (size_t)nl2 * (size_t)nbad * Lmax + 64));
+              SD_TRY(ci2.ensure(sizeof(int32_t) * (size_t)nl2 * (size_t)nbad * Lmax + 64));
+              SD_TRY(fused_slab_topl(ctx, fb, ck2.as<double>(), ci2.as<int32_t>()));
+              const double* lk = ck2.as<double>();
+              const int32_t* li = ci2.as<int32_t>();
+              if (nl2 > 1) {
+                SD_TRY(ck3.ensure(sizeof(double) * (size_t)nbad * Lmax + 64));
+                SD_TRY(ci3.ensure(sizeof(int32_t) * (size_t)nbad * Lmax + 64));
+                SD_TRY(merge_lists(ctx, lk, li, nl2, nbad, Lmax, ck3.as<double>(),
+                                   ci3.as<int32_t>()));
+                lk = ck3.as<double>();
+                li = ci3.as<int32_t>();
+              }
+              SD_TRY(scatter_lists(ctx, lk, li, bpos.as<int32_t>(), nbad, Lmax, mk2.as<double>(),
+                                   mi2.as<int32_t>()));
+            }
             k2 = mk2.as<double>();
             i2 = mi2.as<int32_t>();
           }
-        } else {
-          ++ctx->stats.buffer_escalations;
+          if (!k2) {
+            // second fused pass over just those columns with the widest list
+            FusedArgs fb = fa;
+            fb.qlist = fl.as<int32_t>();
+            fb.nqry = nfail;
+            fb.L = Lmax;
+            const int64_t qt2 = (nfail + 255) / 256;
+            fb.splits = (int)std::min<int64_t>(round_up((2 * (int64_t)ctx->cu_count + qt2 - 1) / qt2, 8), 4096);
+            const int nl2 = fused_effective_splits(fb);
+            DevBuf& ck2 = ctx->scratch[9];
+            DevBuf& ci2 = ctx->scratch[10];
+            SD_TRY(ck2.ensure(sizeof(double) * (size_t)nl2 * (size_t)nfail * Lmax + 64));
+            SD_TRY(ci2.ensure(sizeof(int32_t) * (size_t)nl2 * (size_t)nfail * Lmax + 64));
+            SD_TRY(fused_slab_topl(ctx, fb, ck2.as<double>(), ci2.as<int32_t>()));
+            k2 = ck2.as<double>();
+            i2 = ci2.as<int32_t>();
+            if (nl2 > 1) {
+              SD_TRY(mk2.ensure(sizeof(double) * (size_t)nfail * Lmax + 64));
+              SD_TRY(mi2.ensure(sizeof(int32_t) * (size_t)nfail * Lmax + 64));
+              SD_TRY(merge_lists(ctx, k2, i2, nl2, nfail, Lmax, mk2.as<double>(), mi2.as<int32_t>()));
+              k2 = mk2.as<double>();
+              i2 = mi2.as<int32_t>();
+            }
+          } else {
+            ++ctx->stats.buffer_escalations;
+          }
+          SD_TRY(fl2.ensure(sizeof(int32_t) * (size_t)(nfail + 16)));
+          int32_t* fail_count2 = fl2.as<int32_t>() + nfail;
+          SD_HIP(hipMemsetAsync(fail_count2, 0, sizeof(int32_t), ctx->stream));
+          RerankArgs R2 = R;
+          R2.cand_key = k2;
+          R2.cand_idx = i2;
+          R2.nqry = nfail;
+          R2.L = Lmax;
+          R2.qlist = fl.as<int32_t>();
+          R2.fail_list = fl2.as<int32_t>();
+          R2.fail_count = fail_count2;
+          SD_TRY(rerank(R2));
+          ctx->stats.escalated_queries += nfail;
+          SD_HIP(hipMemcpyAsync(&nfail, fail_count2, sizeof(int32_t), hipMemcpyDeviceToHost,
+                                ctx->stream));
+          SD_HIP(hipStreamSynchronize(ctx->stream));
+          fail_ids = fl2.as<int32_t>();
         }
-        SD_TRY(fl2.ensure(sizeof(int32_t) * (size_t)(nfail + 16)));
-        int32_t* fail_count2 = fl2.as<int32_t>() + nfail;
-        SD_HIP(hipMemsetAsync(fail_count2, 0, sizeof(int32_t), ctx->stream));
-        RerankArgs R2 = R;
-        R2.cand_key = k2;
-        R2.cand_idx = i2;
-        R2.nqry = nfail;
-        R2.L = Lmax;
-        R2.qlist = fl.as<int32_t>();
-        R2.fail_list = fl2.as<int32_t>();
-        R2.fail_count = fail_count2;
-        SD_TRY(rerank(R2));
-        ctx->stats.escalated_queries += nfail;
-        SD_HIP(hipMemcpyAsync(&nfail, fail_count2, sizeof(int32_t), hipMemcpyDeviceToHost,
-                              ctx->stream));
-        SD_HIP(hipStreamSynchronize(ctx->stream));
-        fail_ids = fl2.as<int32_t>();
+        ctx->stats.fallback_queries += nfail;
+        ctx->stats.certified_queries += nq - nfail;
+        if (kind == kDenseBf16Raw && (int64_t)nfail * 100 > nq) {
+          std::lock_guard<std::mutex> lk(ref->prep_mu);
+          ref->raw_split = true;  // later searches on this reference use hi/lo
+        }
+        if (nfail > 0)
+          SD_TRY(exact_knn_columns(ctx, ea, fail_ids, nfail, nq_total, k, plan.dec, 0, out_i,
+                                   out_v));
       }
-      ctx->stats.fallback_queries += nfail;
-      ctx->stats.certified_queries += nq - nfail;
-      if (kind == kDenseBf16Raw && (int64_t)nfail * 100 > nq) {
-        std::lock_guard<std::mutex> lk(ref->prep_mu);
-        ref->raw_split = true;  // later searches on this reference use hi/lo
-      }
-      if (nfail > 0)
-        SD_TRY(exact_knn_columns(ctx, ea, fail_ids, nfail, nq, k, plan.dec, 0,
-                                 out_idx.as<int32_t>(), out_val.as<double>()));
     }
   } else {
     // exact FP64 path over every query column
@@ -1013,6 +1078,15 @@ int sd_context_set_timing(sd_context* ctx, int on) {
   return SD_OK;
 }
 
+int sd_context_set_buffer_budget(sd_context* ctx, int64_t bytes) {
+  return guarded([&]() -> int {
+    if (!ctx) return set_error(SD_EINVAL, "NULL context");
+    if (bytes < 0) return set_error(SD_EINVAL, "'bytes' must be non-negative.");
+    ctx->buffer_budget = bytes;
+    return SD_OK;
+  });
+}
+
 void* sd_context_stream(sd_context* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
 
 int sd_matrix_upload_csc(sd_context* ctx, int64_t nrow, int64_t ncol, const int64_t* colptr,
@@ -1133,10 +1207,11 @@ int sd_knn_cross(sd_context* ctx, const sd_matrix* X, const sd_matrix* Y,
     xm.ready = false;
     xm.target.filled = false;
     if (params->precision != SD_PREC_EXACT && X->ncol >= 128 * 128 && Y->ncol >= 128 * 128) {
-      const int64_t fused = ctx->stats.fused_calls;
+      const int64_t fused = ctx->stats.fused_calls, blocks = ctx->stats.query_blocks;
       xm.phase = 1;
       SD_TRY(run_knn(ctx, Ym, Xm, params, pr, rev_idx, rev_dist));  // writes no results
-      ctx->stats.fused_calls = fused;                               // counted when it resumes
+      ctx->stats.fused_calls = fused;  // counted when it resumes
+      ctx->stats.query_blocks = blocks;
       xm.phase = 2;
     }
     // the forward results wait in host memory until the reverse search has

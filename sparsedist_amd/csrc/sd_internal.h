@@ -154,6 +154,9 @@ struct sd_context {
   // device address of the allocation-free entry point's scratch array on
   // this context's device (resolved when the context is created)
   void* noalloc_scratch = nullptr;
+  // device bytes one fused search may hold in candidate buffers and lists
+  // before its queries are cut into blocks (0: half the free memory)
+  int64_t buffer_budget = 0;
 };
 
 struct sd_matrix {
@@ -245,7 +248,8 @@ struct ExactArgs {
   const double* ref_mean = nullptr; const double* ref_css = nullptr;
   const double* qry_mean = nullptr; const double* qry_css = nullptr;
   const uint8_t* ref_deg = nullptr;   // nullptr: no masking
-  int self_mode = 0;            // query j is reference column j
+  int self_mode = 0;            // query j is reference column j + self_off
+  int64_t self_off = 0;
   int include_self = 0;
 };
 // D[i + b*ld] for refs i in [0, ref.ncol) and queries qcols[b] (b < nb).
@@ -286,6 +290,7 @@ struct FusedArgs {
   int dot_nonneg = 0;
   const uint8_t* ref_deg = nullptr;
   int self_mode = 0, include_self = 0;
+  int64_t self_off = 0;             // self mode: query j is reference column j + self_off
   int L = 0;                        // list length kept per query
   int splits = 1;
   int tile_stride = 1;              // list mode: only every tile_stride-th reference tile
@@ -362,8 +367,10 @@ int gather_i32(sd_context* ctx, const int32_t* src, const int32_t* pos, int64_t 
 int fused_effective_splits(const FusedArgs& a);
 int merge_lists(sd_context* ctx, const double* key, const int32_t* idx, int nlists, int64_t nqry,
                 int L, double* out_key, int32_t* out_idx);
+// Jaccard results (nqry x k, leading dimension out_ld) from exact lists.
 int finish_exact_lists(sd_context* ctx, const double* key, const int32_t* idx, int64_t nqry, int L,
-                       int k, int dist, const uint8_t* qdeg, int32_t* out_idx, double* out_val);
+                       int k, int dist, const uint8_t* qdeg, int32_t* out_idx, double* out_val,
+                       int64_t out_ld);
 int rerank_cosine(sd_context* ctx, const RerankArgs& R);
 // Euclidean: exact FP64 distances of the candidates in the reference's
 // arithmetic; the certificate compares the k-th squared distance with the

@@ -692,3 +692,66 @@ def test_escalation_with_incomplete_buffers(oracle_lib, gpu_ctx):
     same((g.idx, g.dist), o, "incomplete escalation buffers")
     assert st["buffer_escalations"] == 1 and st["escalation_list_queries"] >= 1, st
     assert st["escalated_queries"] < 100, st
+
+
+@pytest.mark.parametrize("method", ["binary", "cosine", "euclidean", "pearson"])
+def test_query_blocks_self_search(oracle_lib, gpu_ctx, method):
+    """Under a small buffer budget a 40,000-column self-search runs its query
+    columns in three blocks (18,432 + 18,432 + 3,136), each a whole search
+    against every reference column that finds the query's own column at the
+    block's offset.  Every row must equal the one-launch (symmetric) search,
+    and sampled rows -- around the block edges, in the partial last block and
+    a degenerate column in the second block -- the oracle, with and without
+    the query's own column."""
+    n = 40000
+    Y = synth.nb_log1p(200, n, 0.1, 120).tolil()
+    Y[:, 20001] = 0
+    Y = Y.tocsc()
+    Y.eliminate_zeros()
+    k = 10
+    rng = np.random.default_rng(121)
+    qcols = sorted(set(rng.choice(n, 200, replace=False).tolist()) |
+                   {0, 18431, 18432, 20001, 36863, 36864, 39999})
+    for include_self in (False, True):
+        lab = f"{method} include_self={include_self}"
+        gpu_ctx.set_buffer_budget(0)
+        gpu_ctx.stats(reset=True)
+        one = sd.sparseKNN(Y, k=k, method=method, include_self=include_self, precision="bf16",
+                           verbose=False)
+        assert gpu_ctx.stats(reset=True)["query_blocks"] == 1, lab
+        gpu_ctx.set_buffer_budget(1 << 30)
+        try:
+            blk = sd.sparseKNN(Y, k=k, method=method, include_self=include_self,
+                               precision="bf16", verbose=False)
+        finally:
+            gpu_ctx.set_buffer_budget(0)
+        st = gpu_ctx.stats(reset=True)
+        assert st["query_blocks"] == 3, (lab, st)
+        same((blk.idx, blk.dist), (one.idx, one.dist), f"{lab}: blocks vs one launch")
+        o = oracle_lib.knn(Y[:, qcols], Y, k=k + 1, method=method, nthreads=NT)
+        same((blk.idx[qcols], blk.dist[qcols]),
+             self_rows_from_cross(o[0], o[1], qcols, k, include_self), f"{lab}: oracle")
+
+
+def test_query_blocks_cross_search(oracle_lib, gpu_ctx):
+    """A cross pair whose searches need blocks gives up the shared filtered
+    pass (the reverse search's probe stands down) and runs each direction in
+    blocks; both directions must equal the unblocked cross search."""
+    X = synth.nb_log1p(200, 20000, 0.1, 122)
+    Y = synth.nb_log1p(200, 18000, 0.1, 123)
+    gpu_ctx.stats(reset=True)
+    one = sd.sparseKNNCross(X, Y, k=8, method="cosine", precision="bf16", verbose=False)
+    assert gpu_ctx.stats(reset=True)["query_blocks"] == 2
+    gpu_ctx.set_buffer_budget(1 << 29)
+    try:
+        blk = sd.sparseKNNCross(X, Y, k=8, method="cosine", precision="bf16", verbose=False)
+    finally:
+        gpu_ctx.set_buffer_budget(0)
+    st = gpu_ctx.stats(reset=True)
+    assert st["query_blocks"] >= 4, st
+    for d in ("forward", "reverse"):
+        same((blk[d].idx, blk[d].dist), (one[d].idx, one[d].dist), f"blocked cross {d}")
+    rng = np.random.default_rng(124)
+    qs = np.sort(rng.choice(20000, 120, replace=False))
+    oi, od, _ = oracle_lib.knn(X[:, qs], Y, k=8, method="cosine", nthreads=NT)
+    same((blk["forward"].idx[qs], blk["forward"].dist[qs]), (oi, od), "blocked cross oracle")
