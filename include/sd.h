@@ -127,6 +127,7 @@ typedef struct {
                                       list-mode pass of their own */
   int64_t query_blocks;       /* blocks of query columns the fused searches ran in (one per
                                  search unless its buffers exceeded the buffer budget) */
+  int64_t snn_device_inputs;  /* sd_snn calls that read their indices from device memory */
 } sd_stats;
 int sd_context_stats(sd_context* ctx, sd_stats* out, int reset);
 /* Enable/disable event timing of kernels (off by default). */
@@ -278,6 +279,17 @@ int sd_snn_jaccard_noalloc(sd_context* ctx, const int32_t* d_idx, int32_t n, int
 int sd_snn(sd_context* ctx, const int32_t* idx, int32_t n, int32_t k, int32_t include_self,
            double prune, int32_t symmetrise, int32_t edges, int32_t idx_on_device,
            sd_graph** out);
+/* The neighbour indices of the context's last sd_knn self-search (Y NULL),
+ * kept in device memory exactly as that call returned them (n x k
+ * column-major int32, 1-based, NA_INTEGER for missing neighbours) until the
+ * next self-search on the context replaces them: pass *d_idx to sd_snn with
+ * idx_on_device = 1 instead of uploading the host copy again.  *serial
+ * numbers the self-searches of the context; a caller that records it right
+ * after its sd_knn call can tell whether the kept indices are still its
+ * result.  *d_idx is NULL and *serial 0 while nothing is kept.  No
+ * counterpart in the reference, where sparseSNN(nn) reads nn$idx
+ * (R/sparseSNN.R:124-166). */
+int sd_knn_kept(sd_context* ctx, const int32_t** d_idx, int32_t* n, int32_t* k, int64_t* serial);
 /* Size of the graph: n and stored entries. */
 int sd_graph_info(const sd_graph* g, int64_t* n, int64_t* nnz);
 /* Copy the graph to host memory in dgCMatrix layout: colptr int64[n+1],

@@ -27,7 +27,7 @@ EXPORTS = (
     "sd_graph_copy", "sd_graph_free", "sd_detect_cores", "sd_server_create", "sd_server_submit",
     "sd_server_wait", "sd_server_error", "sd_server_result", "sd_server_release",
     "sd_server_stats_get", "sd_server_destroy", "sd_autotune", "sd_server_resume",
-    "sd_snn_jaccard_noalloc", "sd_context_set_buffer_budget",
+    "sd_snn_jaccard_noalloc", "sd_context_set_buffer_budget", "sd_knn_kept",
 )
 
 
@@ -56,7 +56,8 @@ class Stats(C.Structure):
                 ("fused_calls", C.c_int64), ("exact_calls", C.c_int64),
                 ("last_margin", C.c_int64), ("slab_products", C.c_int64),
                 ("filter_pairs", C.c_int64), ("buffer_escalations", C.c_int64),
-                ("escalation_list_queries", C.c_int64), ("query_blocks", C.c_int64)]
+                ("escalation_list_queries", C.c_int64), ("query_blocks", C.c_int64),
+                ("snn_device_inputs", C.c_int64)]
 
 
 class TuneInfo(C.Structure):
@@ -113,6 +114,8 @@ def load(path: str = LIB_PATH):
         L.sd_context_stats.argtypes = [vp, C.POINTER(Stats), C.c_int]
         L.sd_context_set_timing.argtypes = [vp, C.c_int]
         L.sd_context_set_buffer_budget.argtypes = [vp, C.c_int64]
+        L.sd_knn_kept.argtypes = [vp, C.POINTER(vp), C.POINTER(i32), C.POINTER(i32),
+                                  C.POINTER(i64)]
         L.sd_context_stream.argtypes = [vp]
         L.sd_context_stream.restype = vp
         L.sd_matrix_upload_csc.argtypes = [vp, i64, i64, vp, vp, vp, C.POINTER(vp)]

@@ -240,10 +240,13 @@ class KNNResult(dict):
     """``list(idx, dist)`` with the ``self_search`` attribute; ``names`` holds
     the row names (the query matrix's column names, when it has any)."""
 
-    def __init__(self, idx, dist, self_search, names=None):
+    def __init__(self, idx, dist, self_search, names=None, kept=None):
         super().__init__(idx=idx, dist=dist)
         self.self_search = self_search
         self.names = names
+        # (context, serial) when the context still may hold these indices on
+        # the device (a self-search's idx, returned read-only)
+        self._kept = kept
 
     @property
     def idx(self):
@@ -322,8 +325,29 @@ def sparseKNN(X, Y=None, k=10, method="binary", dist=True, include_self=False,
     if warn.value:
         warnings.warn(f"k ({k}) exceeds the {ke} available neighbours; returning {ke}.",
                       SparseDistWarning, stacklevel=2)
-    return KNNResult(idx[: nq * ke].reshape((nq, ke), order="F"),
-                     dst[: nq * ke].reshape((nq, ke), order="F"), selfmode, _colnames(X))
+    ix = idx[: nq * ke].reshape((nq, ke), order="F")
+    kept = None
+    if selfmode:
+        # the context keeps this idx on the device for sparseSNN(result); the
+        # array is read-only so that the two copies cannot drift apart
+        serial = _kept_serial(ctx)
+        if serial > 0:
+            ix.setflags(write=False)
+            kept = (ctx, serial)
+    return KNNResult(ix, dst[: nq * ke].reshape((nq, ke), order="F"), selfmode, _colnames(X),
+                     kept)
+
+
+def _kept_info(ctx):
+    """(device pointer, n, k, serial) of the context's kept self-search idx."""
+    d, n, k, serial = C.c_void_p(), C.c_int32(0), C.c_int32(0), C.c_int64(0)
+    check(_lib.load().sd_knn_kept(ctx.handle, C.byref(d), C.byref(n), C.byref(k),
+                                  C.byref(serial)))
+    return d.value, n.value, k.value, serial.value
+
+
+def _kept_serial(ctx):
+    return _kept_info(ctx)[3]
 
 
 def sparseKNNCross(X, Y=None, k=10, method="binary", dist=True, drop_degenerate=True,
@@ -466,9 +490,17 @@ def sparseSNN(idx, include_self=True, prune=0, symmetrise=_MISSING, edges="knn",
     _chk_count(ncores, "ncores", allow_zero=True)
     n, k = idx32.shape
     ctx = context or default_context()
+    # a self-search result whose indices the context still holds on the
+    # device (same search, array left read-only) is not uploaded again
+    src, on_device = idx32.ctypes.data, 0
+    kept = getattr(idx, "_kept", None) if isinstance(idx, KNNResult) else None
+    if kept is not None and kept[0] is ctx and not idx.idx.flags.writeable:
+        d, kn, kk, serial = _kept_info(ctx)
+        if d and serial == kept[1] and (kn, kk) == (n, k):
+            src, on_device = d, 1
     g = C.c_void_p()
-    check(_lib.load().sd_snn(ctx.handle, idx32.ctypes.data, n, k, int(include_self),
-                             float(prune), int(symmetrise), 0 if edges == "knn" else 1, 0,
+    check(_lib.load().sd_snn(ctx.handle, src, n, k, int(include_self), float(prune),
+                             int(symmetrise), 0 if edges == "knn" else 1, on_device,
                              C.byref(g)))
     return _graph_to_csc(g, n)
 

@@ -238,4 +238,28 @@ int topk_rows(sd_context* ctx, const double* D, int64_t nrow, int64_t ncol, int 
   return SD_OK;
 }
 
+// A degenerate query column gets no neighbours: NA index and NA distance in
+// every slot of its row (R/sparseKNN.R:298-301).
+__global__ void k_fill_degenerate(int32_t* __restrict__ idx, double* __restrict__ val,
+                                  const uint8_t* __restrict__ deg, int64_t nq, int64_t n) {
+  const double na = __longlong_as_double((long long)kNaRealBits);
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    if (deg[e % nq]) {
+      idx[e] = kNaInt;
+      val[e] = na;
+    }
+  }
+}
+
+int fill_degenerate(sd_context* ctx, int32_t* idx, double* val, const uint8_t* deg, int64_t nq,
+                    int k) {
+  const int64_t n = nq * k;
+  if (n <= 0) return SD_OK;
+  hipLaunchKernelGGL(k_fill_degenerate, dim3(stride_grid(n)), dim3(256), 0, ctx->stream, idx, val,
+                     deg, nq, n);
+  SD_HIP(hipGetLastError());
+  return SD_OK;
+}
+
 }  // namespace sd

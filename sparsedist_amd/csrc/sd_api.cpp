@@ -176,10 +176,6 @@ using namespace sd;
 
 namespace {
 
-constexpr double kNaReal() {
-  return __builtin_bit_cast(double, kNaRealBits);
-}
-
 int check_cancel(sd_context* ctx) {
   if (ctx->cancel.load(std::memory_order_relaxed))
     return set_error(SD_ECANCELLED, "Computation interrupted by the user.");
@@ -598,7 +594,7 @@ int64_t fused_query_block(sd_context* ctx, int64_t nq, int L) {
 }
 
 int run_knn(sd_context* ctx, sd_matrix* X, sd_matrix* Y, const sd_knn_params* p, const KnnPlan& plan,
-            int32_t* h_idx, double* h_val) {
+            int32_t* h_idx, double* h_val, bool keep = false) {
   sd_matrix* ref = plan.selfmode ? X : Y;
   const int method = p->method;
   const int k = plan.k;
@@ -960,20 +956,26 @@ int run_knn(sd_context* ctx, sd_matrix* X, sd_matrix* Y, const sd_knn_params* p,
                              out_idx.as<int32_t>(), out_val.as<double>()));
   }
   SD_TRY(check_cancel(ctx));
+  // a degenerate query gets no neighbours (R/sparseKNN.R:298-301)
+  if (p->drop_degenerate)
+    SD_TRY(fill_degenerate(ctx, out_idx.as<int32_t>(), out_val.as<double>(),
+                           X->deg.as<uint8_t>(), nq, k));
   SD_HIP(hipMemcpyAsync(h_idx, out_idx.ptr, sizeof(int32_t) * (size_t)nq * (size_t)k,
                         hipMemcpyDeviceToHost, ctx->stream));
   SD_HIP(hipMemcpyAsync(h_val, out_val.ptr, sizeof(double) * (size_t)nq * (size_t)k,
                         hipMemcpyDeviceToHost, ctx->stream));
+  if (keep) {
+    // the indices stay on the device for a following SNN call
+    ctx->knn_serial = 0;
+    SD_TRY(ctx->knn_keep.ensure(sizeof(int32_t) * (size_t)nq * (size_t)k + 64));
+    SD_HIP(hipMemcpyAsync(ctx->knn_keep.ptr, out_idx.ptr, sizeof(int32_t) * (size_t)nq * (size_t)k,
+                          hipMemcpyDeviceToDevice, ctx->stream));
+  }
   SD_HIP(hipStreamSynchronize(ctx->stream));
-  // a degenerate query gets no neighbours (R/sparseKNN.R:298-301)
-  if (p->drop_degenerate) {
-    const double na = kNaReal();
-    for (int64_t q = 0; q < nq; ++q)
-      if (X->hdeg[(size_t)q])
-        for (int t = 0; t < k; ++t) {
-          h_idx[(int64_t)t * nq + q] = kNaInt;
-          h_val[(int64_t)t * nq + q] = na;
-        }
+  if (keep) {
+    ctx->knn_serial = ++ctx->knn_calls;
+    ctx->keep_n = nq;
+    ctx->keep_k = k;
   }
   return SD_OK;
 }
@@ -1050,12 +1052,23 @@ int sd_context_destroy(sd_context* ctx) {
   // pooled buffers are freed on the stream, so before it goes
   for (auto& b : ctx->scratch) b.release();
   for (auto& b : ctx->snn_scratch) b.release();
-  for (DevBuf* b : {&ctx->xm.thr, &ctx->xm.cnt, &ctx->xm.fkey, &ctx->xm.fidx}) b->release();
+  for (DevBuf* b : {&ctx->xm.thr, &ctx->xm.cnt, &ctx->xm.fkey, &ctx->xm.fidx, &ctx->knn_keep})
+    b->release();
   (void)hipStreamSynchronize(ctx->stream);
   if (ctx->ev_a) (void)hipEventDestroy(ctx->ev_a);
   if (ctx->ev_b) (void)hipEventDestroy(ctx->ev_b);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
+  return SD_OK;
+}
+
+int sd_knn_kept(sd_context* ctx, const int32_t** d_idx, int32_t* n, int32_t* k, int64_t* serial) {
+  if (!ctx || !d_idx || !n || !k || !serial) return set_error(SD_EINVAL, "NULL argument");
+  const bool kept = ctx->knn_serial > 0;
+  *d_idx = kept ? ctx->knn_keep.as<int32_t>() : nullptr;
+  *n = kept ? (int32_t)ctx->keep_n : 0;
+  *k = kept ? (int32_t)ctx->keep_k : 0;
+  *serial = ctx->knn_serial;
   return SD_OK;
 }
 
@@ -1150,7 +1163,7 @@ int sd_knn(sd_context* ctx, const sd_matrix* X, const sd_matrix* Y, const sd_knn
     *warn = plan.warn;
     *k_eff = plan.k;
     return run_knn(ctx, const_cast<sd_matrix*>(X), const_cast<sd_matrix*>(Y), params, plan, idx,
-                   dist);
+                   dist, plan.selfmode);
   });
 }
 
@@ -1361,7 +1374,9 @@ int sd_snn(sd_context* ctx, const int32_t* idx, int32_t n, int32_t k, int32_t in
     CancelScope scope(ctx);
     DevBuf& di = ctx->snn_scratch[0];
     const int32_t* d_idx = idx;
-    if (!idx_on_device) {
+    if (idx_on_device) {
+      ++ctx->stats.snn_device_inputs;
+    } else {
       const size_t total = (size_t)n * (size_t)k;
       SD_TRY(di.ensure(sizeof(int32_t) * total));
       SD_HIP(hipMemcpyAsync(di.ptr, idx, sizeof(int32_t) * total, hipMemcpyHostToDevice,

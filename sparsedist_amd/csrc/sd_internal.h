@@ -154,6 +154,11 @@ struct sd_context {
   // device address of the allocation-free entry point's scratch array on
   // this context's device (resolved when the context is created)
   void* noalloc_scratch = nullptr;
+  // the last self-search's neighbour indices (n x k column-major, as
+  // returned), kept on the device for a following SNN call; knn_serial
+  // counts the self-searches, 0 while none is kept
+  sd::DevBuf knn_keep;
+  int64_t knn_serial = 0, knn_calls = 0, keep_n = 0, keep_k = 0;
   // device bytes one fused search may hold in candidate buffers and lists
   // before its queries are cut into blocks (0: half the free memory)
   int64_t buffer_budget = 0;
@@ -367,6 +372,10 @@ int gather_i32(sd_context* ctx, const int32_t* src, const int32_t* pos, int64_t 
 int fused_effective_splits(const FusedArgs& a);
 int merge_lists(sd_context* ctx, const double* key, const int32_t* idx, int nlists, int64_t nqry,
                 int L, double* out_key, int32_t* out_idx);
+// NA rows (index and distance) for the degenerate query columns of a
+// column-major nq x k result.
+int fill_degenerate(sd_context* ctx, int32_t* idx, double* val, const uint8_t* deg, int64_t nq,
+                    int k);
 // Jaccard results (nqry x k, leading dimension out_ld) from exact lists.
 int finish_exact_lists(sd_context* ctx, const double* key, const int32_t* idx, int64_t nqry, int L,
                        int k, int dist, const uint8_t* qdeg, int32_t* out_idx, double* out_val,

@@ -117,3 +117,43 @@ def test_fixture_shared_awkward(gpu_ctx):
     assert sd.sparseSNN(idx, edges="shared", prune=1).nnz == 0
     with pytest.warns(sd.SparseDistWarning, match="symmetrise"):
         sd.sparseSNN(idx, edges="shared", symmetrise=False)
+
+
+def test_snn_reads_kept_device_indices(oracle_lib, gpu_ctx):
+    """sparseSNN(nn) on the context's last self-search reads the indices the
+    search left on the device (no upload), with degenerate columns (NA rows)
+    among them; the graph equals the one built from an uploaded copy and the
+    oracle's.  A later self-search replaces the kept indices: the earlier
+    result then goes up from the host again, and an idx made writable is
+    never read from the device."""
+    X = synth.abs_normal(800, 3000, 0.05, 30).tolil()
+    for c in (7, 1500, 2999):
+        X[:, c] = 0
+    X = X.tocsc()
+    X.eliminate_zeros()
+    nn = sd.sparseKNN(X, k=12, method="cosine", verbose=False)
+    assert not nn.idx.flags.writeable
+    assert (nn.idx[[7, 1500, 2999]] == NA_INT).all()
+    gpu_ctx.stats(reset=True)
+    g_dev = sd.sparseSNN(nn, prune=1 / 15)
+    assert gpu_ctx.stats(reset=True)["snn_device_inputs"] == 1
+    g_host = sd.sparseSNN(nn.idx.copy(), prune=1 / 15)
+    assert gpu_ctx.stats(reset=True)["snn_device_inputs"] == 0
+    p, ri, x = oracle_lib.snn_graph(np.asfortranarray(nn.idx), include_self=True, prune=1 / 15,
+                                    symmetrise=True, nthreads=NT)
+    same_graph(g_dev, p, ri, x)
+    same_graph(g_host, p, ri, x)
+    # another self-search replaces the kept indices
+    nn2 = sd.sparseKNN(X, k=9, method="binary", verbose=False)
+    g_old = sd.sparseSNN(nn, prune=1 / 15)
+    assert gpu_ctx.stats(reset=True)["snn_device_inputs"] == 0
+    same_graph(g_old, p, ri, x)
+    g2 = sd.sparseSNN(nn2, edges="shared")
+    assert gpu_ctx.stats(reset=True)["snn_device_inputs"] == 1
+    p2, ri2, x2 = oracle_lib.snn_graph(np.asfortranarray(nn2.idx), include_self=True, prune=0,
+                                       symmetrise=True, edges="shared", nthreads=NT)
+    same_graph(g2, p2, ri2, x2)
+    # an idx made writable again goes up from the host
+    nn2.idx.setflags(write=True)
+    sd.sparseSNN(nn2, edges="shared")
+    assert gpu_ctx.stats(reset=True)["snn_device_inputs"] == 0
