@@ -205,6 +205,19 @@ def main():
                                idx=nn.idx, dist=nn.dist, graph=g)
         return res
 
+    # setup: the threshold-sample tuner times each of its four candidates
+    # once per class of large searches, one per search (a search that had to
+    # grow a device buffer is not timed), and keeps the fastest; a deployment
+    # tunes once, so the workload runs until both classes are tuned (at most
+    # 10 times) before the warm-up; results are the same whichever candidate
+    # runs
+    sd.tune_samples(reset=True)
+    for t_ in range(10):
+        step()
+        tab = sd.tune_samples()
+        if len(tab) >= 2 and all(min(e["runs"]) >= 1 for e in tab):
+            break
+    log(f"sample tuning took {t_ + 1} runs of the workload")
     for w in range(args.warmup):
         step()
         log(f"warmup {w + 1}/{args.warmup} done")
@@ -220,6 +233,17 @@ def main():
             f"fallback {r['cosine']['stats']['fallback_queries']})")
     ms = float(np.median(times)) * 1e3
     value = 2 * nq / (ms / 1e3)
+    # the tuner's table after the timed steps: per class, the candidates'
+    # measured times, the fixed rule's choice and the candidate now running
+    tuned = []
+    for e in sd.tune_samples():
+        timed = [c for c in range(4) if e["runs"][c] > 0]
+        cand = [f"{e['candidates'][c][0]}/{e['candidates'][c][1]}" for c in range(4)]
+        tuned.append({"method": {0: "binary", 1: "cosine"}.get(e["method"], e["method"]),
+                      "symmetric": e["symmetric"],
+                      "ns_per_mpair": {cand[c]: round(e["ns_per_mpair"][c], 1) for c in timed},
+                      "rule": cand[e["rule"]], "running": cand[e["last_run"]]})
+    log(f"sample tuning (stride/rank): {tuned}")
 
     def med(method, key):
         return float(np.median([p[method][key] for p in per]))
@@ -373,6 +397,7 @@ def main():
         "roofline_other": roof_cos if dominant is roof_bin else roof_bin,
         "cpu_baseline": cpu,
         "symmetric_vs_full_grid": full_grid,
+        "sample_tuning": tuned,
         "snn_vs_oracle": snn_parity,
         "serving": serving,
     }

@@ -128,6 +128,8 @@ typedef struct {
   int64_t query_blocks;       /* blocks of query columns the fused searches ran in (one per
                                  search unless its buffers exceeded the buffer budget) */
   int64_t snn_device_inputs;  /* sd_snn calls that read their indices from device memory */
+  int64_t sample_stride;      /* threshold sample of the last filtered search: tile stride */
+  int64_t sample_rank;        /* ... and rank Lp */
 } sd_stats;
 int sd_context_stats(sd_context* ctx, sd_stats* out, int reset);
 /* Enable/disable event timing of kernels (off by default). */
@@ -173,6 +175,41 @@ typedef struct {
   double step_latency_us; /* merge walk: time per sequential step of one pair */
 } sd_tune_info;
 int sd_autotune(sd_context* ctx, int32_t measure, sd_tune_info* out);
+
+/* The filtered search's threshold sample is tuned online.  Large searches
+ * run in two passes: a list-mode pre-pass over every stride-th reference
+ * tile gives each query its rank-Lp key there as a threshold, and the
+ * filtered pass over all tiles keeps the candidates at or above it.  A
+ * sparser sample costs the pre-pass less and leaves the filtered pass more
+ * candidates (about stride x Lp per query); the balance depends on the row
+ * length, the density, the list length and the symmetry of the pass.  Each
+ * class of searches (method, layout, density bucket, row bytes, list length
+ * class, symmetric or not) times four candidates -- (stride, Lp) = (32, L),
+ * (64, ~L/2), (128, ~L/4) and (64, ~L/4 but >= 8) -- on its first searches
+ * of at least 2^28 scored pairs, one candidate each, and runs the fastest
+ * afterwards.  Results do not depend on the choice.  The table is
+ * process-wide.  No counterpart in the reference, whose design bench sweeps
+ * density to place its dense/sparse crossover (bench/R/20-design.R:115-160).
+ * sd_context_set_tuning: 1 (default) tunes; 0 uses the fixed rule (every
+ * 64th tile at ~L/4 for rows of at most 2 KB, else 128th / 64th / 32nd by the
+ * reference's tile count). */
+int sd_context_set_tuning(sd_context* ctx, int32_t mode);
+typedef struct {
+  int32_t method, layout;     /* SD_* method; densified layout code */
+  int32_t density_bucket;     /* round(-2 log2(stored fraction of the reference)) */
+  int32_t row_bytes_log2;     /* ceil(log2(bytes per densified column)) */
+  int32_t list_len;           /* list length class: 16 (L <= 16) or 64 (longer lists) */
+  int32_t symmetric;          /* 1: a self-search's symmetric pass */
+  int32_t rule;               /* candidate the fixed rule chose for the class's last search */
+  int32_t last_run;           /* candidate that search ran (-1: none noted yet) */
+  int32_t stride[4];          /* the candidates: sampled tile stride ... */
+  int32_t rank[4];            /* ... and rank Lp (for the class's last timed list length) */
+  double ns_per_mpair[4];     /* measured ns per 10^6 scored pairs (0: not yet timed) */
+  int64_t runs[4];            /* timed searches per candidate */
+} sd_tune_sample;
+/* Copies up to max entries of the sample table; *n = the number of entries.
+ * reset != 0 clears the table afterwards. */
+int sd_tune_samples(sd_tune_sample* out, int32_t max, int32_t* n, int32_t reset);
 
 /* ---- matrices ------------------------------------------------------------- */
 /* Copy a CSC matrix to device memory.  Replaces the per-call dgCMatrix ->

@@ -23,6 +23,7 @@ from .api import (  # noqa: F401
     sparseSNN,
     topKBlock,
     topKRowsAccum,
+    tune_samples,
 )
 
 __version__ = "0.1.0"

@@ -28,6 +28,7 @@ EXPORTS = (
     "sd_server_wait", "sd_server_error", "sd_server_result", "sd_server_release",
     "sd_server_stats_get", "sd_server_destroy", "sd_autotune", "sd_server_resume",
     "sd_snn_jaccard_noalloc", "sd_context_set_buffer_budget", "sd_knn_kept",
+    "sd_context_set_tuning", "sd_tune_samples",
 )
 
 
@@ -57,7 +58,16 @@ class Stats(C.Structure):
                 ("last_margin", C.c_int64), ("slab_products", C.c_int64),
                 ("filter_pairs", C.c_int64), ("buffer_escalations", C.c_int64),
                 ("escalation_list_queries", C.c_int64), ("query_blocks", C.c_int64),
-                ("snn_device_inputs", C.c_int64)]
+                ("snn_device_inputs", C.c_int64), ("sample_stride", C.c_int64),
+                ("sample_rank", C.c_int64)]
+
+
+class TuneSample(C.Structure):
+    _fields_ = [("method", C.c_int32), ("layout", C.c_int32), ("density_bucket", C.c_int32),
+                ("row_bytes_log2", C.c_int32), ("list_len", C.c_int32), ("symmetric", C.c_int32),
+                ("rule", C.c_int32), ("last_run", C.c_int32),
+                ("stride", C.c_int32 * 4), ("rank", C.c_int32 * 4),
+                ("ns_per_mpair", C.c_double * 4), ("runs", C.c_int64 * 4)]
 
 
 class TuneInfo(C.Structure):
@@ -116,6 +126,8 @@ def load(path: str = LIB_PATH):
         L.sd_context_set_buffer_budget.argtypes = [vp, C.c_int64]
         L.sd_knn_kept.argtypes = [vp, C.POINTER(vp), C.POINTER(i32), C.POINTER(i32),
                                   C.POINTER(i64)]
+        L.sd_context_set_tuning.argtypes = [vp, i32]
+        L.sd_tune_samples.argtypes = [C.POINTER(TuneSample), i32, C.POINTER(i32), i32]
         L.sd_context_stream.argtypes = [vp]
         L.sd_context_stream.restype = vp
         L.sd_matrix_upload_csc.argtypes = [vp, i64, i64, vp, vp, vp, C.POINTER(vp)]

@@ -95,6 +95,11 @@ class Context:
         check(_lib.load().sd_autotune(self._h, int(bool(measure)), C.byref(t)))
         return {f: getattr(t, f) for f, _ in t._fields_ if f != "reserved"}
 
+    def set_tuning(self, on: bool):
+        """Tune the filtered search's threshold sample online (default) or use
+        the fixed rule; see sd_context_set_tuning."""
+        check(_lib.load().sd_context_set_tuning(self._h, int(bool(on))))
+
     def __del__(self):
         try:
             self.close()
@@ -452,6 +457,25 @@ def _idx_matrix(idx):
                              "indices in [1, nrow(idx)].")
         out = a.astype(np.int32)
     return np.asfortranarray(out)
+
+
+def tune_samples(reset: bool = False) -> list:
+    """The process-wide threshold-sample table (sd_tune_samples): one dict per
+    class of searches with its candidates' (stride, rank), measured ns per
+    10^6 scored pairs and timed runs.  reset=True clears the table."""
+    L = _lib.load()
+    n = C.c_int32(0)
+    check(L.sd_tune_samples(None, 0, C.byref(n), 0))
+    buf = (_lib.TuneSample * max(n.value, 1))()
+    check(L.sd_tune_samples(buf, n.value, C.byref(n), int(bool(reset))))
+    out = []
+    for e in buf[: n.value]:
+        out.append({"method": e.method, "layout": e.layout, "density_bucket": e.density_bucket,
+                    "row_bytes_log2": e.row_bytes_log2, "list_len": e.list_len,
+                    "symmetric": bool(e.symmetric), "rule": e.rule, "last_run": e.last_run,
+                    "candidates": [(e.stride[c], e.rank[c]) for c in range(4)],
+                    "ns_per_mpair": list(e.ns_per_mpair), "runs": list(e.runs)})
+    return out
 
 
 class _Missing:

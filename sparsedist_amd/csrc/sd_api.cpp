@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -28,7 +29,10 @@ int set_error(int code, const char* fmt, ...) {
 
 void clear_error() { g_err.clear(); }
 
+std::atomic<int64_t> g_device_allocs{0};
+
 int DevBuf::alloc(size_t n) {
+  g_device_allocs.fetch_add(1, std::memory_order_relaxed);
   release();
   if (n == 0) n = 16;
   hipError_t e = pool ? hipMallocAsync(&ptr, n, pool) : hipMalloc(&ptr, n);
@@ -329,6 +333,14 @@ struct CandBuffers {
   const int32_t* fidx = nullptr;
   const int32_t* cnt = nullptr;
   int cap = 0;
+  // a search timed for the sample tuner: its class, candidate and scored
+  // pairs, timed from the pre-pass to the end of the search
+  bool record = false;
+  SampleKey key;
+  int cand = 0;
+  double pairs = 0;
+  int64_t allocs = 0;  // g_device_allocs at the start
+  std::chrono::steady_clock::time_point t_start;
 };
 
 int fused_candidate_lists(sd_context* ctx, FusedArgs fa, const double** out_key,
@@ -384,12 +396,43 @@ int fused_candidate_lists(sd_context* ctx, FusedArgs fa, const double** out_key,
   // candidates per query instead of about 1000.  On 256-dimensional
   // embeddings (200,000 x 1,000,000 cross search) every 32nd, 64th and
   // 128th tile gave 339, 318 and 329 ms.
-  const bool short_rows = dense_row_bytes(fa.kind, fa.kpad) <= 2048;
-  const int64_t stride = short_rows ? 64 : rtiles >= 4 * 128 ? 128 : rtiles >= 4 * 64 ? 64 : 32;
-  const int Lp = short_rows      ? fused_list_len(std::max((L + 3) / 4, 8))
-                 : stride == 128 ? fused_list_len((L + 3) / 4)
-                 : stride == 64  ? fused_list_len((L + 1) / 2)
-                                 : L;
+  // That rule is the default; searches of a class large enough to time
+  // choose among the candidates by measured time (sample_plan).
+  const int64_t row_bytes = dense_row_bytes(fa.kind, fa.kpad);
+  const bool short_rows = row_bytes <= 2048;
+  const int def = short_rows ? 3 : rtiles >= 4 * 128 ? 2 : rtiles >= 4 * 64 ? 1 : 0;
+  bool allowed[kSampleCandidates];
+  for (int c = 0; c < kSampleCandidates; ++c)
+    allowed[c] = c == def || rtiles >= 4 * (int64_t)sample_candidate(c, L).stride;
+  // the symmetric pass scores each unordered pair once
+  const bool tri_pass = fa.self_mode && !fa.qlist && fa.nqry == fa.nref;
+  const double pairs = (double)nq * (double)fa.nref * (tri_pass ? 0.5 : 1.0);
+  SampleKey skey;
+  skey.method = fa.method;
+  skey.kind = fa.kind;
+  skey.density = fa.ref_density_bucket;
+  skey.row_bytes = (int)std::ceil(std::log2((double)std::max<int64_t>(row_bytes, 1)));
+  // list lengths move with the re-rank margin; the candidates' ranks scale
+  // with L, so short and long lists form the two classes
+  skey.L = L <= 16 ? 16 : 64;
+  skey.tri = tri_pass ? 1 : 0;
+  bool record = false;
+  // cross pairs sharing one pass keep the rule (both directions' buffers
+  // are sized from it)
+  const int cand = sample_plan(skey, allowed, def, ctx->sample_tuning != 0 && xm.phase == 0,
+                               pairs >= (double)((int64_t)1 << 28), &record);
+  sample_note(skey, def, cand);
+  const SampleChoice sc = sample_candidate(cand, L);
+  const int64_t stride = sc.stride;
+  const int Lp = sc.rank;
+  ctx->stats.sample_stride = stride;
+  ctx->stats.sample_rank = Lp;
+  cb->record = record;
+  cb->key = skey;
+  cb->cand = cand;
+  cb->pairs = pairs;
+  cb->allocs = g_device_allocs.load(std::memory_order_relaxed);
+  cb->t_start = std::chrono::steady_clock::now();
   // room for three times the expected candidates (the count above the
   // Lp-th of a sample varies by about 1/sqrt(Lp))
   int cap = (int)std::min<int64_t>(4096, 3 * Lp * stride + 256);
@@ -699,6 +742,11 @@ int run_knn(sd_context* ctx, sd_matrix* X, sd_matrix* Y, const sd_knn_params* p,
     fa.ref_deg = ref_deg;
     fa.self_mode = plan.selfmode;
     fa.include_self = p->include_self;
+    {
+      const double cells = (double)std::max<int64_t>(ref->nrow, 1) * (double)nref;
+      const double frac = ref->nnz > 0 ? (double)ref->nnz / cells : 0.0;
+      fa.ref_density_bucket = frac > 0 ? (int)std::min(60.0, std::round(-2.0 * std::log2(frac))) : 60;
+    }
     if (fused_ok_euclid) {
       fa.ref_aux = ref->sqf.as<float>();
       fa.qry_aux = X->sqf.as<float>();
@@ -940,6 +988,15 @@ int run_knn(sd_context* ctx, sd_matrix* X, sd_matrix* Y, const sd_knn_params* p,
         if (nfail > 0)
           SD_TRY(exact_knn_columns(ctx, ea, fail_ids, nfail, nq_total, k, plan.dec, 0, out_i,
                                    out_v));
+      }
+      if (cb.record && g_device_allocs.load(std::memory_order_relaxed) == cb.allocs) {
+        // the whole search (pre-pass, filtered pass, selection, re-rank and
+        // escalations) per scored pair; a search that had to allocate device
+        // memory is not timed (the next one of its class retries)
+        SD_HIP(hipStreamSynchronize(ctx->stream));
+        const double sec =
+            std::chrono::duration<double>(std::chrono::steady_clock::now() - cb.t_start).count();
+        sample_record(cb.key, cb.cand, L, sec / cb.pairs);
       }
     }
   } else {

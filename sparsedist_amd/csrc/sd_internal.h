@@ -46,6 +46,10 @@ void clear_error();
 // memory comes from the device's stream-ordered pool on that stream
 // (hipMallocAsync / hipFreeAsync), so growing a buffer never synchronises
 // the device the way hipFree does; otherwise hipMalloc / hipFree.
+// Device allocations made by DevBuf so far (a timed search that allocated
+// is not a fair sample of its steady-state time).
+extern std::atomic<int64_t> g_device_allocs;
+
 struct DevBuf {
   void* ptr = nullptr;
   size_t bytes = 0;
@@ -100,6 +104,35 @@ struct TuneRates {
   double exact_fixed_s = 0.1e-3;
   int calibrated = 0;
 };
+
+// The filtered search's threshold sample (sd_tune.cpp): every stride-th
+// reference tile of 128 columns, each query's rank-Lp key there.  The
+// candidates trade pre-pass work (1/stride of the pairs) against the
+// candidates the filtered pass appends (about stride x Lp per query).
+constexpr int kSampleCandidates = 4;
+struct SampleChoice {
+  int stride = 32;
+  int rank = 0;  // Lp
+};
+// Candidate c for the kept list length L: (32, L), (64, ~L/2), (128, ~L/4),
+// (64, ~L/4 but at least 8).
+SampleChoice sample_candidate(int c, int L);
+// Searches alike in method, layout, reference density, row bytes, list
+// length and symmetry share one tuning entry.
+struct SampleKey {
+  int method = 0, kind = 0, density = 0, row_bytes = 0, L = 0, tri = 0;
+  bool operator<(const SampleKey& o) const;
+};
+// The candidate to run: the first allowed one not yet timed (when the
+// search is large enough to time, *record is then set), else the fastest
+// timed one; def when tuning is off or nothing is known.
+int sample_plan(const SampleKey& key, const bool allowed[kSampleCandidates], int def, bool tuning,
+                bool timeable, bool* record);
+// Notes, for the listing, the rule's candidate and the one a search ran.
+void sample_note(const SampleKey& key, int rule, int ran);
+// Records a timed search (list length L): seconds per scored pair, from its
+// pre-pass to its last re-rank or exact fallback.
+void sample_record(const SampleKey& key, int cand, int L, double sec_per_pair);
 }  // namespace sd
 
 namespace sd {
@@ -159,6 +192,9 @@ struct sd_context {
   // counts the self-searches, 0 while none is kept
   sd::DevBuf knn_keep;
   int64_t knn_serial = 0, knn_calls = 0, keep_n = 0, keep_k = 0;
+  // 1: the filtered search's threshold sample is tuned online per search
+  // class (sd_tune.cpp); 0: the fixed rule
+  int sample_tuning = 1;
   // device bytes one fused search may hold in candidate buffers and lists
   // before its queries are cut into blocks (0: half the free memory)
   int64_t buffer_budget = 0;
@@ -296,6 +332,7 @@ struct FusedArgs {
   const uint8_t* ref_deg = nullptr;
   int self_mode = 0, include_self = 0;
   int64_t self_off = 0;             // self mode: query j is reference column j + self_off
+  int ref_density_bucket = 0;       // round(-2 log2(stored fraction of the reference))
   int L = 0;                        // list length kept per query
   int splits = 1;
   int tile_stride = 1;              // list mode: only every tile_stride-th reference tile

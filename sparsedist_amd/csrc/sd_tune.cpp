@@ -17,11 +17,112 @@
 #include <algorithm>
 #include <chrono>
 #include <cstdint>
+#include <map>
+#include <mutex>
 #include <vector>
 
 #include "sd_internal.h"
 
 namespace sd {
+
+// ---- threshold-sample tuner ------------------------------------------------
+// The filtered search's sample (every stride-th reference tile at rank Lp)
+// costs the pre-pass 1/stride of the pairs and leaves about stride x Lp
+// candidates per query for the filtered pass's epilogue and the selection.
+// Which balance is fastest depends on the row length (a long K amortises the
+// epilogue), the density of the layout's values, the list length and the
+// symmetry of the pass, so each class of searches times its candidates: the
+// first searches of a class large enough to time (at least 2^28 scored
+// pairs) run the candidates not yet timed, one each (a search that had to
+// allocate device memory is not counted); later ones run the fastest -- the
+// fixed rule's choice unless another is at least 3% faster -- and keep
+// refining its time.  Every candidate returns the same
+// results, so exploring changes only the time.  The entries are process-wide
+// (all contexts share what they measured) and listed by sd_tune_samples.
+SampleChoice sample_candidate(int c, int L) {
+  SampleChoice s;
+  switch (c) {
+    case 0: s = {32, L}; break;
+    case 1: s = {64, fused_list_len((L + 1) / 2)}; break;
+    case 2: s = {128, fused_list_len((L + 3) / 4)}; break;
+    default: s = {64, fused_list_len(std::max((L + 3) / 4, 8))}; break;
+  }
+  return s;
+}
+
+bool SampleKey::operator<(const SampleKey& o) const {
+  const int a[6] = {method, kind, density, row_bytes, L, tri};
+  const int b[6] = {o.method, o.kind, o.density, o.row_bytes, o.L, o.tri};
+  return std::lexicographical_compare(a, a + 6, b, b + 6);
+}
+
+namespace {
+struct SampleEntry {
+  double sec_per_pair[kSampleCandidates] = {0, 0, 0, 0};
+  int64_t runs[kSampleCandidates] = {0, 0, 0, 0};
+  int last_L = 0;     // list length of the last timed search (for the listing)
+  int last_rule = -1; // the fixed rule's candidate and the one run, last search
+  int last_run = -1;
+};
+std::mutex g_sample_mu;
+std::map<SampleKey, SampleEntry>& sample_table() {
+  static std::map<SampleKey, SampleEntry> t;
+  return t;
+}
+}  // namespace
+
+int sample_plan(const SampleKey& key, const bool allowed[kSampleCandidates], int def, bool tuning,
+                bool timeable, bool* record) {
+  *record = false;
+  if (!tuning) return def;
+  std::lock_guard<std::mutex> lk(g_sample_mu);
+  auto it = sample_table().find(key);
+  const SampleEntry* e = it == sample_table().end() ? nullptr : &it->second;
+  if (timeable) {
+    // untimed candidates first, the default among them first
+    if (allowed[def] && (!e || e->runs[def] == 0)) {
+      *record = true;
+      return def;
+    }
+    for (int c = 0; c < kSampleCandidates; ++c)
+      if (allowed[c] && (!e || e->runs[c] == 0)) {
+        *record = true;
+        return c;
+      }
+  }
+  if (!e) return def;
+  int best = -1;
+  for (int c = 0; c < kSampleCandidates; ++c)
+    if (allowed[c] && e->runs[c] > 0 && (best < 0 || e->sec_per_pair[c] < e->sec_per_pair[best]))
+      best = c;
+  if (best < 0) return def;
+  // the rule's choice stays unless another candidate is clearly faster
+  // (timing noise between single searches is a few percent)
+  if (best != def && allowed[def] && e->runs[def] > 0 &&
+      e->sec_per_pair[best] > 0.97 * e->sec_per_pair[def])
+    best = def;
+  *record = timeable;
+  return best;
+}
+
+void sample_note(const SampleKey& key, int rule, int ran) {
+  std::lock_guard<std::mutex> lk(g_sample_mu);
+  auto it = sample_table().find(key);
+  if (it == sample_table().end()) return;
+  it->second.last_rule = rule;
+  it->second.last_run = ran;
+}
+
+void sample_record(const SampleKey& key, int cand, int L, double sec_per_pair) {
+  if (cand < 0 || cand >= kSampleCandidates || !(sec_per_pair > 0)) return;
+  std::lock_guard<std::mutex> lk(g_sample_mu);
+  SampleEntry& e = sample_table()[key];
+  e.last_L = L;
+  // the first time stands; later ones move it a quarter of the way
+  e.sec_per_pair[cand] = e.runs[cand] == 0 ? sec_per_pair
+                                           : 0.75 * e.sec_per_pair[cand] + 0.25 * sec_per_pair;
+  ++e.runs[cand];
+}
 
 bool tune_prefer_exact(const sd_context* ctx, int kind, int64_t nq, int64_t nref, int64_t kpad,
                        double nnz_q, double nnz_ref, bool symmetric) {
@@ -209,5 +310,43 @@ extern "C" int sd_autotune(sd_context* ctx, int32_t measure, sd_tune_info* out) 
   *out = sd_tune_info{t.bf16_ops,          t.fp4_ops,     t.merge_steps,
                       t.bf16_fixed_s * 1e3, t.fp4_fixed_s * 1e3, t.exact_fixed_s * 1e3,
                       t.calibrated,         0,             t.step_latency_s * 1e6};
+  return SD_OK;
+}
+
+extern "C" int sd_context_set_tuning(sd_context* ctx, int32_t mode) {
+  if (!ctx) return sd::set_error(SD_EINVAL, "NULL context");
+  if (mode != 0 && mode != 1) return sd::set_error(SD_EINVAL, "tuning mode must be 0 or 1");
+  ctx->sample_tuning = mode;
+  return SD_OK;
+}
+
+extern "C" int sd_tune_samples(sd_tune_sample* out, int32_t max, int32_t* n, int32_t reset) {
+  if (!n || (max > 0 && !out)) return sd::set_error(SD_EINVAL, "NULL argument");
+  std::lock_guard<std::mutex> lk(sd::g_sample_mu);
+  auto& t = sd::sample_table();
+  int32_t i = 0;
+  for (const auto& kv : t) {
+    if (i < max) {
+      sd_tune_sample& o = out[i];
+      o.method = kv.first.method;
+      o.layout = kv.first.kind;
+      o.density_bucket = kv.first.density;
+      o.row_bytes_log2 = kv.first.row_bytes;
+      o.list_len = kv.first.L;
+      o.symmetric = kv.first.tri;
+      o.rule = kv.second.last_rule;
+      o.last_run = kv.second.last_run;
+      for (int c = 0; c < sd::kSampleCandidates; ++c) {
+        const sd::SampleChoice sc = sd::sample_candidate(c, kv.second.last_L);
+        o.stride[c] = sc.stride;
+        o.rank[c] = sc.rank;
+        o.ns_per_mpair[c] = kv.second.sec_per_pair[c] * 1e15;
+        o.runs[c] = kv.second.runs[c];
+      }
+    }
+    ++i;
+  }
+  *n = i;
+  if (reset) t.clear();
   return SD_OK;
 }

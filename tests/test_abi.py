@@ -43,7 +43,8 @@ def test_package_exports_reference_api():
 C_SIZES = {"int32_t": 4, "int64_t": 8, "double": 8, "float": 4}
 STRUCTS = {"sd_stats": "Stats", "sd_tune_info": "TuneInfo", "sd_knn_params": "KnnParams",
            "sd_server_params": "ServerParams", "sd_request": "Request",
-           "sd_request_info": "RequestInfo", "sd_server_stats": "ServerStats"}
+           "sd_request_info": "RequestInfo", "sd_server_stats": "ServerStats",
+           "sd_tune_sample": "TuneSample"}
 
 
 def header_structs():
@@ -64,10 +65,14 @@ def header_structs():
                 nm = nm.strip()
                 if nm.startswith("*"):
                     ptr, nm = "*", nm[1:]
+                arr = re.match(r"([a-z_0-9]+)\[(\d+)\]$", nm)  # fixed-size array field
+                count = 1
+                if arr:
+                    nm, count = arr.group(1), int(arr.group(2))
                 size = 8 if ptr else C_SIZES.get(ctype)
                 if size is None:  # a nested struct declared in the same header
                     size = sum(sz for _, sz in out[ctype]) if ctype in out else None
-                fields.append((nm, size))
+                fields.append((nm, None if size is None else size * count))
         out[name] = fields
     return out
 

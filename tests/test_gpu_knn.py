@@ -755,3 +755,50 @@ def test_query_blocks_cross_search(oracle_lib, gpu_ctx):
     qs = np.sort(rng.choice(20000, 120, replace=False))
     oi, od, _ = oracle_lib.knn(X[:, qs], Y, k=8, method="cosine", nthreads=NT)
     same((blk["forward"].idx[qs], blk["forward"].dist[qs]), (oi, od), "blocked cross oracle")
+
+
+def test_sample_tuner_explores_then_keeps_fastest(gpu_ctx):
+    """A class of large searches times each threshold-sample candidate once
+    (65,536-column self-search: every candidate has at least four sampled
+    tiles; a search that allocated device memory is not timed), then runs
+    the fastest -- the fixed rule's choice unless another is 3% faster; the
+    results never change.  With tuning off the fixed rule applies (rows of
+    400 bytes: every 64th tile at rank ~L/4)."""
+    Y = synth.nb_log1p(200, 65536, 0.1, 130)
+    sd.tune_samples(reset=True)
+    strides = []
+
+    def search():
+        gpu_ctx.stats(reset=True)
+        g = sd.sparseKNN(Y, k=10, method="cosine", precision="bf16", margin=16, verbose=False)
+        st = gpu_ctx.stats(reset=True)
+        strides.append((st["sample_stride"], st["sample_rank"]))
+        return g
+
+    try:
+        first = search()
+        for run in range(1, 8):
+            g = search()
+            same((g.idx, g.dist), (first.idx, first.dist), f"exploring run {run}")
+            tab = sd.tune_samples()
+            if tab and min(tab[0]["runs"]) >= 1:
+                break
+        assert len(tab) == 1, tab
+        e = tab[0]
+        assert e["symmetric"] and e["runs"] == [1, 1, 1, 1], e
+        assert set(strides) == set(e["candidates"]), (strides, e)
+        rule = 3  # rows of 400 bytes
+        t = e["ns_per_mpair"]
+        best = min(range(4), key=lambda c: t[c])
+        if best != rule and t[best] > 0.97 * t[rule]:
+            best = rule
+        g = search()
+        same((g.idx, g.dist), (first.idx, first.dist), "tuned run")
+        assert strides[-1] == e["candidates"][best], (strides, e)
+        assert sd.tune_samples()[0]["runs"][best] == 2
+        gpu_ctx.set_tuning(False)
+        search()
+        assert strides[-1] == e["candidates"][rule], strides
+    finally:
+        gpu_ctx.set_tuning(True)
+        sd.tune_samples(reset=True)
