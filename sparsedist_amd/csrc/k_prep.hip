@@ -86,10 +86,11 @@ __device__ inline uint16_t f32_to_bf16_rne(float f) {
   return (uint16_t)(b >> 16);
 }
 
-// Densify column chunks into [ncol][kpad] column-contiguous storage.  One
-// workgroup per (column, K chunk): the chunk is zeroed in LDS, the column's
-// stored entries that fall into it are scattered, and the chunk is written
-// out with 16-byte stores.  ELEM bytes per element; ELEM == 0 is the FP4
+// Densify column chunks into the chunk-major layout of dense_offset (128-byte
+// K chunks, each a block of ncol columns).  One workgroup per (column, K
+// range): the range is zeroed in LDS, the column's stored entries that fall
+// into it are scattered, and it is written out with 16-byte stores, eight
+// lanes per 128-byte chunk.  ELEM bytes per element; ELEM == 0 is the FP4
 // pattern layout: two rows per byte, row 2i in the low nibble, stored
 // entries as e2m1 1.0 (0x2), set with LDS atomics since neighbouring rows
 // share a byte.
@@ -125,8 +126,10 @@ __global__ __launch_bounds__(256) void k_densify(const int64_t* __restrict__ p,
     }
   }
   __syncthreads();
-  uint4* dst = reinterpret_cast<uint4*>(out + (ELEM ? (j * kpad + k0) * ELEM : (j * kpad + k0) / 2));
-  for (int64_t t = threadIdx.x; t < bytes / 16; t += blockDim.x) dst[t] = l4[t];
+  // chunk-major store: each 128-byte chunk of the column to its own block
+  const int64_t kb0 = ELEM ? k0 * ELEM : k0 / 2;  // a multiple of 128
+  for (int64_t t = threadIdx.x; t < bytes / 16; t += blockDim.x)
+    *reinterpret_cast<uint4*>(out + dense_offset(j, kb0 + 16 * t, ncol)) = l4[t];
 }
 
 // Split layout: one workgroup per (column, K chunk) as above; LDS holds the
@@ -162,15 +165,16 @@ __global__ __launch_bounds__(256) void k_densify_split(const int64_t* __restrict
   }
   __syncthreads();
   const int64_t seg = kpad * 2;  // bytes
-  uint8_t* row = out + j * 3 * seg + k0 * 2;
+  const int64_t kb0 = k0 * 2;    // K byte of the chunk in the first segment (a multiple of 128)
   const int64_t n16 = len * 2 / 16;
   const uint4* h4 = reinterpret_cast<const uint4*>(hi);
   const uint4* o4 = reinterpret_cast<const uint4*>(lo);
   for (int64_t t = threadIdx.x; t < n16; t += blockDim.x) {
     const uint4 hv = h4[t];
-    reinterpret_cast<uint4*>(row)[t] = hv;
-    reinterpret_cast<uint4*>(row + seg)[t] = hv;
-    reinterpret_cast<uint4*>(row + 2 * seg)[t] = o4[t];
+    const int64_t kb = kb0 + 16 * t;
+    *reinterpret_cast<uint4*>(out + dense_offset(j, kb, ncol)) = hv;
+    *reinterpret_cast<uint4*>(out + dense_offset(j, kb + seg, ncol)) = hv;
+    *reinterpret_cast<uint4*>(out + dense_offset(j, kb + 2 * seg, ncol)) = o4[t];
   }
 }
 

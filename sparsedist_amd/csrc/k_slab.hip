@@ -179,8 +179,9 @@ struct Entry {
 }  // namespace
 
 struct FusedKernelArgs {
-  const uint8_t* ref_dense;
-  const uint8_t* qry_dense;
+  const uint8_t* ref_dense;   // chunk-major (dense_offset) with ref_ld ...
+  const uint8_t* qry_dense;   // ... and qry_ld columns per chunk block
+  int64_t ref_ld, qry_ld;
   const int32_t* qlist;  // optional: position -> query column
   int64_t nref, nqry, kpad_bytes;
   const int64_t* ref_p;
@@ -384,20 +385,18 @@ __global__ __launch_bounds__(WM * 256, 1) void k_fused(FusedKernelArgs P) {
   // than global_load_lds: the compiler treats an LDS-writing FLAT load as
   // pending on both counters and would then round every LDS wait in the loop
   // down to lgkmcnt(0), serialising the fragment reads with the MFMAs.  A
-  // resource covers one tile (at most 256 rows x kpad_bytes < 2^31 bytes, which
-  // the host checks); lane offsets are 32-bit and rows past the end are
-  // clamped to the last row (they feed only masked or unused lanes).  Rows
-  // are addressed by query position: the host gathers listed queries first.
+  // resource covers one chunk block of a tile (rows 128 bytes apart in the
+  // chunk-major layout); rows past the end are clamped to the last row (they
+  // feed only masked or unused lanes).  Rows are addressed by query
+  // position: the host gathers listed queries first.
   const int64_t b_rows = (P.nqry - q_base < TN) ? P.nqry - q_base : TN;
-  const __amdgpu_buffer_rsrc_t rb = __builtin_amdgcn_make_buffer_rsrc(
-      (void*)(P.qry_dense + q_base * P.kpad_bytes), 0, (int)(b_rows * P.kpad_bytes), kRsrcWord3);
   uint32_t vb[B_INST];
 #pragma unroll
   for (int u = 0; u < B_INST; ++u) {
     const int i = wave * B_INST + u;
     int row = 8 * i + (lane >> 3);
     if (row >= b_rows) row = (int)b_rows - 1;
-    vb[u] = (uint32_t)(row * P.kpad_bytes) + seg_of(i) * 16;
+    vb[u] = (uint32_t)(row * kDenseChunk) + seg_of(i) * 16;
   }
   typedef __attribute__((address_space(3))) void lds_void;
 
@@ -428,28 +427,32 @@ __global__ __launch_bounds__(WM * 256, 1) void k_fused(FusedKernelArgs P) {
         for (int r = 0; r < 16; ++r) acc[b][j][r] = 0;
 
     const int64_t a_rows = (P.nref - r_base < TM) ? P.nref - r_base : TM;
-    const __amdgpu_buffer_rsrc_t ra = __builtin_amdgcn_make_buffer_rsrc(
-        (void*)(P.ref_dense + r_base * P.kpad_bytes), 0, (int)(a_rows * P.kpad_bytes), kRsrcWord3);
     uint32_t va[A_INST];
 #pragma unroll
     for (int u = 0; u < A_INST; ++u) {
       const int i = wave * A_INST + u;
       int row = 8 * i + (lane >> 3);
       if (row >= a_rows) row = (int)a_rows - 1;  // clamped rows are masked by s_rdeg
-      va[u] = (uint32_t)(row * P.kpad_bytes) + seg_of(i) * 16;
+      va[u] = (uint32_t)(row * kDenseChunk) + seg_of(i) * 16;
     }
+    // chunk c of both tiles: one block each of the chunk-major layouts
     auto issue = [&](uint8_t* st, int64_t c) {
-      const int soff = (int)(c * kRowBytes);
-      const int soff_b = QS ? qseg_offset(soff, P.qseg) : soff;
+      const int kb = (int)(c * kRowBytes);
+      const int kb_b = QS ? qseg_offset(kb, P.qseg) : kb;
+      const __amdgpu_buffer_rsrc_t ra = __builtin_amdgcn_make_buffer_rsrc(
+          (void*)(P.ref_dense + dense_offset(r_base, kb, P.ref_ld)), 0,
+          (int)(a_rows * kDenseChunk), kRsrcWord3);
+      const __amdgpu_buffer_rsrc_t rb = __builtin_amdgcn_make_buffer_rsrc(
+          (void*)(P.qry_dense + dense_offset(q_base, kb_b, P.qry_ld)), 0,
+          (int)(b_rows * kDenseChunk), kRsrcWord3);
 #pragma unroll
       for (int u = 0; u < A_INST; ++u)
         __builtin_amdgcn_raw_ptr_buffer_load_lds(
-            ra, (lds_void*)(st + (wave * A_INST + u) * 1024), 16, (int)va[u], soff, 0, 0);
+            ra, (lds_void*)(st + (wave * A_INST + u) * 1024), 16, (int)va[u], 0, 0, 0);
 #pragma unroll
       for (int u = 0; u < B_INST; ++u)
         __builtin_amdgcn_raw_ptr_buffer_load_lds(
-            rb, (lds_void*)(st + A_BYTES + (wave * B_INST + u) * 1024), 16, (int)vb[u], soff_b, 0,
-            0);
+            rb, (lds_void*)(st + A_BYTES + (wave * B_INST + u) * 1024), 16, (int)vb[u], 0, 0, 0);
     };
     const int swz = (n >> 1) & 7;
     // Operand fragments of one MFMA step (16 K bytes per lane): lanes with h
@@ -758,21 +761,19 @@ __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
     }
   }
 
-  // Buffer resources over the two tiles (rows past the end are clamped to the
-  // last row and feed only masked or unused lanes).  Lane l of load i carries
-  // row kRowsPerInst i + l / kSlots in physical slot l % kSlots, which holds
-  // logical segment (l % kSlots) ^ swz(row).
+  // Chunk c of a tile is one block of the chunk-major layout (rows 128 bytes
+  // apart); a buffer resource per chunk and side covers the tile's rows of it
+  // (rows past the end are clamped to the last row and feed only masked or
+  // unused lanes).  Lane l of load i carries row kRowsPerInst i + l / kSlots
+  // in physical slot l % kSlots, which holds logical segment
+  // (l % kSlots) ^ swz(row).
   const int64_t a_rows = (P.nref - r_base < TM) ? P.nref - r_base : TM;
   const int64_t b_rows = (P.nqry - q_base < TN) ? P.nqry - q_base : TN;
-  const __amdgpu_buffer_rsrc_t ra = __builtin_amdgcn_make_buffer_rsrc(
-      (void*)(P.ref_dense + r_base * P.kpad_bytes), 0, (int)(a_rows * P.kpad_bytes), kRsrcWord3);
-  const __amdgpu_buffer_rsrc_t rb = __builtin_amdgcn_make_buffer_rsrc(
-      (void*)(P.qry_dense + q_base * P.kpad_bytes), 0, (int)(b_rows * P.kpad_bytes), kRsrcWord3);
   auto lane_offset = [&](int i, int64_t rows) {
     const int row = Lay::kRowsPerInst * i + lane / Lay::kSlots;
     const int seg = (lane % Lay::kSlots) ^ Lay::swz(row);
     const int r = row < rows ? row : (int)rows - 1;
-    return (uint32_t)(r * P.kpad_bytes) + seg * 16;
+    return (uint32_t)(r * kDenseChunk) + seg * 16;
   };
   uint32_t va[A_INST], vb[B_INST];
 #pragma unroll
@@ -781,16 +782,22 @@ __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
   for (int u = 0; u < B_INST; ++u) vb[u] = lane_offset(wave * B_INST + u, b_rows);
   typedef __attribute__((address_space(3))) void lds_void;
   auto issue = [&](uint8_t* st, int64_t c) {
-    const int soff = (int)(c * RB);
-    const int soff_b = QS ? qseg_offset(soff, P.qseg) : soff;
+    const int kb = (int)(c * RB);
+    const int kb_b = QS ? qseg_offset(kb, P.qseg) : kb;
+    const __amdgpu_buffer_rsrc_t ra = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)(P.ref_dense + dense_offset(r_base, kb, P.ref_ld)), 0, (int)(a_rows * kDenseChunk),
+        kRsrcWord3);
+    const __amdgpu_buffer_rsrc_t rb = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)(P.qry_dense + dense_offset(q_base, kb_b, P.qry_ld)), 0,
+        (int)(b_rows * kDenseChunk), kRsrcWord3);
 #pragma unroll
     for (int u = 0; u < A_INST; ++u)
       __builtin_amdgcn_raw_ptr_buffer_load_lds(
-          ra, (lds_void*)(st + (wave * A_INST + u) * 1024), 16, (int)va[u], soff, 0, 0);
+          ra, (lds_void*)(st + (wave * A_INST + u) * 1024), 16, (int)va[u], 0, 0, 0);
 #pragma unroll
     for (int u = 0; u < B_INST; ++u)
       __builtin_amdgcn_raw_ptr_buffer_load_lds(
-          rb, (lds_void*)(st + A_BYTES + (wave * B_INST + u) * 1024), 16, (int)vb[u], soff_b, 0, 0);
+          rb, (lds_void*)(st + A_BYTES + (wave * B_INST + u) * 1024), 16, (int)vb[u], 0, 0, 0);
   };
 
   typename Op::Acc acc[4][2];
@@ -1902,35 +1909,39 @@ static int launch_fused_L(sd_context* ctx, FusedKernelArgs& P, int L) {
   }
 }
 
-// Copy the dense rows of listed query columns into a position-indexed block,
-// so the slab kernel's 32-bit buffer offsets stay within one compact tile.
-__global__ void k_gather_rows(const uint4* __restrict__ src, const int32_t* __restrict__ qlist,
-                              int64_t n, int64_t row_vecs, uint4* __restrict__ dst) {
-  const int64_t row = blockIdx.y + (int64_t)blockIdx.z * 65535;
-  if (row >= n) return;
-  const uint4* s = src + (int64_t)qlist[row] * row_vecs;
-  uint4* d = dst + row * row_vecs;
-  for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < row_vecs;
-       v += (int64_t)gridDim.x * blockDim.x)
-    d[v] = s[v];
+// Copy the densified columns of listed queries into a compact chunk-major
+// block of n columns, position b holding column qlist[b] of the source (ld
+// columns per chunk block); 8 lanes move one 128-byte chunk.
+__global__ void k_gather_rows(const uint8_t* __restrict__ src, int64_t src_ld,
+                              const int32_t* __restrict__ qlist, int64_t n, int64_t nchunks,
+                              uint8_t* __restrict__ dst) {
+  const int64_t total = n * nchunks * 8;
+  for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < total;
+       v += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t part = v & 7, cell = v >> 3;
+    const int64_t b = cell % n, c = cell / n;
+    const int64_t kb = c * kDenseChunk + part * 16;
+    *reinterpret_cast<uint4*>(dst + dense_offset(b, kb, n)) =
+        *reinterpret_cast<const uint4*>(src + dense_offset(qlist[b], kb, src_ld));
+  }
 }
 
-// Gathers a.qlist's query rows when a list is given; returns the rows to read.
+// Gathers a.qlist's query columns when a list is given; returns the layout
+// to read and its columns per chunk block.
 static int query_rows(sd_context* ctx, const FusedArgs& a, int64_t kpad_bytes,
-                      const uint8_t** rows) {
+                      const uint8_t** rows, int64_t* ld) {
   *rows = static_cast<const uint8_t*>(a.qry_dense);
+  *ld = a.qry_ld;
   if (!a.qlist || a.nqry <= 0) return SD_OK;
   DevBuf& g = ctx->scratch[16];
   SD_TRY(g.ensure((size_t)a.nqry * (size_t)kpad_bytes + 64));
-  const int64_t vecs = kpad_bytes / 16;
-  const unsigned gx = (unsigned)std::min<int64_t>((vecs + 255) / 256, 64);
-  const unsigned gy = (unsigned)std::min<int64_t>(a.nqry, 65535);
-  const unsigned gz = (unsigned)((a.nqry + 65534) / 65535);
-  hipLaunchKernelGGL(k_gather_rows, dim3(gx, gy, gz), dim3(256), 0, ctx->stream,
-                     static_cast<const uint4*>(a.qry_dense), a.qlist, a.nqry, vecs,
-                     g.as<uint4>());
+  const int64_t nchunks = kpad_bytes / kDenseChunk;
+  hipLaunchKernelGGL(k_gather_rows, dim3(stride_grid(a.nqry * nchunks * 8)), dim3(256), 0,
+                     ctx->stream, static_cast<const uint8_t*>(a.qry_dense), a.qry_ld, a.qlist,
+                     a.nqry, nchunks, g.as<uint8_t>());
   SD_HIP(hipGetLastError());
   *rows = g.as<uint8_t>();
+  *ld = a.nqry;
   return SD_OK;
 }
 
@@ -1956,7 +1967,8 @@ int fused_slab_topl(sd_context* ctx, const FusedArgs& a, double* cand_key, int32
   if (!fused_fits(P.kpad_bytes))
     return set_error(SD_EINVAL, "internal: %lld padded rows exceed the slab kernel's tile range",
                      (long long)a.kpad);
-  SD_TRY(query_rows(ctx, a, P.kpad_bytes, &P.qry_dense));
+  SD_TRY(query_rows(ctx, a, P.kpad_bytes, &P.qry_dense, &P.qry_ld));
+  P.ref_ld = a.ref_ld;
   P.ref_p = a.ref_p;
   P.qry_p = a.qry_p;
   P.ref_aux = a.ref_aux;
@@ -2022,7 +2034,8 @@ int fused_filter_pass(sd_context* ctx, const FusedArgs& a, const double* thr, in
   if (!fused_fits(P.kpad_bytes))
     return set_error(SD_EINVAL, "internal: %lld padded rows exceed the slab kernel's tile range",
                      (long long)a.kpad);
-  SD_TRY(query_rows(ctx, a, P.kpad_bytes, &P.qry_dense));
+  SD_TRY(query_rows(ctx, a, P.kpad_bytes, &P.qry_dense, &P.qry_ld));
+  P.ref_ld = a.ref_ld;
   P.ref_p = a.ref_p;
   P.qry_p = a.qry_p;
   P.ref_aux = a.ref_aux;

@@ -737,6 +737,8 @@ int run_knn(sd_context* ctx, sd_matrix* X, sd_matrix* Y, const sd_knn_params* p,
     fa.kpad = kpad;
     fa.ref_dense = ref->dense[kind].ptr;
     fa.qry_dense = X->dense[kind].ptr;
+    fa.ref_ld = ref->ncol;
+    fa.qry_ld = X->ncol;
     fa.ref_p = ref->p.as<int64_t>();
     fa.qry_p = X->p.as<int64_t>();
     fa.ref_deg = ref_deg;
@@ -782,7 +784,6 @@ int run_knn(sd_context* ctx, sd_matrix* X, sd_matrix* Y, const sd_knn_params* p,
     const FusedArgs fa_all = fa;
     const ExactArgs ea_all = ea;
     const int64_t nq_total = nq;
-    const size_t qrow = (size_t)dense_row_bytes(kind, kpad);
     for (int64_t q0 = 0; q0 < nq_total; q0 += qb) {
       if (q0 > 0) SD_TRY(check_cancel(ctx));
       // block views: query columns [q0, q0 + nq) of X; results at column q0
@@ -791,7 +792,7 @@ int run_knn(sd_context* ctx, sd_matrix* X, sd_matrix* Y, const sd_knn_params* p,
       ExactArgs ea = ea_all;
       if (q0 > 0 || nq < nq_total) {
         fa.nqry = nq;
-        fa.qry_dense = (const char*)fa_all.qry_dense + (size_t)q0 * qrow;
+        fa.qry_dense = (const char*)fa_all.qry_dense + q0 * kDenseChunk;  // same qry_ld
         fa.qry_p = fa_all.qry_p + q0;
         if (fa_all.qry_aux) fa.qry_aux = fa_all.qry_aux + q0;
         if (fa_all.qry_aux2) fa.qry_aux2 = fa_all.qry_aux2 + q0;

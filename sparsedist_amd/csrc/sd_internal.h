@@ -312,8 +312,9 @@ struct FusedArgs {
   int prec = SD_PREC_BF16;
   int kind = kDenseBf16Unit;    // densified layout of ref_dense / qry_dense
   int64_t nref = 0, nqry = 0, kpad = 0;
-  const void* ref_dense = nullptr;
+  const void* ref_dense = nullptr;   // chunk-major layouts (dense_offset) ...
   const void* qry_dense = nullptr;
+  int64_t ref_ld = 0, qry_ld = 0;   // ... of ref_ld / qry_ld columns per chunk block
   const int32_t* qlist = nullptr;   // optional: queries are these columns (positions 0..nqry)
   const int64_t* ref_p = nullptr;   // column pointers (nnz per column)
   const int64_t* qry_p = nullptr;
@@ -363,8 +364,19 @@ struct RerankArgs {
 int rerank_map_words(int64_t nrow);
 // Densified columns are padded with zero rows so that every slab kernel's
 // K loop runs whole groups of chunks: a padded column is a multiple of 256
-// bytes (the filtered pass streams 64-byte chunks in groups of four) and of
-// 256 rows.  The FP4 pattern layout packs two rows per byte.
+// bytes (the slab kernels stream 128-byte chunks in pairs) and of 256 rows.
+// The FP4 pattern layout packs two rows per byte.
+//
+// The densified matrix is stored chunk-major: K byte b of column j of a
+// matrix with ld columns sits at ((b / 128) ld + j) 128 + b % 128, so the
+// 128-byte chunk c of a tile of consecutive columns is one contiguous block
+// (32 KB for 256 columns).  A slab kernel streams a tile's chunks in order:
+// each request then reads whole DRAM pages instead of 128 bytes at a stride
+// of the padded column.
+constexpr int64_t kDenseChunk = 128;
+__host__ __device__ inline int64_t dense_offset(int64_t col, int64_t kbyte, int64_t ld) {
+  return ((kbyte >> 7) * ld + col) * kDenseChunk + (kbyte & (kDenseChunk - 1));
+}
 inline int64_t dense_row_align(int kind) { return kind == kDenseF4Pattern ? 512 : 256; }
 inline int64_t dense_row_bytes(int kind, int64_t kpad) {
   return kind == kDenseF4Pattern                             ? kpad / 2
