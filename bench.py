@@ -38,26 +38,29 @@ SERVE_RATE = 200.0        # requests per second (Poisson)
 SERVE_MAX_COLS = 50_000   # request sizes log-uniform on [1, 50000] columns
 
 
-def serve_arrivals(window_s, seed, pool_cols):
-    """The request stream: Poisson arrivals, log-uniform sizes, method uniform
-    over {cosine, binary}, k uniform over {10, 20, 30}, SNN on half."""
+def serve_arrivals(window_s, seed, pool_cols, max_cols=SERVE_MAX_COLS):
+    """The request stream: Poisson arrivals, log-uniform sizes on [1, max_cols],
+    method uniform over {cosine, binary}, k uniform over {10, 20, 30}, SNN on
+    half."""
     rng = np.random.default_rng(seed)
     t, out = 0.0, []
     while True:
         t += rng.exponential(1.0 / SERVE_RATE)
         if t >= window_s:
             return out
-        n = int(np.exp(rng.uniform(0.0, np.log(SERVE_MAX_COLS + 1))))
-        n = max(1, min(n, SERVE_MAX_COLS, pool_cols))
+        n = int(np.exp(rng.uniform(0.0, np.log(max_cols + 1))))
+        n = max(1, min(n, max_cols, pool_cols))
         out.append(dict(t=t, n=n, method=("cosine", "binary")[int(rng.integers(2))],
                         k=int((10, 20, 30)[int(rng.integers(3))]), snn=bool(rng.random() < 0.5),
                         start=int(rng.integers(0, pool_cols - n + 1))))
 
 
-def serve_run(sd, dY, pool, arrivals, precision, slots=4, chunk_cols=4096, max_pending=16):
+def serve_run(sd, dY, pool, arrivals, precision, slots=4, chunk_cols=4096, max_pending=16,
+              graphs=False):
     """Replays the request stream against a server over dY; each request is a
     contiguous slice of the query pool.  Returns latency and throughput."""
-    srv = sd.Server(dY, slots=slots, chunk_cols=chunk_cols, max_pending=max_pending)
+    srv = sd.Server(dY, slots=slots, chunk_cols=chunk_cols, max_pending=max_pending,
+                    graphs=graphs)
     try:
         # build Y's derived layouts for both methods before the clock starts
         for m in ("binary", "cosine"):
@@ -90,6 +93,7 @@ def serve_run(sd, dY, pool, arrivals, precision, slots=4, chunk_cols=4096, max_p
             infos.append((info, a))
             srv.release(tk)
         t_end = time.perf_counter() - t0
+        gst = srv.stats()
     finally:
         srv.close()
     lat = np.array([i["total_ms"] for i, _ in infos])
@@ -100,7 +104,9 @@ def serve_run(sd, dY, pool, arrivals, precision, slots=4, chunk_cols=4096, max_p
                 p99_ms=float(np.percentile(lat, 99)) if lat.size else None,
                 p99_small_ms=float(np.percentile(small, 99)) if small.size else None,
                 served_cols_per_s=cols / t_end, wall_s=t_end,
-                snn_requests=sum(1 for _, a in infos if a["snn"]))
+                snn_requests=sum(1 for _, a in infos if a["snn"]), graphs=graphs,
+                graph_items=gst["graph_items"], graph_captures=gst["graph_captures"],
+                graph_fallbacks=gst["graph_fallbacks"], packed_items=gst["packed_items"])
 
 
 def main():
@@ -125,6 +131,11 @@ def main():
                          "take far longer to drain)")
     ap.add_argument("--serve", action="store_true",
                     help="serving benchmark only, on the 20000 x 500000 resident reference")
+    ap.add_argument("--serve-ref", default="20000x500000",
+                    help="with --serve: rows x columns of the resident reference")
+    ap.add_argument("--serve-small", action="store_true",
+                    help="with --serve: a stream of small requests (1-256 columns) served with "
+                         "and without the captured graphs of small work items")
     ap.add_argument("--sweep", action="store_true",
                     help="path-chooser sweep: densified vs merge walk vs auto over density "
                          "and batch size (20000-row references of 20000 columns)")
@@ -470,13 +481,30 @@ def cpu_baseline(args, O, X, last, nt):
 def serve_main(args, sd, synth, workers):
     """Serving on the 20000 x 500000 reference (density 0.05, NB/log1p)."""
     t0 = time.time()
-    Y = synth.nb_log1p_parallel(20_000, 500_000, 0.05, 50, workers)
-    pool = synth.nb_log1p_parallel(20_000, SERVE_MAX_COLS, 0.05, 51, workers)
+    nr, nc = (int(v) for v in args.serve_ref.lower().split("x"))
+    Y = synth.nb_log1p_parallel(nr, nc, 0.05, 50, workers)
+    pool = synth.nb_log1p_parallel(nr, SERVE_MAX_COLS, 0.05, 51, workers)
     log(f"generated Y {Y.shape} nnz={Y.nnz} and the query pool in {time.time() - t0:.1f}s")
     ctx = sd.default_context()
     dY = sd.DeviceMatrix(Y, ctx)
     del Y
     window = args.serve_window if args.serve_window > 0 else 10.0
+    if args.serve_small:
+        # small requests only: what the captured graphs of small items change
+        arr = serve_arrivals(window, 53, pool.shape[1], max_cols=256)
+        runs = [serve_run(sd, dY, pool, arr, "auto", graphs=True),
+                serve_run(sd, dY, pool, arr, "auto", graphs=False)]
+        for r in runs:
+            log(f"small requests, graphs {r['graphs']}: served {r['served']}/{r['offered']} p50 "
+                f"{r['p50_ms']:.1f} ms p99 {r['p99_ms']:.1f} ms, graph items {r['graph_items']}, "
+                f"fallbacks {r['graph_fallbacks']}")
+        print(json.dumps({"metric": "serving p50/p99 latency (ms), small requests",
+                          "config": {"workload": f"Y {nr} x {nc} d=0.05 resident; Poisson "
+                                     f"{SERVE_RATE:.0f} req/s for {window:.0f} s; sizes "
+                                     "log-uniform [1, 256]; cosine/binary; k in {10, 20, 30}; "
+                                     "SNN on half", "slots": 4, "chunk_cols": 4096},
+                          "runs": runs}), flush=True)
+        return
     arr = serve_arrivals(window, 52, pool.shape[1])
     runs = [serve_run(sd, dY, pool, arr, "auto")]
     if not args.no_plain:

@@ -358,7 +358,16 @@ int sd_graph_free(sd_graph* g);
  *    admitted requests, always from the request with the fewest unscheduled
  *    columns, so small requests overtake large ones and the work of
  *    different requests overlaps on the device.
- *  - Results do not depend on chunking or on the other requests in flight.
+ *  - Graph items (params.graphs = 1): a work item of at most 256 query columns is padded to 256
+ *    columns with copies of one of its non-empty columns, loaded into query
+ *    buffers the slot sized once, and answered by one replay of the launch
+ *    sequence (query preparation, threshold pre-pass, filtered pass,
+ *    selection, re-rank with the widest list and its certificate) captured
+ *    into a HIP graph the first time the search's parameters occurred; the
+ *    overflow and certificate counters come back with the results, and an
+ *    item that needs a second pass runs as an ordinary search instead.
+ *  - Results do not depend on chunking, graphs or the other requests in
+ *    flight.
  * All functions may be called from any host thread. */
 typedef struct sd_server sd_server;
 
@@ -369,7 +378,11 @@ typedef struct {
   int64_t max_pending;   /* requests allowed to wait for admission (0 = unlimited) */
   int32_t paused;        /* 1: workers take no work until sd_server_resume, so a batch
                             submitted meanwhile is scheduled as a whole */
-  int32_t reserved;
+  int32_t graphs;        /* 1: work items of at most 256 columns (cosine or binary,
+                            precision auto or bf16) replay a HIP graph captured per search
+                            parameters (see below); 0 (default): every item runs as an
+                            ordinary search, which measured faster on MI355X at every
+                            reference size tried */
 } sd_server_params;
 
 typedef struct {
@@ -403,6 +416,11 @@ typedef struct {
   int64_t query_cols;       /* query columns of completed requests */
   int64_t packed_items;     /* work items that carried columns of several requests */
   int64_t waiting_callers;  /* callers blocked in sd_server_wait right now */
+  int64_t graph_items;      /* work items answered by a graph replay */
+  int64_t graph_captures;   /* graphs captured (the first item of its parameters, or after
+                               a buffer the graph reads moved) */
+  int64_t graph_fallbacks;  /* replays whose overflow or certificate counters sent the item
+                               to an ordinary search */
 } sd_server_stats;
 
 /* Y must stay alive until sd_server_destroy; ctx gives the device. */

@@ -80,7 +80,7 @@ class TuneInfo(C.Structure):
 
 class ServerParams(C.Structure):
     _fields_ = [("slots", C.c_int32), ("chunk_cols", C.c_int32), ("budget_bytes", C.c_int64),
-                ("max_pending", C.c_int64), ("paused", C.c_int32), ("reserved", C.c_int32)]
+                ("max_pending", C.c_int64), ("paused", C.c_int32), ("graphs", C.c_int32)]
 
 
 class Request(C.Structure):
@@ -99,7 +99,9 @@ class ServerStats(C.Structure):
     _fields_ = [("submitted", C.c_int64), ("admitted", C.c_int64), ("completed", C.c_int64),
                 ("refused", C.c_int64), ("pending", C.c_int64), ("active", C.c_int64),
                 ("reserved_bytes", C.c_int64), ("query_cols", C.c_int64),
-                ("packed_items", C.c_int64), ("waiting_callers", C.c_int64)]
+                ("packed_items", C.c_int64), ("waiting_callers", C.c_int64),
+                ("graph_items", C.c_int64), ("graph_captures", C.c_int64),
+                ("graph_fallbacks", C.c_int64)]
 
 
 _lib = None

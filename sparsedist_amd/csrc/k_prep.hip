@@ -197,6 +197,24 @@ __global__ void k_to_f32(const double* __restrict__ in, int64_t n, float* __rest
 
 }  // namespace
 
+__global__ void k_fill_i32(int32_t* __restrict__ p, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 0;
+}
+
+int zero_i32(sd_context* ctx, int32_t* p, int64_t n) {
+  if (n <= 0) return SD_OK;
+  hipLaunchKernelGGL(k_fill_i32, dim3(stride_grid(n)), dim3(256), 0, ctx->stream, p, n);
+  SD_HIP(hipGetLastError());
+  return SD_OK;
+}
+
+void matrix_forget_derived(sd_matrix* m) {
+  m->unit_ready = m->centre_ready = m->sq_ready = false;
+  for (int t = 0; t < kDenseKinds; ++t) m->dense_kpad[t] = 0;
+}
+
 int prep_unit(sd_context* ctx, sd_matrix* m) {
   if (m->unit_ready) return SD_OK;
   SD_TRY(m->unit.ensure(sizeof(double) * (size_t)(m->nnz > 0 ? m->nnz : 1)));
@@ -211,7 +229,7 @@ int prep_unit(sd_context* ctx, sd_matrix* m) {
                      m->x.as<double>(), m->ncol, m->scale.as<double>(), sq.as<double>(),
                      m->unit.as<double>());
   SD_HIP(hipGetLastError());
-  SD_HIP(hipStreamSynchronize(ctx->stream));
+  SD_TRY(sync_stream(ctx));
   m->unit_ready = true;
   return SD_OK;
 }
@@ -229,7 +247,7 @@ int prep_centre(sd_context* ctx, sd_matrix* m) {
                      m->mean.as<double>(), m->css.as<double>(), m->ncol, m->nrow,
                      m->cmf.as<float>(), m->cif.as<float>());
   SD_HIP(hipGetLastError());
-  SD_HIP(hipStreamSynchronize(ctx->stream));
+  SD_TRY(sync_stream(ctx));
   m->centre_ready = true;
   return SD_OK;
 }
@@ -244,7 +262,7 @@ int prep_sq(sd_context* ctx, sd_matrix* m) {
   hipLaunchKernelGGL(k_to_f32, dim3((unsigned)nb), dim3(256), 0, ctx->stream, m->sqn.as<double>(),
                      m->ncol, m->sqf.as<float>());
   SD_HIP(hipGetLastError());
-  SD_HIP(hipStreamSynchronize(ctx->stream));
+  SD_TRY(sync_stream(ctx));
   m->sq_ready = true;
   return SD_OK;
 }
@@ -315,7 +333,7 @@ int prep_dense(sd_context* ctx, sd_matrix* m, int kind, int64_t kpad) {
     }
     SD_HIP(hipGetLastError());
   }
-  SD_HIP(hipStreamSynchronize(ctx->stream));
+  SD_TRY(sync_stream(ctx));
   m->dense_kpad[kind] = kpad;
   return SD_OK;
 }

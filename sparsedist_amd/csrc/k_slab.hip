@@ -2059,7 +2059,7 @@ int fused_filter_pass(sd_context* ctx, const FusedArgs& a, const double* thr, in
   P.cnt = cnt;
   P.fkey = fkey;
   P.fidx = fidx;
-  SD_HIP(hipMemsetAsync(cnt, 0, sizeof(int32_t) * (size_t)a.nqry, ctx->stream));
+  SD_TRY(zero_i32(ctx, cnt, a.nqry));
   // A self-search over all columns at once scores each unordered pair once:
   // the mirrored value is the same sum of products (on the split layout the
   // query side reads its segments swapped, so hi.lo and lo.hi are added in
@@ -2091,7 +2091,7 @@ int fused_filter_pass(sd_context* ctx, const FusedArgs& a, const double* thr, in
     P.mfkey = a.mirror->fkey;
     P.mfidx = a.mirror->fidx;
     P.mdeg = a.mirror->deg;
-    SD_HIP(hipMemsetAsync(P.mcnt, 0, sizeof(int32_t) * (size_t)a.nref, ctx->stream));
+    SD_TRY(zero_i32(ctx, P.mcnt, a.nref));
     a.mirror->filled = true;
   }
   KernelTimer timer(ctx, &ctx->stats.filter_ms, &ctx->stats.filter_launches);

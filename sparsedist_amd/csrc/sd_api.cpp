@@ -29,6 +29,12 @@ int set_error(int code, const char* fmt, ...) {
 
 void clear_error() { g_err.clear(); }
 
+int sync_stream(sd_context* ctx) {
+  if (ctx->capturing) return SD_OK;
+  SD_HIP(hipStreamSynchronize(ctx->stream));
+  return SD_OK;
+}
+
 std::atomic<int64_t> g_device_allocs{0};
 
 int DevBuf::alloc(size_t n) {
@@ -420,7 +426,7 @@ int fused_candidate_lists(sd_context* ctx, FusedArgs fa, const double** out_key,
   // cross pairs sharing one pass keep the rule (both directions' buffers
   // are sized from it)
   const int cand = sample_plan(skey, allowed, def, ctx->sample_tuning != 0 && xm.phase == 0,
-                               pairs >= (double)((int64_t)1 << 28), &record);
+                               pairs >= (double)((int64_t)1 << 28) && !ctx->capturing, &record);
   sample_note(skey, def, cand);
   const SampleChoice sc = sample_candidate(cand, L);
   const int64_t stride = sc.stride;
@@ -519,7 +525,7 @@ int fused_candidate_lists(sd_context* ctx, FusedArgs fa, const double** out_key,
   SD_TRY(mi.ensure(sizeof(int32_t) * (size_t)nq * (size_t)L + 64));
   SD_TRY(ov.ensure(sizeof(int32_t) * (size_t)(nq + 16)));
   int32_t* over_count = ov.as<int32_t>() + nq;
-  SD_HIP(hipMemsetAsync(over_count, 0, sizeof(int32_t), ctx->stream));
+  SD_TRY(zero_i32(ctx, over_count, 1));
   // A buffer holds exactly the references whose filtered-pass key is at or
   // above the threshold.  With at least L of them, the L-th key is >= thr and
   // every reference outside keys below it, so the buffer's top-L is the
@@ -530,6 +536,17 @@ int fused_candidate_lists(sd_context* ctx, FusedArgs fa, const double** out_key,
   // candidates send the query to a full pass instead.
   SD_TRY(select_lists(ctx, fk.as<double>(), fi.as<int32_t>(), cnt.as<int32_t>(), cap, nq, L,
                       mk.as<double>(), mi.as<int32_t>(), ov.as<int32_t>(), over_count, L));
+  cb->fkey = fk.as<double>();
+  cb->fidx = fi.as<int32_t>();
+  cb->cnt = cnt.as<int32_t>();
+  cb->cap = cap;
+  *out_key = mk.as<double>();
+  *out_idx = mi.as<int32_t>();
+  if (ctx->capturing) {
+    // a captured search leaves the count for its caller to read after replay
+    ctx->cap_over = over_count;
+    return SD_OK;
+  }
   int32_t nover = 0;
   SD_HIP(hipMemcpyAsync(&nover, over_count, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
   SD_HIP(hipStreamSynchronize(ctx->stream));
@@ -547,12 +564,6 @@ int fused_candidate_lists(sd_context* ctx, FusedArgs fa, const double** out_key,
     SD_TRY(scatter_lists(ctx, ok, oi, ov.as<int32_t>(), nover, L, mk.as<double>(),
                          mi.as<int32_t>()));
   }
-  *out_key = mk.as<double>();
-  *out_idx = mi.as<int32_t>();
-  cb->fkey = fk.as<double>();
-  cb->fidx = fi.as<int32_t>();
-  cb->cnt = cnt.as<int32_t>();
-  cb->cap = cap;
   return SD_OK;
 }
 
@@ -621,6 +632,7 @@ bool dot_bound(const sd_matrix* ref, const sd_matrix* X, int kind, DotBound* out
 // the context's, else half the free device memory -- divided by that gives
 // the block, a multiple of the 256-column query tile.
 int64_t fused_query_block(sd_context* ctx, int64_t nq, int L) {
+  if (ctx->capturing) return nq;  // captured searches are a few query tiles
   int64_t budget = ctx->buffer_budget;
   if (budget <= 0) {
     size_t free_b = 0, total_b = 0;
@@ -645,7 +657,7 @@ int run_knn(sd_context* ctx, sd_matrix* X, sd_matrix* Y, const sd_knn_params* p,
   {
     std::lock_guard<std::mutex> lk(ref->prep_mu);
     SD_TRY(prep_method(ctx, ref, method));
-    SD_HIP(hipStreamSynchronize(ctx->stream));
+    SD_TRY(sync_stream(ctx));
   }
   if (!plan.selfmode) SD_TRY(prep_method(ctx, X, method));
   SD_TRY(check_cancel(ctx));
@@ -720,7 +732,7 @@ int run_knn(sd_context* ctx, sd_matrix* X, sd_matrix* Y, const sd_knn_params* p,
       std::lock_guard<std::mutex> lk(ref->prep_mu);
       SD_TRY(prep_dense(ctx, ref, kind, kpad));
       if (fused_ok_euclid) SD_TRY(prep_sq(ctx, ref));
-      SD_HIP(hipStreamSynchronize(ctx->stream));
+      SD_TRY(sync_stream(ctx));
     }
     if (!plan.selfmode) {
       SD_TRY(prep_dense(ctx, X, kind, kpad));
@@ -829,7 +841,7 @@ int run_knn(sd_context* ctx, sd_matrix* X, sd_matrix* Y, const sd_knn_params* p,
         DevBuf& fl2 = ctx->scratch[8];
         SD_TRY(fl.ensure(sizeof(int32_t) * (size_t)(nq + 16)));
         int32_t* fail_count = fl.as<int32_t>() + nq;
-        SD_HIP(hipMemsetAsync(fail_count, 0, sizeof(int32_t), ctx->stream));
+        SD_TRY(zero_i32(ctx, fail_count, 1));
         RerankArgs R;
         R.ex = ea;
         R.cand_key = lkey;
@@ -850,6 +862,12 @@ int run_knn(sd_context* ctx, sd_matrix* X, sd_matrix* Y, const sd_knn_params* p,
         R.out_ld = nq_total;
         R.map_words = rerank_map_words(ref->nrow);
         SD_TRY(rerank(R));
+        if (ctx->capturing) {
+          // failures are the caller's to find after replay: an ordinary
+          // search answers them
+          ctx->cap_fail = fail_count;
+          continue;
+        }
         int32_t nfail = 0;
         SD_HIP(hipMemcpyAsync(&nfail, fail_count, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
         SD_HIP(hipStreamSynchronize(ctx->stream));
@@ -893,7 +911,7 @@ int run_knn(sd_context* ctx, sd_matrix* X, sd_matrix* Y, const sd_knn_params* p,
             DevBuf& bpos = ctx->scratch[20];
             SD_TRY(bc.ensure(64));
             SD_TRY(bpos.ensure(sizeof(int32_t) * (size_t)(nfail + 16)));
-            SD_HIP(hipMemsetAsync(bc.ptr, 0, sizeof(int32_t), ctx->stream));
+            SD_TRY(zero_i32(ctx, bc.as<int32_t>(), 1));
             SD_TRY(select_lists(ctx, cb.fkey, cb.fidx, cb.cnt, cb.cap, nfail, Lmax, mk2.as<double>(),
                                 mi2.as<int32_t>(), bpos.as<int32_t>(), bc.as<int32_t>(), Lmax,
                                 fl.as<int32_t>()));
@@ -964,7 +982,7 @@ int run_knn(sd_context* ctx, sd_matrix* X, sd_matrix* Y, const sd_knn_params* p,
           }
           SD_TRY(fl2.ensure(sizeof(int32_t) * (size_t)(nfail + 16)));
           int32_t* fail_count2 = fl2.as<int32_t>() + nfail;
-          SD_HIP(hipMemsetAsync(fail_count2, 0, sizeof(int32_t), ctx->stream));
+          SD_TRY(zero_i32(ctx, fail_count2, 1));
           RerankArgs R2 = R;
           R2.cand_key = k2;
           R2.cand_idx = i2;
@@ -1002,6 +1020,8 @@ int run_knn(sd_context* ctx, sd_matrix* X, sd_matrix* Y, const sd_knn_params* p,
     }
   } else {
     // exact FP64 path over every query column
+    if (ctx->capturing)
+      return set_error(SD_EUNSUPPORTED, "internal: the exact path is not captured");
     if (ctx->xm.phase == 1) return SD_OK;  // probe of a cross pair's reverse search
     ++ctx->stats.exact_calls;
     DevBuf& qc = ctx->scratch[7];
@@ -1018,10 +1038,12 @@ int run_knn(sd_context* ctx, sd_matrix* X, sd_matrix* Y, const sd_knn_params* p,
   if (p->drop_degenerate)
     SD_TRY(fill_degenerate(ctx, out_idx.as<int32_t>(), out_val.as<double>(),
                            X->deg.as<uint8_t>(), nq, k));
-  SD_HIP(hipMemcpyAsync(h_idx, out_idx.ptr, sizeof(int32_t) * (size_t)nq * (size_t)k,
-                        hipMemcpyDeviceToHost, ctx->stream));
-  SD_HIP(hipMemcpyAsync(h_val, out_val.ptr, sizeof(double) * (size_t)nq * (size_t)k,
-                        hipMemcpyDeviceToHost, ctx->stream));
+  if (h_idx) {
+    SD_HIP(hipMemcpyAsync(h_idx, out_idx.ptr, sizeof(int32_t) * (size_t)nq * (size_t)k,
+                          hipMemcpyDeviceToHost, ctx->stream));
+    SD_HIP(hipMemcpyAsync(h_val, out_val.ptr, sizeof(double) * (size_t)nq * (size_t)k,
+                          hipMemcpyDeviceToHost, ctx->stream));
+  }
   if (keep) {
     // the indices stay on the device for a following SNN call
     ctx->knn_serial = 0;
@@ -1029,7 +1051,7 @@ int run_knn(sd_context* ctx, sd_matrix* X, sd_matrix* Y, const sd_knn_params* p,
     SD_HIP(hipMemcpyAsync(ctx->knn_keep.ptr, out_idx.ptr, sizeof(int32_t) * (size_t)nq * (size_t)k,
                           hipMemcpyDeviceToDevice, ctx->stream));
   }
-  SD_HIP(hipStreamSynchronize(ctx->stream));
+  SD_TRY(sync_stream(ctx));
   if (keep) {
     ctx->knn_serial = ++ctx->knn_calls;
     ctx->keep_n = nq;
@@ -1066,6 +1088,25 @@ struct CancelScope {
 };
 
 }  // namespace
+
+namespace sd {
+
+int knn_device(sd_context* ctx, sd_matrix* X, sd_matrix* Y, const sd_knn_params* p,
+               int32_t* k_eff, int32_t* warn, const int32_t** d_idx, const double** d_val) {
+  KnnPlan plan;
+  int64_t avail = 0;
+  SD_TRY(plan_knn(X, Y, p, &plan, &avail));
+  *warn = plan.warn;
+  *k_eff = plan.k;
+  ctx->cap_over = nullptr;
+  ctx->cap_fail = nullptr;
+  SD_TRY(run_knn(ctx, X, Y, p, plan, nullptr, nullptr));
+  *d_idx = ctx->scratch[1].as<int32_t>();
+  *d_val = ctx->scratch[2].as<double>();
+  return SD_OK;
+}
+
+}  // namespace sd
 
 extern "C" {
 

@@ -198,6 +198,16 @@ struct sd_context {
   // device bytes one fused search may hold in candidate buffers and lists
   // before its queries are cut into blocks (0: half the free memory)
   int64_t buffer_budget = 0;
+  // Capture of a search's launch sequence into a HIP graph (the server's
+  // graph items): while set, a search synchronises nothing, reads nothing
+  // back and takes no path that needs a host decision; it leaves the
+  // filtered pass's overflow count and the certificate's failure count on
+  // the device (cap_over, cap_fail; null when the search has none), which
+  // the caller reads after each replay and answers with an ordinary search
+  // when either is non-zero.
+  bool capturing = false;
+  const int32_t* cap_over = nullptr;
+  const int32_t* cap_fail = nullptr;
 };
 
 struct sd_matrix {
@@ -272,6 +282,23 @@ void matrix_use_pool(sd_matrix* m, hipStream_t stream);
 // when too small; derived layouts of earlier contents are discarded).
 int matrix_load(sd_context* ctx, sd_matrix* m, int64_t nrow, int64_t ncol, const int64_t* colptr,
                 const int32_t* rowind, const double* values);
+
+// hipStreamSynchronize of ctx's stream, skipped while a search is captured
+int sync_stream(sd_context* ctx);
+// n int32 zeros at p on ctx's stream by a kernel (a captured graph gets no
+// memset nodes)
+int zero_i32(sd_context* ctx, int32_t* p, int64_t n);
+// Forget m's derived layouts (unit values, statistics, densified columns)
+// without freeing their buffers, so the next search builds them again in
+// place.
+void matrix_forget_derived(sd_matrix* m);
+// One search of X against Y (null: self-search) whose results stay on the
+// device: nq x k_eff column-major at *d_idx / *d_val (ctx's output buffers,
+// valid until ctx's next search).  With ctx->capturing the launches go into
+// the stream's capture; a search that would need a host decision or an
+// allocation returns SD_EUNSUPPORTED.
+int knn_device(sd_context* ctx, sd_matrix* X, sd_matrix* Y, const sd_knn_params* p,
+               int32_t* k_eff, int32_t* warn, const int32_t** d_idx, const double** d_val);
 
 // ---- prep (k_prep.hip) ------------------------------------------------------
 int prep_flags(sd_context* ctx, sd_matrix* m);                 // deg, nonfinite, negative

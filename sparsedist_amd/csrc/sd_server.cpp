@@ -16,6 +16,7 @@
 #include <map>
 #include <memory>
 #include <thread>
+#include <tuple>
 
 #include "sd_internal.h"
 
@@ -70,6 +71,48 @@ struct Request {
   Clock::time_point t_submit, t_admit, t_done;
 };
 
+// ---- graph items -------------------------------------------------------------
+// A work item of at most kGraphCols query columns (a small request, or small
+// requests packed together) runs as one replay of a HIP graph.  The slot
+// pads the item to exactly kGraphCols columns with copies of one of its
+// non-empty columns (every query's result is independent of the others, so
+// padding changes no answer), loads it into query buffers sized once for
+// the widest item, and replays the launch sequence captured for the
+// search's parameters: query preparation, threshold pre-pass, filtered pass,
+// selection, FP64 re-rank with the widest list and certificate, results.
+// The filtered pass's overflow count and the certificate's failure count
+// come back with the results; when either is non-zero the item runs as an
+// ordinary search instead.  A graph is captured the first time its
+// parameters occur, right after an ordinary run of the same item has grown
+// every buffer it needs (that run answers the item), and again whenever a
+// buffer it reads has moved.
+constexpr int64_t kGraphCols = 256;
+
+struct GraphKey {
+  int32_t k, method, dist, drop_degenerate, precision, margin;
+  bool operator<(const GraphKey& o) const {
+    return std::tie(k, method, dist, drop_degenerate, precision, margin) <
+           std::tie(o.k, o.method, o.dist, o.drop_degenerate, o.precision, o.margin);
+  }
+};
+
+struct GraphEntry {
+  hipGraphExec_t exec = nullptr;
+  std::vector<const void*> ptrs;  // device buffers the launches were captured with
+  const int32_t* over = nullptr;  // counters the replay leaves (null: none)
+  const int32_t* fail = nullptr;
+  const int32_t* out_idx = nullptr;
+  const double* out_val = nullptr;
+  int32_t k_eff = 0, warn = 0;
+  bool broken = false;  // the search cannot be captured: ordinary searches only
+};
+
+struct GraphSlot {
+  sd_context* ctx = nullptr;  // its own context: no other work grows its buffers
+  sd_matrix* X = nullptr;     // the padded query columns
+  std::map<GraphKey, GraphEntry> graphs;
+};
+
 }  // namespace
 
 struct sd_server {
@@ -83,6 +126,8 @@ struct sd_server {
   // per slot: the query matrix of its current item, reloaded in place so that
   // items allocate no device memory once its buffers have grown
   std::vector<sd_matrix*> qmat;
+  // per slot: captured searches of small items (empty with no_graphs)
+  std::vector<GraphSlot> gslot;
   std::vector<std::thread> workers;
   std::mutex mu;
   std::condition_variable cv_work, cv_done, cv_gone;
@@ -189,16 +234,203 @@ bool same_search(const sd_knn_params& a, const sd_knn_params& b) {
          a.margin == b.margin;
 }
 
+// Results of one search over the pieces' concatenated columns (column-major
+// with leading dimension ld, k_eff columns) go to each piece's request.
+void scatter_results(sd_server* s, const std::vector<Seg>& segs, const int32_t* idx,
+                     const double* dist, int64_t ld, int32_t keff, int32_t warn) {
+  // columns [c0, c1) of each request belong to this item alone, so no lock
+  // is needed for the copies; k_eff is a property of (Y, k) and equal for
+  // every item.
+  int64_t off = 0;
+  for (const Seg& g : segs) {
+    const int64_t n = g.c1 - g.c0;
+    for (int32_t t = 0; t < keff; ++t) {
+      std::copy(idx + (size_t)t * ld + off, idx + (size_t)t * ld + off + n,
+                g.r->idx.begin() + (size_t)t * g.r->ncol + g.c0);
+      std::copy(dist + (size_t)t * ld + off, dist + (size_t)t * ld + off + n,
+                g.r->dist.begin() + (size_t)t * g.r->ncol + g.c0);
+    }
+    off += n;
+  }
+  std::lock_guard<std::mutex> lk(s->mu);
+  for (const Seg& g : segs) {
+    g.r->k_eff = keff;
+    g.r->warn = warn;
+  }
+}
+
+bool graph_eligible(const sd_server* s, const sd_knn_params& k, int64_t ncol) {
+  return !s->gslot.empty() && ncol <= kGraphCols &&
+         (k.method == SD_COSINE || k.method == SD_BINARY) &&
+         (k.precision == SD_PREC_AUTO || k.precision == SD_PREC_BF16);
+}
+
+// Device buffers a captured search reads or writes: the graph context's
+// scratch and the query and reference matrices' arrays and layouts.
+std::vector<const void*> graph_ptrs(const sd_context* c, const sd_matrix* X, const sd_matrix* Y) {
+  std::vector<const void*> v;
+  for (const sd::DevBuf& b : c->scratch) v.push_back(b.ptr);
+  for (const sd_matrix* m : {X, Y}) {
+    for (const sd::DevBuf* b : {&m->p, &m->i, &m->x, &m->deg, &m->unit, &m->scale, &m->unit_ssq,
+                                &m->mean, &m->css, &m->cmf, &m->cif, &m->sqn, &m->sqf})
+      v.push_back(b->ptr);
+    for (const sd::DevBuf& b : m->dense) v.push_back(b.ptr);
+  }
+  return v;
+}
+
+// The item from its graph (see GraphSlot).  *done = false leaves the item to
+// an ordinary search: only empty columns, counters that need a second pass,
+// or any failure here (an ordinary search then reports it to the right
+// request).
+int run_graph_item(sd_server* s, int slot, const std::vector<Seg>& segs, bool* done) {
+  *done = false;
+  GraphSlot& gs = s->gslot[(size_t)slot];
+  sd_context* ctx = gs.ctx;
+  const Request* r0 = segs[0].r;
+  // the item's columns, then copies of its first column with a non-zero value
+  std::vector<int64_t> p(1, 0);
+  std::vector<int32_t> ri;
+  std::vector<double> vx;
+  const Request* pad_r = nullptr;
+  int64_t pad_c = -1;
+  for (const Seg& g : segs)
+    for (int64_t c = g.c0; c < g.c1; ++c) {
+      const int64_t b0 = g.r->colptr[c], b1 = g.r->colptr[c + 1];
+      ri.insert(ri.end(), g.r->rowind + b0, g.r->rowind + b1);
+      vx.insert(vx.end(), g.r->values + b0, g.r->values + b1);
+      p.push_back((int64_t)ri.size());
+      for (int64_t e = b0; pad_c < 0 && e < b1; ++e)
+        if (g.r->values[e] != 0.0) {
+          pad_r = g.r;
+          pad_c = c;
+        }
+    }
+  if (pad_c < 0) return SD_OK;
+  const int64_t nc = (int64_t)p.size() - 1;
+  {
+    const int64_t b0 = pad_r->colptr[pad_c], b1 = pad_r->colptr[pad_c + 1];
+    while ((int64_t)p.size() - 1 < kGraphCols) {
+      ri.insert(ri.end(), pad_r->rowind + b0, pad_r->rowind + b1);
+      vx.insert(vx.end(), pad_r->values + b0, pad_r->values + b1);
+      p.push_back((int64_t)ri.size());
+    }
+  }
+  sd_matrix* X = gs.X;
+  sd_matrix* Y = const_cast<sd_matrix*>(s->Y);
+  // query buffers sized once for the widest item, so that loading an item
+  // never moves them
+  const size_t cap_nnz = (size_t)std::max<int64_t>(Y->nrow, 1) * kGraphCols + 1;
+  SD_TRY(X->i.ensure(sizeof(int32_t) * cap_nnz));
+  SD_TRY(X->x.ensure(sizeof(double) * cap_nnz));
+  SD_TRY(X->unit.ensure(sizeof(double) * cap_nnz));
+  SD_TRY(sd::matrix_load(ctx, X, r0->nrow, kGraphCols, p.data(), ri.data(), vx.data()));
+  sd_knn_params prm = r0->knn;
+  prm.include_self = 0;
+  // cosine: the widest candidate list, so that the certificate clears nearly
+  // every query in the first re-rank (results do not depend on the margin)
+  if (prm.method == SD_COSINE && prm.margin <= 0 && prm.k <= 56) prm.margin = 64 - prm.k;
+  const GraphKey key{prm.k, prm.method, prm.dist, prm.drop_degenerate, prm.precision, prm.margin};
+  GraphEntry& e = gs.graphs[key];
+  if (e.broken) return SD_OK;
+  std::vector<int32_t> idx;
+  std::vector<double> dist;
+  auto fetch = [&](const int32_t* di, const double* dv, int32_t keff) -> int {
+    idx.resize((size_t)kGraphCols * (size_t)keff);
+    dist.resize((size_t)kGraphCols * (size_t)keff);
+    SD_HIP(hipMemcpyAsync(idx.data(), di, sizeof(int32_t) * idx.size(), hipMemcpyDeviceToHost,
+                          ctx->stream));
+    SD_HIP(hipMemcpyAsync(dist.data(), dv, sizeof(double) * dist.size(), hipMemcpyDeviceToHost,
+                          ctx->stream));
+    return SD_OK;
+  };
+  if (e.exec && e.ptrs == graph_ptrs(ctx, X, Y)) {
+    SD_HIP(hipGraphLaunch(e.exec, ctx->stream));
+    int32_t cnt[2] = {0, 0};
+    SD_TRY(fetch(e.out_idx, e.out_val, e.k_eff));
+    if (e.over)
+      SD_HIP(hipMemcpyAsync(&cnt[0], e.over, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+    if (e.fail)
+      SD_HIP(hipMemcpyAsync(&cnt[1], e.fail, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+    SD_HIP(hipStreamSynchronize(ctx->stream));
+    if (cnt[0] != 0 || cnt[1] != 0) {
+      std::lock_guard<std::mutex> lk(s->mu);
+      ++s->st.graph_fallbacks;
+      return SD_OK;
+    }
+    scatter_results(s, segs, idx.data(), dist.data(), kGraphCols, e.k_eff, e.warn);
+    std::lock_guard<std::mutex> lk(s->mu);
+    ++s->st.graph_items;
+    *done = true;
+    return SD_OK;
+  }
+  // An ordinary run answers the item and grows every buffer the search
+  // needs; then the same search is captured (capture runs nothing).
+  if (e.exec) {
+    (void)hipGraphExecDestroy(e.exec);
+    e.exec = nullptr;
+  }
+  int32_t keff = 0, warn = 0;
+  const int32_t* di = nullptr;
+  const double* dv = nullptr;
+  SD_TRY(sd::knn_device(ctx, X, Y, &prm, &keff, &warn, &di, &dv));
+  SD_TRY(fetch(di, dv, keff));
+  SD_HIP(hipStreamSynchronize(ctx->stream));
+  scatter_results(s, segs, idx.data(), dist.data(), kGraphCols, keff, warn);
+  *done = true;
+  const std::vector<const void*> before = graph_ptrs(ctx, X, Y);
+  sd::matrix_forget_derived(X);  // the capture records the query preparation too
+  hipGraph_t g = nullptr;
+  int rc = SD_EDEVICE;
+  if (hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+    ctx->capturing = true;
+    try {
+      rc = sd::knn_device(ctx, X, Y, &prm, &keff, &warn, &di, &dv);
+    } catch (...) {
+      rc = SD_EDEVICE;
+    }
+    ctx->capturing = false;
+    if (hipStreamEndCapture(ctx->stream, &g) != hipSuccess && rc == SD_OK) rc = SD_EDEVICE;
+  }
+  if (rc == SD_OK && g && graph_ptrs(ctx, X, Y) == before &&
+      hipGraphInstantiate(&e.exec, g, nullptr, nullptr, 0) == hipSuccess) {
+    e.ptrs = before;
+    e.over = ctx->cap_over;
+    e.fail = ctx->cap_fail;
+    e.out_idx = di;
+    e.out_val = dv;
+    e.k_eff = keff;
+    e.warn = warn;
+    std::lock_guard<std::mutex> lk(s->mu);
+    ++s->st.graph_captures;
+  } else {
+    e.exec = nullptr;
+    e.broken = rc == SD_EUNSUPPORTED;  // a path with host decisions: never captured
+  }
+  if (g) (void)hipGraphDestroy(g);
+  (void)hipGetLastError();
+  sd::clear_error();
+  (void)nc;
+  return SD_OK;
+}
+
 // One search of the pieces' columns, concatenated in order, against Y on the
 // slot's context; each piece's results go to its own request.  A query's
 // result does not depend on the other columns of the search, so packing
-// changes no answer.
-int run_segs(sd_server* s, sd_context* ctx, sd_matrix* X, const std::vector<Seg>& segs) {
+// changes no answer.  Items of at most kGraphCols columns replay a captured
+// graph when they can.
+int run_segs(sd_server* s, int slot, sd_context* ctx, sd_matrix* X, const std::vector<Seg>& segs) {
   const Request* r0 = segs[0].r;
   int64_t nc = 0, nnz = 0;
   for (const Seg& g : segs) {
     nc += g.c1 - g.c0;
     nnz += g.r->colptr[g.c1] - g.r->colptr[g.c0];
+  }
+  if (graph_eligible(s, r0->knn, nc)) {
+    bool done = false;
+    const int grc = guarded_call([&] { return run_graph_item(s, slot, segs, &done); });
+    if (grc == SD_OK && done) return SD_OK;
+    sd::clear_error();
   }
   std::vector<int64_t> p((size_t)nc + 1);
   std::vector<int32_t> ri;
@@ -233,24 +465,7 @@ int run_segs(sd_server* s, sd_context* ctx, sd_matrix* X, const std::vector<Seg>
   sd_knn_params prm = r0->knn;
   prm.include_self = 0;
   SD_TRY(sd_knn(ctx, X, s->Y, &prm, idx.data(), dist.data(), &keff, &warn));
-  // columns [c0, c1) of each request belong to this item alone, so no lock
-  // is needed here; k_eff is a property of (Y, k) and equal for every item.
-  int64_t off = 0;
-  for (const Seg& g : segs) {
-    const int64_t n = g.c1 - g.c0;
-    for (int32_t t = 0; t < keff; ++t) {
-      std::copy(idx.begin() + (size_t)t * nc + off, idx.begin() + (size_t)t * nc + off + n,
-                g.r->idx.begin() + (size_t)t * g.r->ncol + g.c0);
-      std::copy(dist.begin() + (size_t)t * nc + off, dist.begin() + (size_t)t * nc + off + n,
-                g.r->dist.begin() + (size_t)t * g.r->ncol + g.c0);
-    }
-    off += n;
-  }
-  std::lock_guard<std::mutex> lk(s->mu);
-  for (const Seg& g : segs) {
-    g.r->k_eff = keff;
-    g.r->warn = warn;
-  }
+  scatter_results(s, segs, idx.data(), dist.data(), nc, keff, warn);
   return SD_OK;
 }
 
@@ -332,10 +547,10 @@ void worker(sd_server* s, int slot) {
     // (say, non-finite input of one request) is charged to its own request
     std::vector<int> rcs(segs.size(), SD_OK);
     std::vector<std::string> msgs(segs.size());
-    int rc = guarded_call([&] { return run_segs(s, ctx, X, segs); });
+    int rc = guarded_call([&] { return run_segs(s, slot, ctx, X, segs); });
     if (rc != SD_OK && rc != SD_ECANCELLED && segs.size() > 1) {
       for (size_t i = 0; i < segs.size(); ++i) {
-        rcs[i] = guarded_call([&] { return run_segs(s, ctx, X, {segs[i]}); });
+        rcs[i] = guarded_call([&] { return run_segs(s, slot, ctx, X, {segs[i]}); });
         if (rcs[i] != SD_OK) msgs[i] = sd_last_error();
       }
     } else if (rc != SD_OK) {
@@ -406,6 +621,10 @@ int sd_server_create(sd_context* ctx, const sd_matrix* Y, const sd_server_params
     auto undo = [&] {
       for (sd_matrix* m : s->qmat) sd_matrix_free(m);
       for (sd_context* d : s->ctx) sd_context_destroy(d);
+      for (GraphSlot& gs : s->gslot) {
+        if (gs.X) sd_matrix_free(gs.X);
+        if (gs.ctx) sd_context_destroy(gs.ctx);
+      }
     };
     for (int i = 0; i < s->slots; ++i) {
       sd_context* c = nullptr;
@@ -423,6 +642,25 @@ int sd_server_create(sd_context* ctx, const sd_matrix* Y, const sd_server_params
       m->device = c->device;
       sd::matrix_use_pool(m, c->stream);
       s->qmat.push_back(m);
+    }
+    if (p.graphs) {
+      // each slot's graph items run on a context of their own, with the
+      // fixed threshold-sample rule (a tuned choice could change buffer sizes
+      // between the ordinary run and the capture)
+      s->gslot.resize((size_t)s->slots);
+      for (GraphSlot& gs : s->gslot) {
+        int rc = sd_context_create(s->device, &gs.ctx);
+        if (rc == SD_OK) rc = sd::context_use_pool(gs.ctx);
+        if (rc != SD_OK) {
+          undo();
+          return rc;
+        }
+        gs.ctx->sample_tuning = 0;
+        gs.X = new sd_matrix();
+        gs.X->ctx = gs.ctx;
+        gs.X->device = gs.ctx->device;
+        sd::matrix_use_pool(gs.X, gs.ctx->stream);
+      }
     }
     for (int i = 0; i < s->slots; ++i) s->workers.emplace_back(worker, s.get(), i);
     *out = s.release();
@@ -605,6 +843,7 @@ int sd_server_destroy(sd_server* s) {
     s->stop = true;
   }
   for (sd_context* c : s->ctx) sd_cancel(c);
+  for (GraphSlot& gs : s->gslot) sd_cancel(gs.ctx);
   s->cv_work.notify_all();
   for (auto& t : s->workers) t.join();
   {
@@ -629,6 +868,12 @@ int sd_server_destroy(sd_server* s) {
   }
   for (sd_matrix* m : s->qmat) sd_matrix_free(m);
   for (sd_context* c : s->ctx) sd_context_destroy(c);
+  for (GraphSlot& gs : s->gslot) {
+    for (auto& kv : gs.graphs)
+      if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
+    sd_matrix_free(gs.X);
+    sd_context_destroy(gs.ctx);
+  }
   delete s;
   return SD_OK;
 }

@@ -74,10 +74,11 @@ def ref_and_requests():
     return Y, reqs
 
 
-def test_server_same_alone_and_in_mix(ref_and_requests, oracle_lib, gpu_ctx):
+@pytest.mark.parametrize("graphs", [False, True])
+def test_server_same_alone_and_in_mix(ref_and_requests, oracle_lib, gpu_ctx, graphs):
     Y, reqs = ref_and_requests
     dY = sd.DeviceMatrix(Y)
-    with sd.Server(dY, slots=4, chunk_cols=256) as srv:
+    with sd.Server(dY, slots=4, chunk_cols=256, graphs=graphs) as srv:
         tickets = [srv.submit(r["X"], k=r["k"], method=r["method"], snn=r["snn"], prune=PRUNE)
                    for r in reqs]
         mixed = [srv.result(t) for t in tickets]
@@ -281,3 +282,51 @@ def test_small_requests_share_a_tile(gpu_ctx):
         assert np.array_equal(bits(r.dist), bits(alone.dist))
     alone = sd.sparseKNN(other, Y, k=15, method="cosine", verbose=False)
     assert np.array_equal(ro.idx, alone.idx)
+
+
+def test_graph_items_replay_and_match_oracle(oracle_lib, gpu_ctx):
+    """Work items of at most 256 columns replay a captured HIP graph.  The
+    reference has 20000 columns (157 reference tiles), so the captured
+    search is the two-pass one: threshold pre-pass, filtered pass, selection,
+    re-rank.  One slot serves the requests one after another, so each graph
+    is replayed several times, with ordinary items (more than 256 columns),
+    one-shot searches on another context and host synchronisations in
+    between; every served row equals the oracle's bit for bit."""
+    Y = synth.nb_log1p(1000, 20000, 0.05, 81)
+    dY = sd.DeviceMatrix(Y, gpu_ctx)
+    plan = [(5, "cosine", 20), (256, "binary", 10), (40, "cosine", 20), (1, "binary", 10),
+            (130, "cosine", 20), (300, "cosine", 20), (77, "binary", 10), (256, "cosine", 20),
+            (12, "binary", 10), (200, "cosine", 20), (3, "cosine", 20), (90, "binary", 10)]
+    with sd.Server(dY, slots=1, graphs=True) as srv:
+        for t, (n, method, k) in enumerate(plan):
+            X = synth.nb_log1p(1000, n, 0.05, 200 + t)
+            res = srv.result(srv.submit(X, k=k, method=method))
+            same_as_oracle(oracle_lib, res, X, Y, k, method)
+            if t in (4, 8):  # eager work between replays
+                one = sd.sparseKNN(X, dY, k=k, method=method, verbose=False)
+                assert np.array_equal(one.idx, res.idx)
+                assert np.array_equal(bits(one.dist), bits(res.dist))
+        st = srv.stats()
+    print("graph stats:", {f: st[f] for f in ("graph_items", "graph_captures", "graph_fallbacks")})
+    assert st["graph_captures"] >= 2  # one per search parameters (cosine k=20, binary k=10)
+    assert st["graph_items"] + st["graph_fallbacks"] >= 8  # 11 small items, 2 of them captured
+    assert st["graph_items"] >= 5
+
+
+def test_graph_items_off_gives_same_results(oracle_lib, gpu_ctx):
+    """graphs=False (the default) serves every item as an ordinary search,
+    with the same rows as the graph replays."""
+    Y = synth.nb_log1p(1000, 20000, 0.05, 82)
+    dY = sd.DeviceMatrix(Y, gpu_ctx)
+    Xs = [synth.nb_log1p(1000, n, 0.05, 300 + n) for n in (9, 120, 9)]
+    out = {}
+    for graphs in (True, False):
+        with sd.Server(dY, slots=1, graphs=graphs) as srv:
+            out[graphs] = [srv.result(srv.submit(X, k=15, method="cosine")) for X in Xs]
+            st = srv.stats()
+        if not graphs:
+            assert st["graph_items"] == 0 and st["graph_captures"] == 0
+    for a, b, X in zip(out[True], out[False], Xs):
+        assert np.array_equal(a.idx, b.idx)
+        assert np.array_equal(bits(a.dist), bits(b.dist))
+    same_as_oracle(oracle_lib, out[True][2], Xs[2], Y, 15, "cosine")
