@@ -347,9 +347,12 @@ int sd_graph_free(sd_graph* g);
  *    and SNN workspace).  Requests are admitted in arrival order while the
  *    reservations fit in budget_bytes; one that does not fit yet lets later
  *    requests that fit pass it until it has waited 2 s or been passed 64
- *    times, after which it is admitted next.  A request that cannot fit even
- *    alone, or that arrives while max_pending requests wait, is refused with
- *    SD_EBUDGET.
+ *    times, after which it is admitted next.  Requests of at most 1000
+ *    columns that the budget cannot take yet (or that wait behind such a
+ *    request) may use a lane of budget_bytes / 16 beyond the budget, so a
+ *    large request holding the queue delays only other large ones.  A
+ *    request that cannot fit even alone, or that arrives while max_pending
+ *    requests wait, is refused with SD_EBUDGET.
  *  - Memory: each worker reloads its query columns into buffers it keeps,
  *    and grows its scratch from the device's stream-ordered pool, so items
  *    do not allocate or free device memory once warm.
@@ -421,6 +424,7 @@ typedef struct {
                                a buffer the graph reads moved) */
   int64_t graph_fallbacks;  /* replays whose overflow or certificate counters sent the item
                                to an ordinary search */
+  int64_t lane_admitted;    /* requests admitted into the small-request lane */
 } sd_server_stats;
 
 /* Y must stay alive until sd_server_destroy; ctx gives the device. */

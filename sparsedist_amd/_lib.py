@@ -101,7 +101,7 @@ class ServerStats(C.Structure):
                 ("reserved_bytes", C.c_int64), ("query_cols", C.c_int64),
                 ("packed_items", C.c_int64), ("waiting_callers", C.c_int64),
                 ("graph_items", C.c_int64), ("graph_captures", C.c_int64),
-                ("graph_fallbacks", C.c_int64)]
+                ("graph_fallbacks", C.c_int64), ("lane_admitted", C.c_int64)]
 
 
 _lib = None

@@ -54,6 +54,7 @@ struct Request {
   int snn = 0;
   double prune = 0.0;
   int64_t reserve = 0;  // bytes reserved while admitted
+  bool in_lane = false;  // admitted into the small-request lane
   int bypassed = 0;     // requests admitted ahead of it while it waited
   // progress (guarded by the server mutex)
   int64_t next_col = 0;   // first unscheduled column
@@ -140,6 +141,7 @@ struct sd_server {
   std::deque<Request*> pending;
   std::vector<Request*> active;
   int64_t reserved = 0;
+  int64_t reserved_lane = 0;  // reservations of requests admitted into the small lane
   sd_server_stats st{};
   std::string last_msg;  // storage for sd_server_error's return value
 };
@@ -172,25 +174,51 @@ int64_t reserve_estimate(const sd_server* s, const Request& r) {
 constexpr double kAgeMs = 2000.0;
 constexpr int kMaxBypass = 64;
 
+// Small requests (at most kLaneCols columns) that the budget cannot take
+// yet, or that wait behind a request which has stopped the queue, may use a
+// lane of their own: up to budget / kLaneShare bytes beyond the budget.  A
+// large request that holds the queue then delays only other large ones; the
+// lane is bounded, so it delays no large request either.
+constexpr int64_t kLaneCols = 1000;
+constexpr int64_t kLaneShare = 16;
+
 // Admit waiting requests in arrival order while their reservations fit,
-// letting fitting requests pass blocked ones that have not aged yet.
+// letting fitting requests pass blocked ones that have not aged yet; then
+// small requests still waiting into the lane while it has room.
 void admit_locked(sd_server* s) {
   const auto now = Clock::now();
+  auto admit = [&](std::deque<Request*>::iterator it, bool lane) {
+    Request* r = *it;
+    // every request still waiting ahead of r is passed once more
+    for (auto jt = s->pending.begin(); jt != it; ++jt) ++(*jt)->bypassed;
+    if (lane) {
+      s->reserved_lane += r->reserve;
+      ++s->st.lane_admitted;
+    } else {
+      s->reserved += r->reserve;
+    }
+    r->in_lane = lane;
+    r->admitted = true;
+    r->t_admit = now;
+    s->active.push_back(r);
+    ++s->st.admitted;
+    return s->pending.erase(it);
+  };
   for (auto it = s->pending.begin(); it != s->pending.end();) {
     Request* r = *it;
     if (s->reserved + r->reserve <= s->budget) {
-      // every request still waiting ahead of r is passed once more
-      for (auto jt = s->pending.begin(); jt != it; ++jt) ++(*jt)->bypassed;
-      it = s->pending.erase(it);
-      s->reserved += r->reserve;
-      r->admitted = true;
-      r->t_admit = now;
-      s->active.push_back(r);
-      ++s->st.admitted;
+      it = admit(it, false);
       continue;
     }
     if (r->bypassed >= kMaxBypass || ms_between(r->t_submit, now) >= kAgeMs) break;
     ++it;
+  }
+  for (auto it = s->pending.begin(); it != s->pending.end();) {
+    Request* r = *it;
+    if (r->ncol <= kLaneCols && s->reserved_lane + r->reserve <= s->budget / kLaneShare)
+      it = admit(it, true);
+    else
+      ++it;
   }
 }
 
@@ -198,7 +226,7 @@ void finish_locked(sd_server* s, Request* r) {
   r->finished = true;
   r->finish_order = ++s->finished_count;
   r->t_done = Clock::now();
-  s->reserved -= r->reserve;
+  (r->in_lane ? s->reserved_lane : s->reserved) -= r->reserve;
   s->active.erase(std::remove(s->active.begin(), s->active.end(), r), s->active.end());
   ++s->st.completed;
   s->st.query_cols += r->ncol;
@@ -821,7 +849,7 @@ int sd_server_stats_get(sd_server* s, sd_server_stats* out) {
   *out = s->st;
   out->pending = (int64_t)s->pending.size();
   out->active = (int64_t)s->active.size();
-  out->reserved_bytes = s->reserved;
+  out->reserved_bytes = s->reserved + s->reserved_lane;
   out->waiting_callers = s->waiters;
   return SD_OK;
 }

@@ -154,6 +154,38 @@ def test_admission_lets_fitting_requests_pass(oracle_lib, gpu_ctx):
     assert rb["snn"].shape == (3000, 3000)
 
 
+def test_small_lane_admits_behind_a_full_budget(oracle_lib, gpu_ctx):
+    """A small request that the budget cannot take (a large one holds it and
+    another large one waits) is admitted into the small-request lane, a
+    sixteenth of the budget beyond it, instead of waiting for the large
+    ones; the waiting large request is admitted once the budget frees."""
+    Y = synth.nb_log1p(200, 3000, 0.05, 95)
+    big1 = synth.nb_log1p(200, 3000, 0.05, 96)
+    big2 = synth.nb_log1p(200, 3000, 0.05, 97)
+    tiny = synth.nb_log1p(200, 3, 0.05, 98)
+    reserve = {}
+    with sd.Server(Y, slots=1, paused=True) as probe:
+        for name, X in (("big", big1), ("tiny", tiny)):
+            before = probe.stats()["reserved_bytes"]
+            probe.submit(X, k=20, method="cosine", snn=True, prune=PRUNE)
+            reserve[name] = probe.stats()["reserved_bytes"] - before
+    budget = max(reserve["big"], 16 * reserve["tiny"])
+    assert 2 * reserve["big"] > budget  # the second large request waits
+    with sd.Server(Y, slots=1, budget_bytes=budget, paused=True) as srv:
+        t1 = srv.submit(big1, k=20, method="cosine", snn=True, prune=PRUNE)
+        t2 = srv.submit(big2, k=20, method="cosine", snn=True, prune=PRUNE)
+        tt = srv.submit(tiny, k=20, method="cosine", snn=True, prune=PRUNE)
+        st = srv.stats()
+        assert st["admitted"] == 2 and st["pending"] == 1 and st["lane_admitted"] == 1, st
+        srv.resume()
+        r1, r2, rt = srv.result(t1), srv.result(t2), srv.result(tt)
+        st = srv.stats()
+    assert st["completed"] == 3 and st["reserved_bytes"] == 0, st
+    same_as_oracle(oracle_lib, rt, tiny, Y, 20, "cosine")
+    for r, X in ((r1, big1), (r2, big2)):
+        same_as_oracle(oracle_lib, r, X, Y, 20, "cosine", cols=np.arange(0, 3000, 37))
+
+
 def test_destroy_while_a_caller_waits(gpu_ctx):
     """sd_server_destroy while another thread is blocked in sd_server_wait:
     the waiter returns (its request cancelled) before the server is freed."""
