@@ -718,6 +718,8 @@ __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
   const int64_t q_base = (int64_t)qt * TN;
   const int64_t r_base = rt * TM;
   const bool mirror = SYM && (!P.tri || rt > qt);  // workgroup-uniform
+  // per-reference arrays of the epilogue, filled while the first chunks load
+  auto fill_ref = [&]() {
   if (SYM && tid == 0) s_npool = 0;
   for (int t = tid; t < TM; t += NT) {
     const int64_t r = r_base + t;
@@ -760,6 +762,7 @@ __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
       }
     }
   }
+  };
 
   // Chunk c of a tile is one block of the chunk-major layout (rows 128 bytes
   // apart); a buffer resource per chunk and side covers the tile's rows of it
@@ -838,15 +841,18 @@ __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
   // LDS read sits behind a branch.
   const int64_t nchunks = P.kpad_bytes / RB;
   const int64_t last = nchunks - 1;
-  lds_barrier();  // s_rnnz and friends are visible
   issue(st0, 0);
   issue(st1, last < 1 ? last : 1);
   if constexpr (NS > 2) {
     issue(st2, last < 2 ? last : 2);
     issue(st3, last < 3 ? last : 3);
   }
+  // the per-reference loads go out after the first chunks', so the two
+  // latencies overlap (a tile of short rows is a few chunks long)
+  __builtin_amdgcn_sched_barrier(0);
+  fill_ref();
   wait_vm<(NS - 1) * INST>();
-  lds_barrier();
+  lds_barrier();  // the per-reference arrays are visible and chunk 0 has landed
   load_frag(f0, st0, 0);
   // Chunk c in stage CUR (its step-0 fragments already requested): the steps
   // alternate fragment sets, each requesting the next step's fragments before
