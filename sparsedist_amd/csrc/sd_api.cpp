@@ -35,6 +35,34 @@ int sync_stream(sd_context* ctx) {
   return SD_OK;
 }
 
+// Blocking copies go through a stream of the library's, never the legacy
+// (null) stream: while any thread of the process captures a stream into a
+// HIP graph (a server's graph items do), HIP refuses synchronous work on the
+// legacy stream in every thread (hipErrorStreamCaptureImplicit).
+static int copy_wait(void* dst, const void* src, size_t bytes, hipMemcpyKind kind,
+                     hipStream_t s) {
+  if (bytes == 0) return SD_OK;
+  SD_HIP(hipMemcpyAsync(dst, src, bytes, kind, s));
+  SD_HIP(hipStreamSynchronize(s));
+  return SD_OK;
+}
+
+// One non-blocking stream per device for blocking copies that have no
+// context (sd_graph_copy); created on first use, kept until the process ends.
+static int transfer_stream(int device, hipStream_t* out) {
+  static std::mutex mu;
+  static std::vector<hipStream_t> streams;
+  std::lock_guard<std::mutex> lk(mu);
+  if (device < 0) return set_error(SD_EINVAL, "bad device %d", device);
+  if (streams.size() <= (size_t)device) streams.resize((size_t)device + 1, nullptr);
+  if (!streams[(size_t)device]) {
+    SD_HIP(hipSetDevice(device));
+    SD_HIP(hipStreamCreateWithFlags(&streams[(size_t)device], hipStreamNonBlocking));
+  }
+  *out = streams[(size_t)device];
+  return SD_OK;
+}
+
 std::atomic<int64_t> g_device_allocs{0};
 
 int DevBuf::alloc(size_t n) {
@@ -1363,7 +1391,8 @@ int sd_slab(sd_context* ctx, int32_t method, const sd_matrix* X, const sd_matrix
     SD_TRY(qc.alloc(sizeof(int32_t) * (size_t)nc));
     std::vector<int32_t> iota((size_t)nc);
     for (int64_t j = 0; j < nc; ++j) iota[(size_t)j] = (int32_t)j;
-    SD_HIP(hipMemcpy(qc.ptr, iota.data(), sizeof(int32_t) * (size_t)nc, hipMemcpyHostToDevice));
+    SD_TRY(copy_wait(qc.ptr, iota.data(), sizeof(int32_t) * (size_t)nc,
+                     hipMemcpyHostToDevice, ctx->stream));
     for (int64_t c0 = 0; c0 < nc; c0 += nb) {
       const int64_t n = std::min(nb, nc - c0);
       SD_TRY(exact_slab(ctx, a, qc.as<int32_t>() + c0, n, dD.as<double>(), nr, 0));
@@ -1394,13 +1423,17 @@ int sd_topk_block(sd_context* ctx, const double* D, int64_t nrow, int64_t ncol, 
     SD_TRY(ds.alloc(sizeof(int32_t) * (size_t)ncol));
     SD_TRY(di.alloc(sizeof(int32_t) * (size_t)ncol * (size_t)k));
     SD_TRY(dv.alloc(sizeof(double) * (size_t)ncol * (size_t)k));
-    SD_HIP(hipMemcpy(dD.ptr, D, sizeof(double) * (size_t)nrow * (size_t)ncol, hipMemcpyHostToDevice));
-    SD_HIP(hipMemcpy(ds.ptr, skip.data(), sizeof(int32_t) * (size_t)ncol, hipMemcpyHostToDevice));
+    SD_TRY(copy_wait(dD.ptr, D, sizeof(double) * (size_t)nrow * (size_t)ncol,
+                     hipMemcpyHostToDevice, ctx->stream));
+    SD_TRY(copy_wait(ds.ptr, skip.data(), sizeof(int32_t) * (size_t)ncol,
+                     hipMemcpyHostToDevice, ctx->stream));
     SD_TRY(topk_dense(ctx, dD.as<double>(), nrow, ncol, nrow, k, decreasing, ds.as<int32_t>(),
                       di.as<int32_t>(), dv.as<double>(), ncol, nullptr));
     SD_HIP(hipStreamSynchronize(ctx->stream));
-    SD_HIP(hipMemcpy(idx, di.ptr, sizeof(int32_t) * (size_t)ncol * (size_t)k, hipMemcpyDeviceToHost));
-    SD_HIP(hipMemcpy(val, dv.ptr, sizeof(double) * (size_t)ncol * (size_t)k, hipMemcpyDeviceToHost));
+    SD_TRY(copy_wait(idx, di.ptr, sizeof(int32_t) * (size_t)ncol * (size_t)k,
+                     hipMemcpyDeviceToHost, ctx->stream));
+    SD_TRY(copy_wait(val, dv.ptr, sizeof(double) * (size_t)ncol * (size_t)k,
+                     hipMemcpyDeviceToHost, ctx->stream));
     return SD_OK;
   });
 }
@@ -1425,20 +1458,21 @@ int sd_topk_rows_accum(sd_context* ctx, const double* D, int64_t nrow, int64_t n
     SD_TRY(di.alloc(sizeof(int32_t) * (size_t)nrow * (size_t)k));
     SD_TRY(dv.alloc(sizeof(double) * (size_t)nrow * (size_t)k));
     if (ncol > 0)
-      SD_HIP(hipMemcpy(dD.ptr, D, sizeof(double) * (size_t)nrow * (size_t)ncol,
-                       hipMemcpyHostToDevice));
-    SD_HIP(hipMemcpy(dsc.ptr, self_col, sizeof(int32_t) * (size_t)nrow, hipMemcpyHostToDevice));
-    SD_HIP(hipMemcpy(di.ptr, best_idx, sizeof(int32_t) * (size_t)nrow * (size_t)k,
-                     hipMemcpyHostToDevice));
-    SD_HIP(hipMemcpy(dv.ptr, best_val, sizeof(double) * (size_t)nrow * (size_t)k,
-                     hipMemcpyHostToDevice));
+      SD_TRY(copy_wait(dD.ptr, D, sizeof(double) * (size_t)nrow * (size_t)ncol,
+                       hipMemcpyHostToDevice, ctx->stream));
+    SD_TRY(copy_wait(dsc.ptr, self_col, sizeof(int32_t) * (size_t)nrow,
+                     hipMemcpyHostToDevice, ctx->stream));
+    SD_TRY(copy_wait(di.ptr, best_idx, sizeof(int32_t) * (size_t)nrow * (size_t)k,
+                     hipMemcpyHostToDevice, ctx->stream));
+    SD_TRY(copy_wait(dv.ptr, best_val, sizeof(double) * (size_t)nrow * (size_t)k,
+                     hipMemcpyHostToDevice, ctx->stream));
     SD_TRY(topk_rows(ctx, dD.as<double>(), nrow, ncol, k, decreasing, di.as<int32_t>(),
                      dv.as<double>(), col_offset, dsc.as<int32_t>()));
     SD_HIP(hipStreamSynchronize(ctx->stream));
-    SD_HIP(hipMemcpy(best_idx, di.ptr, sizeof(int32_t) * (size_t)nrow * (size_t)k,
-                     hipMemcpyDeviceToHost));
-    SD_HIP(hipMemcpy(best_val, dv.ptr, sizeof(double) * (size_t)nrow * (size_t)k,
-                     hipMemcpyDeviceToHost));
+    SD_TRY(copy_wait(best_idx, di.ptr, sizeof(int32_t) * (size_t)nrow * (size_t)k,
+                     hipMemcpyDeviceToHost, ctx->stream));
+    SD_TRY(copy_wait(best_val, dv.ptr, sizeof(double) * (size_t)nrow * (size_t)k,
+                     hipMemcpyDeviceToHost, ctx->stream));
     return SD_OK;
   });
 }
@@ -1453,9 +1487,9 @@ int sd_snn_jaccard(sd_context* ctx, const int32_t* idx, int32_t n, int32_t k,
     const size_t total = (size_t)n * (size_t)k;
     SD_TRY(di.alloc(sizeof(int32_t) * total));
     SD_TRY(dw.alloc(sizeof(double) * total));
-    SD_HIP(hipMemcpy(di.ptr, idx, sizeof(int32_t) * total, hipMemcpyHostToDevice));
+    SD_TRY(copy_wait(di.ptr, idx, sizeof(int32_t) * total, hipMemcpyHostToDevice, ctx->stream));
     SD_TRY(snn_build(ctx, di.as<int32_t>(), n, k, include_self, 0.0, 0, 0, nullptr, dw.as<double>()));
-    SD_HIP(hipMemcpy(w, dw.ptr, sizeof(double) * total, hipMemcpyDeviceToHost));
+    SD_TRY(copy_wait(w, dw.ptr, sizeof(double) * total, hipMemcpyDeviceToHost, ctx->stream));
     return SD_OK;
   });
 }
@@ -1483,6 +1517,7 @@ int sd_snn(sd_context* ctx, const int32_t* idx, int32_t n, int32_t k, int32_t in
       d_idx = di.as<int32_t>();
     }
     auto* g = new sd_graph();
+    g->device = ctx->device;
     int rc = snn_build(ctx, d_idx, n, k, include_self, prune, symmetrise, edges, g, nullptr);
     if (rc != SD_OK) {
       delete g;
@@ -1503,13 +1538,15 @@ int sd_graph_info(const sd_graph* g, int64_t* n, int64_t* nnz) {
 int sd_graph_copy(const sd_graph* g, int64_t* colptr, int32_t* rowind, double* val) {
   return guarded([&]() -> int {
     if (!g || !colptr) return set_error(SD_EINVAL, "NULL argument");
-    SD_HIP(hipMemcpy(colptr, g->colptr.ptr, sizeof(int64_t) * (size_t)(g->n + 1),
-                     hipMemcpyDeviceToHost));
+    hipStream_t s = nullptr;
+    SD_TRY(transfer_stream(g->device, &s));
+    SD_TRY(copy_wait(colptr, g->colptr.ptr, sizeof(int64_t) * (size_t)(g->n + 1),
+                     hipMemcpyDeviceToHost, s));
     if (g->nnz > 0) {
       if (!rowind || !val) return set_error(SD_EINVAL, "NULL argument");
-      SD_HIP(hipMemcpy(rowind, g->rowind.ptr, sizeof(int32_t) * (size_t)g->nnz,
-                       hipMemcpyDeviceToHost));
-      SD_HIP(hipMemcpy(val, g->val.ptr, sizeof(double) * (size_t)g->nnz, hipMemcpyDeviceToHost));
+      SD_TRY(copy_wait(rowind, g->rowind.ptr, sizeof(int32_t) * (size_t)g->nnz,
+                       hipMemcpyDeviceToHost, s));
+      SD_TRY(copy_wait(val, g->val.ptr, sizeof(double) * (size_t)g->nnz, hipMemcpyDeviceToHost, s));
     }
     return SD_OK;
   });

@@ -257,6 +257,7 @@ struct sd_matrix {
 
 struct sd_graph {
   int64_t n = 0, nnz = 0;
+  int device = 0;
   sd::DevBuf colptr, rowind, val;
 };
 

@@ -511,6 +511,7 @@ int run_snn(sd_server* s, sd_context* ctx, Request* r) {
   SD_HIP(hipMemcpyAsync(di.ptr, r->idx.data(), sizeof(int32_t) * (size_t)n * (size_t)k,
                         hipMemcpyHostToDevice, ctx->stream));
   auto* g = new sd_graph();
+  g->device = ctx->device;
   const int rc = sd::snn_build(ctx, di.as<int32_t>(), (int)n, k, 0, r->prune, 0, 1, g, nullptr,
                                (int)s->Y->ncol);
   if (rc != SD_OK) {
