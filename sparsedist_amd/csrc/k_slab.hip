@@ -159,6 +159,19 @@ __device__ inline float next_float_up(float t) {
   return __int_as_float(t > 0.f ? b + 1 : b - 1);
 }
 
+// Jaccard screen in one FMA: with d = ni + nj - c > 0, c / d >= t exactly
+// when c >= t (ni + nj) / (1 + t), so a value passes the screen when
+// c >= fma(a, ni, a nj) with a = t / (1 + t), t the screen threshold (a
+// little below the exact one, at least 0).  a is lowered by a further 2^-18
+// relative: the FP32 quotient, product and FMA round by at most 2^-22
+// relative in all, so the screen never rejects a value the exact test keeps.
+// Two empty columns (c = ni = nj = 0) pass (J = 1).  An infinite threshold
+// (no query) gives a NaN coefficient, which nothing passes.
+__device__ inline float jac_screen_coef(float t) {
+  const float tt = fmaxf(t, 0.f);
+  return (tt / (1.f + tt)) * 0.999996185302734375f;  // 1 - 2^-18
+}
+
 // Split layout's query offset: segment 1 <-> 2.  Written as arithmetic, not
 // selects, so that the K loop stays one basic block (a branch there makes the
 // wait-count pass merge states at the join and wait for every load), and
@@ -696,6 +709,12 @@ __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
   constexpr bool MW = SYM && MK != kKeyScore;
   __shared__ __align__(16) float s_rw1[MW ? TM : 4];
   __shared__ __align__(16) float s_rw2[MW ? TM : 4];
+  // Jaccard: stored entries as a float, and the mirror screen's FMA
+  // coefficients (jac_screen_coef) of the reference column as a query
+  constexpr bool JC = sizeof(K) == 8;
+  __shared__ __align__(16) float s_rnf[JC ? TM : 4];
+  __shared__ __align__(16) float s_rja[JC && SYM ? TM : 4];
+  __shared__ __align__(16) float s_rjb[JC && SYM ? TM : 4];
 
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
@@ -745,7 +764,12 @@ __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
           s_rw1[t] = w.w1;
           s_rw2[t] = w.w2;
         }
+        if constexpr (JC) {
+          s_rja[t] = jac_screen_coef(s_rthr[t]);
+          s_rjb[t] = s_rja[t] * (float)s_rnnz[t];
+        }
       }
+      if constexpr (JC) s_rnf[t] = (float)s_rnnz[t];
     } else {
       s_rnnz[t] = 0;
       s_raux[t] = 0.f;
@@ -759,6 +783,13 @@ __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
       if constexpr (MW) {
         s_rw1[t] = 0.f;
         s_rw2[t] = 0.f;
+      }
+      if constexpr (JC) {
+        s_rnf[t] = 0.f;
+        if constexpr (SYM) {
+          s_rja[t] = __int_as_float(0x7fc00000);  // NaN: padding passes no screen
+          s_rjb[t] = 0.f;
+        }
       }
     }
   }
@@ -934,6 +965,7 @@ __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
   bool qv[2];
   uint32_t qn[2];
   float qa[2], qa2[2], thr_f[2];
+  float ja[2], jb[2], qnf[2];  // Jaccard screen: c >= fma(ja, ni, jb), jb = ja nj
   KeyWeights kw[2];
   double thr_d[2];
 #pragma unroll
@@ -956,6 +988,9 @@ __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
       // float screen a little below the threshold; the exact test follows
       thr_f[j] = t - fabsf(t) * 9.5367431640625e-07f - 1e-30f;
     }
+    qnf[j] = (float)qn[j];
+    ja[j] = jac_screen_coef(thr_f[j]);
+    jb[j] = ja[j] * qnf[j];
   }
   const int64_t rg0 = r_base + wm * 128 + 4 * h;
   // exact test of one value: reference rg (tile row rloc, ni stored entries)
@@ -985,11 +1020,9 @@ __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
   };
   // cheap screen (may keep too much, never too little); the self value is
   // marked separately below
-  auto screen = [&](const float x, int j, int ni, float ra, float rc) -> bool {
+  auto screen = [&](const float x, int j, float nif, float ra, float rc) -> bool {
     if constexpr (sizeof(K) == 8) {
-      // c / (ni + nj - c) >= thr  <=>  c >= thr (ni + nj - c)
-      const float d = (float)((uint32_t)ni + qn[j]) - x;
-      return x >= thr_f[j] * d || d <= 0.f;
+      return x >= fmaf(ja[j], nif, jb[j]);  // jac_screen_coef
     } else if constexpr (MK == kKeyEuclid) {
       return euc_key(kw[j], qa[j], ra, x) >= thr_f[j];
     } else if constexpr (MK == kKeyPearson) {
@@ -1016,12 +1049,12 @@ __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
     for (int q4 = 0; q4 < 4; ++q4) {
       // the four references 8 q4 + 4h + 0..3 of block b are consecutive
       const int rl = wm * 128 + b * 32 + 8 * q4 + 4 * h;
-      int4 nn = make_int4(0, 0, 0, 0);
+      float4 nn = make_float4(0.f, 0.f, 0.f, 0.f);
       float4 ra4 = make_float4(0.f, 0.f, 0.f, 0.f), rc4 = ra4;
-      if constexpr (sizeof(K) == 8) nn = *reinterpret_cast<const int4*>(&s_rnnz[rl]);
+      if constexpr (sizeof(K) == 8) nn = *reinterpret_cast<const float4*>(&s_rnf[rl]);
       if constexpr (MK != kKeyScore) ra4 = *reinterpret_cast<const float4*>(&s_raux[rl]);
       if constexpr (MK == kKeyPearson) rc4 = *reinterpret_cast<const float4*>(&s_raux2[rl]);
-      const int nis[4] = {nn.x, nn.y, nn.z, nn.w};
+      const float nis[4] = {nn.x, nn.y, nn.z, nn.w};
       const float ras[4] = {ra4.x, ra4.y, ra4.z, ra4.w};
       const float rcs[4] = {rc4.x, rc4.y, rc4.z, rc4.w};
 #pragma unroll
@@ -1160,10 +1193,14 @@ __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
 #pragma unroll
         for (int q4 = 0; q4 < 4; ++q4) {
           const int rl = wm * 128 + b * 32 + 8 * q4 + 4 * h;
-          const float4 t4 = *reinterpret_cast<const float4*>(&s_rthr[rl]);
-          int4 nn = make_int4(0, 0, 0, 0);
-          float4 w14 = make_float4(0.f, 0.f, 0.f, 0.f), w24 = w14, ra4 = w14, rc4 = w14;
-          if constexpr (sizeof(K) == 8) nn = *reinterpret_cast<const int4*>(&s_rnnz[rl]);
+          float4 t4 = make_float4(0.f, 0.f, 0.f, 0.f);
+          float4 w14 = t4, w24 = t4, ra4 = t4, rc4 = t4, ja4 = t4, jb4 = t4;
+          if constexpr (sizeof(K) == 8) {
+            ja4 = *reinterpret_cast<const float4*>(&s_rja[rl]);
+            jb4 = *reinterpret_cast<const float4*>(&s_rjb[rl]);
+          } else {
+            t4 = *reinterpret_cast<const float4*>(&s_rthr[rl]);
+          }
           if constexpr (MW) {
             w14 = *reinterpret_cast<const float4*>(&s_rw1[rl]);
             w24 = *reinterpret_cast<const float4*>(&s_rw2[rl]);
@@ -1171,7 +1208,8 @@ __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
           }
           if constexpr (MW && MK == kKeyPearson) rc4 = *reinterpret_cast<const float4*>(&s_raux2[rl]);
           const float ts[4] = {t4.x, t4.y, t4.z, t4.w};
-          const int nis[4] = {nn.x, nn.y, nn.z, nn.w};
+          const float jas[4] = {ja4.x, ja4.y, ja4.z, ja4.w};
+          const float jbs[4] = {jb4.x, jb4.y, jb4.z, jb4.w};
           const float w1s[4] = {w14.x, w14.y, w14.z, w14.w};
           const float w2s[4] = {w24.x, w24.y, w24.z, w24.w};
           const float ras[4] = {ra4.x, ra4.y, ra4.z, ra4.w};
@@ -1184,8 +1222,7 @@ __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
               const float x = acc[b][j][r];
               bool pass;
               if constexpr (sizeof(K) == 8) {
-                const float d = (float)((uint32_t)nis[e] + qn[j]) - x;
-                pass = x >= ts[e] * d || d <= 0.f;
+                pass = x >= fmaf(jas[e], qnf[j], jbs[e]);  // jac_screen_coef
               } else if constexpr (MK == kKeyEuclid) {
                 pass = euc_key(KeyWeights{w1s[e], w2s[e], 0.f}, ras[e], qa[j], x) >= ts[e];
               } else if constexpr (MK == kKeyPearson) {
