@@ -113,6 +113,9 @@ def test_knn_matches_full_matrix(oracle_lib, method):
         # the driver reports exactly the slab kernel's values ...
         own = oracle_lib.slab(method, X, X, dist=dist)
         assert np.array_equal(bits(dst), bits(np.take_along_axis(own.T, idx - 1, 1)))
+        # ... and its ranks are exactly those of a plain sort of that slab
+        oidx, _ = brute.ref_topk(own, 5, dec, True)
+        assert np.array_equal(idx, oidx)
         # ... which match the closed form to rounding
         np.testing.assert_allclose(dst, np.take_along_axis(full.T, idx - 1, 1), rtol=1e-9,
                                    atol=1e-12)
