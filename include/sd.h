@@ -369,6 +369,10 @@ int sd_graph_free(sd_graph* g);
  *    into a HIP graph the first time the search's parameters occurred; the
  *    overflow and certificate counters come back with the results, and an
  *    item that needs a second pass runs as an ordinary search instead.
+ *    While a capture runs, HIP refuses synchronous work on the legacy
+ *    (null) stream from every thread of the process; this library makes
+ *    no such call, but a caller's own legacy-stream work on other threads
+ *    can fail then.  Captures happen on the first item of each parameter set.
  *  - Results do not depend on chunking, graphs or the other requests in
  *    flight.
  * All functions may be called from any host thread. */
