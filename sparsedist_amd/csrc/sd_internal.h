@@ -184,6 +184,10 @@ struct sd_context {
   sd::DevBuf snn_scratch[24];
   sd::CrossMirror xm;
   std::vector<uint8_t> host_scratch;
+  // pinned host memory that large results are copied into before the host
+  // spreads them over the caller's arrays (grow-only; hipHostMalloc)
+  void* host_stage = nullptr;
+  size_t host_stage_bytes = 0;
   // device address of the allocation-free entry point's scratch array on
   // this context's device (resolved when the context is created)
   void* noalloc_scratch = nullptr;
