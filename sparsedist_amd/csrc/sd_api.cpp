@@ -149,11 +149,16 @@ static int download(sd_context* ctx, const HostCopy* parts, int n) {
     (void)hipGetLastError();
   }
   if (total < ((size_t)1 << 20) || !ctx->host_stage) {
-    for (int i = 0; i < n; ++i)
+    // the stream is drained before any return: no copy is left writing the
+    // caller's arrays
+    hipError_t e = hipSuccess;
+    for (int i = 0; i < n && e == hipSuccess; ++i)
       if (parts[i].bytes)
-        SD_HIP(hipMemcpyAsync(parts[i].dst, parts[i].src, parts[i].bytes, hipMemcpyDeviceToHost,
-                              ctx->stream));
-    SD_HIP(hipStreamSynchronize(ctx->stream));
+        e = hipMemcpyAsync(parts[i].dst, parts[i].src, parts[i].bytes, hipMemcpyDeviceToHost,
+                           ctx->stream);
+    const hipError_t es = hipStreamSynchronize(ctx->stream);
+    SD_HIP(e);
+    SD_HIP(es);
     return SD_OK;
   }
   struct Piece {
@@ -1674,16 +1679,22 @@ int sd_graph_info(const sd_graph* g, int64_t* n, int64_t* nnz) {
 int sd_graph_copy(const sd_graph* g, int64_t* colptr, int32_t* rowind, double* val) {
   return guarded([&]() -> int {
     if (!g || !colptr) return set_error(SD_EINVAL, "NULL argument");
+    if (g->nnz > 0 && (!rowind || !val)) return set_error(SD_EINVAL, "NULL argument");
     hipStream_t s = nullptr;
     SD_TRY(transfer_stream(g->device, &s));
-    SD_TRY(copy_wait(colptr, g->colptr.ptr, sizeof(int64_t) * (size_t)(g->n + 1),
-                     hipMemcpyDeviceToHost, s));
-    if (g->nnz > 0) {
-      if (!rowind || !val) return set_error(SD_EINVAL, "NULL argument");
-      SD_TRY(copy_wait(rowind, g->rowind.ptr, sizeof(int32_t) * (size_t)g->nnz,
-                       hipMemcpyDeviceToHost, s));
-      SD_TRY(copy_wait(val, g->val.ptr, sizeof(double) * (size_t)g->nnz, hipMemcpyDeviceToHost, s));
-    }
+    // the three arrays in one round trip; the stream is drained before any
+    // return, so no copy is left writing the caller's arrays
+    hipError_t e = hipMemcpyAsync(colptr, g->colptr.ptr, sizeof(int64_t) * (size_t)(g->n + 1),
+                                  hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess && g->nnz > 0)
+      e = hipMemcpyAsync(rowind, g->rowind.ptr, sizeof(int32_t) * (size_t)g->nnz,
+                         hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess && g->nnz > 0)
+      e = hipMemcpyAsync(val, g->val.ptr, sizeof(double) * (size_t)g->nnz,
+                         hipMemcpyDeviceToHost, s);
+    const hipError_t es = hipStreamSynchronize(s);
+    SD_HIP(e);
+    SD_HIP(es);
     return SD_OK;
   });
 }
