@@ -1031,11 +1031,24 @@ __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
       return x >= thr_f[j];
     }
   };
+  // value r (0..15, per lane) of block b, query j: a select tree on r's four
+  // bits (4 bit tests, 15 selects) rather than a compare and select per value,
+  // each select waiting on its own compare.  The tree selects between
+  // shuffled halves of the accumulator vector: selects between elements of
+  // one vector are folded back into an element read at a variable index,
+  // which is lowered to the compare chain.
   auto value_of = [&](int b, int j, int r) -> float {
-    float x = 0.f;
-#pragma unroll
-    for (int t = 0; t < 16; ++t) x = (r == t) ? (float)acc[b][j][t] : x;
-    return x;
+    typedef float v8f __attribute__((ext_vector_type(8)));
+    typedef float v4f __attribute__((ext_vector_type(4)));
+    typedef float v2f __attribute__((ext_vector_type(2)));
+    const v16f a = acc[b][j];
+    const v8f l0 = (r & 1) ? __builtin_shufflevector(a, a, 1, 3, 5, 7, 9, 11, 13, 15)
+                           : __builtin_shufflevector(a, a, 0, 2, 4, 6, 8, 10, 12, 14);
+    const v4f l1 = (r & 2) ? __builtin_shufflevector(l0, l0, 1, 3, 5, 7)
+                           : __builtin_shufflevector(l0, l0, 0, 2, 4, 6);
+    const v2f l2 = (r & 4) ? __builtin_shufflevector(l1, l1, 1, 3)
+                           : __builtin_shufflevector(l1, l1, 0, 2);
+    return (r & 8) ? l2.y : l2.x;
   };
   // Pass 1, unrolled over the 128 values: plain compares only, one bit per
   // value (16 b + r of pm[j]).  The exact test and the append run in loops
@@ -1188,6 +1201,13 @@ __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
         cand_ok[j] = qv[j] & (dg == 0);
       }
       uint64_t pmr[2] = {0ull, 0ull};
+      // cosine: each group's thresholds are read one group ahead, so their LDS
+      // latency overlaps the previous group's compares (the barrier at each
+      // group's end keeps the compiler from moving the reads itself).  The
+      // other keys read several operands per group and gain nothing from it.
+      float4 t4n = make_float4(0.f, 0.f, 0.f, 0.f);
+      if constexpr (MK == kKeyScore && sizeof(K) != 8)
+        t4n = *reinterpret_cast<const float4*>(&s_rthr[wm * 128 + 4 * h]);
 #pragma unroll
       for (int b = 0; b < 4; ++b)
 #pragma unroll
@@ -1198,6 +1218,13 @@ __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
           if constexpr (sizeof(K) == 8) {
             ja4 = *reinterpret_cast<const float4*>(&s_rja[rl]);
             jb4 = *reinterpret_cast<const float4*>(&s_rjb[rl]);
+          } else if constexpr (MK == kKeyScore) {
+            t4 = t4n;
+            if (b * 4 + q4 < 15) {
+              const int g = b * 4 + q4 + 1;
+              t4n = *reinterpret_cast<const float4*>(
+                  &s_rthr[wm * 128 + (g >> 2) * 32 + 8 * (g & 3) + 4 * h]);
+            }
           } else {
             t4 = *reinterpret_cast<const float4*>(&s_rthr[rl]);
           }
