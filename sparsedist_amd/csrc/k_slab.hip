@@ -1200,7 +1200,8 @@ __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
         const int dg = P.mdeg ? (int)P.mdeg[cq] : 0;
         cand_ok[j] = qv[j] & (dg == 0);
       }
-      uint64_t pmr[2] = {0ull, 0ull};
+      // marks of blocks 2w and 2w + 1 in pw[j][w] (bit 16 (b & 1) + r)
+      uint32_t pw[2][2] = {{0u, 0u}, {0u, 0u}};
       // cosine: each group's thresholds are read one group ahead, so their LDS
       // latency overlaps the previous group's compares (the barrier at each
       // group's end keeps the compiler from moving the reads itself).  The
@@ -1241,6 +1242,10 @@ __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
           const float w2s[4] = {w24.x, w24.y, w24.z, w24.w};
           const float ras[4] = {ra4.x, ra4.y, ra4.z, ra4.w};
           const float rcs[4] = {rc4.x, rc4.y, rc4.z, rc4.w};
+          // each value's bit is selected on its own and the group's four are
+          // ORed together, so the marks add one dependent step per group
+          // rather than two per value
+          uint32_t c[2][4];
 #pragma unroll
           for (int e = 0; e < 4; ++e) {
             const int r = 4 * q4 + e;
@@ -1258,14 +1263,19 @@ __global__ __launch_bounds__(512, 2) void k_filter_wide(FusedKernelArgs P) {
               } else {
                 pass = x >= ts[e];
               }
-              if (pass & cand_ok[j]) pmr[j] |= 1ull << (16 * b + r);
+              c[j][e] = (pass & cand_ok[j]) ? 1u << (16 * (b & 1) + r) : 0u;
             }
           }
+#pragma unroll
+          for (int j = 0; j < 2; ++j) pw[j][b >> 1] |= (c[j][0] | c[j][1]) | (c[j][2] | c[j][3]);
           // the group's compares complete here: otherwise they sink below
           // every group's LDS reads, whose results then spill
-          asm volatile("" : "+v"(pmr[0]), "+v"(pmr[1]));
+          asm volatile("" : "+v"(pw[0][b >> 1]), "+v"(pw[1][b >> 1]));
           __builtin_amdgcn_sched_barrier(0);
         }
+      uint64_t pmr[2];
+#pragma unroll
+      for (int j = 0; j < 2; ++j) pmr[j] = (uint64_t)pw[j][0] | ((uint64_t)pw[j][1] << 32);
       // Gather the marked pairs per reference column in LDS (the stage
       // buffers are free once every wave's refills have landed): kMirrorSlots
       // per column in stage 0, further ones in a pool in stage 1.  Then one
