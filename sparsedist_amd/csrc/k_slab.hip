@@ -574,9 +574,25 @@ __global__ __launch_bounds__(WM * 256, 1) void k_fused(FusedKernelArgs P) {
     // (the other four rows of every group of eight).  Values pass through
     // LDS (rows of 33 words against bank conflicts) so the scan is a loop.
     const int rloc_base = wm * kTileRefs;
+    // Plain float keys (cosine): the lane that stores a row's 32 values also
+    // marks those at or above the owner's current L-th key in the row's
+    // spare word, and the owner visits the marked values only.  The L-th key
+    // only rises during the scan, so every value the list takes is marked.
+    // A tile that may hold a query's own column marks everything (that
+    // value's key is restored above its slab value).
+    constexpr bool kListScreen = MK == kKeyScore && sizeof(K) != 8;
+    auto diag = [&](int64_t off) { return r_base < q_base + off + TN && q_base + off < r_base + TM; };
+    const bool mark_all = P.self_mode && (P.qlist != nullptr || diag(0) || diag(P.self_off));
 #pragma unroll
     for (int sub = 0; sub < 4; ++sub) {
       const int phase = sub >> 1, b0 = (sub & 1) * 2;
+      float thr_w = 0.f;  // the L-th key of the row's owner (this lane, then lane l ^ 32)
+      if constexpr (kListScreen) {
+        const float own = (float)list[L - 1].key;
+        const float other = __shfl_xor(own, 32);
+        thr_w = (phase == 0) ? own : other;
+      }
+      uint32_t marks = 0u;
 #pragma unroll
       for (int bb = 0; bb < 2; ++bb)
 #pragma unroll
@@ -589,47 +605,61 @@ __global__ __launch_bounds__(WM * 256, 1) void k_fused(FusedKernelArgs P) {
           else
             __builtin_memcpy(&bits, &x, 4);
           xch[((phase == 0) ? tid : (tid ^ 32)) * 33 + bb * 16 + r] = bits;
+          if constexpr (kListScreen) marks |= ((float)x >= thr_w) ? 1u << (bb * 16 + r) : 0u;
         }
+      if constexpr (kListScreen)
+        xch[((phase == 0) ? tid : (tid ^ 32)) * 33 + 32] = mark_all ? 0xFFFFFFFFu : marks;
       lds_barrier();
       const int hh = (phase == 0) ? h : 1 - h;
-      if (q_ok) {
-#pragma unroll 1
-        for (int v = 0; v < 32; ++v) {
-          const int b = b0 + (v >> 4), rr = v & 15;
-          const int rloc = rloc_base + b * 32 + 8 * (rr >> 2) + 4 * hh + (rr & 3);
-          const int64_t rg = r_base + rloc;
-          if (s_rdeg[rloc] || (P.self_mode && rg == my_q + P.self_off && !P.include_self)) continue;
-          const uint32_t bits = xch[tid * 33 + v];
-          K key;
-          if constexpr (sizeof(K) == 8) {
-            const uint32_t c = bits;
-            const uint32_t d = (uint32_t)s_rnnz[rloc] + nj - c;
-            key = (d == 0) ? KeyJ{1u, 1u} : KeyJ{c, d};
+      // value v of the row: the exact key and the list insertion
+      auto visit = [&](int v) {
+        const int b = b0 + (v >> 4), rr = v & 15;
+        const int rloc = rloc_base + b * 32 + 8 * (rr >> 2) + 4 * hh + (rr & 3);
+        const int64_t rg = r_base + rloc;
+        if (s_rdeg[rloc] || (P.self_mode && rg == my_q + P.self_off && !P.include_self)) return;
+        const uint32_t bits = xch[tid * 33 + v];
+        K key;
+        if constexpr (sizeof(K) == 8) {
+          const uint32_t c = bits;
+          const uint32_t d = (uint32_t)s_rnnz[rloc] + nj - c;
+          key = (d == 0) ? KeyJ{1u, 1u} : KeyJ{c, d};
+        } else {
+          float sv;
+          __builtin_memcpy(&sv, &bits, 4);
+          if constexpr (MK == kKeyEuclid) {
+            key = euc_key(kw, qaux, s_raux[rloc], sv);
+            if (P.self_mode && rg == my_q) key = FLT_MAX;  // distance 0
+          } else if constexpr (MK == kKeyPearson) {
+            key = pear_key(kw, qaux, qaux2, s_raux[rloc], s_raux2[rloc], sv);
+            if (P.self_mode && rg == my_q) key = FLT_MAX;  // restored self correlation
           } else {
-            float sv;
-            __builtin_memcpy(&sv, &bits, 4);
-            if constexpr (MK == kKeyEuclid) {
-              key = euc_key(kw, qaux, s_raux[rloc], sv);
-              if (P.self_mode && rg == my_q) key = FLT_MAX;  // distance 0
-            } else if constexpr (MK == kKeyPearson) {
-              key = pear_key(kw, qaux, qaux2, s_raux[rloc], s_raux2[rloc], sv);
-              if (P.self_mode && rg == my_q) key = FLT_MAX;  // restored self correlation
-            } else {
-              key = sv;
-              if (P.self_mode && rg == my_q) key = 2.0f;  // restored self similarity
-            }
+            key = sv;
+            if (P.self_mode && rg == my_q) key = 2.0f;  // restored self similarity
           }
-          if (better(key, (uint32_t)rg, list[L - 1].key, list[L - 1].idx)) {
-            Entry<K> carry{key, (uint32_t)rg};
+        }
+        if (better(key, (uint32_t)rg, list[L - 1].key, list[L - 1].idx)) {
+          Entry<K> carry{key, (uint32_t)rg};
 #pragma unroll
-            for (int p = 0; p < L; ++p) {
-              if (better(carry.key, carry.idx, list[p].key, list[p].idx)) {
-                const Entry<K> tmp = list[p];
-                list[p] = carry;
-                carry = tmp;
-              }
+          for (int p = 0; p < L; ++p) {
+            if (better(carry.key, carry.idx, list[p].key, list[p].idx)) {
+              const Entry<K> tmp = list[p];
+              list[p] = carry;
+              carry = tmp;
             }
           }
+        }
+      };
+      if (q_ok) {
+        if constexpr (kListScreen) {
+          uint32_t m = xch[tid * 33 + 32];
+          while (m) {
+            const int v = __builtin_ctz(m);
+            m &= m - 1u;
+            visit(v);
+          }
+        } else {
+#pragma unroll 1
+          for (int v = 0; v < 32; ++v) visit(v);
         }
       }
       lds_barrier();
