@@ -522,8 +522,12 @@ int fused_candidate_lists(sd_context* ctx, FusedArgs fa, const double** out_key,
   auto list_pass = [&](FusedArgs a, DevBuf& kbuf, DevBuf& ibuf, DevBuf& mkbuf, DevBuf& mibuf,
                        const double** rk, const int32_t** ri) -> int {
     const int64_t qtiles = (a.nqry + 255) / 256;
+    // two workgroups per CU at least, the references split in multiples of
+    // 8 (one per XCD); query sets of eight query tiles per CU or more fill
+    // the GPU with one split, whose lists then need no merge
     int64_t splits = (2 * (int64_t)ctx->cu_count + qtiles - 1) / qtiles;
-    a.splits = (int)std::min<int64_t>(round_up(std::max<int64_t>(splits, 1), 8), 4096);
+    splits = qtiles >= 8 * (int64_t)ctx->cu_count ? 1 : round_up(std::max<int64_t>(splits, 1), 8);
+    a.splits = (int)std::min<int64_t>(splits, 4096);
     const int nlists = fused_effective_splits(a);
     const int La = a.L;
     SD_TRY(kbuf.ensure(sizeof(double) * (size_t)nlists * (size_t)a.nqry * (size_t)La + 64));
