@@ -349,7 +349,7 @@ __global__ __launch_bounds__(WM * 256, 1) void k_fused(FusedKernelArgs P) {
   __shared__ __align__(16) uint8_t st1[STAGE_BYTES];
   __shared__ __align__(16) uint8_t st2[STAGE_BYTES];
   uint32_t* xch = reinterpret_cast<uint32_t*>(st0);
-  __shared__ int32_t s_rnnz[TM];
+  __shared__ __align__(16) int32_t s_rnnz[TM];  // read four at a time by the Jaccard marks
   __shared__ float s_raux[TM];
   __shared__ float s_raux2[TM];
   __shared__ uint8_t s_rdeg[TM];
@@ -574,27 +574,55 @@ __global__ __launch_bounds__(WM * 256, 1) void k_fused(FusedKernelArgs P) {
     // (the other four rows of every group of eight).  Values pass through
     // LDS (rows of 33 words against bank conflicts) so the scan is a loop.
     const int rloc_base = wm * kTileRefs;
-    // Plain float keys (cosine): the lane that stores a row's 32 values also
-    // marks those at or above the owner's current L-th key in the row's
-    // spare word, and the owner visits the marked values only.  The L-th key
-    // only rises during the scan, so every value the list takes is marked.
-    // A tile that may hold a query's own column marks everything (that
-    // value's key is restored above its slab value).
-    constexpr bool kListScreen = MK == kKeyScore && sizeof(K) != 8;
+    // Plain float keys (cosine) and Jaccard: the lane that stores a row's 32
+    // values also marks those at or above the owner's current L-th key in the
+    // row's spare word, and the owner visits the marked values only.  The
+    // L-th key only rises during the scan, so every value the list takes is
+    // marked.  Jaccard marks use the one-FMA screen c >= a n_r + b with a and
+    // b from the owner's L-th key (jac_screen_coef, which never drops a count
+    // whose exact key reaches the threshold).  A tile that may hold a query's
+    // own column marks everything (that value's key is restored above its
+    // slab value).
+    constexpr bool kListScreen = MK == kKeyScore;
     auto diag = [&](int64_t off) { return r_base < q_base + off + TN && q_base + off < r_base + TM; };
     const bool mark_all = P.self_mode && (P.qlist != nullptr || diag(0) || diag(P.self_off));
 #pragma unroll
     for (int sub = 0; sub < 4; ++sub) {
       const int phase = sub >> 1, b0 = (sub & 1) * 2;
-      float thr_w = 0.f;  // the L-th key of the row's owner (this lane, then lane l ^ 32)
-      if constexpr (kListScreen) {
+      // the row owner's (this lane, then lane l ^ 32) L-th key as a float
+      // threshold, or its Jaccard screen coefficients
+      float thr_w = 0.f, ja_w = 0.f, jb_w = 0.f;
+      if constexpr (kListScreen && sizeof(K) == 8) {
+        const KeyJ lk = list[L - 1].key;  // d >= 1
+        const float f = (float)((double)lk.c / (double)lk.d);
+        const float a_own = jac_screen_coef(f - fabsf(f) * 9.5367431640625e-07f - 1e-30f);
+        const float b_own = a_own * (float)nj;
+        const float a_oth = __shfl_xor(a_own, 32), b_oth = __shfl_xor(b_own, 32);
+        ja_w = (phase == 0) ? a_own : a_oth;
+        jb_w = (phase == 0) ? b_own : b_oth;
+      } else if constexpr (kListScreen) {
         const float own = (float)list[L - 1].key;
         const float other = __shfl_xor(own, 32);
         thr_w = (phase == 0) ? own : other;
       }
       uint32_t marks = 0u;
 #pragma unroll
-      for (int bb = 0; bb < 2; ++bb)
+      for (int bb = 0; bb < 2; ++bb) {
+        // stored entries of the references of this lane's values: the row
+        // owner's values of a block are the references of this lane's layout
+        // (32 b + 8 (r / 4) + 4 h + r % 4) in either phase
+        int32_t rn[16];
+        if constexpr (kListScreen && sizeof(K) == 8) {
+#pragma unroll
+          for (int g = 0; g < 4; ++g) {
+            const int4 n4 =
+                *reinterpret_cast<const int4*>(&s_rnnz[rloc_base + (b0 + bb) * 32 + 8 * g + 4 * h]);
+            rn[4 * g] = n4.x;
+            rn[4 * g + 1] = n4.y;
+            rn[4 * g + 2] = n4.z;
+            rn[4 * g + 3] = n4.w;
+          }
+        }
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
           const auto v0 = acc[b0 + bb][0][r], v1 = acc[b0 + bb][1][r];
@@ -605,8 +633,12 @@ __global__ __launch_bounds__(WM * 256, 1) void k_fused(FusedKernelArgs P) {
           else
             __builtin_memcpy(&bits, &x, 4);
           xch[((phase == 0) ? tid : (tid ^ 32)) * 33 + bb * 16 + r] = bits;
-          if constexpr (kListScreen) marks |= ((float)x >= thr_w) ? 1u << (bb * 16 + r) : 0u;
+          if constexpr (kListScreen && sizeof(K) == 8)
+            marks |= ((float)x >= fmaf(ja_w, (float)rn[r], jb_w)) ? 1u << (bb * 16 + r) : 0u;
+          else if constexpr (kListScreen)
+            marks |= ((float)x >= thr_w) ? 1u << (bb * 16 + r) : 0u;
         }
+      }
       if constexpr (kListScreen)
         xch[((phase == 0) ? tid : (tid ^ 32)) * 33 + 32] = mark_all ? 0xFFFFFFFFu : marks;
       lds_barrier();

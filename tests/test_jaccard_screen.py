@@ -1,6 +1,9 @@
-"""The filtered pass's one-FMA Jaccard screen (jac_screen_coef in
+"""The one-FMA Jaccard screen (jac_screen_coef in
 sparsedist_amd/csrc/k_slab.hip), checked against an emulation of its FP32
-arithmetic on the CPU.
+arithmetic on the CPU.  The filtered pass screens against each query's
+threshold with it, and the list-mode pass marks the values it visits with it
+against the list's current L-th key (an exact rational whose FP64 quotient is
+the T below).
 
 A candidate with intersection c of a reference column with ni stored entries
 and a query with nj enters the buffers when its exact key, the FP64 quotient
