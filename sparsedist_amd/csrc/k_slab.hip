@@ -1578,7 +1578,9 @@ __device__ inline uint64_t wave_shfl_u64(uint64_t v, int src) {
 // than the current L-th is skipped, otherwise it is sorted across the wave
 // (bitonic), merged with the list (the elementwise minimum against the
 // reversed chunk is a bitonic sequence holding the 64 best of both) and
-// cleaned back into order.  Over-full and short buffers are listed for a full
+// cleaned back into order; a chunk with at most 12 such entries inserts them
+// one by one at their ranks instead.  Lanes below L always hold the L best
+// so far; the lanes above them stay in order.  Over-full and short buffers are listed for a full
 // pass as in k_select.
 __global__ __launch_bounds__(256) void k_select_wave(
     const double* __restrict__ fkey, const int32_t* __restrict__ fidx,
@@ -1626,7 +1628,37 @@ __global__ __launch_bounds__(256) void k_select_wave(
       ck = kNone;
       ci = 0xFFFFFFFFu;
     }
-    if (__ballot(ck != kNone) == 0) continue;
+    const uint64_t pass = __ballot(ck != kNone);
+    if (pass == 0) continue;
+    if (__popcll(pass) <= 12) {
+      // a few entries (every chunk after the first two or so): insert each
+      // at its rank in the list, the entries from that rank on moving up one
+      // lane, which costs less than sorting and merging the chunk
+      uint64_t m = pass;
+      const int prev = lane > 0 ? lane - 1 : 0;
+      while (m) {
+        const int src = (int)__builtin_ctzll(m);
+        m &= m - 1;
+        const uint64_t nk = wave_shfl_u64(ck, src);
+        const uint32_t ni = (uint32_t)__shfl((int)ci, src);
+        const double nv = __shfl(cv, src);
+        const int rank = (int)__popcll(__ballot(less(lk, li, nk, ni)));
+        if (rank >= L) continue;  // no longer among the L best (wave-uniform)
+        const uint64_t pk = wave_shfl_u64(lk, prev);
+        const uint32_t pi = (uint32_t)__shfl((int)li, prev);
+        const double pv = __shfl(lv, prev);
+        if (lane > rank) {
+          lk = pk;
+          li = pi;
+          lv = pv;
+        } else if (lane == rank) {
+          lk = nk;
+          li = ni;
+          lv = nv;
+        }
+      }
+      continue;
+    }
     // sort the chunk ascending across the wave
     for (int size = 2; size <= 64; size <<= 1) {
       for (int stride = size >> 1; stride > 0; stride >>= 1) {
