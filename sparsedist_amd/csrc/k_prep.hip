@@ -215,6 +215,12 @@ void matrix_forget_derived(sd_matrix* m) {
   for (int t = 0; t < kDenseKinds; ++t) m->dense_kpad[t] = 0;
 }
 
+__global__ void k_rows16(const int32_t* __restrict__ in, int64_t n, uint16_t* __restrict__ out) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n;
+       e += (int64_t)gridDim.x * blockDim.x)
+    out[e] = (uint16_t)in[e];
+}
+
 int prep_unit(sd_context* ctx, sd_matrix* m) {
   if (m->unit_ready) return SD_OK;
   SD_TRY(m->unit.ensure(sizeof(double) * (size_t)(m->nnz > 0 ? m->nnz : 1)));
@@ -228,6 +234,11 @@ int prep_unit(sd_context* ctx, sd_matrix* m) {
   hipLaunchKernelGGL(k_unit_vals, dim3((unsigned)nw), dim3(256), 0, ctx->stream, m->p.as<int64_t>(),
                      m->x.as<double>(), m->ncol, m->scale.as<double>(), sq.as<double>(),
                      m->unit.as<double>());
+  if (matrix_rows16(m)) {
+    SD_TRY(m->rows16.ensure(sizeof(uint16_t) * (size_t)m->nnz + 16));
+    hipLaunchKernelGGL(k_rows16, dim3(stride_grid(m->nnz)), dim3(256), 0, ctx->stream,
+                       m->i.as<int32_t>(), m->nnz, m->rows16.as<uint16_t>());
+  }
   SD_HIP(hipGetLastError());
   SD_TRY(sync_stream(ctx));
   m->unit_ready = true;

@@ -1727,6 +1727,10 @@ __host__ __device__ inline int rerank_wave_bytes(int nwords) {
   return ((((nwords + 3) & ~3) * 4 + ((nwords + 7) & ~7) * 2) + 15) & ~15;
 }
 
+// kRows16: the candidates' row indices are read from the uint16 copy (a
+// separate instantiation, so the int32 walk keeps its register count and
+// occupancy)
+template <bool kRows16>
 __global__ __launch_bounds__(256) void k_rerank(RerankArgs R) {
   extern __shared__ __align__(16) uint8_t rsm[];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -1804,30 +1808,63 @@ __global__ __launch_bounds__(256) void k_rerank(RerankArgs R) {
               if (word & bit) acc += ua[e] * qv[pre[r >> 5] + __popc(word & (bit - 1u))];
             };
             int64_t e = 0;
-            const int64_t head = min(na, (int64_t)((4 - (a0 & 3)) & 3));
-            for (; e < head; ++e) step(ra[e], e);
-            // eight 16-byte loads in flight per lane before the walk uses them
-            for (; e + 32 <= na; e += 32) {
-              const int4* r32 = reinterpret_cast<const int4*>(ra + e);
-              int4 x[8];
+            if constexpr (kRows16) {
+              // the same walk over the uint16 copy of the row indices (8 per
+              // 16-byte load, entry a0 + e aligned when (a0 + e) % 8 == 0)
+              const uint16_t* r16 = R.ex.ref_rows16 + a0;
+              const int64_t head16 = min(na, (int64_t)((8 - (a0 & 7)) & 7));
+              for (; e < head16; ++e) step(r16[e], e);
+              for (; e + 64 <= na; e += 64) {
+                const uint4* p64 = reinterpret_cast<const uint4*>(r16 + e);
+                uint4 x[8];
 #pragma unroll
-              for (int u = 0; u < 8; ++u) x[u] = r32[u];
+                for (int u = 0; u < 8; ++u) x[u] = p64[u];
 #pragma unroll
-              for (int u = 0; u < 8; ++u) {
-                step(x[u].x, e + 4 * u);
-                step(x[u].y, e + 4 * u + 1);
-                step(x[u].z, e + 4 * u + 2);
-                step(x[u].w, e + 4 * u + 3);
+                for (int u = 0; u < 8; ++u) {
+                  const uint32_t w[4] = {x[u].x, x[u].y, x[u].z, x[u].w};
+#pragma unroll
+                  for (int t = 0; t < 4; ++t) {
+                    step((int32_t)(w[t] & 0xFFFFu), e + 8 * u + 2 * t);
+                    step((int32_t)(w[t] >> 16), e + 8 * u + 2 * t + 1);
+                  }
+                }
               }
+              for (; e + 8 <= na; e += 8) {
+                const uint4 x = *reinterpret_cast<const uint4*>(r16 + e);
+                const uint32_t w[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                  step((int32_t)(w[t] & 0xFFFFu), e + 2 * t);
+                  step((int32_t)(w[t] >> 16), e + 2 * t + 1);
+                }
+              }
+              for (; e < na; ++e) step(r16[e], e);
+            } else {
+              const int64_t head = min(na, (int64_t)((4 - (a0 & 3)) & 3));
+              for (; e < head; ++e) step(ra[e], e);
+              // eight 16-byte loads in flight per lane before the walk uses them
+              for (; e + 32 <= na; e += 32) {
+                const int4* r32 = reinterpret_cast<const int4*>(ra + e);
+                int4 x[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) x[u] = r32[u];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                  step(x[u].x, e + 4 * u);
+                  step(x[u].y, e + 4 * u + 1);
+                  step(x[u].z, e + 4 * u + 2);
+                  step(x[u].w, e + 4 * u + 3);
+                }
+              }
+              for (; e + 4 <= na; e += 4) {
+                const int4 r4 = *reinterpret_cast<const int4*>(ra + e);
+                step(r4.x, e);
+                step(r4.y, e + 1);
+                step(r4.z, e + 2);
+                step(r4.w, e + 3);
+              }
+              for (; e < na; ++e) step(ra[e], e);
             }
-            for (; e + 4 <= na; e += 4) {
-              const int4 r4 = *reinterpret_cast<const int4*>(ra + e);
-              step(r4.x, e);
-              step(r4.y, e + 1);
-              step(r4.z, e + 2);
-              step(r4.w, e + 3);
-            }
-            for (; e < na; ++e) step(ra[e], e);
           }
           c = acc;
           if (!(c == c)) c = 0.0;
@@ -2383,8 +2420,13 @@ int rerank_map_words(int64_t nrow) {
 int rerank_cosine(sd_context* ctx, const RerankArgs& R) {
   if (R.nqry <= 0) return SD_OK;
   KernelTimer timer(ctx, &ctx->stats.rerank_ms, &ctx->stats.rerank_launches);
-  hipLaunchKernelGGL(k_rerank, dim3((unsigned)((R.nqry + 3) / 4)), dim3(256),
-                     R.map_words > 0 ? 4 * rerank_wave_bytes(R.map_words) : 0, ctx->stream, R);
+  const unsigned lds = R.map_words > 0 ? 4 * rerank_wave_bytes(R.map_words) : 0;
+  if (R.ex.ref_rows16)
+    hipLaunchKernelGGL(k_rerank<true>, dim3((unsigned)((R.nqry + 3) / 4)), dim3(256), lds,
+                       ctx->stream, R);
+  else
+    hipLaunchKernelGGL(k_rerank<false>, dim3((unsigned)((R.nqry + 3) / 4)), dim3(256), lds,
+                       ctx->stream, R);
   SD_HIP(hipGetLastError());
   return SD_OK;
 }

@@ -414,6 +414,7 @@ ExactArgs exact_args(const sd_matrix* ref, const sd_matrix* qry, int method, int
   a.qry = qry->view();
   a.ref_unit = ref->unit.as<double>();
   a.ref_scale = ref->scale.as<double>();
+  a.ref_rows16 = (ref->unit_ready && matrix_rows16(ref)) ? ref->rows16.as<uint16_t>() : nullptr;
   a.qry_unit = qry->unit.as<double>();
   a.qry_scale = qry->scale.as<double>();
   a.ref_mean = ref->mean.as<double>();
@@ -1406,6 +1407,7 @@ int sd_matrix_drop_cache(sd_matrix* m) {
   m->unit.release();
   m->scale.release();
   m->unit_ssq.release();
+  m->rows16.release();
   m->unit_ready = false;
   m->mean.release();
   m->css.release();

@@ -234,6 +234,9 @@ struct sd_matrix {
   // cosine: unit-normalised values (nnz) and per-column max |v|
   bool unit_ready = false;
   sd::DevBuf unit, scale, unit_ssq;
+  // row indices as uint16 (matrices of at most 65536 rows, built with the
+  // unit values): the cosine re-rank streams half the index bytes
+  sd::DevBuf rows16;
   // pearson / covariance: per-column mean and centred sum of squares, and
   // their FP32 forms for the slab epilogue (sqrt(nrow) mean, 1 / sqrt(css))
   bool centre_ready = false;
@@ -258,6 +261,13 @@ struct sd_matrix {
     return c;
   }
 };
+
+// whether prep_unit keeps the uint16 row indices: at most 65536 rows, and
+// some column not fully stored (fully stored columns are re-ranked over their
+// value arrays without row indices)
+inline bool matrix_rows16(const sd_matrix* m) {
+  return m->nrow <= 65536 && m->nnz > 0 && m->nnz < m->nrow * m->ncol;
+}
 
 struct sd_graph {
   int64_t n = 0, nnz = 0;
@@ -321,6 +331,7 @@ struct ExactArgs {
   const double* ref_mean = nullptr; const double* ref_css = nullptr;
   const double* qry_mean = nullptr; const double* qry_css = nullptr;
   const uint8_t* ref_deg = nullptr;   // nullptr: no masking
+  const uint16_t* ref_rows16 = nullptr;  // ref.i as uint16, or nullptr
   int self_mode = 0;            // query j is reference column j + self_off
   int64_t self_off = 0;
   int include_self = 0;
