@@ -524,10 +524,14 @@ int fused_candidate_lists(sd_context* ctx, FusedArgs fa, const double** out_key,
                        const double** rk, const int32_t** ri) -> int {
     const int64_t qtiles = (a.nqry + 255) / 256;
     // two workgroups per CU at least, the references split in multiples of
-    // 8 (one per XCD); query sets of eight query tiles per CU or more fill
-    // the GPU with one split, whose lists then need no merge
+    // 8 (one per XCD); query sets of eight query tiles per CU or more with
+    // short rows (2 KB or less) fill the GPU with one split, whose lists then
+    // need no merge.  With long rows one split was slower: each workgroup
+    // streams every sampled reference tile (7.7 MB each at 30,000 rows).
     int64_t splits = (2 * (int64_t)ctx->cu_count + qtiles - 1) / qtiles;
-    splits = qtiles >= 8 * (int64_t)ctx->cu_count ? 1 : round_up(std::max<int64_t>(splits, 1), 8);
+    const bool one_split =
+        qtiles >= 8 * (int64_t)ctx->cu_count && dense_row_bytes(a.kind, a.kpad) <= 2048;
+    splits = one_split ? 1 : round_up(std::max<int64_t>(splits, 1), 8);
     a.splits = (int)std::min<int64_t>(splits, 4096);
     const int nlists = fused_effective_splits(a);
     const int La = a.L;
