@@ -258,8 +258,8 @@ int context_use_pool(sd_context* ctx) {
 }
 
 void matrix_use_pool(sd_matrix* m, hipStream_t stream) {
-  for (DevBuf* b : {&m->p, &m->i, &m->x, &m->deg, &m->unit, &m->scale, &m->unit_ssq, &m->mean,
-                    &m->css, &m->cmf, &m->cif, &m->sqn, &m->sqf})
+  for (DevBuf* b : {&m->p, &m->i, &m->x, &m->deg, &m->unit, &m->scale, &m->unit_ssq, &m->rows16,
+                    &m->mean, &m->css, &m->cmf, &m->cif, &m->sqn, &m->sqf})
     b->pool = stream;
   for (auto& b : m->dense) b.pool = stream;
 }

@@ -304,6 +304,9 @@ std::vector<const void*> graph_ptrs(const sd_context* c, const sd_matrix* X, con
       v.push_back(b->ptr);
     for (const sd::DevBuf& b : m->dense) v.push_back(b.ptr);
   }
+  // the uint16 row indices are read on the reference side only (the cosine
+  // re-rank); the query's copy moves as it grows and is not watched
+  v.push_back((Y ? Y : X)->rows16.ptr);
   return v;
 }
 
