@@ -215,8 +215,9 @@ int sd_tune_samples(sd_tune_sample* out, int32_t max, int32_t* n, int32_t reset)
 /* Copy a CSC matrix to device memory.  Replaces the per-call dgCMatrix ->
  * arma::sp_mat conversion at the reference's .Call boundary
  * (src/RcppExports.cpp:124-137, 229-242; R/sparseDist.R:198-202): the matrix
- * stays resident and its derived layouts (norms, densified tiles) are cached
- * with it. */
+ * stays resident and its derived layouts (norms, densified tiles, and for
+ * cosine on matrices of at most 65536 rows a uint16 copy of the row indices,
+ * 2 bytes per stored entry) are cached with it. */
 int sd_matrix_upload_csc(sd_context* ctx, int64_t nrow, int64_t ncol, const int64_t* colptr,
                          const int32_t* rowind, const double* values, sd_matrix** out);
 int sd_matrix_free(sd_matrix* m);
