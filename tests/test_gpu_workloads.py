@@ -132,3 +132,29 @@ def test_dense_embedding_workload_cross(oracle_lib, gpu_ctx):
     orv = oracle_lib.knn(Ys[:, sr], Es, k=10, method="cosine", dist=False, nthreads=NT)
     same((cr["forward"].idx[sf], cr["forward"].dist[sf]), of, "forward vs oracle")
     same((cr["reverse"].idx[sr], cr["reverse"].dist[sr]), orv, "reverse vs oracle")
+
+
+def test_large_query_set_one_split_prepass(oracle_lib, gpu_ctx):
+    """A short-row query set of 2048 query tiles (eight per CU on 256 CUs)
+    runs the threshold pre-pass as one split per query tile, whose lists need
+    no merge: 524,288 256-dimensional query columns against 16,384 references
+    (128 reference tiles, so the search takes the two-pass path).  Sampled
+    rows equal the oracle's bit for bit."""
+    nq, nr = 2048 * 256, 16384
+    Q = synth.unit_normal_dense(256, nq, 531).astype(np.float64)
+    Y = synth.unit_normal_dense(256, nr, 532).astype(np.float64)
+    dQ, dY = sd.DeviceMatrix(Q, gpu_ctx), sd.DeviceMatrix(Y, gpu_ctx)
+    gpu_ctx.stats(reset=True)
+    gpu_ctx.set_timing(True)  # launches are counted by the kernel timers
+    try:
+        nn = sd.sparseKNN(dQ, dY, k=10, method="cosine", verbose=False, context=gpu_ctx)
+    finally:
+        gpu_ctx.set_timing(False)
+    st = gpu_ctx.stats(reset=True)
+    assert st["filter_launches"] == 1 and st["certified_queries"] == nq, st
+    rng = np.random.default_rng(533)
+    s = np.sort(np.concatenate([rng.choice(nq, 298, replace=False), [0, nq - 1]]))
+    s = np.unique(s)
+    o = oracle_lib.knn(sp.csc_matrix(Q[:, s]), sp.csc_matrix(Y), k=10, method="cosine",
+                       nthreads=NT)
+    same((nn.idx[s], nn.dist[s]), o, "one-split pre-pass vs oracle")
